@@ -1,0 +1,220 @@
+/*
+ * spops_hip.h -- C ABI of libsparseops_hip.so, the MI355X (gfx950) implementation of the
+ * SparseOps hot path: SpMV in CSR / ELL / SELL-C-sigma / SELL-C-R / BSR and the LeSpGEMM
+ * pipeline (MinHash-LSH -> hierarchical clustering -> cluster-wise SpGEMM), plus batched
+ * SpMM and the SpMM/SDDMM pair used by autograd.
+ *
+ * Plain C: pointers, sizes and int status codes; no HIP or torch types.  `stream` is a
+ * hipStream_t passed as void* (NULL = the null stream).  Device pointers are marked d_.
+ * Every entry point returns SPOPS_SUCCESS (0) or an error code; spops_last_error()
+ * gives the detail for the calling thread.
+ *
+ * Value storage types and the vector types that go with them:
+ *   SPOPS_VALUE_F64  : values double, x/y double, FP64 accumulation
+ *   SPOPS_VALUE_F32  : values float,  x/y float,  FP32 accumulation (compensated on long rows)
+ *   SPOPS_VALUE_BF16 : values bf16,   x/y float,  FP32 accumulation (compensated on long rows)
+ *
+ * Reference interfaces replaced (lsl036/SparseOps):
+ *   SpOps_csr          include/spmv_csr.h:20,        src/spmv_csr.cpp:165
+ *   SpOps_ell          include/spmv_ell.h:53,        src/spmv_ell.cpp:225
+ *   SpOps_sell_c_sigma include/spmv_sell_c_sigma.h:61, src/spmv_sell_c_sigma.cpp:190
+ *   SpOps_sell_c_R     include/spmv_sell_c_R.h:61,   src/spmv_sell_c_R.cpp:190
+ *   SpOps_bsr          include/spmv_bsr.h:59,        src/spmv_bsr.cpp:215
+ *   (each entry below cites the one it backs)
+ */
+#ifndef SPOPS_HIP_H
+#define SPOPS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+#define SPOPS_SUCCESS               0
+#define SPOPS_ERROR_INVALID_VALUE   1   /* bad argument (shape, type code, null pointer)  */
+#define SPOPS_ERROR_OUT_OF_MEMORY   2   /* device or host allocation failed               */
+#define SPOPS_ERROR_HIP             3   /* a HIP runtime call failed                        */
+#define SPOPS_ERROR_NOT_SUPPORTED   4   /* valid request this build does not implement      */
+#define SPOPS_ERROR_LIMIT           5   /* exceeds a documented size limit                  */
+
+#define SPOPS_INDEX_I32  0
+#define SPOPS_INDEX_I64  1
+#define SPOPS_VALUE_F64  0
+#define SPOPS_VALUE_F32  1
+#define SPOPS_VALUE_BF16 2
+
+#define SPOPS_FORMAT_CSR   0
+#define SPOPS_FORMAT_SELL  1   /* SELL-C-sigma (SELL-C-R is sigma = n_rows) */
+#define SPOPS_FORMAT_ELL   2
+#define SPOPS_FORMAT_BSR   3
+
+const char *spops_status_string(int status);
+const char *spops_last_error(void);
+int spops_version(int *major, int *minor, int *patch);
+
+/* ---- device memory and streams (so C++ callers need no HIP headers) --------------------- */
+int spops_device_count(int *count);
+int spops_set_device(int device);
+int spops_malloc(void **d_ptr, size_t bytes);
+int spops_free(void *d_ptr);
+int spops_memcpy_htod(void *d_dst, const void *src, size_t bytes, void *stream);
+int spops_memcpy_dtoh(void *dst, const void *d_src, size_t bytes, void *stream);
+int spops_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes, void *stream);
+int spops_memset(void *d_ptr, int value, size_t bytes, void *stream);
+int spops_stream_synchronize(void *stream);
+int spops_stream_create(void **stream);
+int spops_stream_destroy(void *stream);
+/* events for kernel timing on a given stream */
+int spops_event_create(void **event);
+int spops_event_destroy(void *event);
+int spops_event_record(void *event, void *stream);
+int spops_event_elapsed_ms(float *ms, void *start, void *stop);
+
+/* ---- SpMV matrices ------------------------------------------------------------------------
+ * A spops_matrix is a device-resident, immutable matrix in one format with its launch plan
+ * (row blocks, long-row lists, chunk descriptors).  Creation uploads and converts; it is
+ * the place where the reference's csr_to_* conversions happen.  Every creator takes
+ * HOST arrays; values are given as `in_value_type` (F64 or F32 host arrays) and stored as
+ * `value_type` (F64, F32 or BF16, rounded to nearest even).  Row/column indices are `index_type` (int32 or int64 host arrays). */
+typedef struct spops_matrix_s *spops_matrix;
+
+/* CSR (include/sparse_format.h:96-104). Backs SpOps_csr. */
+int spops_csr_create(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t nnz,
+                     int index_type, const void *rowptr, const void *col,
+                     const void *values, int in_value_type, int value_type);
+
+/* SELL-C-sigma from CSR with the reference's ordering: each sigma-slice sorted by
+ * (nnz desc, row desc), chunks of C sorted rows (include/sparse_conversion.h:345-487).
+ * sigma = n_rows gives SELL-C-R (:491-606).  sigma % C != 0 -> SPOPS_ERROR_INVALID_VALUE.
+ * Backs csr_to_sell_c_sigma + SpOps_sell_c_sigma and csr_to_sell_c_R + SpOps_sell_c_R. */
+int spops_sell_create(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t nnz,
+                      int index_type, const void *rowptr, const void *col,
+                      const void *values, int in_value_type, int value_type, int64_t sigma, int64_t C);
+
+/* SELL-C-sigma from the reference's own host layout (SELL_C_Sigma_Matrix,
+ * include/sparse_format.h:245-272): reorder[n_rows], chunk_len[nchunks], and per chunk
+ * arrays col[chunk][chunk_len*C], val[chunk][chunk_len*C] row-major inside the chunk with
+ * col = -1 padding.  `in_value_type` (F64 or F32) is the host value type.  Indices int32
+ * or int64 per index_type. */
+int spops_sell_create_from_chunks(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t nnz,
+                                  int64_t C, int64_t nchunks, int index_type,
+                                  const void *reorder, const void *chunk_len,
+                                  const void *const *chunk_col, const void *const *chunk_val,
+                                  int in_value_type, int value_type);
+
+/* ELL (include/sparse_format.h:192-206, csr_to_ell :195-256). Backs SpOps_ell. */
+int spops_ell_create(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t nnz,
+                     int index_type, const void *rowptr, const void *col,
+                     const void *values, int in_value_type, int value_type);
+
+/* BSR with bR x bC row-major blocks (include/sparse_format.h:113-131,
+ * csr_to_bsr :777-934). Backs SpOps_bsr. */
+int spops_bsr_create(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t nnz,
+                     int index_type, const void *rowptr, const void *col,
+                     const void *values, int in_value_type, int value_type, int64_t bR, int64_t bC);
+
+/* y = alpha*A*x + beta*y on the device; beta == 0 does not read y (BLAS convention;
+ * the reference's OMP kernels read y there, src/spmv_csr.cpp:119).  x/y element type
+ * follows the value type (see top).  Asynchronous on `stream`. */
+int spops_spmv(spops_matrix A, double alpha, const void *d_x, double beta, void *d_y, void *stream);
+
+/* Y = A*X (+ beta*Y) for k right-hand sides, X row-major (n_cols x k), Y (n_rows x k),
+ * CSR and SELL matrices.  x/y element types as for spops_spmv. */
+int spops_spmm(spops_matrix A, int64_t k, double alpha, const void *d_X, double beta, void *d_Y, void *stream);
+
+/* info[0]=format info[1]=n_rows info[2]=n_cols info[3]=nnz info[4]=value_type
+ * info[5]=device bytes owned info[6]=algorithmic bytes of one SpMV (no padding)
+ * info[7]=stored slots (incl. padding) info[8]=long rows info[9]=wave tasks
+ * info[10]=C info[11]=sigma info[12]=bR info[13]=bC info[14]=nchunks info[15]=index_type */
+int spops_matrix_info(spops_matrix A, int64_t info[16]);
+
+/* Device copies of the layout arrays a format conversion produced, for parity checks:
+ * which=0: reorder (int32, n_rows; SELL only); which=1: chunk_len in the reference's
+ * sense (int32, nchunks; SELL only, rounded to the reference alignment 256/8/sizeof(V)).
+ * `host_out` must hold the element count given by spops_matrix_info. */
+int spops_matrix_layout(spops_matrix A, int which, void *host_out);
+
+int spops_matrix_destroy(spops_matrix A);
+
+/* ---- LeSpGEMM preprocessing: MinHash + LSH (device) --------------------------------------- */
+typedef struct spops_pairs_s *spops_pairs;
+
+/* d_sigs[r*k + d] = min over the columns c of row r of (a_d*c + b_d) mod 2^64, a_d/b_d from
+ * the LCG of include/spgemm_MinHashLSH.h:40-48 seeded with `seed`; empty rows UINT64_MAX.
+ * Backs minhash_signatures_flat (include/spgemm_MinHashLSH.h:119-180).  k <= 256. */
+int spops_minhash_signatures(const void *d_rowptr, const void *d_col, int index_type, int64_t n_rows, int k,
+                             uint64_t seed, uint64_t *d_sigs, void *stream);
+
+/* Candidate pairs (i < j, sorted by (i,j), unique) of rows that share a band bucket, with
+ * score = matching signature dimensions / k stored as `score_type` (F64 or F32).  Buckets
+ * of more than 256 rows pair each row with its next 16 when `constrain` != 0.
+ * Backs lsh_candidate_pairs_vector (include/spgemm_MinHashLSH.h:777-791). */
+int spops_lsh_candidate_pairs(const void *d_rowptr, const void *d_col, int index_type, int64_t n_rows, int k,
+                              int bands, uint64_t seed, int constrain, int score_type, spops_pairs *out,
+                              void *stream);
+/* Same, from precomputed device signatures (lsh_candidate_pairs_from_flat_vector, :427-605). */
+int spops_lsh_pairs_from_signatures(const uint64_t *d_sigs, int64_t n_rows, int k, int bands, int constrain,
+                                    int score_type, spops_pairs *out, void *stream);
+int spops_pairs_count(spops_pairs P, int64_t *count);
+int spops_pairs_device(spops_pairs P, const int64_t **d_i, const int64_t **d_j, const void **d_score);
+int spops_pairs_download(spops_pairs P, int64_t *i, int64_t *j, void *score);
+int spops_pairs_destroy(spops_pairs P);
+
+/* ---- LeSpGEMM preprocessing: hierarchical clustering (host, sequential, bit-exact) --------
+ * version 0: hierarchical_clustering_v0_fast (include/spgemm_utility.h:743-915)
+ * version 1: hierarchical_clustering_v1      (include/spgemm_utility.h:609-733)
+ * Host arrays.  pairs (pi, pj, score) as produced by LSH (sorted by (i,j)); score_type F64
+ * or F32 selects the reference's ValueType.  perm[n_rows] receives the row permutation
+ * (perm[new] = original row), offsets must hold n_rows + 1 entries; *nclusters is set. */
+int spops_hierarchical_clustering(int version, int64_t n_rows, const int64_t *rowptr, const int64_t *col,
+                                  int64_t npairs, const int64_t *pi, const int64_t *pj, const void *score,
+                                  int score_type, int64_t cluster_size, int64_t *perm, int64_t *offsets,
+                                  int64_t *nclusters);
+
+/* ---- LeSpGEMM: cluster format and cluster-wise SpGEMM (device) ----------------------------
+ * A spops_cluster is a device CSR_VlengthCluster (include/spgemm_cluster.h:108-143):
+ * cluster c = rows offsets[c]..offsets[c+1] of the permuted matrix, its distinct columns
+ * (ascending) and a dense cluster_sz-wide value block per column. */
+typedef struct spops_cluster_s *spops_cluster;
+
+/* csr_reorder_rows + csr_to_vlength_cluster (include/sparse_conversion.h:1879, :1641).
+ * A on the device (int64 rowptr/col, values F64 or F32); perm[n_rows] and
+ * offsets[nclusters+1] on the host (as produced by spops_hierarchical_clustering). */
+int spops_cluster_create(spops_cluster *out, int64_t n_rows, int64_t n_cols, int64_t nnz, const int64_t *d_rowptr,
+                         const int64_t *d_col, const void *d_values, int value_type, const int64_t *perm,
+                         int64_t nclusters, const int64_t *offsets, void *stream);
+/* info[0]=clusters info[1]=csr rows info[2]=cols info[3]=nnzc info[4]=nnzv
+ * info[5]=max cluster size info[6]=value type info[7]=device bytes */
+int spops_cluster_info(spops_cluster A, int64_t info[8]);
+/* host copies (int64 indices; values in the cluster's value type); any pointer may be NULL */
+int spops_cluster_download(spops_cluster A, int64_t *cluster_sz, int64_t *rowptr, int64_t *rowptr_val,
+                           int64_t *colids, void *values);
+/* per-cluster symbolic results of a product (nz, min and max column), as kept by kernel 3 */
+int spops_cluster_symbolic_download(spops_cluster C, int64_t *nz, int64_t *cmin, int64_t *cmax);
+int spops_cluster_destroy(spops_cluster A);
+
+/* C = A * B, cluster-wise with the hybrid accumulator (LeSpGEMM_VLength kernel 3,
+ * src/spgemm.cpp:545-680).  B on the device: int64 rowptr/col, values of A's value type.
+ * C's columns ascend inside each cluster.  stats (may be NULL): [1]=nnzc [2]=nnzv
+ * [3]=LDS-path clusters [4]=HBM-path clusters [5..8]=setup/symbolic/malloc/numeric in us
+ * [9]=cluster-wise products. */
+int spops_lespgemm(spops_cluster A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *d_browptr,
+                   const int64_t *d_bcol, const void *d_bval, spops_cluster *C, int64_t stats[16], void *stream);
+
+/* vlength_cluster2csr + inverse permutation (include/sparse_conversion.h:1778-1862,
+ * test/test_spgemm_hc_lsh.cpp:296-303): C in the original row order, exact zeros dropped,
+ * columns ascending.  The three device arrays are allocated here; free with spops_free. */
+int spops_cluster_to_csr(spops_cluster C, spops_cluster A, int64_t *nnz, int64_t **d_rowptr, int64_t **d_col,
+                         void **d_val, void *stream);
+
+/* get_spgemm_flop (src/spgemm.cpp:24-42): 2 * sum over A entries (i,k), k < b_rows, of nnz(B_k). */
+int spops_spgemm_flop(int64_t a_nnz, const int64_t *d_acol, int64_t b_rows, const int64_t *d_browptr, int64_t *flops,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPOPS_HIP_H */

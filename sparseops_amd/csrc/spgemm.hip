@@ -1,0 +1,926 @@
+// Cluster-wise SpGEMM (LeSpGEMM, kernel 3 semantics) on MI355X.
+//
+// Reference path (lsl036/SparseOps):
+//   csr_reorder_rows + csr_to_vlength_cluster   include/sparse_conversion.h:1641-1976
+//   LeSpGEMM_mixed_acc                          src/spgemm.cpp:545-659
+//   spgemm_mixed_symbolic_omp_lb                src/spgemm_Vlength_mixed.cpp:197-261
+//   classify_clusters                           src/spgemm_Vlength_mixed.cpp:268-319
+//   spgemm_mixed_numeric_omp_lb                 src/spgemm_Vlength_mixed.cpp:326-434
+//   vlength_cluster2csr (+ inverse permutation) include/sparse_conversion.h:1778-1862,
+//                                               test/test_spgemm_hc_lsh.cpp:296-303
+//
+// Cluster matrix layout (identical meaning to CSR_VlengthCluster,
+// include/spgemm_cluster.h:108-143): cluster c holds csz = cluster_sz[c] consecutive rows
+// of the reordered matrix, its distinct columns colids[rowptr[c] .. rowptr[c+1]) and a
+// dense csz-wide value block per column at values[rowptr_val[c] + t*csz + l].  Here the
+// columns of each cluster are ascending (A: the reference iterates an unordered_map; C:
+// the reference's sortOutput=1 order), which changes only FP summation order.
+//
+// GPU design:
+//   * cluster build: keys (cluster << 32 | col) of every entry in reordered order, stable
+//     radix sort, head flags + scan give colids and the column offsets; values scatter.
+//   * per-cluster work flop_c = sum over its columns of nnz(B row) bins the clusters:
+//       small  (flop_c <= kSmallFlop, output block fits LDS): one workgroup, LDS hash
+//              table of columns, LDS bitonic sort, LDS f64 accumulator, coalesced store;
+//       large  : persistent workgroups, each owning a column bitmap in HBM; the bitmap
+//              gives the sorted column list and O(1) column ranks (word prefix counts),
+//              values accumulate with HBM f64 atomics into the zeroed output block.
+//   * symbolic (counts, min/max column) and numeric (columns + values) are separate
+//     passes; C's offsets come from a device scan in between.
+#include <algorithm>
+#include <memory>
+#include <numeric>
+
+#include "primitives.hpp"
+#include "spgemm_internal.hpp"
+
+namespace spops {
+
+namespace {
+
+inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+__global__ __launch_bounds__(kBlock) void k_narrow(const int64_t *__restrict__ in, int64_t n, int32_t *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = (int32_t)in[i];
+}
+
+// Loads that bypass the per-CU vector L1, for words other waves of the workgroup updated
+// with L2 atomics or stores in the same kernel.
+template <typename T>
+__device__ __forceinline__ T ld_l2(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------------------------
+// cluster build
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_row_cluster(const int64_t *__restrict__ offsets, int64_t nc,
+                                                        int32_t *__restrict__ cid, int32_t *__restrict__ lrow,
+                                                        int32_t *__restrict__ csz) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nc) return;
+  const int64_t a = offsets[c], b = offsets[c + 1];
+  csz[c] = (int32_t)(b - a);
+  for (int64_t t = a; t < b; ++t) { cid[t] = (int32_t)c; lrow[t] = (int32_t)(t - a); }
+}
+
+__global__ __launch_bounds__(kBlock) void k_reordered_len(const int64_t *__restrict__ rowptr, const int64_t *__restrict__ perm,
+                                                          int64_t n, int64_t *__restrict__ len) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  const int64_t r = perm[t];
+  len[t] = rowptr[r + 1] - rowptr[r];
+}
+
+// one wave per reordered row: key = cluster << 32 | col, payload = local row << 40 | entry
+__global__ __launch_bounds__(kBlock) void k_cluster_keys(const int64_t *__restrict__ rowptr, const int64_t *__restrict__ col,
+                                                         const int64_t *__restrict__ perm, const int64_t *__restrict__ rptr,
+                                                         const int32_t *__restrict__ cid, const int32_t *__restrict__ lrow,
+                                                         int64_t n, uint64_t *__restrict__ keys, uint64_t *__restrict__ pay) {
+  const int64_t t = (int64_t)__builtin_amdgcn_readfirstlane((int)blockIdx.x) * (kBlock / kWave) + wave_id();
+  if (t >= n) return;
+  const int64_t r = perm[t], p0 = rowptr[r], p1 = rowptr[r + 1], o = rptr[t];
+  const uint64_t c = (uint64_t)cid[t], l = (uint64_t)lrow[t];
+  for (int64_t j = p0 + lane_id(); j < p1; j += kWave) {
+    keys[o + (j - p0)] = (c << 32) | (uint64_t)col[j];
+    pay[o + (j - p0)] = (l << 40) | (uint64_t)j;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_key_heads(const uint64_t *__restrict__ keys, int64_t m, int64_t *__restrict__ head,
+                                                      int64_t *__restrict__ ccount) {
+  const int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (a >= m) return;
+  const bool h = (a == 0 || keys[a] != keys[a - 1]);
+  head[a] = h ? 1 : 0;
+  if (h) atomicAdd((unsigned long long *)&ccount[keys[a] >> 32], 1ull);
+}
+
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_cluster_fill(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ pay,
+                                                         const int64_t *__restrict__ uidx, int64_t m,
+                                                         const int64_t *__restrict__ crp, const int64_t *__restrict__ crpv,
+                                                         const int32_t *__restrict__ csz, const V *__restrict__ aval,
+                                                         int32_t *__restrict__ colids, V *__restrict__ values) {
+  const int64_t a = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (a >= m) return;
+  const uint64_t key = keys[a];
+  const int64_t c = (int64_t)(key >> 32);
+  const int64_t u = uidx[a] + ((a == 0 || key != keys[a - 1]) ? 0 : -1);   // unique index of this key
+  const int64_t t = u - crp[c];
+  const uint64_t p = pay[a];
+  const int64_t l = (int64_t)(p >> 40), j = (int64_t)(p & ((1ull << 40) - 1));
+  if (a == 0 || key != keys[a - 1]) colids[u] = (int32_t)(key & 0xffffffffull);
+  // a later entry of the same row and column overwrites (the reference's map assignment)
+  if (a + 1 < m && keys[a + 1] == key && (int64_t)(pay[a + 1] >> 40) == l) return;
+  values[crpv[c] + t * csz[c] + l] = aval[j];
+}
+
+__global__ __launch_bounds__(kBlock) void k_mul_csz(const int64_t *__restrict__ nz, const int32_t *__restrict__ csz, int64_t n,
+                                                    int64_t *__restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c < n) out[c] = nz[c] * (int64_t)csz[c];
+}
+
+// ---------------------------------------------------------------------------------------------
+// SpGEMM: work estimate
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_cluster_flop(const int64_t *__restrict__ crp, const int32_t *__restrict__ colids,
+                                                         const int64_t *__restrict__ brp, int64_t nc,
+                                                         int64_t *__restrict__ flop) {
+  const int64_t c = (int64_t)__builtin_amdgcn_readfirstlane((int)blockIdx.x) * (kBlock / kWave) + wave_id();
+  if (c >= nc) return;
+  int64_t f = 0;
+  for (int64_t t = crp[c] + lane_id(); t < crp[c + 1]; t += kWave) {
+    const int32_t b = colids[t];
+    f += brp[b + 1] - brp[b];
+  }
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) f += __shfl_xor(f, m, kWave);
+  if (lane_id() == 0) flop[c] = f;
+}
+
+// reference flop count (src/spgemm.cpp:24-42): 2 * sum over A entries of nnz(B row)
+__global__ __launch_bounds__(kBlock) void k_ref_flop(const int64_t *__restrict__ acol, int64_t annz, int64_t b_rows,
+                                                     const int64_t *__restrict__ brp, unsigned long long *__restrict__ out) {
+  __shared__ unsigned long long part[kBlock / kWave];
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  unsigned long long f = 0;
+  if (j < annz) {
+    const int64_t a = acol[j];
+    if (a < b_rows) f = (unsigned long long)(brp[a + 1] - brp[a]);
+  }
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) f += __shfl_xor(f, m, kWave);
+  if (lane_id() == 0) part[wave_id()] = f;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// small clusters: one workgroup each, everything in LDS
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t hslot(int32_t key, uint32_t mask) { return ((uint32_t)key * 107u) & mask; }
+
+// insert key; returns true when newly inserted
+__device__ __forceinline__ bool lds_insert(int32_t *tab, uint32_t mask, int32_t key) {
+  uint32_t h = hslot(key, mask);
+  for (;;) {
+    int32_t old = atomicCAS(&tab[h], -1, key);
+    if (old == -1) return true;
+    if (old == key) return false;
+    h = (h + 1) & mask;
+  }
+}
+
+__device__ __forceinline__ uint32_t lds_find(const int32_t *tab, uint32_t mask, int32_t key) {
+  uint32_t h = hslot(key, mask);
+  while (tab[h] != key) h = (h + 1) & mask;
+  return h;
+}
+
+// Walk all (t, k) items of cluster c: t over the cluster's columns, k over B row colids[t].
+// Waves take columns t in turn; lanes take entries of the B row.
+template <typename F>
+__device__ __forceinline__ void for_each_product(int64_t t0, int64_t t1, const int32_t *__restrict__ colids,
+                                                 const int64_t *__restrict__ brp, F &&f) {
+  for (int64_t t = t0 + wave_id(); t < t1; t += kBlock / kWave) {
+    const int32_t b = colids[t];
+    const int64_t k0 = brp[b], k1 = brp[b + 1];
+    for (int64_t k = k0 + lane_id(); k < k1; k += kWave) f(t, k);
+  }
+}
+
+// symbolic for small clusters: count distinct columns, min and max
+__global__ __launch_bounds__(kBlock) void k_sym_small(const int32_t *__restrict__ list, int64_t nlist,
+                                                      const int64_t *__restrict__ arp, const int32_t *__restrict__ acol,
+                                                      const int64_t *__restrict__ brp, const int32_t *__restrict__ bcol,
+                                                      int64_t *__restrict__ nz, int64_t *__restrict__ cmin,
+                                                      int64_t *__restrict__ cmax) {
+  __shared__ int32_t tab[kSmallTable];
+  __shared__ int cnt, mn, mx;
+  const int64_t c = list[blockIdx.x];
+  for (int i = threadIdx.x; i < kSmallTable; i += kBlock) tab[i] = -1;
+  if (threadIdx.x == 0) { cnt = 0; mn = INT32_MAX; mx = -1; }
+  __syncthreads();
+  for_each_product(arp[c], arp[c + 1], acol, brp, [&](int64_t, int64_t k) {
+    const int32_t key = bcol[k];
+    if (lds_insert(tab, kSmallTable - 1, key)) {
+      atomicAdd(&cnt, 1);
+      atomicMin(&mn, key);
+      atomicMax(&mx, key);
+    }
+  });
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    nz[c] = cnt;
+    cmin[c] = cnt ? mn : 0;
+    cmax[c] = cnt ? mx : 0;
+  }
+}
+
+// numeric for small clusters: sorted columns + LDS accumulation of the csz-wide blocks
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_num_small(const int32_t *__restrict__ list, int64_t nlist,
+                                                      const int64_t *__restrict__ arp, const int64_t *__restrict__ arpv,
+                                                      const int32_t *__restrict__ csz_a, const int32_t *__restrict__ acol,
+                                                      const V *__restrict__ aval, const int64_t *__restrict__ brp,
+                                                      const int32_t *__restrict__ bcol, const V *__restrict__ bval,
+                                                      const int64_t *__restrict__ crp, const int64_t *__restrict__ crpv,
+                                                      int32_t *__restrict__ ccol, V *__restrict__ cval) {
+  __shared__ int32_t tab[kSmallTable];
+  __shared__ int32_t rank[kSmallTable];
+  __shared__ int32_t keys[kSmallTable];
+  __shared__ V acc[kSmallAcc];
+  __shared__ int cnt;
+  const int64_t c = list[blockIdx.x];
+  const int csz = csz_a[c];
+  const int64_t nzc = crp[c + 1] - crp[c];
+  for (int i = threadIdx.x; i < kSmallTable; i += kBlock) { tab[i] = -1; keys[i] = INT32_MAX; }
+  for (int i = threadIdx.x; i < kSmallAcc; i += kBlock) acc[i] = V(0);
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  for_each_product(arp[c], arp[c + 1], acol, brp, [&](int64_t, int64_t k) {
+    const int32_t key = bcol[k];
+    if (lds_insert(tab, kSmallTable - 1, key)) keys[atomicAdd(&cnt, 1)] = key;
+  });
+  __syncthreads();
+  // bitonic sort of keys[0 .. P) with P = next power of two >= cnt (padding INT32_MAX)
+  const int n = cnt;
+  int P = 1;
+  while (P < n) P <<= 1;
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < P; i += kBlock) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const int32_t a = keys[i], b = keys[j];
+          if ((a > b) == up) { keys[i] = b; keys[j] = a; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    rank[lds_find(tab, kSmallTable - 1, keys[i])] = i;
+    ccol[crp[c] + i] = keys[i];
+  }
+  __syncthreads();
+  const int64_t a0 = arp[c], av0 = arpv[c];
+  for_each_product(a0, arp[c + 1], acol, brp, [&](int64_t t, int64_t k) {
+    const int r = rank[lds_find(tab, kSmallTable - 1, bcol[k])];
+    const V b = bval[k];
+    const V *a = aval + av0 + (t - a0) * csz;
+    for (int l = 0; l < csz; ++l) {
+      const V av = a[l];
+      if (av != V(0)) atomicAdd(&acc[r * csz + l], av * b);
+    }
+  });
+  __syncthreads();
+  const int64_t o = crpv[c];
+  for (int64_t i = threadIdx.x; i < nzc * csz; i += kBlock) cval[o + i] = acc[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// large clusters: persistent workgroups, one HBM column bitmap per workgroup
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void bitmap_build(uint32_t *bm, int64_t t0, int64_t t1, const int32_t *__restrict__ acol,
+                                             const int64_t *__restrict__ brp, const int32_t *__restrict__ bcol,
+                                             int *cnt, int *mn, int *mx) {
+  for_each_product(t0, t1, acol, brp, [&](int64_t, int64_t k) {
+    const int32_t key = bcol[k];
+    const uint32_t bit = 1u << (key & 31);
+    const uint32_t old = atomicOr(&bm[key >> 5], bit);
+    if (!(old & bit)) {
+      if (cnt) atomicAdd(cnt, 1);
+      atomicMin(mn, key);
+      atomicMax(mx, key);
+    }
+  });
+}
+
+__device__ __forceinline__ void bitmap_clear(uint32_t *bm, int mn, int mx) {
+  if (mx < mn) return;
+  for (int w = (mn >> 5) + (int)threadIdx.x; w <= (mx >> 5); w += kBlock) bm[w] = 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sym_large(const int32_t *__restrict__ list, int64_t nlist,
+                                                      unsigned int *__restrict__ next, uint32_t *__restrict__ bitmaps,
+                                                      int64_t words, const int64_t *__restrict__ arp,
+                                                      const int32_t *__restrict__ acol, const int64_t *__restrict__ brp,
+                                                      const int32_t *__restrict__ bcol, int64_t *__restrict__ nz,
+                                                      int64_t *__restrict__ cmin, int64_t *__restrict__ cmax) {
+  __shared__ int cnt, mn, mx;
+  __shared__ int64_t item;
+  uint32_t *bm = bitmaps + (int64_t)blockIdx.x * words;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      item = (int64_t)atomicAdd(next, 1u);
+      cnt = 0; mn = INT32_MAX; mx = -1;
+    }
+    __syncthreads();
+    const int64_t it = item;
+    if (it >= nlist) break;
+    const int64_t c = list[it];
+    bitmap_build(bm, arp[c], arp[c + 1], acol, brp, bcol, &cnt, &mn, &mx);
+    __syncthreads();
+    const int lo = mn, hi = mx, n = cnt;
+    if (threadIdx.x == 0) {
+      nz[c] = n;
+      cmin[c] = n ? lo : 0;
+      cmax[c] = n ? hi : 0;
+    }
+    bitmap_clear(bm, lo, hi);
+    __syncthreads();
+  }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_num_large(const int32_t *__restrict__ list, int64_t nlist,
+                                                      unsigned int *__restrict__ next, uint32_t *__restrict__ bitmaps,
+                                                      int32_t *__restrict__ prefixes, int64_t words,
+                                                      const int64_t *__restrict__ arp, const int64_t *__restrict__ arpv,
+                                                      const int32_t *__restrict__ csz_a, const int32_t *__restrict__ acol,
+                                                      const V *__restrict__ aval, const int64_t *__restrict__ brp,
+                                                      const int32_t *__restrict__ bcol, const V *__restrict__ bval,
+                                                      const int64_t *__restrict__ crp, const int64_t *__restrict__ crpv,
+                                                      const int64_t *__restrict__ cmin, const int64_t *__restrict__ cmax,
+                                                      int32_t *__restrict__ ccol, V *__restrict__ cval) {
+  __shared__ int mn, mx;
+  __shared__ int64_t item;
+  __shared__ int32_t wsum[kBlock / kWave];
+  uint32_t *bm = bitmaps + (int64_t)blockIdx.x * words;
+  int32_t *pre = prefixes + (int64_t)blockIdx.x * words;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      item = (int64_t)atomicAdd(next, 1u);
+      mn = INT32_MAX; mx = -1;
+    }
+    __syncthreads();
+    const int64_t it = item;
+    if (it >= nlist) break;
+    const int64_t c = list[it];
+    const int64_t a0 = arp[c], a1 = arp[c + 1];
+    bitmap_build(bm, a0, a1, acol, brp, bcol, nullptr, &mn, &mx);
+    __syncthreads();
+    const int w0 = (int)(cmin[c] >> 5), w1 = (int)(cmax[c] >> 5);
+    // exclusive prefix of popcounts over words [w0, w1], in chunks of kBlock words
+    int32_t carry = 0;
+    for (int base = w0; base <= w1; base += kBlock) {
+      const int w = base + (int)threadIdx.x;
+      const uint32_t word = (w <= w1) ? ld_l2(&bm[w]) : 0u;
+      const int32_t pc = __popc(word);
+      int32_t inc = pc;
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const int32_t o = __shfl_up(inc, d, kWave);
+        if (lane_id() >= d) inc += o;
+      }
+      if (lane_id() == kWave - 1) wsum[wave_id()] = inc;
+      __syncthreads();
+      int32_t wofs = 0, tot = 0;
+      for (int q = 0; q < kBlock / kWave; ++q) {
+        if (q < wave_id()) wofs += wsum[q];
+        tot += wsum[q];
+      }
+      if (w <= w1) {
+        const int32_t excl = carry + wofs + inc - pc;
+        __hip_atomic_store(&pre[w], excl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t bits = word;
+        int32_t r = excl;
+        while (bits) {                            // ascending columns of this word
+          const int b = __ffs(bits) - 1;
+          ccol[crp[c] + r] = w * 32 + b;
+          bits &= bits - 1;
+          ++r;
+        }
+      }
+      carry += tot;
+      __syncthreads();
+    }
+    __syncthreads();
+    const int csz = csz_a[c];
+    const int64_t av0 = arpv[c], cbase = crpv[c];
+    for_each_product(a0, a1, acol, brp, [&](int64_t t, int64_t k) {
+      const int32_t key = bcol[k];
+      const int w = key >> 5;
+      const int64_t r = ld_l2(&pre[w]) + __popc(ld_l2(&bm[w]) & ((1u << (key & 31)) - 1u));
+      const V b = bval[k];
+      const V *a = aval + av0 + (t - a0) * csz;
+      V *dst = cval + cbase + r * csz;
+      for (int l = 0; l < csz; ++l) {
+        const V av = a[l];
+        if (av != V(0)) atomicAdd(&dst[l], av * b);
+      }
+    });
+    __syncthreads();
+    bitmap_clear(bm, (int)cmin[c], (int)cmax[c]);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// C (cluster) -> CSR in original row order, dropping exact zeros
+// ---------------------------------------------------------------------------------------------
+// one wave per (cluster, local row)
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_c2csr_count(const int64_t *__restrict__ offsets, const int64_t *__restrict__ perm,
+                                                        const int32_t *__restrict__ cid, const int32_t *__restrict__ lrow,
+                                                        const int32_t *__restrict__ csz, const int64_t *__restrict__ crp,
+                                                        const int64_t *__restrict__ crpv, const V *__restrict__ cval,
+                                                        int64_t n, int64_t *__restrict__ cnt) {
+  const int64_t t = (int64_t)__builtin_amdgcn_readfirstlane((int)blockIdx.x) * (kBlock / kWave) + wave_id();
+  if (t >= n) return;
+  const int64_t c = cid[t];
+  const int l = lrow[t], s = csz[c];
+  int64_t m = 0;
+  for (int64_t q = lane_id(); q < crp[c + 1] - crp[c]; q += kWave) m += (cval[crpv[c] + q * s + l] != V(0)) ? 1 : 0;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) m += __shfl_xor(m, d, kWave);
+  if (lane_id() == 0) cnt[perm[t]] = m;
+}
+
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_c2csr_fill(const int64_t *__restrict__ perm, const int32_t *__restrict__ cid,
+                                                       const int32_t *__restrict__ lrow, const int32_t *__restrict__ csz,
+                                                       const int64_t *__restrict__ crp, const int64_t *__restrict__ crpv,
+                                                       const int32_t *__restrict__ ccol, const V *__restrict__ cval, int64_t n,
+                                                       const int64_t *__restrict__ orp, int64_t *__restrict__ ocol,
+                                                       V *__restrict__ oval) {
+  const int64_t t = (int64_t)__builtin_amdgcn_readfirstlane((int)blockIdx.x) * (kBlock / kWave) + wave_id();
+  if (t >= n) return;
+  const int64_t c = cid[t];
+  const int l = lrow[t], s = csz[c];
+  const int64_t nzc = crp[c + 1] - crp[c];
+  int64_t o = orp[perm[t]];
+  const uint64_t lt = lane_id() == 0 ? 0ull : (~0ull >> (64 - lane_id()));
+  for (int64_t q0 = 0; q0 < nzc; q0 += kWave) {
+    const int64_t q = q0 + lane_id();
+    V v = V(0);
+    if (q < nzc) v = cval[crpv[c] + q * s + l];
+    const bool keep = q < nzc && v != V(0);
+    const uint64_t bal = __ballot(keep);
+    if (keep) {
+      const int64_t p = o + __popcll(bal & lt);
+      ocol[p] = ccol[crp[c] + q];
+      oval[p] = v;
+    }
+    o += __popcll(bal);
+  }
+}
+
+}  // namespace
+
+// =============================================================================================
+// host orchestration
+// =============================================================================================
+void cluster_build(spops_cluster_s &A, int64_t n, int64_t n_cols, int64_t nnz, const int64_t *rowptr, const int64_t *col,
+                   const void *values, int vt, const int64_t *h_perm, int64_t nc, const int64_t *h_offsets, hipStream_t s) {
+  A.csr_rows = n;
+  A.rows = nc;
+  A.cols = n_cols;
+  A.value_type = vt;
+  A.perm.upload(std::vector<int64_t>(h_perm, h_perm + n));
+  A.offsets.upload(std::vector<int64_t>(h_offsets, h_offsets + nc + 1));
+  A.max_cluster_sz = 0;
+  for (int64_t c = 0; c < nc; ++c) A.max_cluster_sz = std::max(A.max_cluster_sz, h_offsets[c + 1] - h_offsets[c]);
+  A.cid.alloc((size_t)std::max<int64_t>(n, 1) * 4);
+  A.lrow.alloc((size_t)std::max<int64_t>(n, 1) * 4);
+  A.cluster_sz.alloc((size_t)std::max<int64_t>(nc, 1) * 4);
+  if (nc > 0)
+    hipLaunchKernelGGL(k_row_cluster, dim3(nblk(nc, kBlock)), dim3(kBlock), 0, s, A.offsets.as<int64_t>(), nc,
+                       A.cid.as<int32_t>(), A.lrow.as<int32_t>(), A.cluster_sz.as<int32_t>());
+  // reordered row offsets
+  DevBuf len, rptr;
+  len.alloc((size_t)std::max<int64_t>(n, 1) * 8);
+  rptr.alloc((size_t)std::max<int64_t>(n, 1) * 8);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_reordered_len, dim3(nblk(n, kBlock)), dim3(kBlock), 0, s, rowptr, A.perm.as<int64_t>(), n,
+                       len.as<int64_t>());
+    SPOPS_HIP_CHECK(hipGetLastError());
+    exclusive_scan_i64(len.as<int64_t>(), rptr.as<int64_t>(), n, s, nullptr);
+  }
+  DevBuf keys, pay, head, uidx, ccount;
+  keys.alloc((size_t)std::max<int64_t>(nnz, 1) * 8);
+  pay.alloc((size_t)std::max<int64_t>(nnz, 1) * 8);
+  if (n > 0)
+    hipLaunchKernelGGL(k_cluster_keys, dim3(nblk(n, kBlock / kWave)), dim3(kBlock), 0, s, rowptr, col,
+                       A.perm.as<int64_t>(), rptr.as<int64_t>(), A.cid.as<int32_t>(), A.lrow.as<int32_t>(), n,
+                       keys.as<uint64_t>(), pay.as<uint64_t>());
+  SPOPS_HIP_CHECK(hipGetLastError());
+  len.release();
+  rptr.release();
+  const int cbits = bit_width_u64((uint64_t)std::max<int64_t>(nc - 1, 0));
+  radix_sort_u64(keys.as<uint64_t>(), pay.p, 8, nnz, 0, 32 + cbits, s);
+  head.alloc((size_t)std::max<int64_t>(nnz, 1) * 8);
+  uidx.alloc((size_t)std::max<int64_t>(nnz, 1) * 8);
+  ccount.alloc((size_t)(nc + 1) * 8);
+  SPOPS_HIP_CHECK(hipMemsetAsync(ccount.p, 0, (size_t)(nc + 1) * 8, s));
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_key_heads, dim3(nblk(nnz, kBlock)), dim3(kBlock), 0, s, keys.as<uint64_t>(), nnz,
+                       head.as<int64_t>(), ccount.as<int64_t>());
+  SPOPS_HIP_CHECK(hipGetLastError());
+  int64_t nnzc = 0;
+  exclusive_scan_i64(head.as<int64_t>(), uidx.as<int64_t>(), nnz, s, &nnzc);
+  // uidx[a] = number of heads before a; for a head that is its unique index, for a
+  // non-head it is one past the index of its key (k_cluster_fill corrects by -1)
+  A.rowptr.alloc((size_t)(nc + 1) * 8);
+  exclusive_scan_i64(ccount.as<int64_t>(), A.rowptr.as<int64_t>(), nc + 1, s, nullptr);
+  DevBuf vcount;
+  vcount.alloc((size_t)(nc + 1) * 8);
+  SPOPS_HIP_CHECK(hipMemsetAsync(vcount.p, 0, (size_t)(nc + 1) * 8, s));
+  if (nc > 0)
+    hipLaunchKernelGGL(k_mul_csz, dim3(nblk(nc, kBlock)), dim3(kBlock), 0, s, ccount.as<int64_t>(),
+                       A.cluster_sz.as<int32_t>(), nc, vcount.as<int64_t>());
+  A.rowptr_val.alloc((size_t)(nc + 1) * 8);
+  int64_t nnzv = 0;
+  exclusive_scan_i64(vcount.as<int64_t>(), A.rowptr_val.as<int64_t>(), nc + 1, s, &nnzv);
+  A.nnzc = nnzc;
+  A.nnzv = nnzv;
+  A.colids.alloc((size_t)std::max<int64_t>(nnzc, 1) * 4);
+  A.values.alloc((size_t)std::max<int64_t>(nnzv, 1) * value_bytes(vt));
+  SPOPS_HIP_CHECK(hipMemsetAsync(A.values.p, 0, (size_t)std::max<int64_t>(nnzv, 1) * value_bytes(vt), s));
+  if (nnz > 0) {
+    if (vt == SPOPS_VALUE_F64)
+      hipLaunchKernelGGL(k_cluster_fill<double>, dim3(nblk(nnz, kBlock)), dim3(kBlock), 0, s, keys.as<uint64_t>(),
+                         pay.as<uint64_t>(), uidx.as<int64_t>(), nnz, A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(),
+                         A.cluster_sz.as<int32_t>(), static_cast<const double *>(values), A.colids.as<int32_t>(),
+                         A.values.as<double>());
+    else
+      hipLaunchKernelGGL(k_cluster_fill<float>, dim3(nblk(nnz, kBlock)), dim3(kBlock), 0, s, keys.as<uint64_t>(),
+                         pay.as<uint64_t>(), uidx.as<int64_t>(), nnz, A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(),
+                         A.cluster_sz.as<int32_t>(), static_cast<const float *>(values), A.colids.as<int32_t>(),
+                         A.values.as<float>());
+    SPOPS_HIP_CHECK(hipGetLastError());
+  }
+  SPOPS_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+namespace {
+
+struct Timer {
+  hipEvent_t a, b;
+  hipStream_t s;
+  explicit Timer(hipStream_t st) : s(st) {
+    SPOPS_HIP_CHECK(hipEventCreate(&a));
+    SPOPS_HIP_CHECK(hipEventCreate(&b));
+    SPOPS_HIP_CHECK(hipEventRecord(a, s));
+  }
+  double stop() {
+    SPOPS_HIP_CHECK(hipEventRecord(b, s));
+    SPOPS_HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0;
+    SPOPS_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    SPOPS_HIP_CHECK(hipEventRecord(a, s));
+    return ms;
+  }
+  ~Timer() {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+};
+
+int persistent_slots(int64_t words) {
+  int dev = 0, cus = 0;
+  SPOPS_HIP_CHECK(hipGetDevice(&dev));
+  SPOPS_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  int64_t slots = (int64_t)std::max(cus, 1) * kLargeSlotsPerCU;
+  const int64_t budget = kLargeScratchBytes / std::max<int64_t>(words * 8, 1);   // bitmap + prefix per slot
+  return (int)std::max<int64_t>(1, std::min(slots, budget));
+}
+
+}  // namespace
+
+void lespgemm(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *brp64,
+              const int64_t *bcol64, const void *bval, spops_cluster_s &C, int64_t stats[kSpgemmStats], hipStream_t s) {
+  const int vt = A.value_type;
+  const int64_t nc = A.rows;
+  Timer tm(s);
+  // B with int32 columns
+  DevBuf bcol32;
+  bcol32.alloc((size_t)std::max<int64_t>(b_nnz, 1) * 4);
+  if (b_nnz > 0) hipLaunchKernelGGL(k_narrow, dim3(nblk(std::max<int64_t>(b_nnz, 1), kBlock)), dim3(kBlock), 0, s, bcol64, b_nnz,
+                     bcol32.as<int32_t>());
+  SPOPS_HIP_CHECK(hipGetLastError());
+  // work estimate and bins
+  DevBuf flop;
+  flop.alloc((size_t)std::max<int64_t>(nc, 1) * 8);
+  if (nc > 0)
+    hipLaunchKernelGGL(k_cluster_flop, dim3(nblk(nc, kBlock / kWave)), dim3(kBlock), 0, s, A.rowptr.as<int64_t>(),
+                       A.colids.as<int32_t>(), brp64, nc, flop.as<int64_t>());
+  SPOPS_HIP_CHECK(hipGetLastError());
+  std::vector<int64_t> hflop(nc);
+  std::vector<int32_t> hcsz(nc);
+  if (nc > 0) {
+    SPOPS_HIP_CHECK(hipMemcpyAsync(hflop.data(), flop.p, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
+    SPOPS_HIP_CHECK(hipMemcpyAsync(hcsz.data(), A.cluster_sz.p, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
+  }
+  SPOPS_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<int32_t> small, large;
+  int64_t total_cluster_flop = 0;
+  for (int64_t c = 0; c < nc; ++c) {
+    total_cluster_flop += hflop[c];
+    if (hflop[c] == 0) continue;
+    if (hflop[c] <= kSmallFlop && hflop[c] * hcsz[c] <= kSmallAcc) small.push_back((int32_t)c);
+    else large.push_back((int32_t)c);
+  }
+  // large clusters in descending work so the persistent workgroups end together
+  std::stable_sort(large.begin(), large.end(), [&](int32_t a, int32_t b) { return hflop[a] > hflop[b]; });
+  DevBuf dsmall, dlarge, nz, cmin, cmax, counter;
+  dsmall.upload(small);
+  dlarge.upload(large);
+  nz.alloc((size_t)std::max<int64_t>(nc, 1) * 8);
+  cmin.alloc((size_t)std::max<int64_t>(nc, 1) * 8);
+  cmax.alloc((size_t)std::max<int64_t>(nc, 1) * 8);
+  SPOPS_HIP_CHECK(hipMemsetAsync(nz.p, 0, (size_t)std::max<int64_t>(nc, 1) * 8, s));
+  SPOPS_HIP_CHECK(hipMemsetAsync(cmin.p, 0, (size_t)std::max<int64_t>(nc, 1) * 8, s));
+  SPOPS_HIP_CHECK(hipMemsetAsync(cmax.p, 0, (size_t)std::max<int64_t>(nc, 1) * 8, s));
+  counter.alloc(16);
+  const int64_t words = (b_cols + 31) / 32 + 1;
+  const int slots = large.empty() ? 0 : persistent_slots(words);
+  DevBuf bitmaps, prefixes;
+  if (slots) {
+    bitmaps.alloc((size_t)slots * words * 4);
+    prefixes.alloc((size_t)slots * words * 4);
+    SPOPS_HIP_CHECK(hipMemsetAsync(bitmaps.p, 0, (size_t)slots * words * 4, s));
+  }
+  double ms_setup = tm.stop();
+  // ---- symbolic
+  if (!small.empty())
+    hipLaunchKernelGGL(k_sym_small, dim3((unsigned)small.size()), dim3(kBlock), 0, s, dsmall.as<int32_t>(),
+                       (int64_t)small.size(), A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64, bcol32.as<int32_t>(),
+                       nz.as<int64_t>(), cmin.as<int64_t>(), cmax.as<int64_t>());
+  if (slots) {
+    SPOPS_HIP_CHECK(hipMemsetAsync(counter.p, 0, 16, s));
+    hipLaunchKernelGGL(k_sym_large, dim3((unsigned)slots), dim3(kBlock), 0, s, dlarge.as<int32_t>(), (int64_t)large.size(),
+                       counter.as<unsigned int>(), bitmaps.as<uint32_t>(), words, A.rowptr.as<int64_t>(),
+                       A.colids.as<int32_t>(), brp64, bcol32.as<int32_t>(), nz.as<int64_t>(), cmin.as<int64_t>(),
+                       cmax.as<int64_t>());
+  }
+  SPOPS_HIP_CHECK(hipGetLastError());
+  C.csr_rows = A.csr_rows;
+  C.rows = nc;
+  C.cols = b_cols;
+  C.value_type = vt;
+  C.max_cluster_sz = A.max_cluster_sz;
+  C.rowptr.alloc((size_t)(nc + 1) * 8);
+  C.rowptr_val.alloc((size_t)(nc + 1) * 8);
+  int64_t nnzc = 0, nnzv = 0;
+  {
+    DevBuf nzp;
+    nzp.alloc((size_t)(nc + 1) * 8);
+    SPOPS_HIP_CHECK(hipMemsetAsync(nzp.p, 0, (size_t)(nc + 1) * 8, s));
+    SPOPS_HIP_CHECK(hipMemcpyAsync(nzp.p, nz.p, (size_t)nc * 8, hipMemcpyDeviceToDevice, s));
+    exclusive_scan_i64(nzp.as<int64_t>(), C.rowptr.as<int64_t>(), nc + 1, s, &nnzc);
+    if (nc > 0)
+      hipLaunchKernelGGL(k_mul_csz, dim3(nblk(nc, kBlock)), dim3(kBlock), 0, s, nz.as<int64_t>(),
+                         A.cluster_sz.as<int32_t>(), nc, nzp.as<int64_t>());
+    exclusive_scan_i64(nzp.as<int64_t>(), C.rowptr_val.as<int64_t>(), nc + 1, s, &nnzv);
+  }
+  C.nnzc = nnzc;
+  C.nnzv = nnzv;
+  double ms_symbolic = tm.stop();
+  // ---- allocate output (the reference's "malloc" phase)
+  C.colids.alloc((size_t)std::max<int64_t>(nnzc, 1) * 4);
+  C.values.alloc((size_t)std::max<int64_t>(nnzv, 1) * value_bytes(vt));
+  SPOPS_HIP_CHECK(hipMemsetAsync(C.values.p, 0, (size_t)std::max<int64_t>(nnzv, 1) * value_bytes(vt), s));
+  C.cluster_sz.alloc((size_t)std::max<int64_t>(nc, 1) * 4);
+  SPOPS_HIP_CHECK(hipMemcpyAsync(C.cluster_sz.p, A.cluster_sz.p, (size_t)nc * 4, hipMemcpyDeviceToDevice, s));
+  double ms_malloc = tm.stop();
+  // ---- numeric
+  if (vt == SPOPS_VALUE_F64) {
+    if (!small.empty())
+      hipLaunchKernelGGL(k_num_small<double>, dim3((unsigned)small.size()), dim3(kBlock), 0, s, dsmall.as<int32_t>(),
+                         (int64_t)small.size(), A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(),
+                         A.cluster_sz.as<int32_t>(), A.colids.as<int32_t>(), A.values.as<double>(), brp64,
+                         bcol32.as<int32_t>(), static_cast<const double *>(bval), C.rowptr.as<int64_t>(),
+                         C.rowptr_val.as<int64_t>(), C.colids.as<int32_t>(), C.values.as<double>());
+    if (slots) {
+      SPOPS_HIP_CHECK(hipMemsetAsync(counter.p, 0, 16, s));
+      hipLaunchKernelGGL(k_num_large<double>, dim3((unsigned)slots), dim3(kBlock), 0, s, dlarge.as<int32_t>(),
+                         (int64_t)large.size(), counter.as<unsigned int>(), bitmaps.as<uint32_t>(), prefixes.as<int32_t>(),
+                         words, A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(), A.cluster_sz.as<int32_t>(),
+                         A.colids.as<int32_t>(), A.values.as<double>(), brp64, bcol32.as<int32_t>(),
+                         static_cast<const double *>(bval), C.rowptr.as<int64_t>(), C.rowptr_val.as<int64_t>(),
+                         cmin.as<int64_t>(), cmax.as<int64_t>(), C.colids.as<int32_t>(), C.values.as<double>());
+    }
+  } else {
+    if (!small.empty())
+      hipLaunchKernelGGL(k_num_small<float>, dim3((unsigned)small.size()), dim3(kBlock), 0, s, dsmall.as<int32_t>(),
+                         (int64_t)small.size(), A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(),
+                         A.cluster_sz.as<int32_t>(), A.colids.as<int32_t>(), A.values.as<float>(), brp64,
+                         bcol32.as<int32_t>(), static_cast<const float *>(bval), C.rowptr.as<int64_t>(),
+                         C.rowptr_val.as<int64_t>(), C.colids.as<int32_t>(), C.values.as<float>());
+    if (slots) {
+      SPOPS_HIP_CHECK(hipMemsetAsync(counter.p, 0, 16, s));
+      hipLaunchKernelGGL(k_num_large<float>, dim3((unsigned)slots), dim3(kBlock), 0, s, dlarge.as<int32_t>(),
+                         (int64_t)large.size(), counter.as<unsigned int>(), bitmaps.as<uint32_t>(), prefixes.as<int32_t>(),
+                         words, A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(), A.cluster_sz.as<int32_t>(),
+                         A.colids.as<int32_t>(), A.values.as<float>(), brp64, bcol32.as<int32_t>(),
+                         static_cast<const float *>(bval), C.rowptr.as<int64_t>(), C.rowptr_val.as<int64_t>(),
+                         cmin.as<int64_t>(), cmax.as<int64_t>(), C.colids.as<int32_t>(), C.values.as<float>());
+    }
+  }
+  SPOPS_HIP_CHECK(hipGetLastError());
+  double ms_numeric = tm.stop();
+  // per-cluster min/max kept for the classification report
+  C.cmin = std::move(cmin);
+  C.cmax = std::move(cmax);
+  C.nzc = std::move(nz);
+  stats[0] = 0;
+  stats[1] = nnzc;
+  stats[2] = nnzv;
+  stats[3] = (int64_t)small.size();
+  stats[4] = (int64_t)large.size();
+  stats[5] = (int64_t)(ms_setup * 1e3);
+  stats[6] = (int64_t)(ms_symbolic * 1e3);
+  stats[7] = (int64_t)(ms_malloc * 1e3);
+  stats[8] = (int64_t)(ms_numeric * 1e3);
+  stats[9] = total_cluster_flop;
+}
+
+void cluster_to_csr(const spops_cluster_s &C, const spops_cluster_s &A, int64_t &nnz, DevBuf &rowptr, DevBuf &col,
+                    DevBuf &val, hipStream_t s) {
+  const int64_t n = C.csr_rows;
+  DevBuf cnt;
+  cnt.alloc((size_t)(n + 1) * 8);
+  SPOPS_HIP_CHECK(hipMemsetAsync(cnt.p, 0, (size_t)(n + 1) * 8, s));
+  const unsigned grid = nblk(std::max<int64_t>(n, 1), kBlock / kWave);
+  if (C.value_type == SPOPS_VALUE_F64)
+    hipLaunchKernelGGL(k_c2csr_count<double>, dim3(grid), dim3(kBlock), 0, s, A.offsets.as<int64_t>(), A.perm.as<int64_t>(),
+                       A.cid.as<int32_t>(), A.lrow.as<int32_t>(), C.cluster_sz.as<int32_t>(), C.rowptr.as<int64_t>(),
+                       C.rowptr_val.as<int64_t>(), C.values.as<double>(), n, cnt.as<int64_t>());
+  else
+    hipLaunchKernelGGL(k_c2csr_count<float>, dim3(grid), dim3(kBlock), 0, s, A.offsets.as<int64_t>(), A.perm.as<int64_t>(),
+                       A.cid.as<int32_t>(), A.lrow.as<int32_t>(), C.cluster_sz.as<int32_t>(), C.rowptr.as<int64_t>(),
+                       C.rowptr_val.as<int64_t>(), C.values.as<float>(), n, cnt.as<int64_t>());
+  SPOPS_HIP_CHECK(hipGetLastError());
+  rowptr.alloc((size_t)(n + 1) * 8);
+  exclusive_scan_i64(cnt.as<int64_t>(), rowptr.as<int64_t>(), n + 1, s, &nnz);
+  col.alloc((size_t)std::max<int64_t>(nnz, 1) * 8);
+  val.alloc((size_t)std::max<int64_t>(nnz, 1) * value_bytes(C.value_type));
+  if (C.value_type == SPOPS_VALUE_F64)
+    hipLaunchKernelGGL(k_c2csr_fill<double>, dim3(grid), dim3(kBlock), 0, s, A.perm.as<int64_t>(), A.cid.as<int32_t>(),
+                       A.lrow.as<int32_t>(), C.cluster_sz.as<int32_t>(), C.rowptr.as<int64_t>(), C.rowptr_val.as<int64_t>(),
+                       C.colids.as<int32_t>(), C.values.as<double>(), n, rowptr.as<int64_t>(), col.as<int64_t>(),
+                       val.as<double>());
+  else
+    hipLaunchKernelGGL(k_c2csr_fill<float>, dim3(grid), dim3(kBlock), 0, s, A.perm.as<int64_t>(), A.cid.as<int32_t>(),
+                       A.lrow.as<int32_t>(), C.cluster_sz.as<int32_t>(), C.rowptr.as<int64_t>(), C.rowptr_val.as<int64_t>(),
+                       C.colids.as<int32_t>(), C.values.as<float>(), n, rowptr.as<int64_t>(), col.as<int64_t>(),
+                       val.as<float>());
+  SPOPS_HIP_CHECK(hipGetLastError());
+  SPOPS_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace spops
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+using namespace spops;
+
+namespace {
+unsigned nblocks(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+}  // namespace
+
+extern "C" int spops_cluster_create(spops_cluster *out, int64_t n_rows, int64_t n_cols, int64_t nnz,
+                                    const int64_t *d_rowptr, const int64_t *d_col, const void *d_values, int value_type,
+                                    const int64_t *perm, int64_t nclusters, const int64_t *offsets, void *stream) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(out != nullptr, SPOPS_ERROR_INVALID_VALUE, "null output handle");
+    SPOPS_REQUIRE(value_type == SPOPS_VALUE_F64 || value_type == SPOPS_VALUE_F32, SPOPS_ERROR_INVALID_VALUE,
+                  "cluster values must be F64 or F32");
+    SPOPS_REQUIRE(n_rows >= 0 && nnz >= 0 && nclusters >= 0, SPOPS_ERROR_INVALID_VALUE, "negative size");
+    SPOPS_REQUIRE(n_rows < ((int64_t)1 << 31) && n_cols < ((int64_t)1 << 31) && nclusters < ((int64_t)1 << 31),
+                  SPOPS_ERROR_LIMIT, "rows, columns and clusters must be < 2^31");
+    SPOPS_REQUIRE(nnz < ((int64_t)1 << 40), SPOPS_ERROR_LIMIT, "nnz must be < 2^40");
+    SPOPS_REQUIRE(perm && offsets && (n_rows == 0 || (d_rowptr && d_col && (nnz == 0 || d_values))),
+                  SPOPS_ERROR_INVALID_VALUE, "null pointer");
+    // csr_to_vlength_cluster's checks (include/sparse_conversion.h:1656-1667)
+    SPOPS_REQUIRE(nclusters >= 1 || n_rows == 0, SPOPS_ERROR_INVALID_VALUE, "offsets must have at least 2 elements");
+    SPOPS_REQUIRE(offsets[0] == 0 && offsets[nclusters] == n_rows, SPOPS_ERROR_INVALID_VALUE,
+                  "Invalid offsets: must be [0, ..., csr.num_rows]");
+    for (int64_t c = 0; c < nclusters; ++c)
+      SPOPS_REQUIRE(offsets[c + 1] > offsets[c], SPOPS_ERROR_INVALID_VALUE, "empty or decreasing cluster %lld",
+                    (long long)c);
+    std::vector<char> seen(n_rows, 0);
+    for (int64_t t = 0; t < n_rows; ++t) {
+      SPOPS_REQUIRE(perm[t] >= 0 && perm[t] < n_rows && !seen[perm[t]], SPOPS_ERROR_INVALID_VALUE,
+                    "perm is not a permutation (entry %lld)", (long long)t);
+      seen[perm[t]] = 1;
+    }
+    auto A = std::make_unique<spops_cluster_s>();
+    cluster_build(*A, n_rows, n_cols, nnz, d_rowptr, d_col, d_values, value_type, perm, nclusters, offsets,
+                  static_cast<hipStream_t>(stream));
+    *out = A.release();
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_cluster_info(spops_cluster A, int64_t info[8]) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(A && info, SPOPS_ERROR_INVALID_VALUE, "null argument");
+    info[0] = A->rows; info[1] = A->csr_rows; info[2] = A->cols; info[3] = A->nnzc; info[4] = A->nnzv;
+    info[5] = A->max_cluster_sz; info[6] = A->value_type;
+    info[7] = (int64_t)(A->cluster_sz.bytes + A->rowptr.bytes + A->rowptr_val.bytes + A->colids.bytes + A->values.bytes);
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_cluster_download(spops_cluster A, int64_t *cluster_sz, int64_t *rowptr, int64_t *rowptr_val,
+                                      int64_t *colids, void *values) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(A, SPOPS_ERROR_INVALID_VALUE, "null cluster matrix");
+    const size_t nc = (size_t)A->rows;
+    if (cluster_sz && nc) {
+      std::vector<int32_t> t(nc);
+      SPOPS_HIP_CHECK(hipMemcpy(t.data(), A->cluster_sz.p, nc * 4, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < nc; ++i) cluster_sz[i] = t[i];
+    }
+    if (rowptr) SPOPS_HIP_CHECK(hipMemcpy(rowptr, A->rowptr.p, (nc + 1) * 8, hipMemcpyDeviceToHost));
+    if (rowptr_val) SPOPS_HIP_CHECK(hipMemcpy(rowptr_val, A->rowptr_val.p, (nc + 1) * 8, hipMemcpyDeviceToHost));
+    if (colids && A->nnzc) {
+      std::vector<int32_t> t((size_t)A->nnzc);
+      SPOPS_HIP_CHECK(hipMemcpy(t.data(), A->colids.p, t.size() * 4, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < t.size(); ++i) colids[i] = t[i];
+    }
+    if (values && A->nnzv)
+      SPOPS_HIP_CHECK(hipMemcpy(values, A->values.p, (size_t)A->nnzv * value_bytes(A->value_type), hipMemcpyDeviceToHost));
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_cluster_symbolic_download(spops_cluster C, int64_t *nz, int64_t *cmin, int64_t *cmax) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(C && C->nzc.p, SPOPS_ERROR_INVALID_VALUE, "not a product cluster matrix");
+    const size_t nc = (size_t)C->rows;
+    if (!nc) return SPOPS_SUCCESS;
+    if (nz) SPOPS_HIP_CHECK(hipMemcpy(nz, C->nzc.p, nc * 8, hipMemcpyDeviceToHost));
+    if (cmin) SPOPS_HIP_CHECK(hipMemcpy(cmin, C->cmin.p, nc * 8, hipMemcpyDeviceToHost));
+    if (cmax) SPOPS_HIP_CHECK(hipMemcpy(cmax, C->cmax.p, nc * 8, hipMemcpyDeviceToHost));
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_cluster_destroy(spops_cluster A) {
+  delete A;
+  return SPOPS_SUCCESS;
+}
+
+extern "C" int spops_lespgemm(spops_cluster A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *d_browptr,
+                              const int64_t *d_bcol, const void *d_bval, spops_cluster *C_out, int64_t stats[16],
+                              void *stream) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(A && C_out, SPOPS_ERROR_INVALID_VALUE, "null argument");
+    SPOPS_REQUIRE(A->cols == b_rows, SPOPS_ERROR_INVALID_VALUE, "A.cols (%lld) != B.rows (%lld)", (long long)A->cols,
+                  (long long)b_rows);
+    SPOPS_REQUIRE(b_cols < ((int64_t)1 << 31), SPOPS_ERROR_LIMIT, "B columns must be < 2^31");
+    SPOPS_REQUIRE(b_rows == 0 || (d_browptr && (b_nnz == 0 || (d_bcol && d_bval))), SPOPS_ERROR_INVALID_VALUE,
+                  "null B array");
+    int64_t st[kSpgemmStats] = {0};
+    auto C = std::make_unique<spops_cluster_s>();
+    lespgemm(*A, b_rows, b_cols, b_nnz, d_browptr, d_bcol, d_bval, *C, st, static_cast<hipStream_t>(stream));
+    if (stats)
+      for (int i = 0; i < 16; ++i) stats[i] = i < kSpgemmStats ? st[i] : 0;
+    *C_out = C.release();
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_cluster_to_csr(spops_cluster C, spops_cluster A, int64_t *nnz, int64_t **d_rowptr, int64_t **d_col,
+                                    void **d_val, void *stream) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(C && A && nnz && d_rowptr && d_col && d_val, SPOPS_ERROR_INVALID_VALUE, "null argument");
+    SPOPS_REQUIRE(C->rows == A->rows && C->csr_rows == A->csr_rows && A->perm.p, SPOPS_ERROR_INVALID_VALUE,
+                  "C must be the product of the clustered matrix A");
+    DevBuf rp, col, val;
+    cluster_to_csr(*C, *A, *nnz, rp, col, val, static_cast<hipStream_t>(stream));
+    *d_rowptr = static_cast<int64_t *>(rp.p); rp.p = nullptr;
+    *d_col = static_cast<int64_t *>(col.p); col.p = nullptr;
+    *d_val = val.p; val.p = nullptr;
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_spgemm_flop(int64_t a_nnz, const int64_t *d_acol, int64_t b_rows, const int64_t *d_browptr,
+                                 int64_t *flops, void *stream) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(flops, SPOPS_ERROR_INVALID_VALUE, "null output");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DevBuf acc;
+    acc.alloc(16);
+    SPOPS_HIP_CHECK(hipMemsetAsync(acc.p, 0, 16, s));
+    if (a_nnz > 0)
+      hipLaunchKernelGGL(k_ref_flop, dim3(nblocks(a_nnz, kBlock)), dim3(kBlock), 0, s, d_acol, a_nnz, b_rows, d_browptr,
+                         acc.as<unsigned long long>());
+    SPOPS_HIP_CHECK(hipGetLastError());
+    unsigned long long f = 0;
+    SPOPS_HIP_CHECK(hipMemcpyAsync(&f, acc.p, 8, hipMemcpyDeviceToHost, s));
+    SPOPS_HIP_CHECK(hipStreamSynchronize(s));
+    *flops = 2 * (int64_t)f;
+    return SPOPS_SUCCESS;
+  });
+}

@@ -1,0 +1,211 @@
+"""LeSpGEMM pipeline on the GPU through the C ABI (mirrors test/test_spgemm_hc_lsh.cpp:49-203).
+
+    pairs   = lsh_candidate_pairs(A, k, bands, seed, constrain)      # MinHash + LSH (device)
+    perm, offsets = hierarchical_clustering(A, pairs, cluster_size)  # host, bit-exact v0_fast / v1
+    A_cl    = ClusterMatrix(A, perm, offsets)                        # csr_reorder_rows + csr_to_vlength_cluster
+    C_cl    = A_cl.multiply(B)                                       # LeSpGEMM_VLength kernel 3
+    C       = C_cl.to_csr(A_cl)                                      # vlength_cluster2csr + inverse permutation
+
+Every step runs in libsparseops_hip.so; this module only moves arrays.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .gen import Csr
+
+c_i64 = ctypes.c_int64
+
+
+def _dev(a, dtype, device):
+    return torch.as_tensor(np.ascontiguousarray(a), device=device).to(dtype)
+
+
+class DeviceCsr:
+    """A CSR matrix with int64 indices resident on the device (torch allocations)."""
+
+    def __init__(self, A: Csr, device, value_dtype=torch.float64):
+        self.n_rows, self.n_cols, self.nnz = A.n_rows, A.n_cols, A.nnz
+        self.rowptr = _dev(A.rowptr.astype(np.int64), torch.int64, device)
+        self.col = _dev(A.col.astype(np.int64), torch.int64, device)
+        self.val = _dev(A.val, value_dtype, device)
+        self.value_type = L.VALUE_F64 if value_dtype == torch.float64 else L.VALUE_F32
+
+
+def minhash_signatures(dA: DeviceCsr, k: int = 64, seed: int = 12345) -> torch.Tensor:
+    sigs = torch.empty((max(dA.n_rows, 1), k), dtype=torch.int64, device=dA.rowptr.device)
+    L.call("spops_minhash_signatures", dA.rowptr.data_ptr(), dA.col.data_ptr(), L.INDEX_I64, dA.n_rows, k,
+           ctypes.c_uint64(seed), sigs.data_ptr(), L.current_stream())
+    return sigs[: dA.n_rows]
+
+
+def lsh_candidate_pairs(dA: DeviceCsr, k: int = 64, bands: int = 16, seed: int = 12345, constrain: bool = True,
+                        score_f32: bool = False):
+    """(pi, pj, score) numpy arrays, sorted by (i, j); scores in float64 (or float32)."""
+    h = ctypes.c_void_p()
+    st = L.VALUE_F32 if score_f32 else L.VALUE_F64
+    L.call("spops_lsh_candidate_pairs", dA.rowptr.data_ptr(), dA.col.data_ptr(), L.INDEX_I64, dA.n_rows, k, bands,
+           ctypes.c_uint64(seed), int(constrain), st, ctypes.byref(h), L.current_stream())
+    try:
+        n = c_i64()
+        L.call("spops_pairs_count", h, ctypes.byref(n))
+        pi = np.zeros(n.value, np.int64)
+        pj = np.zeros(n.value, np.int64)
+        sc = np.zeros(n.value, np.float32 if score_f32 else np.float64)
+        L.call("spops_pairs_download", h, pi.ctypes.data, pj.ctypes.data, sc.ctypes.data)
+    finally:
+        L.lib.spops_pairs_destroy(h)
+    return pi, pj, sc
+
+
+def hierarchical_clustering(A: Csr, pi, pj, score, cluster_size: int = 8, version: int = 0):
+    rp = np.ascontiguousarray(A.rowptr, dtype=np.int64)
+    ci = np.ascontiguousarray(A.col, dtype=np.int64)
+    pi = np.ascontiguousarray(pi, dtype=np.int64)
+    pj = np.ascontiguousarray(pj, dtype=np.int64)
+    score = np.ascontiguousarray(score)
+    st = L.VALUE_F32 if score.dtype == np.float32 else L.VALUE_F64
+    perm = np.zeros(A.n_rows, np.int64)
+    offsets = np.zeros(A.n_rows + 1, np.int64)
+    nc = c_i64()
+    L.call("spops_hierarchical_clustering", version, A.n_rows, rp.ctypes.data, ci.ctypes.data, len(pi),
+           pi.ctypes.data, pj.ctypes.data, score.ctypes.data, st, cluster_size, perm.ctypes.data,
+           offsets.ctypes.data, ctypes.byref(nc))
+    return perm, offsets[: nc.value + 1].copy()
+
+
+class ClusterMatrix:
+    """Device CSR_VlengthCluster (owned by the library)."""
+
+    def __init__(self, handle, value_type):
+        self._h = handle
+        self.value_type = value_type
+        info = (c_i64 * 8)()
+        L.call("spops_cluster_info", self._h, info)
+        self.rows, self.csr_rows, self.cols, self.nnzc, self.nnzv, self.max_cluster_sz = list(info)[:6]
+        self.stats = None
+
+    @classmethod
+    def build(cls, dA: DeviceCsr, perm, offsets):
+        perm = np.ascontiguousarray(perm, dtype=np.int64)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        h = ctypes.c_void_p()
+        L.call("spops_cluster_create", ctypes.byref(h), dA.n_rows, dA.n_cols, dA.nnz, dA.rowptr.data_ptr(),
+               dA.col.data_ptr(), dA.val.data_ptr(), dA.value_type, perm.ctypes.data, len(offsets) - 1,
+               offsets.ctypes.data, L.current_stream())
+        return cls(h, dA.value_type)
+
+    def multiply(self, dB: DeviceCsr) -> "ClusterMatrix":
+        h = ctypes.c_void_p()
+        stats = (c_i64 * 16)()
+        L.call("spops_lespgemm", self._h, dB.n_rows, dB.n_cols, dB.nnz, dB.rowptr.data_ptr(), dB.col.data_ptr(),
+               dB.val.data_ptr(), ctypes.byref(h), stats, L.current_stream())
+        C = ClusterMatrix(h, self.value_type)
+        s = list(stats)
+        C.stats = {"nnzc": s[1], "nnzv": s[2], "small_clusters": s[3], "large_clusters": s[4],
+                   "setup_ms": s[5] / 1e3, "symbolic_ms": s[6] / 1e3, "malloc_ms": s[7] / 1e3,
+                   "numeric_ms": s[8] / 1e3, "cluster_products": s[9]}
+        return C
+
+    def download(self):
+        vt = np.float64 if self.value_type == L.VALUE_F64 else np.float32
+        csz = np.zeros(self.rows, np.int64)
+        rp = np.zeros(self.rows + 1, np.int64)
+        rpv = np.zeros(self.rows + 1, np.int64)
+        col = np.zeros(self.nnzc, np.int64)
+        val = np.zeros(self.nnzv, vt)
+        L.call("spops_cluster_download", self._h, csz.ctypes.data, rp.ctypes.data, rpv.ctypes.data, col.ctypes.data,
+               val.ctypes.data)
+        return csz, rp, rpv, col, val
+
+    def symbolic(self):
+        nz, mn, mx = (np.zeros(self.rows, np.int64) for _ in range(3))
+        L.call("spops_cluster_symbolic_download", self._h, nz.ctypes.data, mn.ctypes.data, mx.ctypes.data)
+        return nz, mn, mx
+
+    def to_csr(self, A_cl: "ClusterMatrix") -> Csr:
+        nnz = c_i64()
+        drp, dcol, dval = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        L.call("spops_cluster_to_csr", self._h, A_cl._h, ctypes.byref(nnz), ctypes.byref(drp), ctypes.byref(dcol),
+               ctypes.byref(dval), L.current_stream())
+        try:
+            n = self.csr_rows
+            vt = np.float64 if self.value_type == L.VALUE_F64 else np.float32
+            rp = np.zeros(n + 1, np.int64)
+            col = np.zeros(nnz.value, np.int64)
+            val = np.zeros(nnz.value, vt)
+            L.call("spops_memcpy_dtoh", rp.ctypes.data, drp, rp.nbytes, None)
+            L.call("spops_memcpy_dtoh", col.ctypes.data, dcol, col.nbytes, None)
+            L.call("spops_memcpy_dtoh", val.ctypes.data, dval, val.nbytes, None)
+        finally:
+            for p in (drp, dcol, dval):
+                L.lib.spops_free(p)
+        return Csr(n, self.cols, rp, col, val)
+
+    def close(self):
+        if self._h:
+            L.lib.spops_cluster_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def spgemm_flop(dA: DeviceCsr, dB: DeviceCsr) -> int:
+    f = c_i64()
+    L.call("spops_spgemm_flop", dA.nnz, dA.col.data_ptr(), dB.n_rows, dB.rowptr.data_ptr(), ctypes.byref(f),
+           L.current_stream())
+    return f.value
+
+
+def lespgemm(A: Csr, B: Csr, k: int = 64, bands: int = 16, seed: int = 7, cluster_size: int = 8, hc_v: int = 0,
+             device="cuda:0", precision: int = 64, iterations: int = 0):
+    """Full test_spgemm_hc_lsh pipeline; returns host arrays and phase timings (ms).
+
+    hc_v: 0 = constrained LSH + v0_fast, 1 = full LSH + v1, 2 = full LSH + v0_fast.
+    iterations > 0 additionally times that many LeSpGEMM runs (symbolic + malloc + numeric).
+    """
+    if A.n_cols != B.n_rows:
+        raise ValueError("Error: A.cols != B.rows")
+    if k <= 0 or bands <= 0 or k % bands != 0:
+        raise ValueError(f"Error: k must be positive and k % bands == 0. k={k} bands={bands}")
+    vdt = torch.float64 if precision == 64 else torch.float32
+    dA = DeviceCsr(A, device, vdt)
+    dB = dA if B is A else DeviceCsr(B, device, vdt)
+    t = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pi, pj, ps = lsh_candidate_pairs(dA, k, bands, seed, constrain=(hc_v == 0), score_f32=(precision == 32))
+    t["genpairs_ms"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    perm, offsets = hierarchical_clustering(A, pi, pj, ps, cluster_size, version=1 if hc_v == 1 else 0)
+    t["hc_ms"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    Acl = ClusterMatrix.build(dA, perm, offsets)
+    torch.cuda.synchronize()
+    t["conversion_ms"] = (time.perf_counter() - t0) * 1e3
+    Ccl = Acl.multiply(dB)
+    runs = []
+    for _ in range(iterations):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        C2 = Acl.multiply(dB)
+        torch.cuda.synchronize()
+        runs.append((time.perf_counter() - t0) * 1e3)
+        C2.close()
+    C = Ccl.to_csr(Acl)
+    nz, mn, mx = Ccl.symbolic()
+    out = dict(pi=pi, pj=pj, ps=ps, perm=perm, offsets=offsets, nz=nz, cmin=mn, cmax=mx,
+               C_rowptr=C.rowptr, C_col=C.col, C_val=C.val, flop=spgemm_flop(dA, dB), stats=Ccl.stats,
+               nclusters=len(offsets) - 1, nnzc_A=Acl.nnzc, nnzv_A=Acl.nnzv, times=t, runs_ms=runs)
+    Ccl.close()
+    Acl.close()
+    return out

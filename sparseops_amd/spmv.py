@@ -1,0 +1,152 @@
+"""Python handle over the C-ABI SpMV matrices (device-resident CSR / SELL-C-sigma / ELL / BSR).
+
+Mirrors the reference's operator surface for the SpMV path:
+``csr_to_sell_c_sigma`` / ``csr_to_sell_c_R`` / ``csr_to_ell`` / ``csr_to_bsr`` build a
+device matrix (include/sparse_conversion.h:195, :345, :491, :777) and
+``SpOps(alpha, A, x, beta, y)`` computes y = alpha*A*x + beta*y
+(src/spmv_csr.cpp:165, spmv_ell.cpp:225, spmv_sell_c_sigma.cpp:190,
+spmv_sell_c_R.cpp:190, spmv_bsr.cpp:215) on device tensors.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .gen import Csr
+
+VALUE_TYPES = {"f64": L.VALUE_F64, "f32": L.VALUE_F32, "bf16": L.VALUE_BF16}
+XY_DTYPE = {L.VALUE_F64: torch.float64, L.VALUE_F32: torch.float32, L.VALUE_BF16: torch.float32}
+FORMAT_NAMES = {L.FORMAT_CSR: "csr", L.FORMAT_SELL: "sell", L.FORMAT_ELL: "ell", L.FORMAT_BSR: "bsr"}
+
+
+def _vt(v) -> int:
+    return VALUE_TYPES[v] if isinstance(v, str) else int(v)
+
+
+def _index_arrays(A: Csr):
+    it = L.INDEX_I32 if A.rowptr.dtype == np.int32 else L.INDEX_I64
+    dt = np.int32 if it == L.INDEX_I32 else np.int64
+    rp = np.ascontiguousarray(A.rowptr, dtype=dt)
+    ci = np.ascontiguousarray(A.col, dtype=dt)
+    return it, rp, ci
+
+
+def _host_values(A: Csr):
+    v = np.ascontiguousarray(A.val)
+    if v.dtype == np.float32:
+        return v, L.VALUE_F32
+    return np.ascontiguousarray(v, dtype=np.float64), L.VALUE_F64
+
+
+class SpMatrix:
+    """A device matrix handle (owns the HIP allocations made by the library)."""
+
+    def __init__(self, handle: ctypes.c_void_p, keep=()):
+        self._h = handle
+        self._keep = keep
+        info = (ctypes.c_int64 * 16)()
+        L.call("spops_matrix_info", self._h, info)
+        self.info = list(info)
+        self.format = FORMAT_NAMES[self.info[0]]
+        self.n_rows, self.n_cols, self.nnz = self.info[1], self.info[2], self.info[3]
+        self.value_type = self.info[4]
+        self.xy_dtype = XY_DTYPE[self.value_type]
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def device_bytes(self) -> int:
+        return self.info[5]
+
+    @property
+    def algo_bytes(self) -> int:
+        """Compulsory bytes of one SpMV (beta = 0): indices, values, x once, y once."""
+        return self.info[6]
+
+    def spmv(self, x: torch.Tensor, y: torch.Tensor, alpha: float = 1.0, beta: float = 0.0, stream=None):
+        if x.dtype != self.xy_dtype or y.dtype != self.xy_dtype:
+            raise TypeError(f"x/y must be {self.xy_dtype} for this matrix")
+        if not (x.is_cuda and y.is_cuda and x.is_contiguous() and y.is_contiguous()):
+            raise ValueError("x and y must be contiguous device tensors")
+        if x.numel() < self.n_cols or y.numel() < self.n_rows:
+            raise ValueError("x or y too short")
+        s = L.current_stream() if stream is None else stream
+        L.call("spops_spmv", self._h, float(alpha), x.data_ptr(), float(beta), y.data_ptr(), s)
+        return y
+
+    def layout(self, which: int) -> np.ndarray:
+        n = self.n_rows if which == 0 else self.info[14]
+        out = np.zeros(n, dtype=np.int32)
+        L.call("spops_matrix_layout", self._h, which, out.ctypes.data)
+        return out
+
+    def close(self):
+        if self._h:
+            L.lib.spops_matrix_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _create(fn: str, A: Csr, value_type, *extra) -> SpMatrix:
+    it, rp, ci = _index_arrays(A)
+    v, in_vt = _host_values(A)
+    h = ctypes.c_void_p()
+    L.call(fn, ctypes.byref(h), A.n_rows, A.n_cols, A.nnz, it, rp.ctypes.data, ci.ctypes.data,
+           v.ctypes.data, in_vt, _vt(value_type), *extra)
+    return SpMatrix(h)
+
+
+def csr(A: Csr, value_type="f64") -> SpMatrix:
+    return _create("spops_csr_create", A, value_type)
+
+
+def csr_to_sell_c_sigma(A: Csr, sigma: int, C: int, value_type="f64") -> SpMatrix:
+    return _create("spops_sell_create", A, value_type, int(sigma), int(C))
+
+
+def csr_to_sell_c_R(A: Csr, C: int, value_type="f64") -> SpMatrix:
+    """SELL-C-R: one slice over the whole matrix (sigma = C * ceil(n / C) >= n)."""
+    sigma = int(C) * max(1, -(-A.n_rows // int(C)))
+    return _create("spops_sell_create", A, value_type, sigma, int(C))
+
+
+def csr_to_ell(A: Csr, value_type="f64") -> SpMatrix:
+    return _create("spops_ell_create", A, value_type)
+
+
+def csr_to_bsr(A: Csr, bR: int, bC: int, value_type="f64") -> SpMatrix:
+    return _create("spops_bsr_create", A, value_type, int(bR), int(bC))
+
+
+def sell_from_chunks(n_rows, n_cols, nnz, C, reorder, chunk_len, chunk_cols, chunk_vals, value_type="f64") -> SpMatrix:
+    """Device SELL from the reference's own host layout (per-chunk row-major arrays)."""
+    reorder = np.ascontiguousarray(reorder)
+    it = L.INDEX_I32 if reorder.dtype == np.int32 else L.INDEX_I64
+    dt = reorder.dtype
+    chunk_len = np.ascontiguousarray(chunk_len, dtype=dt)
+    cols = [np.ascontiguousarray(c, dtype=dt) for c in chunk_cols]
+    vals = [np.ascontiguousarray(v) for v in chunk_vals]
+    in_vt = L.VALUE_F32 if vals and vals[0].dtype == np.float32 else L.VALUE_F64
+    vals = [np.ascontiguousarray(v, dtype=np.float32 if in_vt == L.VALUE_F32 else np.float64) for v in vals]
+    nch = len(cols)
+    cp = (ctypes.c_void_p * nch)(*[c.ctypes.data for c in cols])
+    vp = (ctypes.c_void_p * nch)(*[v.ctypes.data for v in vals])
+    h = ctypes.c_void_p()
+    L.call("spops_sell_create_from_chunks", ctypes.byref(h), n_rows, n_cols, nnz, C, nch, it,
+           reorder.ctypes.data, chunk_len.ctypes.data, cp, vp, in_vt, _vt(value_type))
+    return SpMatrix(h)
+
+
+def SpOps(alpha: float, A: SpMatrix, x: torch.Tensor, beta: float, y: torch.Tensor):
+    """y = alpha*A*x + beta*y (the reference's SpOps_<fmt> argument order)."""
+    return A.spmv(x, y, alpha, beta)

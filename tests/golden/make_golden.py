@@ -1,0 +1,127 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of tests/golden/ from the reference itself.
+
+The reference (lsl036/SparseOps) ships no golden vectors, so its own functions are run
+here, built from its sources by oracle/Makefile into oracle/_ref/ref_harness, on seeded
+synthetic inputs (sparseops_amd.gen).  Only inputs' fingerprints and outputs are stored
+(.npz, no pickles); the inputs are regenerated from their seeds by the tests.
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from sparseops_amd.gen import laplacian_2d, powerlaw, rmat, transpose, uniform_x  # noqa: E402
+from sparseops_amd.io import write_csr_bin  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def digest(*arrays) -> str:
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def rd(d, name, dt):
+    p = os.path.join(d, name)
+    return np.fromfile(p, dtype=dt) if os.path.exists(p) else None
+
+
+def run(args, d):
+    subprocess.check_call([HARNESS, *args], stdout=subprocess.DEVNULL, env=dict(os.environ, OMP_NUM_THREADS="4"))
+
+
+def spmv_case(name, A, sigma, C, bR, bC, prec, full_layout):
+    with tempfile.TemporaryDirectory() as d:
+        write_csr_bin(os.path.join(d, "A.csr"), A)
+        x = uniform_x(A.n_cols, 1)
+        x.tofile(os.path.join(d, "x.f64"))
+        run(["spmv", os.path.join(d, "A.csr"), os.path.join(d, "x.f64"), d, str(sigma), str(C), str(bR), str(bC),
+             str(prec)], d)
+        vt = np.float64 if prec == 64 else np.float32
+        out = {"input_digest": digest(A.rowptr, A.col, A.val), "sigma": sigma, "C": C, "bR": bR, "bC": bC,
+               "x_seed": 1, "alpha": 0.8, "beta": 0.7}
+        for y in ["y0", "y_csr_a1b0", "y_csr_ab", "y_csr_lb_ab", "y_sell_ab", "y_sellr_ab", "y_ell_ab", "y_bsr_ab"]:
+            v = rd(d, y, vt)
+            if v is not None:
+                out[y] = v
+        out["sell_reorder"] = rd(d, "sell_reorder", np.int32)
+        out["sell_chunk_len"] = rd(d, "sell_chunk_len", np.int32)
+        out["sellr_reorder"] = rd(d, "sellr_reorder", np.int32)
+        out["sellr_chunk_len"] = rd(d, "sellr_chunk_len", np.int32)
+        out["sell_meta"] = rd(d, "sell_meta", np.int64)
+        out["ell_width"] = rd(d, "ell_width", np.int64)
+        sc, sv = rd(d, "sell_col", np.int32), rd(d, "sell_val", vt)
+        bp, bc, bd = rd(d, "bsr_row_ptr", np.int32), rd(d, "bsr_colidx", np.int32), rd(d, "bsr_data", vt)
+        if full_layout:
+            out.update(sell_col=sc, sell_val=sv, bsr_row_ptr=bp, bsr_colidx=bc, bsr_data=bd)
+        out["sell_layout_digest"] = digest(sc, sv)
+        if bp is not None:
+            out["bsr_layout_digest"] = digest(bp, bc, bd)
+    np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **out)
+    print("wrote", name)
+
+
+def lespgemm_case(name, A, B, k=64, bands=16, seed=7, hc_v=0, cs=8, prec=64, full_pairs=True, full_c=False):
+    with tempfile.TemporaryDirectory() as d:
+        write_csr_bin(os.path.join(d, "A.csr"), A)
+        write_csr_bin(os.path.join(d, "B.csr"), B)
+        run(["lespgemm", os.path.join(d, "A.csr"), os.path.join(d, "B.csr"), d, str(k), str(bands), str(seed),
+             str(hc_v), str(cs), "3", str(prec)], d)
+        sig = rd(d, "signatures", np.uint64).reshape(A.n_rows, k)
+        pi, pj, ps = rd(d, "pairs_i", np.int64), rd(d, "pairs_j", np.int64), rd(d, "pairs_score", np.float64)
+        perm, off = rd(d, "perm", np.int64), rd(d, "offsets", np.int64)
+        crp, ccol, cval = rd(d, "C_rowptr", np.int64), rd(d, "C_col", np.int64), rd(d, "C_val", np.float64)
+        # the reference emits C's columns in cluster column order; sort within rows
+        rows = np.repeat(np.arange(A.n_rows), np.diff(crp))
+        o = np.lexsort((ccol, rows))
+        ccol, cval = ccol[o], cval[o]
+        out = {"input_digest": digest(A.rowptr, A.col, A.val), "B_digest": digest(B.rowptr, B.col, B.val),
+               "k": k, "bands": bands, "seed": seed, "hc_v": hc_v, "cluster_size": cs, "precision": prec,
+               "sig_head": sig[:64].copy(), "sig_digest": digest(sig), "npairs": len(pi),
+               "pairs_digest": digest(pi, pj, ps), "perm": perm, "offsets": off,
+               "acl_meta": rd(d, "acl_meta", np.int64), "ccl_meta": rd(d, "ccl_meta", np.int64),
+               "ccl_rowptr": rd(d, "ccl_rowptr", np.int64), "ccl_min": rd(d, "ccl_min", np.int64),
+               "ccl_max": rd(d, "ccl_max", np.int64), "flops": rd(d, "flops", np.int64),
+               "C_rowptr": crp, "C_col_digest": digest(ccol),
+               "C_rowsum": np.bincount(rows[o], weights=cval, minlength=A.n_rows)}
+        if full_pairs:
+            out.update(pairs_i=pi, pairs_j=pj, pairs_score=ps)
+        if full_c:
+            out.update(C_col=ccol, C_val=cval)
+    np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **out)
+    print("wrote", name, "pairs", out["npairs"], "clusters", len(out["offsets"]) - 1, "nnz(C)", out["C_rowptr"][-1])
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit(f"{HARNESS} missing: run make -C oracle ref (needs the reference sources)")
+    spmv_case("spmv_lap16_f64", laplacian_2d(16), 64, 8, 4, 4, 64, True)
+    spmv_case("spmv_lap16_f32", laplacian_2d(16), 64, 8, 4, 8, 32, True)
+    spmv_case("spmv_pl3000_f64", powerlaw(3000, lmax=300, seed=2), 256, 32, 16, 4, 64, False)
+    spmv_case("spmv_pl3000_f32", powerlaw(3000, lmax=300, seed=2), 256, 32, 16, 8, 32, False)
+    R8, R10, R12 = rmat(8, 16, seed=8), rmat(10, 16, seed=10), rmat(12, 16, seed=12)
+    lespgemm_case("lsh_rmat8", R8, R8, full_pairs=True, full_c=True)
+    lespgemm_case("lsh_rmat10", R10, R10, full_pairs=True)
+    lespgemm_case("lsh_rmat10_AAt", R10, transpose(R10), full_pairs=False)
+    lespgemm_case("lsh_rmat10_hc1", R10, R10, hc_v=1, full_pairs=False)
+    lespgemm_case("lsh_rmat10_hc2", R10, R10, hc_v=2, full_pairs=False)
+    lespgemm_case("lsh_rmat10_f32", R10, R10, prec=32, full_pairs=False)
+    lespgemm_case("lsh_rmat10_k128", R10, R10, k=128, bands=32, seed=42, full_pairs=False)
+    lespgemm_case("lsh_rmat12", R12, R12, full_pairs=False)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,257 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so, oracle/oracle.c) for the tests.
+
+Test infrastructure only: the oracle is the checker, never the thing measured.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+i64 = ctypes.c_int64
+c_int = ctypes.c_int
+c_dbl = ctypes.c_double
+
+
+def _load():
+    if not os.path.exists(ORACLE_SO):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "oracle"])
+    lib = ctypes.CDLL(ORACLE_SO)
+    for name in ["orc_sell_layout", "orc_ell_width", "orc_bsr_count", "orc_lsh_pairs", "orc_hc",
+                 "orc_cluster_build", "orc_spgemm_flop", "orc_cluster_classify", "orc_cluster_to_csr"]:
+        getattr(lib, name).restype = i64
+    lib.orc_band_hash.restype = ctypes.c_uint64
+    lib.orc_jaccard.restype = c_dbl
+    return lib
+
+
+lib = _load()
+
+
+def P(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _take(p, n, ctype, dtype):
+    if n == 0:
+        lib.orc_free(p)
+        return np.zeros(0, dtype=dtype)
+    arr = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctype)), (n,)).copy()
+    lib.orc_free(p)
+    return arr.astype(dtype, copy=False)
+
+
+# ---- SpMV ------------------------------------------------------------------------------------
+def spmv_csr(A, x, alpha=1.0, beta=0.0, y0=None):
+    rp, ci, v = _i64(A.rowptr), _i64(A.col), _f64(A.val)
+    y = np.zeros(A.n_rows) if y0 is None else _f64(y0).copy()
+    lib.orc_spmv_csr(i64(A.n_rows), P(rp), P(ci), P(v), c_dbl(alpha), P(_f64(x)), c_dbl(beta), P(y))
+    return y
+
+
+def spmv_csr_f32(A, x, alpha=1.0, beta=0.0, y0=None):
+    rp, ci = _i64(A.rowptr), _i64(A.col)
+    v = np.ascontiguousarray(A.val, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros(A.n_rows, np.float32) if y0 is None else np.ascontiguousarray(y0, dtype=np.float32).copy()
+    lib.orc_spmv_csr_f32(i64(A.n_rows), P(rp), P(ci), P(v), ctypes.c_float(alpha), P(xx), ctypes.c_float(beta), P(y))
+    return y
+
+
+def spmv_exact(A, x, alpha=1.0, beta=0.0, y0=None, values=None):
+    """Extended-precision y and the row condition scale sum_j |a_rj x_j|."""
+    rp, ci = _i64(A.rowptr), _i64(A.col)
+    v = _f64(A.val if values is None else values)
+    y = np.zeros(A.n_rows) if y0 is None else _f64(y0).copy()
+    absrow = np.zeros(A.n_rows)
+    lib.orc_spmv_csr_ld(i64(A.n_rows), P(rp), P(ci), P(v), c_dbl(alpha), P(_f64(x)), c_dbl(beta), P(y), P(absrow))
+    return y, absrow
+
+
+def sell_layout(A, sigma, C, align):
+    n = A.n_rows
+    rp = _i64(A.rowptr)
+    reorder = np.zeros(n, np.int64)
+    nch = (n + C - 1) // C
+    cl = np.zeros(nch, np.int64)
+    co = np.zeros(nch, np.int64)
+    tot = lib.orc_sell_layout(i64(n), P(rp), i64(sigma), i64(C), i64(align), P(reorder), P(cl), P(co))
+    if tot < 0:
+        raise ValueError("sigma % C != 0")
+    scol = np.zeros(tot, np.int64)
+    sval = np.zeros(tot)
+    lib.orc_sell_fill(i64(n), P(rp), P(_i64(A.col)), P(_f64(A.val)), i64(C), P(reorder), P(cl), P(co), P(scol), P(sval))
+    return reorder, cl, co, scol, sval
+
+
+def spmv_sell(n, C, reorder, cl, co, scol, sval, x, alpha, beta, y0):
+    y = _f64(y0).copy()
+    lib.orc_spmv_sell(i64(n), i64(C), P(_i64(reorder)), P(_i64(cl)), P(_i64(co)), P(_i64(scol)), P(_f64(sval)),
+                      c_dbl(alpha), P(_f64(x)), c_dbl(beta), P(y))
+    return y
+
+
+def ell(A, col_major=False):
+    rp = _i64(A.rowptr)
+    w = lib.orc_ell_width(i64(A.n_rows), P(rp))
+    ec = np.zeros(A.n_rows * w, np.int64)
+    ev = np.zeros(A.n_rows * w)
+    lib.orc_ell_fill(i64(A.n_rows), P(rp), P(_i64(A.col)), P(_f64(A.val)), i64(w), c_int(int(col_major)), P(ec), P(ev))
+    return w, ec, ev
+
+
+def spmv_ell(n, w, ec, ev, x, alpha, beta, y0, col_major=False):
+    y = _f64(y0).copy()
+    lib.orc_spmv_ell(i64(n), i64(w), c_int(int(col_major)), P(ec), P(ev), c_dbl(alpha), P(_f64(x)), c_dbl(beta), P(y))
+    return y
+
+
+def bsr(A, bR, bC):
+    rp, ci = _i64(A.rowptr), _i64(A.col)
+    mb = (A.n_rows + bR - 1) // bR
+    brp = np.zeros(mb + 1, np.int64)
+    nnzb = lib.orc_bsr_count(i64(A.n_rows), i64(A.n_cols), P(rp), P(ci), i64(bR), i64(bC), P(brp))
+    bcol = np.zeros(nnzb, np.int64)
+    bd = np.zeros(nnzb * bR * bC)
+    lib.orc_bsr_fill(i64(A.n_rows), i64(A.n_cols), P(rp), P(ci), P(_f64(A.val)), i64(bR), i64(bC), P(brp), P(bcol), P(bd))
+    return brp, bcol, bd
+
+
+def spmv_bsr(n_rows, n_cols, bR, bC, brp, bcol, bd, x, alpha, beta, y0):
+    y = _f64(y0).copy()
+    lib.orc_spmv_bsr(i64(n_rows), i64(n_cols), i64(bR), i64(bC), P(brp), P(bcol), P(bd), c_dbl(alpha), P(_f64(x)),
+                     c_dbl(beta), P(y))
+    return y
+
+
+# ---- LeSpGEMM ----------------------------------------------------------------------------------
+def minhash(A, k, seed):
+    sig = np.zeros(A.n_rows * k, np.uint64)
+    lib.orc_minhash(i64(A.n_rows), P(_i64(A.rowptr)), P(_i64(A.col)), c_int(k), ctypes.c_uint64(seed), P(sig))
+    return sig.reshape(A.n_rows, k)
+
+
+def band_hash(band_sig):
+    b = np.ascontiguousarray(band_sig, dtype=np.uint64)
+    return int(lib.orc_band_hash(P(b), c_int(b.size)))
+
+
+def lsh_pairs(sigs, k, bands, constrain=True):
+    s = np.ascontiguousarray(sigs, dtype=np.uint64)
+    n = s.shape[0]
+    pi, pj, ps = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    m = lib.orc_lsh_pairs(P(s), i64(n), c_int(k), c_int(bands), c_int(int(constrain)), ctypes.byref(pi),
+                          ctypes.byref(pj), ctypes.byref(ps))
+    return (_take(pi, m, ctypes.c_int64, np.int64), _take(pj, m, ctypes.c_int64, np.int64),
+            _take(ps, m, ctypes.c_double, np.float64))
+
+
+def hc(A, pi, pj, score, cluster_size=8, version=0, score_f32=False):
+    perm = np.zeros(A.n_rows, np.int64)
+    off = ctypes.c_void_p()
+    nc = lib.orc_hc(c_int(version), i64(A.n_rows), P(_i64(A.rowptr)), P(_i64(A.col)), i64(len(pi)), P(_i64(pi)),
+                    P(_i64(pj)), P(_f64(score)), c_int(int(score_f32)), i64(cluster_size), P(perm), ctypes.byref(off))
+    return perm, _take(off, nc + 1, ctypes.c_int64, np.int64)
+
+
+def jaccard(A, a, b, sorted_unique=True):
+    return lib.orc_jaccard(P(_i64(A.rowptr)), P(_i64(A.col)), i64(a), i64(b), c_int(int(sorted_unique)))
+
+
+def cluster_build(A, perm, offsets):
+    nc = len(offsets) - 1
+    clr = np.zeros(nc + 1, np.int64)
+    clrv = np.zeros(nc + 1, np.int64)
+    csz = np.zeros(nc, np.int64)
+    cid, cv = ctypes.c_void_p(), ctypes.c_void_p()
+    nnzc = lib.orc_cluster_build(i64(A.n_rows), P(_i64(A.rowptr)), P(_i64(A.col)), P(_f64(A.val)), P(_i64(perm)),
+                                 i64(nc), P(_i64(offsets)), P(clr), P(clrv), P(csz), ctypes.byref(cid), ctypes.byref(cv))
+    return (clr, clrv, csz, _take(cid, nnzc, ctypes.c_int64, np.int64),
+            _take(cv, int(clrv[-1]), ctypes.c_double, np.float64))
+
+
+def spgemm_flop(A, B):
+    return int(lib.orc_spgemm_flop(i64(A.n_rows), P(_i64(A.rowptr)), P(_i64(A.col)), i64(B.n_rows), P(_i64(B.rowptr))))
+
+
+def cluster_symbolic(clr, colids, B):
+    nc = len(clr) - 1
+    nz, mn, mx = np.zeros(nc, np.int64), np.zeros(nc, np.int64), np.zeros(nc, np.int64)
+    lib.orc_cluster_symbolic(i64(nc), P(clr), P(_i64(colids)), P(_i64(B.rowptr)), P(_i64(B.col)), i64(B.n_cols),
+                             P(nz), P(mn), P(mx))
+    return nz, mn, mx
+
+
+def cluster_numeric(clr, clrv, csz, colids, values, B, nz, use_long_double=False):
+    nc = len(clr) - 1
+    crp = np.zeros(nc + 1, np.int64)
+    crp[1:] = np.cumsum(nz)
+    crv = np.zeros(nc + 1, np.int64)
+    crv[1:] = np.cumsum(nz * csz)
+    cc = np.zeros(int(crp[-1]), np.int64)
+    cvals = np.zeros(int(crv[-1]))
+    lib.orc_cluster_numeric(i64(nc), P(clr), P(clrv), P(csz), P(_i64(colids)), P(_f64(values)), P(_i64(B.rowptr)),
+                            P(_i64(B.col)), P(_f64(B.val)), P(crp), P(crv), P(cc), P(cvals), c_int(int(use_long_double)))
+    return crp, crv, cc, cvals
+
+
+def cluster_to_csr(n_rows, csz, perm, crp, crv, cc, cvals):
+    nc = len(csz)
+    rp = np.zeros(n_rows + 1, np.int64)
+    co, vo = ctypes.c_void_p(), ctypes.c_void_p()
+    nnz = lib.orc_cluster_to_csr(i64(n_rows), i64(nc), P(_i64(csz)), P(_i64(perm)), P(_i64(crp)), P(_i64(crv)),
+                                 P(_i64(cc)), P(_f64(cvals)), P(rp), ctypes.byref(co), ctypes.byref(vo))
+    return rp, _take(co, nnz, ctypes.c_int64, np.int64), _take(vo, nnz, ctypes.c_double, np.float64)
+
+
+def lespgemm(A, B, k=64, bands=16, seed=7, cluster_size=8, hc_version=0, constrain=True, use_long_double=False):
+    """Full pipeline on the CPU: returns dict with pairs, perm, offsets and C (original row order)."""
+    sig = minhash(A, k, seed)
+    pi, pj, ps = lsh_pairs(sig, k, bands, constrain)
+    perm, off = hc(A, pi, pj, ps, cluster_size, hc_version)
+    clr, clrv, csz, colids, vals = cluster_build(A, perm, off)
+    nz, mn, mx = cluster_symbolic(clr, colids, B)
+    crp, crv, cc, cvals = cluster_numeric(clr, clrv, csz, colids, vals, B, nz, use_long_double)
+    rp, cols, cv = cluster_to_csr(A.n_rows, csz, perm, crp, crv, cc, cvals)
+    return dict(sig=sig, pi=pi, pj=pj, ps=ps, perm=perm, offsets=off, nz=nz, cmin=mn, cmax=mx,
+                C_rowptr=rp, C_col=cols, C_val=cv, flop=spgemm_flop(A, B))
+
+
+# ---- AI forms ----------------------------------------------------------------------------------
+def spmm(A, X):
+    X = _f64(X)
+    k = X.shape[1]
+    Y = np.zeros((A.n_rows, k))
+    lib.orc_spmm_csr(i64(A.n_rows), P(_i64(A.rowptr)), P(_i64(A.col)), P(_f64(A.val)), i64(k), P(X), P(Y))
+    return Y
+
+
+def sddmm(A, G, X):
+    G, X = _f64(G), _f64(X)
+    d = np.zeros(A.nnz)
+    lib.orc_sddmm_csr(i64(A.n_rows), P(_i64(A.rowptr)), P(_i64(A.col)), i64(G.shape[1]), P(G), P(X), P(d))
+    return d
+
+
+def transpose(A):
+    trp = np.zeros(A.n_cols + 1, np.int64)
+    tc = np.zeros(A.nnz, np.int64)
+    tv = np.zeros(A.nnz)
+    tp = np.zeros(A.nnz, np.int64)
+    lib.orc_csr_transpose(i64(A.n_rows), i64(A.n_cols), P(_i64(A.rowptr)), P(_i64(A.col)), P(_f64(A.val)), P(trp),
+                          P(tc), P(tv), P(tp))
+    return trp, tc, tv, tp

@@ -1,0 +1,132 @@
+"""LeSpGEMM parity on the GPU (through the C ABI): MinHash signatures, LSH pairs, clusters and
+permutations bit-exact; C structure (row_ptr, sorted col_idx) bit-exact; FP64 values within
+1e-12 relative (FP32: 2e-6) of the oracle and of the reference fixtures."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle_bind as orc
+from sparseops_amd import lespgemm as LG
+from sparseops_amd.gen import Csr, rmat, transpose
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+
+
+def digest(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+@pytest.mark.parametrize("scale,k,seed", [(10, 64, 7), (10, 128, 42), (9, 32, 12345), (12, 64, 7), (8, 200, 3)])
+def test_minhash_bit_exact(scale, k, seed, dev):
+    A = rmat(scale, 16, seed=scale)
+    got = LG.minhash_signatures(LG.DeviceCsr(A, dev), k, seed).cpu().numpy().view(np.uint64)
+    np.testing.assert_array_equal(got, orc.minhash(A, k, seed))
+
+
+@pytest.mark.parametrize("name", ["lsh_rmat8", "lsh_rmat10", "lsh_rmat10_k128", "lsh_rmat12", "lsh_rmat10_hc1"])
+def test_lsh_pairs_match_reference(name, dev):
+    g = gold(name)
+    scale = {"lsh_rmat8": 8, "lsh_rmat12": 12}.get(name, 10)
+    A = rmat(scale, 16, seed=scale)
+    dA = LG.DeviceCsr(A, dev)
+    sig = LG.minhash_signatures(dA, int(g["k"]), int(g["seed"])).cpu().numpy().view(np.uint64)
+    assert digest(sig) == str(g["sig_digest"])
+    pi, pj, ps = LG.lsh_candidate_pairs(dA, int(g["k"]), int(g["bands"]), int(g["seed"]),
+                                        constrain=int(g["hc_v"]) == 0)
+    assert len(pi) == int(g["npairs"])
+    assert digest(pi, pj, ps) == str(g["pairs_digest"])
+
+
+def test_lsh_large_bucket_window(dev):
+    """300 identical rows + 700 empty rows: buckets above 256 members use the 16-window."""
+    n = 1000
+    lens = np.r_[np.ones(300, np.int64), np.zeros(700, np.int64)]
+    rp = np.r_[0, np.cumsum(lens)]
+    A = Csr(n, 50, rp, np.full(300, 7, np.int64), np.ones(300))
+    dA = LG.DeviceCsr(A, dev)
+    for constrain in (True, False):
+        got = LG.lsh_candidate_pairs(dA, 8, 2, 5, constrain=constrain)
+        want = orc.lsh_pairs(orc.minhash(A, 8, 5), 8, 2, constrain)
+        for a, b in zip(got, want):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_lsh_score_f32(dev):
+    A = rmat(10, 16, seed=10)
+    pi, pj, ps = LG.lsh_candidate_pairs(LG.DeviceCsr(A, dev), 64, 16, 7, score_f32=True)
+    wi, wj, ws = orc.lsh_pairs(orc.minhash(A, 64, 7), 64, 16)
+    np.testing.assert_array_equal(pi, wi)
+    np.testing.assert_array_equal(ps, ws.astype(np.float32))
+
+
+@pytest.mark.parametrize("scale,cs", [(10, 8), (12, 8), (11, 4)])
+def test_cluster_build_matches_oracle(scale, cs, dev):
+    A = rmat(scale, 16, seed=scale)
+    sig = orc.minhash(A, 64, 7)
+    perm, off = orc.hc(A, *orc.lsh_pairs(sig, 64, 16), cluster_size=cs)
+    Acl = LG.ClusterMatrix.build(LG.DeviceCsr(A, dev), perm, off)
+    csz, rp, rpv, col, val = Acl.download()
+    wr, wrv, wcsz, wcol, wval = orc.cluster_build(A, perm, off)
+    np.testing.assert_array_equal(csz, wcsz)
+    np.testing.assert_array_equal(rp, wr)
+    np.testing.assert_array_equal(rpv, wrv)
+    np.testing.assert_array_equal(col, wcol)
+    np.testing.assert_array_equal(val, wval)
+
+
+CASES = [("lsh_rmat8", 8, None, 64), ("lsh_rmat10", 10, None, 64), ("lsh_rmat10_AAt", 10, "T", 64),
+         ("lsh_rmat10_hc1", 10, None, 64), ("lsh_rmat10_hc2", 10, None, 64), ("lsh_rmat10_f32", 10, None, 32),
+         ("lsh_rmat12", 12, None, 64)]
+
+
+@pytest.mark.parametrize("name,scale,B_of,prec", CASES)
+def test_lespgemm_pipeline_matches_reference(name, scale, B_of, prec, dev):
+    g = gold(name)
+    A = rmat(scale, 16, seed=scale)
+    B = transpose(A) if B_of == "T" else A
+    out = LG.lespgemm(A, B, k=int(g["k"]), bands=int(g["bands"]), seed=int(g["seed"]),
+                      cluster_size=int(g["cluster_size"]), hc_v=int(g["hc_v"]), device=dev, precision=prec)
+    np.testing.assert_array_equal(out["perm"], g["perm"])
+    np.testing.assert_array_equal(out["offsets"], g["offsets"])
+    assert out["flop"] == int(g["flops"][0])
+    assert out["nclusters"] == int(g["acl_meta"][0]) and out["nnzc_A"] == int(g["acl_meta"][1])
+    np.testing.assert_array_equal(out["cmin"], g["ccl_min"])
+    np.testing.assert_array_equal(out["cmax"], g["ccl_max"])
+    np.testing.assert_array_equal(np.r_[0, np.cumsum(out["nz"])], g["ccl_rowptr"])
+    np.testing.assert_array_equal(out["C_rowptr"], g["C_rowptr"])
+    assert digest(out["C_col"]) == str(g["C_col_digest"])
+    rows = np.repeat(np.arange(A.n_rows), np.diff(out["C_rowptr"]))
+    rs = np.bincount(rows, weights=out["C_val"].astype(np.float64), minlength=A.n_rows)
+    np.testing.assert_allclose(rs, g["C_rowsum"], rtol=1e-12 if prec == 64 else 2e-6, atol=0)
+    if prec == 64:
+        want = orc.lespgemm(A, B, k=int(g["k"]), bands=int(g["bands"]), seed=int(g["seed"]),
+                            cluster_size=int(g["cluster_size"]), hc_version=1 if int(g["hc_v"]) == 1 else 0,
+                            constrain=int(g["hc_v"]) == 0, use_long_double=True)
+        np.testing.assert_array_equal(out["C_col"], want["C_col"])
+        rel = np.abs(out["C_val"] - want["C_val"]) / np.abs(want["C_val"])
+        assert rel.max() <= 1e-12, rel.max()
+
+
+def test_lespgemm_exercises_both_accumulators(dev):
+    A = rmat(12, 16, seed=12)
+    out = LG.lespgemm(A, A, device=dev)
+    assert out["stats"]["small_clusters"] > 0 and out["stats"]["large_clusters"] > 0
+
+
+def test_lespgemm_error_paths(dev):
+    A = rmat(6, 4, seed=1)
+    with pytest.raises(ValueError):
+        LG.lespgemm(A, rmat(7, 4, seed=1), device=dev)
+    with pytest.raises(ValueError):
+        LG.lespgemm(A, A, k=10, bands=3, device=dev)

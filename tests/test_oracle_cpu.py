@@ -1,0 +1,313 @@
+"""Pins the CPU oracle (oracle/oracle.c) before anything is checked against it.
+
+Three kinds of evidence, per function:
+  * the reference's own outputs (tests/golden/*.npz, written by tests/golden/make_golden.py
+    from the reference built from its sources),
+  * closed forms / brute force that do not share the oracle's algorithm (numpy, scipy,
+    Python integers, pairwise enumeration),
+  * invariants a plausible mistake would break.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle_bind as orc
+from sparseops_amd.gen import Csr, laplacian_2d, powerlaw, rmat, transpose, uniform_x
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False)
+
+
+def digest(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def scipy_csr(A):
+    return sp.csr_matrix((A.val, A.col.astype(np.int64), A.rowptr.astype(np.int64)), shape=(A.n_rows, A.n_cols))
+
+
+SPMV_CASES = {
+    "spmv_lap16_f64": lambda: laplacian_2d(16),
+    "spmv_pl3000_f64": lambda: powerlaw(3000, lmax=300, seed=2),
+}
+
+
+# ---------------------------------------------------------------------------------------- SpMV
+@pytest.mark.parametrize("name", list(SPMV_CASES))
+def test_spmv_formats_match_reference(name):
+    g = gold(name)
+    A = SPMV_CASES[name]()
+    assert digest(A.rowptr, A.col, A.val) == str(g["input_digest"]), "generator drifted from the fixture"
+    x = uniform_x(A.n_cols, int(g["x_seed"]))
+    a, b, y0 = float(g["alpha"]), float(g["beta"]), g["y0"]
+    sigma, C, bR, bC = int(g["sigma"]), int(g["C"]), int(g["bR"]), int(g["bC"])
+    tol = 1e-13 * (np.abs(scipy_csr(A)) @ np.abs(x) + np.abs(y0) + 1e-300)
+    y = orc.spmv_csr(A, x, a, b, y0)
+    assert np.all(np.abs(y - g["y_csr_ab"]) <= tol)
+    assert np.all(np.abs(orc.spmv_csr(A, x) - g["y_csr_a1b0"]) <= tol)
+    reorder, cl, co, scol, sval = orc.sell_layout(A, sigma, C, 4)
+    np.testing.assert_array_equal(reorder, g["sell_reorder"])
+    np.testing.assert_array_equal(cl, g["sell_chunk_len"])
+    assert digest(scol.astype(np.int32), sval) == str(g["sell_layout_digest"])
+    ys = orc.spmv_sell(A.n_rows, C, reorder, cl, co, scol, sval, x, a, b, y0)
+    assert np.all(np.abs(ys - g["y_sell_ab"]) <= tol)
+    sigR = C * -(-A.n_rows // C)
+    np.testing.assert_array_equal(orc.sell_layout(A, sigR, C, 4)[0], g["sellr_reorder"])
+    w, ec, ev = orc.ell(A)
+    assert w == int(g["ell_width"][0])
+    assert np.all(np.abs(orc.spmv_ell(A.n_rows, w, ec, ev, x, a, b, y0) - g["y_ell_ab"]) <= tol)
+    brp, bcol, bd = orc.bsr(A, bR, bC)
+    assert digest(brp.astype(np.int32), bcol.astype(np.int32), bd) == str(g["bsr_layout_digest"])
+    yb = orc.spmv_bsr(A.n_rows, A.n_cols, bR, bC, brp, bcol, bd, x, a, b, y0)
+    assert np.all(np.abs(yb - g["y_bsr_ab"]) <= tol)
+
+
+def test_spmv_laplacian_closed_form():
+    A = laplacian_2d(32)
+    y = orc.spmv_csr(A, np.ones(A.n_cols))
+    np.testing.assert_array_equal(y, 4.0 - (np.diff(A.rowptr) - 1))
+
+
+def test_spmv_against_scipy_and_col_major_ell():
+    A = powerlaw(5000, lmax=700, seed=4)
+    x = uniform_x(A.n_cols, 9)
+    y0 = np.linspace(-1, 1, A.n_rows)
+    ref = 0.3 * (scipy_csr(A) @ x) - 1.25 * y0
+    scale = 0.3 * (np.abs(scipy_csr(A)) @ np.abs(x)) + 1.25 * np.abs(y0)
+    assert np.all(np.abs(orc.spmv_csr(A, x, 0.3, -1.25, y0) - ref) <= 1e-13 * scale)
+    w, ec, ev = orc.ell(A, col_major=True)
+    assert np.all(np.abs(orc.spmv_ell(A.n_rows, w, ec, ev, x, 0.3, -1.25, y0, col_major=True) - ref) <= 1e-13 * scale)
+    for bR, bC in [(1, 1), (3, 7), (16, 4)]:
+        brp, bcol, bd = orc.bsr(A, bR, bC)
+        yb = orc.spmv_bsr(A.n_rows, A.n_cols, bR, bC, brp, bcol, bd, x, 0.3, -1.25, y0)
+        assert np.all(np.abs(yb - ref) <= 1e-13 * scale)
+
+
+def test_sell_reorder_closed_form():
+    """reorder = per-slice sort by (nnz desc, row desc), computed with numpy lexsort."""
+    A = powerlaw(4000, lmax=200, seed=5)
+    lens = np.diff(A.rowptr.astype(np.int64))
+    for sigma, C in [(64, 8), (1024, 32), (48, 12)]:
+        want = []
+        for lo in range(0, A.n_rows, sigma):
+            rows = np.arange(lo, min(A.n_rows, lo + sigma))
+            o = np.lexsort((-rows, -lens[rows]))
+            want.append(rows[o])
+        want = np.concatenate(want)
+        reorder, cl, _, _, _ = orc.sell_layout(A, sigma, C, 4)
+        np.testing.assert_array_equal(reorder, want)
+        chunk_max = np.array([lens[want[c * C:(c + 1) * C]].max() for c in range(len(cl))])
+        np.testing.assert_array_equal(cl, -(-chunk_max // 4) * 4)
+
+
+def test_sell_sigma_not_multiple_of_c():
+    with pytest.raises(ValueError):
+        orc.sell_layout(laplacian_2d(4), 10, 4, 4)
+
+
+# ------------------------------------------------------------------------------ MinHash + LSH
+def _lcg_coefs(k, seed):
+    M = 1 << 64
+    s = seed
+    a, b = [], []
+    for _ in range(k):
+        s = (s * 6364136223846793005 + 1442695040888963407) % M
+        a.append(((s >> 1) * 2 + 1) % M)
+        s = (s * 6364136223846793005 + 1442695040888963407) % M
+        b.append(s)
+    return a, b
+
+
+def test_minhash_brute_force_python_integers():
+    A = rmat(7, 8, seed=3)
+    k, seed = 24, 99
+    sig = orc.minhash(A, k, seed)
+    a, b = _lcg_coefs(k, seed)
+    M = 1 << 64
+    for r in range(0, A.n_rows, 7):
+        cols = A.col[A.rowptr[r]:A.rowptr[r + 1]]
+        for d in range(k):
+            want = min(((a[d] * int(c) + b[d]) % M for c in cols), default=M - 1)
+            assert int(sig[r, d]) == want
+
+
+@pytest.mark.parametrize("name", ["lsh_rmat8", "lsh_rmat10", "lsh_rmat10_k128", "lsh_rmat12", "lsh_rmat10_hc1"])
+def test_lsh_matches_reference(name):
+    g = gold(name)
+    scale = {"lsh_rmat8": 8, "lsh_rmat12": 12}.get(name, 10)
+    A = rmat(scale, 16, seed=scale)
+    assert digest(A.rowptr, A.col, A.val) == str(g["input_digest"])
+    k, bands, seed = int(g["k"]), int(g["bands"]), int(g["seed"])
+    sig = orc.minhash(A, k, seed)
+    assert digest(sig) == str(g["sig_digest"])
+    np.testing.assert_array_equal(sig[:64], g["sig_head"])
+    pi, pj, ps = orc.lsh_pairs(sig, k, bands, constrain=int(g["hc_v"]) == 0)
+    assert len(pi) == int(g["npairs"])
+    assert digest(pi, pj, ps) == str(g["pairs_digest"])
+
+
+def test_lsh_brute_force_pairwise():
+    """Pairs from all-pairs band comparison (no sorting/grouping), buckets <= 256."""
+    A = rmat(6, 4, seed=1)
+    k, bands = 16, 4
+    r = k // bands
+    sig = orc.minhash(A, k, 5)
+    n = A.n_rows
+    bh = np.array([[orc.band_hash(sig[i, b * r:(b + 1) * r]) for b in range(bands)] for i in range(n)], dtype=object)
+    want = set()
+    for i in range(n):
+        for j in range(i + 1, n):
+            if any(bh[i, b] == bh[j, b] for b in range(bands)):
+                want.add((i, j))
+    pi, pj, ps = orc.lsh_pairs(sig, k, bands)
+    assert set(zip(pi.tolist(), pj.tolist())) == want
+    for i, j, s in zip(pi[:200], pj[:200], ps[:200]):
+        assert s == np.count_nonzero(sig[i] == sig[j]) / k
+
+
+def test_lsh_window_on_large_bucket():
+    """A bucket of 300 identical rows: constrain -> each row pairs with its next 16."""
+    n = 300
+    A = Csr(n, 10, np.arange(n + 1, dtype=np.int64), np.full(n, 3, np.int64), np.ones(n))
+    sig = orc.minhash(A, 8, 1)
+    pi, pj, _ = orc.lsh_pairs(sig, 8, 2, constrain=True)
+    want = {(i, j) for i in range(n) for j in range(i + 1, min(n, i + 17))}
+    assert set(zip(pi.tolist(), pj.tolist())) == want
+    pi2, _, _ = orc.lsh_pairs(sig, 8, 2, constrain=False)
+    assert len(pi2) == n * (n - 1) // 2
+
+
+def test_lsh_bad_band_count_returns_empty():
+    A = rmat(5, 4, seed=2)
+    sig = orc.minhash(A, 10, 1)
+    assert len(orc.lsh_pairs(sig, 10, 3)[0]) == 0
+
+
+# ---------------------------------------------------------------------------------------- HC
+HC_CASES = ["lsh_rmat8", "lsh_rmat10", "lsh_rmat10_hc1", "lsh_rmat10_hc2", "lsh_rmat10_f32", "lsh_rmat10_k128",
+            "lsh_rmat12"]
+
+
+def _case(name):
+    g = gold(name)
+    scale = {"lsh_rmat8": 8, "lsh_rmat12": 12}.get(name, 10)
+    A = rmat(scale, 16, seed=scale)
+    k, bands, seed, hc_v = int(g["k"]), int(g["bands"]), int(g["seed"]), int(g["hc_v"])
+    sig = orc.minhash(A, k, seed)
+    pi, pj, ps = orc.lsh_pairs(sig, k, bands, constrain=hc_v == 0)
+    return g, A, pi, pj, ps, hc_v
+
+
+@pytest.mark.parametrize("name", HC_CASES)
+def test_hc_matches_reference(name):
+    g, A, pi, pj, ps, hc_v = _case(name)
+    f32 = int(g["precision"]) == 32
+    perm, off = orc.hc(A, pi, pj, ps, int(g["cluster_size"]), version=1 if hc_v == 1 else 0, score_f32=f32)
+    np.testing.assert_array_equal(perm, g["perm"])
+    np.testing.assert_array_equal(off, g["offsets"])
+
+
+@pytest.mark.parametrize("version", [0, 1])
+def test_hc_invariants(version):
+    A = rmat(11, 16, seed=21)
+    sig = orc.minhash(A, 64, 7)
+    pi, pj, ps = orc.lsh_pairs(sig, 64, 16)
+    cs = 6
+    perm, off = orc.hc(A, pi, pj, ps, cs, version=version)
+    assert np.array_equal(np.sort(perm), np.arange(A.n_rows))
+    sizes = np.diff(off)
+    assert sizes.min() >= 1 and sizes.max() <= 2 * cs - 2
+    for c in range(len(sizes)):
+        rows = perm[off[c]:off[c + 1]]
+        assert np.all(np.diff(rows) > 0)
+
+
+def test_hc_mutations_change_output():
+    """Planted mistakes (scores in the wrong order; a different cluster size) must change the
+    clustering, so the fixture comparison above is sensitive to the merge order."""
+    g, A, pi, pj, ps, _ = _case("lsh_rmat10")
+    perm, off = orc.hc(A, pi, pj, 1.0 - ps, 8)
+    assert not (np.array_equal(perm, g["perm"]) and np.array_equal(off, g["offsets"]))
+    perm, off = orc.hc(A, pi, pj, ps, 7)
+    assert not (np.array_equal(perm, g["perm"]) and np.array_equal(off, g["offsets"]))
+
+
+def test_jaccard_closed_form():
+    A = Csr(3, 10, np.array([0, 3, 6, 6]), np.array([1, 2, 5, 2, 5, 9]), np.ones(6))
+    assert orc.jaccard(A, 0, 1) == 2 / 4
+    assert orc.jaccard(A, 0, 2) == 0.0
+    assert orc.jaccard(A, 0, 1, sorted_unique=False) == 2 / 4
+
+
+# ------------------------------------------------------------------------- cluster SpGEMM
+@pytest.mark.parametrize("name,B_of", [("lsh_rmat8", None), ("lsh_rmat10", None), ("lsh_rmat10_AAt", "T"),
+                                       ("lsh_rmat12", None)])
+def test_lespgemm_matches_reference_and_scipy(name, B_of):
+    g = gold(name)
+    scale = {"lsh_rmat8": 8, "lsh_rmat12": 12}.get(name, 10)
+    A = rmat(scale, 16, seed=scale)
+    B = transpose(A) if B_of == "T" else A
+    assert digest(B.rowptr, B.col, B.val) == str(g["B_digest"])
+    out = orc.lespgemm(A, B)
+    np.testing.assert_array_equal(out["perm"], g["perm"])
+    assert out["flop"] == int(g["flops"][0])
+    clr, clrv, csz, colids, vals = orc.cluster_build(A, out["perm"], out["offsets"])
+    assert [len(csz), len(colids), clrv[-1]] == list(g["acl_meta"])
+    nz, mn, mx = orc.cluster_symbolic(clr, colids, B)
+    np.testing.assert_array_equal(mn, g["ccl_min"])
+    np.testing.assert_array_equal(mx, g["ccl_max"])
+    np.testing.assert_array_equal(np.concatenate([[0], np.cumsum(nz)]), g["ccl_rowptr"])
+    np.testing.assert_array_equal(out["C_rowptr"], g["C_rowptr"])
+    assert digest(out["C_col"]) == str(g["C_col_digest"])
+    rows = np.repeat(np.arange(A.n_rows), np.diff(out["C_rowptr"]))
+    rs = np.bincount(rows, weights=out["C_val"], minlength=A.n_rows)
+    np.testing.assert_allclose(rs, g["C_rowsum"], rtol=1e-12, atol=0)
+    # independent product (scipy): same structure, values within 1e-12
+    P = (scipy_csr(A) @ scipy_csr(B)).tocsr()
+    P.sort_indices()
+    np.testing.assert_array_equal(out["C_rowptr"], P.indptr)
+    np.testing.assert_array_equal(out["C_col"], P.indices)
+    np.testing.assert_allclose(out["C_val"], P.data, rtol=1e-12, atol=0)
+    if "C_val" in g:
+        np.testing.assert_array_equal(out["C_col"], g["C_col"])
+        np.testing.assert_allclose(out["C_val"], g["C_val"], rtol=1e-12, atol=0)
+
+
+def test_cluster_build_closed_form():
+    """Every A entry appears exactly once at its (cluster, column, local row) slot."""
+    A = rmat(9, 8, seed=4)
+    sig = orc.minhash(A, 32, 1)
+    perm, off = orc.hc(A, *orc.lsh_pairs(sig, 32, 8), cluster_size=5)
+    clr, clrv, csz, colids, vals = orc.cluster_build(A, perm, off)
+    D = scipy_csr(A).toarray()
+    for c in range(len(csz)):
+        cols = colids[clr[c]:clr[c + 1]]
+        assert np.all(np.diff(cols) > 0)
+        block = vals[clrv[c]:clrv[c + 1]].reshape(len(cols), csz[c])
+        rows = perm[off[c]:off[c + 1]]
+        np.testing.assert_array_equal(block.T, D[np.ix_(rows, cols)])
+        assert set(cols) == set(np.nonzero(D[rows].any(axis=0))[0])
+
+
+def test_spmm_sddmm_transpose_dense():
+    A = rmat(7, 8, seed=6)
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((A.n_cols, 5))
+    G = rng.standard_normal((A.n_rows, 5))
+    D = scipy_csr(A).toarray()
+    np.testing.assert_allclose(orc.spmm(A, X), D @ X, rtol=1e-13, atol=1e-13)
+    rows = np.repeat(np.arange(A.n_rows), np.diff(A.rowptr))
+    np.testing.assert_allclose(orc.sddmm(A, G, X), np.einsum("ij,ij->i", G[rows], X[A.col]), rtol=1e-13, atol=1e-13)
+    trp, tc, tv, tp = orc.transpose(A)
+    T = sp.csr_matrix((tv, tc, trp), shape=(A.n_cols, A.n_rows)).toarray()
+    np.testing.assert_array_equal(T, D.T)
+    np.testing.assert_array_equal(tv, A.val[tp])

@@ -1,0 +1,174 @@
+"""SpMV parity: HIP library (through the C ABI) vs the CPU oracle and the reference layouts.
+
+Tolerances (relative to the row condition scale s_r = sum_j |a_rj x_j|, computed in
+extended precision by the oracle):
+  FP64 values, FP64 accumulation:            |y - y*| <= 1e-13 * s_r
+  FP32 values, FP32 (compensated) sums:      |y - y*(rounded inputs)| <= 4e-7 * s_r
+  BF16 values, FP32 (compensated) sums:      |y - y*(rounded inputs)| <= 4e-7 * s_r
+where y*(rounded inputs) is the exact product of the values and x as actually stored.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle_bind as orc
+from sparseops_amd import spmv as S
+from sparseops_amd.gen import Csr, laplacian_2d, powerlaw, uniform_x
+
+pytestmark = pytest.mark.gpu
+
+ALPHA, BETA = 0.8, 0.7
+
+
+def _y0(n):
+    return 0.25 * ((np.arange(n) * 7919) % 13) - 1.0
+
+
+def _bf16_round(v):
+    """Round-to-nearest-even to bf16 (8 significant bits), computed from the float64 value."""
+    m, e = np.frexp(np.asarray(v, dtype=np.float64))
+    return np.ldexp(np.rint(np.ldexp(m, 8)), e - 8)
+
+
+def _run(M, x, y0, dev, alpha=ALPHA, beta=BETA):
+    dt = M.xy_dtype
+    xt = torch.tensor(x, dtype=dt, device=dev)
+    yt = torch.tensor(y0, dtype=dt, device=dev)
+    M.spmv(xt, yt, alpha, beta)
+    torch.cuda.synchronize()
+    return yt.cpu().double().numpy()
+
+
+def _check(A, M, x, y0, vt, alpha=ALPHA, beta=BETA, dev=None):
+    y = _run(M, x, y0, dev, alpha, beta)
+    if vt == "f64":
+        ref, s = orc.spmv_exact(A, x, alpha, beta, y0)
+        tol = 1e-13
+    else:
+        vals = A.val.astype(np.float32).astype(np.float64) if vt == "f32" else _bf16_round(A.val)
+        xr = x.astype(np.float32).astype(np.float64)
+        y0r = y0.astype(np.float32).astype(np.float64)
+        ref, s = orc.spmv_exact(A, xr, alpha, beta, y0r, values=vals)
+        tol = 4e-7
+    scale = np.abs(alpha) * s + np.abs(beta) * np.abs(y0) + 1e-300
+    err = np.abs(y - ref) / scale
+    assert np.all(err <= tol), f"max conditioned error {err.max():.3e} > {tol}"
+    return y
+
+
+def _edge_matrix(seed=5):
+    """Rows of every kernel path: empty, short, stream-block, medium (>512), long (>2048)."""
+    rng = np.random.default_rng(seed)
+    n = 6000
+    lens = rng.integers(0, 40, n)
+    lens[::97] = 0
+    lens[10] = 700
+    lens[11] = 3000
+    lens[3000] = 513
+    lens[4000] = 2049
+    lens[5999] = 5000
+    rows, cols = [], []
+    for r, L in enumerate(lens):
+        c = np.sort(rng.choice(n, size=L, replace=False))
+        rows.append(np.full(L, r))
+        cols.append(c)
+    rows = np.concatenate(rows)
+    cols = np.concatenate(cols)
+    rp = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=rp[1:])
+    return Csr(n, n, rp.astype(np.int32), cols.astype(np.int32), rng.standard_normal(rows.size))
+
+
+@pytest.fixture(scope="module")
+def mats():
+    return {
+        "lap": laplacian_2d(64),
+        "pl": powerlaw(20000, lmax=3000, seed=11),
+        "edge": _edge_matrix(),
+    }
+
+
+def test_laplacian_closed_form(dev):
+    A = laplacian_2d(256)
+    M = S.csr(A, "f64")
+    y = _run(M, np.ones(A.n_cols), np.zeros(A.n_rows), dev, 1.0, 0.0)
+    deg = np.diff(A.rowptr) - 1            # neighbours per row
+    np.testing.assert_array_equal(y, 4.0 - deg)   # A*1 is exact in FP64
+
+
+@pytest.mark.parametrize("vt", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("name", ["lap", "pl", "edge"])
+def test_csr(mats, name, vt, dev):
+    A = mats[name]
+    x = uniform_x(A.n_cols, 1)
+    _check(A, S.csr(A, vt), x, _y0(A.n_rows), vt, dev=dev)
+
+
+@pytest.mark.parametrize("vt", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("name", ["lap", "pl", "edge"])
+@pytest.mark.parametrize("sigma,C", [(1024, 32), (64, 8), (16, 4), (256, 64), (48, 12)])
+def test_sell(mats, name, vt, sigma, C, dev):
+    A = mats[name]
+    x = uniform_x(A.n_cols, 2)
+    M = S.csr_to_sell_c_sigma(A, sigma, C, vt)
+    _check(A, M, x, _y0(A.n_rows), vt, dev=dev)
+    # layout parity with the oracle (pinned to the reference's csr_to_sell_c_sigma)
+    align = 4 if vt == "f64" else 8
+    reorder, cl, _, _, _ = orc.sell_layout(A, sigma, C, align)
+    np.testing.assert_array_equal(M.layout(0), reorder)
+    np.testing.assert_array_equal(M.layout(1), cl)
+
+
+@pytest.mark.parametrize("vt", ["f64", "f32"])
+def test_sell_c_R(mats, vt, dev):
+    A = mats["pl"]
+    M = S.csr_to_sell_c_R(A, 32, vt)
+    _check(A, M, uniform_x(A.n_cols, 3), _y0(A.n_rows), vt, dev=dev)
+    np.testing.assert_array_equal(M.layout(0), orc.sell_layout(A, M.info[11], 32, 4 if vt == "f64" else 8)[0])
+
+
+@pytest.mark.parametrize("vt", ["f64", "f32"])
+def test_sell_from_reference_chunks(mats, vt, dev):
+    """The drop-in path: device SELL built from the reference's own host layout."""
+    A = mats["edge"]
+    C, sigma = 8, 64
+    reorder, cl, co, scol, sval = orc.sell_layout(A, sigma, C, 4)
+    ccols = [scol[co[c]:co[c] + cl[c] * C].astype(np.int32) for c in range(len(cl))]
+    cvals = [sval[co[c]:co[c] + cl[c] * C] for c in range(len(cl))]
+    if vt == "f32":
+        cvals = [v.astype(np.float32) for v in cvals]
+    M = S.sell_from_chunks(A.n_rows, A.n_cols, A.nnz, C, reorder.astype(np.int32), cl.astype(np.int32), ccols, cvals, vt)
+    _check(A, M, uniform_x(A.n_cols, 4), _y0(A.n_rows), vt, dev=dev)
+
+
+@pytest.mark.parametrize("vt", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("name", ["lap", "edge"])
+def test_ell(mats, name, vt, dev):
+    A = mats[name]
+    _check(A, S.csr_to_ell(A, vt), uniform_x(A.n_cols, 5), _y0(A.n_rows), vt, dev=dev)
+
+
+@pytest.mark.parametrize("vt", ["f64", "f32"])
+@pytest.mark.parametrize("bR,bC", [(16, 4), (16, 8), (4, 4), (1, 1), (8, 8), (3, 5)])
+def test_bsr(mats, vt, bR, bC, dev):
+    A = mats["edge"]
+    _check(A, S.csr_to_bsr(A, bR, bC, vt), uniform_x(A.n_cols, 6), _y0(A.n_rows), vt, dev=dev)
+
+
+def test_beta_zero_does_not_read_y(mats, dev):
+    A = mats["pl"]
+    M = S.csr(A, "f64")
+    x = uniform_x(A.n_cols, 7)
+    y0 = np.full(A.n_rows, np.nan)
+    y = _run(M, x, y0, dev, 1.0, 0.0)
+    ref, _ = orc.spmv_exact(A, x)
+    assert np.all(np.isfinite(y))
+    np.testing.assert_allclose(y, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def test_empty_and_tiny(dev):
+    for n in (1, 2, 63, 65):
+        A = Csr(n, n, np.zeros(n + 1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+        for M in (S.csr(A), S.csr_to_sell_c_sigma(A, 32, 32), S.csr_to_ell(A), S.csr_to_bsr(A, 4, 4)):
+            y = _run(M, np.ones(n), np.ones(n), dev, 1.0, 0.5)
+            np.testing.assert_array_equal(y, 0.5)
