@@ -96,6 +96,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variants", default="sell_f32,csr_f64,sell_f64,sell_bf16,csr_f32",
+                    help="comma list of <fmt>_<value type>; the first one is the headline")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="only run a device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -109,6 +113,14 @@ def main():
     from sparseops_amd.gen import powerlaw, uniform_x
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration_mb:
+        a = torch.ones(args.copy_calibration_mb << 18, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "launches": 3}))
+        return
     t = time.time()
     A = powerlaw(args.n, seed=0)
     x64 = uniform_x(A.n_cols, 0)
@@ -118,8 +130,8 @@ def main():
     absrow = abs(As) @ np.abs(x64)
 
     stream = torch.cuda.current_stream().cuda_stream
-    variants = [("sell_f32", "sell", "f32"), ("csr_f64", "csr", "f64"), ("sell_f64", "sell", "f64"),
-                ("sell_bf16", "sell", "bf16"), ("csr_f32", "csr", "f32")]
+    variants = [(v, v.split("_")[0], v.split("_")[1]) for v in args.variants.split(",")]
+    head_name = variants[0][0]
     res = {}
     for name, fmt, vt in variants:
         t = time.time()
@@ -139,13 +151,13 @@ def main():
         M.close()
         del x, y
 
-    head = res["sell_f32"]
+    head = res[head_name]
     cpu = None
     if not args.no_cpu:
         t = time.time()
         c = cpu_baseline(A, args.cpu_seconds, args.cpu_threads)
         if c:
-            cpu = {"value": res["sell_f32"]["algo_bytes"] / c["seconds_per_iter"] / 1e9, "unit": "GB/s",
+            cpu = {"value": head["algo_bytes"] / c["seconds_per_iter"] / 1e9, "unit": "GB/s",
                    "cores": c["threads"], "kind": "reference",
                    "sample": f"the full matrix, {c['iterations']} SpMV iterations of the reference's OpenMP "
                              f"SELL-C-sigma kernel (kernel_flag 2, FP32, C=32, sigma=1024), "
@@ -176,7 +188,8 @@ def main():
                             for k, v in res.items()},
         "variants": {k: {"gbs": round(v["gbs"], 1), "kernel_us": round(v["kernel_ms"] * 1e3, 2),
                          "pct_hbm_peak": round(100 * v["kernel_gbs"] / HBM_PEAK_GBS, 2)} for k, v in res.items()},
-        "fp32_sell_over_fp64_csr": round(res["csr_f64"]["kernel_ms"] / head["kernel_ms"], 3),
+        "fp32_sell_over_fp64_csr": (round(res["csr_f64"]["kernel_ms"] / res["sell_f32"]["kernel_ms"], 3)
+                                    if "csr_f64" in res and "sell_f32" in res else None),
     }
     print(json.dumps(out), flush=True)
 

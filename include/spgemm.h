@@ -1,0 +1,127 @@
+/* Cluster-wise SpGEMM of the LeSpGEMM path (drop-in for the reference's include/spgemm.h):
+ *   get_spgemm_flop   src/spgemm.cpp:24-42
+ *   LeSpGEMM_VLength  src/spgemm.cpp:662-680 (template <bool sortOutput, I, V>)
+ * C = A_cluster * B on the GPU (spops_lespgemm, the hybrid LDS-hash / HBM-bitmap
+ * accumulator).  kernel_flag 1, 2 and 3 all produce the same C here (they differ only in
+ * the reference's CPU accumulator); for kernel 3 the caller-allocated acc_flag, min_ccol and
+ * max_ccol arrays are filled as the reference does (src/spgemm.cpp:598-622, the
+ * classification using the reference's L2 budget so dense_cluster_count is comparable) and
+ * the phase timings land in mixed_acc_*_ms.  C's columns ascend inside each cluster (the
+ * reference's sortOutput=1 order; with sortOutput=0 it leaves hash-slot order). */
+#ifndef SPOPS_DROPIN_SPGEMM_H
+#define SPOPS_DROPIN_SPGEMM_H
+
+#include <stdexcept>
+#include <vector>
+
+#include "sparse_format.h"
+#include "spgemm_cluster.h"
+#include "spgemm_MinHashLSH.h"
+#include "spgemm_utility.h"
+#include "spops_dropin.h"
+
+#ifndef MIXED_DENSITY_THRESHOLD
+#define MIXED_DENSITY_THRESHOLD 0.15
+#endif
+#ifndef MIXED_L2_FRACTION
+#define MIXED_L2_FRACTION 0.75
+#endif
+#ifndef CPU_L2CACHE_SIZE
+#define CPU_L2CACHE_SIZE 67108864   /* the reference's plat_config.h value */
+#endif
+
+template <typename IndexType, typename ValueType>
+long long int get_spgemm_flop(const CSR_Matrix<IndexType, ValueType> &A, const CSR_Matrix<IndexType, ValueType> &B) {
+  long long int flops = 0;
+  for (IndexType i = 0; i < A.num_rows; i++)
+    for (IndexType j = A.row_offset[i]; j < A.row_offset[i + 1]; j++) {
+      const IndexType a = A.col_index[j];
+      if (a < B.num_rows) flops += B.row_offset[a + 1] - B.row_offset[a];
+    }
+  return flops * 2;
+}
+
+template <bool sortOutput, typename IndexType, typename ValueType>
+void LeSpGEMM_VLength(const CSR_VlengthCluster<IndexType, ValueType> &A, const CSR_Matrix<IndexType, ValueType> &B,
+                      CSR_VlengthCluster<IndexType, ValueType> &C, int kernel_flag = 1,
+                      double mixed_l2_fraction = -1.0) {
+  using namespace spops_dropin;
+  typedef IndexType I;
+  typedef ValueType V;
+  if (A.cols != B.num_rows) throw std::invalid_argument("LeSpGEMM_VLength: A.cols != B.num_rows");
+  std::lock_guard<std::mutex> g(state().mu);
+  const int64_t nc = A.rows;
+  std::vector<int64_t> csz(A.cluster_sz, A.cluster_sz + nc), rp(A.rowptr, A.rowptr + nc + 1),
+      rpv(A.rowptr_val, A.rowptr_val + nc + 1), col(A.colids, A.colids + A.nnzc);
+  spops_cluster hA = nullptr, hC = nullptr;
+  check(spops_cluster_create_from_host(&hA, A.csr_rows, A.cols, nc, csz.data(), rp.data(), rpv.data(), col.data(),
+                                       A.values, value_code<V>()),
+        "spops_cluster_create_from_host");
+  int64_t stats[16] = {0};
+  int st;
+  {
+    const int64_t *drp = upload_index64(0, B.row_offset, (int64_t)B.num_rows + 1);
+    const int64_t *dcol = upload_index64(1, B.col_index, (int64_t)B.num_nnzs);
+    void *dval = state().scratch[2].get((size_t)(B.num_nnzs ? B.num_nnzs : 1) * sizeof(V));
+    check(spops_memcpy_htod(dval, B.values, (size_t)B.num_nnzs * sizeof(V), nullptr), "upload B values");
+    st = spops_lespgemm(hA, B.num_rows, B.num_cols, B.num_nnzs, drp, dcol, dval, &hC, stats, nullptr);
+  }
+  spops_cluster_destroy(hA);
+  check(st, "spops_lespgemm");
+  int64_t info[8];
+  spops_cluster_info(hC, info);
+  C.csr_rows = A.csr_rows;
+  C.rows = A.rows;
+  C.cols = B.num_cols;
+  C.nnzc = (I)info[3];
+  C.nnzv = (I)info[4];
+  C.max_cluster_sz = A.max_cluster_sz;
+  std::vector<int64_t> ccsz(nc), crp(nc + 1), crpv(nc + 1), ccol(C.nnzc);
+  C.values = new_array<V>(C.nnzv);
+  st = spops_cluster_download(hC, ccsz.data(), crp.data(), crpv.data(), ccol.data(), C.values);
+  std::vector<int64_t> nz(nc), mn(nc), mx(nc);
+  if (st == SPOPS_SUCCESS) st = spops_cluster_symbolic_download(hC, nz.data(), mn.data(), mx.data());
+  spops_cluster_destroy(hC);
+  check(st, "download C");
+  C.cluster_sz = new_array<I>(nc);
+  C.rowptr = new_array<I>(nc + 1);
+  C.rowptr_val = new_array<I>(nc + 1);
+  C.colids = new_array<I>(C.nnzc);
+  std::copy(ccsz.begin(), ccsz.end(), C.cluster_sz);
+  std::copy(crp.begin(), crp.end(), C.rowptr);
+  std::copy(crpv.begin(), crpv.end(), C.rowptr_val);
+  std::copy(ccol.begin(), ccol.end(), C.colids);
+  C.num_rows = C.csr_rows;
+  C.num_cols = C.cols;
+  C.mixed_acc_setup_ms = C.mixed_acc_malloc_ms = C.mixed_acc_symbolic_ms = 0.0;
+  C.mixed_acc_dispatcher_ms = C.mixed_acc_numeric_ms = 0.0;
+  C.dense_cluster_count = 0;
+  if (kernel_flag == 3) {
+    C.mixed_acc_setup_ms = stats[5] / 1e3;
+    C.mixed_acc_symbolic_ms = stats[6] / 1e3;
+    C.mixed_acc_malloc_ms = stats[7] / 1e3;
+    C.mixed_acc_numeric_ms = stats[8] / 1e3;
+    // classify_clusters (src/spgemm_Vlength_mixed.cpp:268-319): dense iff the B rows plus
+    // the dense block fit the L2 budget and nz / col_range >= 0.15
+    const double l2f = (mixed_l2_fraction >= 0.0 && mixed_l2_fraction <= 2.0) ? mixed_l2_fraction : MIXED_L2_FRACTION;
+    const size_t budget = (size_t)((double)CPU_L2CACHE_SIZE * l2f);
+    const double b_rowdense = B.num_rows ? (double)B.num_nnzs / (double)B.num_rows : 0.0;
+    for (int64_t c = 0; c < nc; ++c) {
+      if (C.min_ccol) C.min_ccol[c] = (I)mn[c];
+      if (C.max_ccol) C.max_ccol[c] = (I)mx[c];
+      char dense = 0;
+      if (nz[c] > 0) {
+        const int64_t range = mx[c] - mn[c] + 1;
+        const double density = (double)nz[c] / (double)range;
+        const size_t bbytes = (size_t)((double)(rp[c + 1] - rp[c]) * b_rowdense * (double)(sizeof(I) + sizeof(V)));
+        const size_t dbytes = (size_t)range * (size_t)csz[c] * sizeof(V);
+        dense = (bbytes + dbytes <= budget && density >= MIXED_DENSITY_THRESHOLD) ? 1 : 0;
+      }
+      if (C.acc_flag) C.acc_flag[c] = dense;
+      C.dense_cluster_count += dense;
+    }
+  }
+  (void)sortOutput;
+}
+
+#endif

@@ -116,6 +116,13 @@ int spops_bsr_create(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t nn
                      int index_type, const void *rowptr, const void *col,
                      const void *values, int in_value_type, int value_type, int64_t bR, int64_t bC);
 
+/* BSR from the reference's own block layout (row_ptr[mb+1], block_colindex[nnzb], row-major
+ * blocks block_data[nnzb*bR*bC]); backs SpOps_bsr on a BSR_Matrix built by the caller. */
+int spops_bsr_create_from_blocks(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t bR, int64_t bC,
+                                 int64_t mb, int64_t nnzb, int index_type, const void *row_ptr,
+                                 const void *block_colindex, const void *block_data, int in_value_type,
+                                 int value_type);
+
 /* y = alpha*A*x + beta*y on the device; beta == 0 does not read y (BLAS convention;
  * the reference's OMP kernels read y there, src/spmv_csr.cpp:119).  x/y element type
  * follows the value type (see top).  Asynchronous on `stream`. */
@@ -186,6 +193,11 @@ typedef struct spops_cluster_s *spops_cluster;
 int spops_cluster_create(spops_cluster *out, int64_t n_rows, int64_t n_cols, int64_t nnz, const int64_t *d_rowptr,
                          const int64_t *d_col, const void *d_values, int value_type, const int64_t *perm,
                          int64_t nclusters, const int64_t *offsets, void *stream);
+/* Device cluster matrix from a host CSR_VlengthCluster (the drop-in path of
+ * LeSpGEMM_VLength, src/spgemm.cpp:662): arrays as in the struct, any column order. */
+int spops_cluster_create_from_host(spops_cluster *out, int64_t csr_rows, int64_t cols, int64_t nclusters,
+                                   const int64_t *cluster_sz, const int64_t *rowptr, const int64_t *rowptr_val,
+                                   const int64_t *colids, const void *values, int value_type);
 /* info[0]=clusters info[1]=csr rows info[2]=cols info[3]=nnzc info[4]=nnzv
  * info[5]=max cluster size info[6]=value type info[7]=device bytes */
 int spops_cluster_info(spops_cluster A, int64_t info[8]);

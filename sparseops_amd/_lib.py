@@ -50,6 +50,8 @@ _SIGNATURES = [
      [c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_int, c_int]),
     ("spops_ell_create", c_int, [c_vp, c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_int, c_int]),
     ("spops_bsr_create", c_int, [c_vp, c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_int, c_int, c_i64, c_i64]),
+    ("spops_bsr_create_from_blocks", c_int,
+     [c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_int, c_int]),
     ("spops_spmv", c_int, [c_vp, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
     ("spops_spmm", c_int, [c_vp, c_i64, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
     ("spops_matrix_info", c_int, [c_vp, c_vp]),
@@ -66,6 +68,7 @@ _SIGNATURES = [
     ("spops_hierarchical_clustering", c_int,
      [c_int, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_int, c_i64, c_vp, c_vp, c_vp]),
     ("spops_cluster_create", c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_int, c_vp, c_i64, c_vp, c_vp]),
+    ("spops_cluster_create_from_host", c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_int]),
     ("spops_cluster_info", c_int, [c_vp, c_vp]),
     ("spops_cluster_download", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("spops_cluster_symbolic_download", c_int, [c_vp, c_vp, c_vp, c_vp]),

@@ -129,6 +129,41 @@ inline size_t xy_bytes(int vt) { return vt == SPOPS_VALUE_F64 ? 8 : 4; }
 // ---- device helpers -------------------------------------------------------------------------------
 __device__ __forceinline__ float bf16_to_float(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// Raw buffer loads: an offset past the resource's size returns 0 and touches no memory, so
+// masked lanes cost no traffic and the loads need no branches (all loads of an unrolled
+// loop issue back to back).  Offsets are bytes, 32-bit; resources are kept below 2 GiB.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+constexpr uint32_t kOOB = 0x7FFFFFF0u;           // an offset no resource reaches
+constexpr uint64_t kMaxBufBytes = 0x7FFFFFF0ull;
+constexpr int kAuxNT = 2;                        // non-temporal (streamed once)
+typedef unsigned bu2 __attribute__((ext_vector_type(2)));
+typedef unsigned bu4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, uint64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0,
+                                           (int)(bytes < kMaxBufBytes ? bytes : kMaxBufBytes), 0x00020000);
+}
+
+template <typename T, int aux = 0>
+__device__ __forceinline__ T bload(rsrc_t r, uint32_t off) {
+  static_assert(sizeof(T) == 2 || sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "load size");
+  T out;
+  if constexpr (sizeof(T) == 2) {
+    unsigned short v = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, aux);
+    __builtin_memcpy(&out, &v, 2);
+  } else if constexpr (sizeof(T) == 4) {
+    unsigned v = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, aux);
+    __builtin_memcpy(&out, &v, 4);
+  } else if constexpr (sizeof(T) == 8) {
+    bu2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, aux);
+    __builtin_memcpy(&out, &v, 8);
+  } else {
+    bu4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, aux);
+    __builtin_memcpy(&out, &v, 16);
+  }
+  return out;
+}
+
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 __device__ __forceinline__ int wave_id() { return (int)(threadIdx.x >> 6); }
 

@@ -924,3 +924,49 @@ extern "C" int spops_spgemm_flop(int64_t a_nnz, const int64_t *d_acol, int64_t b
     return SPOPS_SUCCESS;
   });
 }
+
+extern "C" int spops_cluster_create_from_host(spops_cluster *out, int64_t csr_rows, int64_t cols, int64_t nclusters,
+                                              const int64_t *cluster_sz, const int64_t *rowptr,
+                                              const int64_t *rowptr_val, const int64_t *colids, const void *values,
+                                              int value_type) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(out && cluster_sz && rowptr && rowptr_val, SPOPS_ERROR_INVALID_VALUE, "null argument");
+    SPOPS_REQUIRE(value_type == SPOPS_VALUE_F64 || value_type == SPOPS_VALUE_F32, SPOPS_ERROR_INVALID_VALUE,
+                  "cluster values must be F64 or F32");
+    SPOPS_REQUIRE(csr_rows < ((int64_t)1 << 31) && cols < ((int64_t)1 << 31) && nclusters <= csr_rows,
+                  SPOPS_ERROR_LIMIT, "rows and columns must be < 2^31");
+    const int64_t nnzc = rowptr[nclusters], nnzv = rowptr_val[nclusters];
+    std::vector<int64_t> offsets(nclusters + 1, 0);
+    for (int64_t c = 0; c < nclusters; ++c) {
+      SPOPS_REQUIRE(cluster_sz[c] > 0 && rowptr[c + 1] >= rowptr[c] &&
+                        rowptr_val[c + 1] - rowptr_val[c] == (rowptr[c + 1] - rowptr[c]) * cluster_sz[c],
+                    SPOPS_ERROR_INVALID_VALUE, "inconsistent cluster %lld", (long long)c);
+      offsets[c + 1] = offsets[c] + cluster_sz[c];
+    }
+    SPOPS_REQUIRE(offsets[nclusters] == csr_rows, SPOPS_ERROR_INVALID_VALUE, "cluster sizes must sum to csr_rows");
+    for (int64_t t = 0; t < nnzc; ++t)
+      SPOPS_REQUIRE(colids[t] >= 0 && colids[t] < cols, SPOPS_ERROR_INVALID_VALUE, "column out of range");
+    auto A = std::make_unique<spops_cluster_s>();
+    A->csr_rows = csr_rows; A->rows = nclusters; A->cols = cols; A->nnzc = nnzc; A->nnzv = nnzv;
+    A->value_type = value_type;
+    std::vector<int32_t> csz(cluster_sz, cluster_sz + nclusters), cid(csr_rows), lrow(csr_rows);
+    A->max_cluster_sz = 0;
+    for (int64_t c = 0; c < nclusters; ++c) {
+      A->max_cluster_sz = std::max<int64_t>(A->max_cluster_sz, csz[c]);
+      for (int64_t t = offsets[c]; t < offsets[c + 1]; ++t) { cid[t] = (int32_t)c; lrow[t] = (int32_t)(t - offsets[c]); }
+    }
+    std::vector<int64_t> perm(csr_rows);
+    std::iota(perm.begin(), perm.end(), 0);
+    A->cluster_sz.upload(csz);
+    A->rowptr.upload(std::vector<int64_t>(rowptr, rowptr + nclusters + 1));
+    A->rowptr_val.upload(std::vector<int64_t>(rowptr_val, rowptr_val + nclusters + 1));
+    A->colids.upload(std::vector<int32_t>(colids, colids + nnzc));
+    A->values.upload_raw(values, (size_t)nnzv * value_bytes(value_type));
+    A->perm.upload(perm);
+    A->offsets.upload(offsets);
+    A->cid.upload(cid);
+    A->lrow.upload(lrow);
+    *out = A.release();
+    return SPOPS_SUCCESS;
+  });
+}

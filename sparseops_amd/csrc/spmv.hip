@@ -57,9 +57,6 @@ template <> struct VT_<SPOPS_VALUE_BF16> {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-template <typename T>
-__device__ __forceinline__ T ld_stream(const T *p) { return __builtin_nontemporal_load(p); }
-
 template <typename X, typename Acc>
 __device__ __forceinline__ void store_y(X *y, long row, Acc alpha, Acc beta, Acc sum) {
   Acc r = alpha * sum;
@@ -67,45 +64,62 @@ __device__ __forceinline__ void store_y(X *y, long row, Acc alpha, Acc beta, Acc
   y[row] = (X)r;
 }
 
-// Sum of entries [p0, p1) of a row, lanes of one wave striding by 64 (unrolled by 4).
-template <int VT, typename P>
-__device__ __forceinline__ void row_sum_wave(const int *__restrict__ col, const typename VT_<VT>::V *__restrict__ val,
-                                             const typename VT_<VT>::X *__restrict__ x, P p0, P p1,
+// Sum over entries [0, len) of one row whose col/val start at the given pointers: lanes of
+// one wave stride by 64, four entries per lane in flight, every load a buffer load (masked
+// lanes fetch nothing), gathers of x through the x resource.
+template <int VT>
+__device__ __forceinline__ void row_sum_wave(const int *col, const typename VT_<VT>::V *val, int64_t len, rsrc_t rx,
                                              Accum<typename VT_<VT>::Acc, VT_<VT>::kComp> &a) {
   using T = VT_<VT>;
+  constexpr uint32_t sV = sizeof(typename T::V), sX = sizeof(typename T::X);
   const int lane = lane_id();
-  for (P p = p0 + lane; p < p1; p += 4 * kWave) {
-    int c[4];
-    typename T::V v[4];
+  for (int64_t seg = 0; seg < len; seg += kRowSegment) {
+    const int n = (int)((len - seg) < kRowSegment ? (len - seg) : kRowSegment);
+    const rsrc_t rc = make_rsrc(col + seg, (uint64_t)n * 4), rv = make_rsrc(val + seg, (uint64_t)n * sV);
+    for (int p = lane; p < n; p += 4 * kWave) {
+      int c[4];
+      typename T::V v[4];
+      typename T::X xv[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      P e = p + u * kWave;
-      if (e < p1) { c[u] = ld_stream(col + e); v[u] = ld_stream(val + e); }
+      for (int u = 0; u < 4; ++u) {
+        const int e = p + u * kWave;
+        c[u] = bload<int, kAuxNT>(rc, e < n ? (uint32_t)e * 4u : kOOB);
+        v[u] = bload<typename T::V, kAuxNT>(rv, e < n ? (uint32_t)e * sV : kOOB);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xv[u] = bload<typename T::X>(rx, (p + u * kWave < n) ? (uint32_t)c[u] * sX : kOOB);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a.fma(T::cvt(v[u]), (typename T::Acc)xv[u]);
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (p + u * kWave < p1) a.fma(T::cvt(v[u]), (typename T::Acc)x[c[u]]);
   }
 }
 
-// Whole-workgroup sum of entries [p0, p1); result valid in thread 0.
-template <int VT, typename P>
-__device__ __forceinline__ typename VT_<VT>::Acc row_sum_block(const int *__restrict__ col, const typename VT_<VT>::V *__restrict__ val,
-                                                              const typename VT_<VT>::X *__restrict__ x, P p0, P p1,
-                                                              typename VT_<VT>::Acc *scratch /* 2*4 entries of LDS */) {
+// Whole-workgroup sum of one long row; result valid in thread 0.
+template <int VT>
+__device__ __forceinline__ typename VT_<VT>::Acc row_sum_block(const int *col, const typename VT_<VT>::V *val,
+                                                              int64_t len, rsrc_t rx,
+                                                              typename VT_<VT>::Acc *scratch /* 8 entries of LDS */) {
   using T = VT_<VT>;
+  constexpr uint32_t sV = sizeof(typename T::V), sX = sizeof(typename T::X);
   Accum<typename T::Acc, T::kComp> a;
-  for (P p = p0 + (P)threadIdx.x; p < p1; p += 4 * kBlock) {
-    int c[4];
-    typename T::V v[4];
+  for (int64_t seg = 0; seg < len; seg += kRowSegment) {
+    const int n = (int)((len - seg) < kRowSegment ? (len - seg) : kRowSegment);
+    const rsrc_t rc = make_rsrc(col + seg, (uint64_t)n * 4), rv = make_rsrc(val + seg, (uint64_t)n * sV);
+    for (int p = (int)threadIdx.x; p < n; p += 4 * kBlock) {
+      int c[4];
+      typename T::V v[4];
+      typename T::X xv[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      P e = p + u * kBlock;
-      if (e < p1) { c[u] = ld_stream(col + e); v[u] = ld_stream(val + e); }
+      for (int u = 0; u < 4; ++u) {
+        const int e = p + u * kBlock;
+        c[u] = bload<int, kAuxNT>(rc, e < n ? (uint32_t)e * 4u : kOOB);
+        v[u] = bload<typename T::V, kAuxNT>(rv, e < n ? (uint32_t)e * sV : kOOB);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xv[u] = bload<typename T::X>(rx, (p + u * kBlock < n) ? (uint32_t)c[u] * sX : kOOB);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a.fma(T::cvt(v[u]), (typename T::Acc)xv[u]);
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (p + u * kBlock < p1) a.fma(T::cvt(v[u]), (typename T::Acc)x[c[u]]);
   }
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) a.merge_xor(m);
@@ -133,18 +147,21 @@ __device__ __forceinline__ typename VT_<VT>::Acc row_sum_block(const int *__rest
 template <int VT, typename RP>
 __global__ __launch_bounds__(kBlock) void k_csr_spmv(const RP *__restrict__ rowptr, const int *__restrict__ col,
                                                      const typename VT_<VT>::V *__restrict__ val,
-                                                     const typename VT_<VT>::X *__restrict__ x,
+                                                     const typename VT_<VT>::X *__restrict__ x, int n_cols,
                                                      typename VT_<VT>::X *__restrict__ y,
                                                      typename VT_<VT>::Acc alpha, typename VT_<VT>::Acc beta,
                                                      const int2 *__restrict__ tasks, int ntasks,
                                                      const int *__restrict__ long_rows, int nlong) {
   using T = VT_<VT>;
   using Acc = typename T::Acc;
+  constexpr uint32_t sV = sizeof(typename T::V), sX = sizeof(typename T::X);
   __shared__ Acc lds[kBlock / kWave][kCsrStreamNnz];
+  const rsrc_t rx = make_rsrc(x, (uint64_t)n_cols * sX);
 
   if ((int)blockIdx.x < nlong) {
     const int row = long_rows[blockIdx.x];
-    Acc r = row_sum_block<VT, RP>(col, val, x, rowptr[row], rowptr[row + 1], &lds[0][0]);
+    const RP p0 = rowptr[row], p1 = rowptr[row + 1];
+    Acc r = row_sum_block<VT>(col + p0, val + p0, (int64_t)(p1 - p0), rx, &lds[0][0]);
     if (threadIdx.x == 0) store_y(y, row, alpha, beta, r);
     return;
   }
@@ -157,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmv(const RP *__restrict__ rowp
 
   if (r1 - r0 == 1 && p1 - p0 > kCsrStreamNnz) {        // one medium row: wave-per-row
     Accum<Acc, T::kComp> a;
-    row_sum_wave<VT, RP>(col, val, x, p0, p1, a);
+    row_sum_wave<VT>(col + p0, val + p0, (int64_t)(p1 - p0), rx, a);
 #pragma unroll
     for (int m = 32; m > 0; m >>= 1) a.merge_xor(m);
     if (lane == 0) store_y(y, r0, alpha, beta, a.value());
@@ -168,18 +185,21 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmv(const RP *__restrict__ rowp
   // each row with 64 / 2^ceil(log2 R) lanes (R = rows in the block).
   Acc *sh = lds[wave_id()];
   const int cnt = (int)(p1 - p0);
-  int c[kCsrStreamNnz / kWave];
-  typename T::V v[kCsrStreamNnz / kWave];
+  const rsrc_t rc = make_rsrc(col + p0, (uint64_t)cnt * 4), rv = make_rsrc(val + p0, (uint64_t)cnt * sV);
+  constexpr int K = kCsrStreamNnz / kWave;
+  int c[K];
+  typename T::V v[K];
+  typename T::X xv[K];
 #pragma unroll
-  for (int i = 0; i < kCsrStreamNnz / kWave; ++i) {
-    int e = lane + i * kWave;
-    if (e < cnt) { c[i] = ld_stream(col + p0 + e); v[i] = ld_stream(val + p0 + e); }
+  for (int i = 0; i < K; ++i) {
+    const int e = lane + i * kWave;
+    c[i] = bload<int, kAuxNT>(rc, e < cnt ? (uint32_t)e * 4u : kOOB);
+    v[i] = bload<typename T::V, kAuxNT>(rv, e < cnt ? (uint32_t)e * sV : kOOB);
   }
 #pragma unroll
-  for (int i = 0; i < kCsrStreamNnz / kWave; ++i) {
-    int e = lane + i * kWave;
-    if (e < cnt) sh[e] = T::cvt(v[i]) * (Acc)x[c[i]];
-  }
+  for (int i = 0; i < K; ++i) xv[i] = bload<typename T::X>(rx, (lane + i * kWave < cnt) ? (uint32_t)c[i] * sX : kOOB);
+#pragma unroll
+  for (int i = 0; i < K; ++i) sh[lane + i * kWave] = T::cvt(v[i]) * (Acc)xv[i];
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -208,15 +228,18 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmv(const v4i *__restrict__ gc
                                                       const long *__restrict__ optr, const int *__restrict__ ocol,
                                                       const typename VT_<VT>::V *__restrict__ oval,
                                                       const int *__restrict__ orow, int n_olong, int n_ovec,
-                                                      const typename VT_<VT>::X *__restrict__ x,
+                                                      const typename VT_<VT>::X *__restrict__ x, int n_cols,
                                                       typename VT_<VT>::X *__restrict__ y,
                                                       typename VT_<VT>::Acc alpha, typename VT_<VT>::Acc beta) {
   using T = VT_<VT>;
   using Acc = typename T::Acc;
+  constexpr uint32_t sX = sizeof(typename T::X), sG = sizeof(typename T::G);
   __shared__ Acc lds[8];
+  const rsrc_t rx = make_rsrc(x, (uint64_t)n_cols * sX);
   if ((int)blockIdx.x < n_olong) {
     const int o = blockIdx.x;
-    Acc r = row_sum_block<VT, long>(ocol, oval, x, optr[o], optr[o + 1], lds);
+    const long p0 = optr[o];
+    Acc r = row_sum_block<VT>(ocol + p0, oval + p0, optr[o + 1] - p0, rx, lds);
     if (threadIdx.x == 0) store_y(y, orow[o], alpha, beta, r);
     return;
   }
@@ -224,8 +247,9 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmv(const v4i *__restrict__ gc
   const int lane = lane_id();
   if (t < n_ovec) {
     const int o = n_olong + t;
+    const long p0 = optr[o];
     Accum<Acc, T::kComp> a;
-    row_sum_wave<VT, long>(ocol, oval, x, optr[o], optr[o + 1], a);
+    row_sum_wave<VT>(ocol + p0, oval + p0, optr[o + 1] - p0, rx, a);
 #pragma unroll
     for (int m = 32; m > 0; m >>= 1) a.merge_xor(m);
     if (lane == 0) store_y(y, orow[o], alpha, beta, a.value());
@@ -237,6 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmv(const v4i *__restrict__ gc
   const int l = lane % C, h = lane / C;
   const long base = cbase[ch];
   const int W = cwidth[ch];
+  const rsrc_t rc = make_rsrc(gcol + base, (uint64_t)W * C * 16), rv = make_rsrc(gval + base, (uint64_t)W * C * sG);
   const int gr = ch * C + l;
   const int2 ri = gr < n_rows ? rinfo[gr] : make_int2(-1, -1);
   const int len = ri.y;                     // -1: padding row or row moved to the overflow
@@ -248,21 +273,28 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmv(const v4i *__restrict__ gc
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int q = q0 + u * H;
-      if (4 * q < len) {
-        cc[u] = ld_stream(gcol + base + (long)q * C + l);
-        vv[u] = ld_stream(gval + base + (long)q * C + l);
+      const bool act = 4 * q < len;
+      const uint32_t slot = (uint32_t)(q * C + l);
+      cc[u] = bload<v4i, kAuxNT>(rc, act ? slot * 16u : kOOB);
+      if constexpr (sG == 32) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const d2 lo = bload<d2, kAuxNT>(rv, act ? slot * 32u : kOOB);
+        const d2 hi = bload<d2, kAuxNT>(rv, act ? slot * 32u + 16u : kOOB);
+        vv[u] = typename T::G{lo.x, lo.y, hi.x, hi.y};
+      } else {
+        vv[u] = bload<typename T::G, kAuxNT>(rv, act ? slot * sG : kOOB);
       }
     }
+    typename T::X xv[U][4];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int q = q0 + u * H;
-      const int rem = len - 4 * q;
-      if (rem > 0) {
+    for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
-          if (e < rem) a.fma(T::cvt(vv[u][e]), (Acc)x[cc[u][e]]);
-      }
-    }
+      for (int e = 0; e < 4; ++e)
+        xv[u][e] = bload<typename T::X>(rx, (4 * (q0 + u * H) + e < len) ? (uint32_t)cc[u][e] * sX : kOOB);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) a.fma(T::cvt(vv[u][e]), (Acc)xv[u][e]);
   }
 #pragma unroll
   for (int m = C; m < kWave; m <<= 1) a.merge_xor(m);
@@ -281,29 +313,33 @@ __global__ __launch_bounds__(kBlock) void k_bsr_spmv(const int *__restrict__ brp
                                                      typename VT_<VT>::Acc alpha, typename VT_<VT>::Acc beta) {
   using T = VT_<VT>;
   using Acc = typename T::Acc;
+  constexpr uint32_t sV = sizeof(typename T::V), sX = sizeof(typename T::X);
   __shared__ Acc lds[kBlock / kWave][kBsrMaxBlock];
   const int i = __builtin_amdgcn_readfirstlane((int)blockIdx.x * (kBlock / kWave) + wave_id());
   if (i >= mb) return;
   const int lane = lane_id();
   const int bn = bR * bC;
+  const rsrc_t rx = make_rsrc(x, (uint64_t)n_cols * sX);
   Accum<Acc, T::kComp> a[kBsrMaxBlock / kWave];
-  for (int b = brp[i]; b < brp[i + 1]; ++b) {
+  const int b0 = brp[i], b1 = brp[i + 1];
+  const rsrc_t rd = make_rsrc(bdata + (long)b0 * bn, (uint64_t)(b1 - b0) * bn * sV);
+  for (int b = b0; b < b1; ++b) {
     const long xb = (long)bcol[b] * bC;
+    typename T::V v[kBsrMaxBlock / kWave];
+    typename T::X xv[kBsrMaxBlock / kWave];
 #pragma unroll
     for (int k = 0; k < kBsrMaxBlock / kWave; ++k) {
       const int e = lane + k * kWave;
-      if (e < bn) {
-        const long xi = xb + e % bC;
-        if (xi < n_cols) a[k].fma(T::cvt(ld_stream(bdata + (long)b * bn + e)), (Acc)x[xi]);
-      }
+      v[k] = bload<typename T::V, kAuxNT>(rd, e < bn ? (uint32_t)((b - b0) * bn + e) * sV : kOOB);
+      const long xi = xb + e % bC;
+      xv[k] = bload<typename T::X>(rx, (e < bn && xi < n_cols) ? (uint32_t)xi * sX : kOOB);
     }
+#pragma unroll
+    for (int k = 0; k < kBsrMaxBlock / kWave; ++k) a[k].fma(T::cvt(v[k]), (Acc)xv[k]);
   }
   Acc *sh = lds[wave_id()];
 #pragma unroll
-  for (int k = 0; k < kBsrMaxBlock / kWave; ++k) {
-    const int e = lane + k * kWave;
-    if (e < bn) sh[e] = a[k].value();
-  }
+  for (int k = 0; k < kBsrMaxBlock / kWave; ++k) sh[lane + k * kWave] = a[k].value();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -353,6 +389,8 @@ void pack_values(std::vector<uint8_t> &out, int vt, double v) {
 void check_csr(int64_t n_rows, int64_t n_cols, int64_t nnz, const std::vector<int64_t> &rp,
                const std::vector<int64_t> &ci) {
   SPOPS_REQUIRE(n_rows >= 0 && n_cols >= 0 && nnz >= 0, SPOPS_ERROR_INVALID_VALUE, "negative matrix size");
+  SPOPS_REQUIRE((uint64_t)n_cols * 8 <= kMaxBufBytes, SPOPS_ERROR_LIMIT,
+                "x must fit one 2 GiB buffer resource (n_cols <= 268435454)");
   SPOPS_REQUIRE(n_rows < (int64_t)INT32_MAX && n_cols < (int64_t)INT32_MAX, SPOPS_ERROR_LIMIT,
                 "rows and columns must be < 2^31 (got %lld x %lld)", (long long)n_rows, (long long)n_cols);
   SPOPS_REQUIRE(rp[0] == 0 && rp[n_rows] == nnz, SPOPS_ERROR_INVALID_VALUE,
@@ -507,11 +545,11 @@ static void launch_spmv_vt(const spops_matrix_s &M, double alpha, const void *x,
     if (grid == 0) return;
     if (P.rowptr64)
       hipLaunchKernelGGL((k_csr_spmv<VT, long>), dim3(grid), dim3(kBlock), 0, s, P.rowptr.as<long>(), P.col.as<int>(),
-                         P.val.as<typename T::V>(), xx, yy, (Acc)alpha, (Acc)beta, P.tasks.as<int2>(), P.ntasks,
+                         P.val.as<typename T::V>(), xx, (int)M.n_cols, yy, (Acc)alpha, (Acc)beta, P.tasks.as<int2>(), P.ntasks,
                          P.longs.as<int>(), P.nlong);
     else
       hipLaunchKernelGGL((k_csr_spmv<VT, int>), dim3(grid), dim3(kBlock), 0, s, P.rowptr.as<int>(), P.col.as<int>(),
-                         P.val.as<typename T::V>(), xx, yy, (Acc)alpha, (Acc)beta, P.tasks.as<int2>(), P.ntasks,
+                         P.val.as<typename T::V>(), xx, (int)M.n_cols, yy, (Acc)alpha, (Acc)beta, P.tasks.as<int2>(), P.ntasks,
                          P.longs.as<int>(), P.nlong);
   } else if (M.format == SPOPS_FORMAT_SELL || M.format == SPOPS_FORMAT_ELL) {
     const auto &S = M.sell;
@@ -521,7 +559,8 @@ static void launch_spmv_vt(const spops_matrix_s &M, double alpha, const void *x,
   hipLaunchKernelGGL((k_sell_spmv<VT, CC>), dim3(grid), dim3(kBlock), 0, s, S.gcol.as<v4i>(),                  \
                      S.gval.as<typename T::G>(), S.rinfo.as<int2>(), S.cbase.as<long>(), S.cwidth.as<int>(),   \
                      (int)S.nchunks, (int)M.n_rows, S.optr.as<long>(), S.ocol.as<int>(),                       \
-                     S.oval.as<typename T::V>(), S.orow.as<int>(), S.n_olong, S.n_ovec, xx, yy, (Acc)alpha,    \
+                     S.oval.as<typename T::V>(), S.orow.as<int>(), S.n_olong, S.n_ovec, xx, (int)M.n_cols, yy,     \
+                     (Acc)alpha,                                                                             \
                      (Acc)beta)
     switch (S.Cd) {
       case 4: SPOPS_SELL_LAUNCH(4); break;
@@ -897,5 +936,47 @@ extern "C" int spops_spmm(spops_matrix A, int64_t k, double alpha, const void *d
     (void)alpha; (void)d_X; (void)beta; (void)d_Y; (void)stream;
     throw_error(SPOPS_ERROR_NOT_SUPPORTED, "spops_spmm: not implemented for this format yet");
     return SPOPS_ERROR_NOT_SUPPORTED;
+  });
+}
+
+extern "C" int spops_bsr_create_from_blocks(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t bR, int64_t bC,
+                                            int64_t mb, int64_t nnzb, int index_type, const void *row_ptr,
+                                            const void *block_colindex, const void *block_data, int in_vt, int vt) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(A != nullptr, SPOPS_ERROR_INVALID_VALUE, "null output handle");
+    check_vt(in_vt, vt);
+    SPOPS_REQUIRE(bR > 0 && bC > 0 && mb == (n_rows + bR - 1) / bR, SPOPS_ERROR_INVALID_VALUE,
+                  "mb must be ceil(n_rows / bR)");
+    SPOPS_REQUIRE(bR * bC <= kBsrMaxBlock && bR <= kWave, SPOPS_ERROR_LIMIT, "block of %lld x %lld exceeds %d entries",
+                  (long long)bR, (long long)bC, kBsrMaxBlock);
+    SPOPS_REQUIRE((uint64_t)n_cols * 8 <= kMaxBufBytes, SPOPS_ERROR_LIMIT, "x must fit one 2 GiB buffer resource");
+    std::vector<int64_t> rp = host_index(row_ptr, mb + 1, index_type);
+    std::vector<int64_t> bc = host_index(block_colindex, nnzb, index_type);
+    SPOPS_REQUIRE(rp[0] == 0 && rp[mb] == nnzb, SPOPS_ERROR_INVALID_VALUE, "row_ptr must run from 0 to nnzb");
+    const int64_t nb = (n_cols + bC - 1) / bC;
+    for (int64_t b : bc) SPOPS_REQUIRE(b >= 0 && b < nb, SPOPS_ERROR_INVALID_VALUE, "block column out of range");
+    std::vector<uint8_t> v;
+    v.reserve((size_t)(nnzb * bR * bC) * value_bytes(vt));
+    int64_t nnz = 0;
+    for (int64_t e = 0; e < nnzb * bR * bC; ++e) {
+      double d = host_value(block_data, in_vt, e);
+      nnz += d != 0.0;
+      pack_values(v, vt, d);
+    }
+    auto M = std::make_unique<spops_matrix_s>();
+    M->format = SPOPS_FORMAT_BSR;
+    M->value_type = vt;
+    M->index_type = index_type;
+    M->n_rows = n_rows; M->n_cols = n_cols; M->nnz = nnz;
+    M->bsr.brp.upload(std::vector<int32_t>(rp.begin(), rp.end()));
+    M->bsr.bcol.upload(std::vector<int32_t>(bc.begin(), bc.end()));
+    M->bsr.data.upload(v);
+    M->bsr.mb = mb; M->bsr.bR = bR; M->bsr.bC = bC; M->bsr.nnzb = nnzb;
+    M->slots = nnzb * bR * bC;
+    M->dev_bytes = M->bsr.brp.bytes + M->bsr.bcol.bytes + M->bsr.data.bytes;
+    int64_t sV = (int64_t)value_bytes(vt), sX = (int64_t)xy_bytes(vt), sI = index_type == SPOPS_INDEX_I32 ? 4 : 8;
+    M->algo_bytes = (mb + 1) * sI + nnzb * sI + nnzb * bR * bC * sV + n_cols * sX + n_rows * sX;
+    *A = M.release();
+    return SPOPS_SUCCESS;
   });
 }

@@ -11,6 +11,7 @@ constexpr int kCsrStreamNnz = 512;     // entries staged per wave stream block
 constexpr int64_t kCsrLongRow = 2048;  // longer rows: one workgroup each
 constexpr int64_t kSellOvfMin = 64;    // rows shorter than this never leave their chunk
 constexpr int kBsrMaxBlock = 256;      // bR * bC limit of the wave-per-block-row kernel
+constexpr int64_t kRowSegment = (int64_t)1 << 24;   // entries per buffer resource of a long row
 
 struct CsrDev {
   DevBuf rowptr, col, val, tasks, longs;

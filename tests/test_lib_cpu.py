@@ -19,7 +19,7 @@ GOLD = os.path.join(ROOT, "tests", "golden")
 def declared_in_headers():
     names = set()
     for h in os.listdir(os.path.join(ROOT, "include")):
-        if h.endswith(".h"):
+        if h == "spops_hip.h":
             txt = open(os.path.join(ROOT, "include", h)).read()
             names |= set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(spops_\w+)\s*\(", txt, re.M))
     return names
