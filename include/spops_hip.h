@@ -135,8 +135,16 @@ int spops_spmm(spops_matrix A, int64_t k, double alpha, const void *d_X, double 
 /* info[0]=format info[1]=n_rows info[2]=n_cols info[3]=nnz info[4]=value_type
  * info[5]=device bytes owned info[6]=algorithmic bytes of one SpMV (no padding)
  * info[7]=stored slots (incl. padding) info[8]=long rows info[9]=wave tasks
- * info[10]=C info[11]=sigma info[12]=bR info[13]=bC info[14]=nchunks info[15]=index_type */
+ * info[10]=C info[11]=sigma info[12]=bR (BSR) or column slices (SELL/ELL) info[13]=bC
+ * (BSR) or max row width (ELL) info[14]=nchunks (reference layout) info[15]=index_type */
 int spops_matrix_info(spops_matrix A, int64_t info[16]);
+
+/* Layout tuning for matrices created afterwards (process-wide; no reference counterpart).
+ *   SPOPS_TUNE_SLICE_BYTES: bytes of x per column slice of the SELL/ELL layout (default
+ *   2 MiB, one XCD's share of L2); the slice count is the smallest power of two <= 8 with
+ *   n_cols * sizeof(x) / slices <= this.  Returns the previous value in *old (may be null). */
+#define SPOPS_TUNE_SLICE_BYTES 0
+int spops_set_tuning(int key, int64_t value, int64_t *old);
 
 /* Device copies of the layout arrays a format conversion produced, for parity checks:
  * which=0: reorder (int32, n_rows; SELL only); which=1: chunk_len in the reference's

@@ -18,6 +18,7 @@ SUCCESS = 0
 INDEX_I32, INDEX_I64 = 0, 1
 VALUE_F64, VALUE_F32, VALUE_BF16 = 0, 1, 2
 FORMAT_CSR, FORMAT_SELL, FORMAT_ELL, FORMAT_BSR = 0, 1, 2, 3
+TUNE_SLICE_BYTES = 0
 
 c_i64 = ctypes.c_int64
 c_int = ctypes.c_int
@@ -56,6 +57,7 @@ _SIGNATURES = [
     ("spops_spmm", c_int, [c_vp, c_i64, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
     ("spops_matrix_info", c_int, [c_vp, c_vp]),
     ("spops_matrix_layout", c_int, [c_vp, c_int, c_vp]),
+    ("spops_set_tuning", c_int, [c_int, c_i64, c_vp]),
     ("spops_matrix_destroy", c_int, [c_vp]),
     ("spops_minhash_signatures", c_int, [c_vp, c_vp, c_int, c_i64, c_int, ctypes.c_uint64, c_vp, c_vp]),
     ("spops_lsh_candidate_pairs", c_int,

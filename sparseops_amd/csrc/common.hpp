@@ -139,9 +139,14 @@ constexpr int kAuxNT = 2;                        // non-temporal (streamed once)
 typedef unsigned bu2 __attribute__((ext_vector_type(2)));
 typedef unsigned bu4 __attribute__((ext_vector_type(4)));
 
+// p and bytes must be the same in every lane of the wave: they are moved to scalar registers
+// (a resource in vector registers would make the compiler loop over lanes for every load).
 __device__ __forceinline__ rsrc_t make_rsrc(const void *p, uint64_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0,
-                                           (int)(bytes < kMaxBufBytes ? bytes : kMaxBufBytes), 0x00020000);
+  const uint64_t a = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a), hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  const int n = __builtin_amdgcn_readfirstlane((int)(bytes < kMaxBufBytes ? bytes : kMaxBufBytes));
+  return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(((uint64_t)hi << 32) | lo), (short)0, n,
+                                           0x00020000);
 }
 
 template <typename T, int aux = 0>

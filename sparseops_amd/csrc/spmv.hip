@@ -12,14 +12,20 @@
 //   CSR  : rowptr (int32, or int64 when nnz >= 2^31), col int32, values V; a launch plan of
 //          wave tasks (<= 512-entry blocks of consecutive rows with <= 64 rows, or one
 //          medium row) and a list of long rows (> 2048 entries), each one workgroup.
-//   SELL : rows grouped in chunks of C sorted rows.  Inside a chunk the entries of row l
-//          are stored in groups of 4 consecutive entries, group q of row l at
-//          base + q*C + l  (so one half-wave loads C*16 contiguous bytes with 128-bit
-//          loads per lane).  Each sorted row carries (original row, length); loads past a
-//          row's length are masked, so padding is never fetched.  Rows much longer than
-//          their chunk's median are moved to an overflow CSR and handled by one wave or
-//          one workgroup each, which keeps every chunk's wave short.
-//   ELL  : the SELL layout with sigma = 1 (no reordering), C = 32.
+//   SELL : (SELL-C-sigma and SELL-C-R) x is cut into P column slices of about equal entry
+//          count, P the smallest power of two <= 8 that keeps a slice of x within 2 MiB
+//          (one XCD's share of L2).  Workgroup g runs on XCD g % 8 and only tasks of slice
+//          (g % 8) % P run there, so every XCD gathers from an L2-resident slice of x:
+//          random gathers that hit L2 run at about twice the rate of gathers that miss
+//          to the memory side.  Per slice the row segments are stored packed in row order
+//          (no padding) with uint16 lengths; a wave task covers up to 64 rows and 512
+//          entries (one lane per row), a segment longer than 64 entries is a wave task of
+//          its own, rows longer than 2048 entries are workgroup tasks over all of x.
+//          Products and sums are in double; each slice writes one double partial per
+//          row and a second pass forms y = alpha * sum of partials + beta * y (P == 1:
+//          the tasks write y directly).  The reference's chunk_len / reorder arrays are
+//          kept on the host for inspection.
+//   ELL  : the same layout.
 //   BSR  : the reference layout (row_ptr, block_colindex, row-major bR x bC blocks).
 // The streams of A (indices, values) use non-temporal loads so that L2 keeps x.
 #include <algorithm>
@@ -67,9 +73,9 @@ __device__ __forceinline__ void store_y(X *y, long row, Acc alpha, Acc beta, Acc
 // Sum over entries [0, len) of one row whose col/val start at the given pointers: lanes of
 // one wave stride by 64, four entries per lane in flight, every load a buffer load (masked
 // lanes fetch nothing), gathers of x through the x resource.
-template <int VT>
+template <int VT, typename A = typename VT_<VT>::Acc, bool kC = VT_<VT>::kComp>
 __device__ __forceinline__ void row_sum_wave(const int *col, const typename VT_<VT>::V *val, int64_t len, rsrc_t rx,
-                                             Accum<typename VT_<VT>::Acc, VT_<VT>::kComp> &a) {
+                                             Accum<A, kC> &a) {
   using T = VT_<VT>;
   constexpr uint32_t sV = sizeof(typename T::V), sX = sizeof(typename T::X);
   const int lane = lane_id();
@@ -89,19 +95,18 @@ __device__ __forceinline__ void row_sum_wave(const int *col, const typename VT_<
 #pragma unroll
       for (int u = 0; u < 4; ++u) xv[u] = bload<typename T::X>(rx, (p + u * kWave < n) ? (uint32_t)c[u] * sX : kOOB);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) a.fma(T::cvt(v[u]), (typename T::Acc)xv[u]);
+      for (int u = 0; u < 4; ++u) a.fma((A)T::cvt(v[u]), (A)xv[u]);
     }
   }
 }
 
 // Whole-workgroup sum of one long row; result valid in thread 0.
-template <int VT>
-__device__ __forceinline__ typename VT_<VT>::Acc row_sum_block(const int *col, const typename VT_<VT>::V *val,
-                                                              int64_t len, rsrc_t rx,
-                                                              typename VT_<VT>::Acc *scratch /* 8 entries of LDS */) {
+template <int VT, typename A = typename VT_<VT>::Acc, bool kC = VT_<VT>::kComp>
+__device__ __forceinline__ A row_sum_block(const int *col, const typename VT_<VT>::V *val, int64_t len, rsrc_t rx,
+                                           A *scratch /* 8 entries of LDS */) {
   using T = VT_<VT>;
   constexpr uint32_t sV = sizeof(typename T::V), sX = sizeof(typename T::X);
-  Accum<typename T::Acc, T::kComp> a;
+  Accum<A, kC> a;
   for (int64_t seg = 0; seg < len; seg += kRowSegment) {
     const int n = (int)((len - seg) < kRowSegment ? (len - seg) : kRowSegment);
     const rsrc_t rc = make_rsrc(col + seg, (uint64_t)n * 4), rv = make_rsrc(val + seg, (uint64_t)n * sV);
@@ -118,22 +123,22 @@ __device__ __forceinline__ typename VT_<VT>::Acc row_sum_block(const int *col, c
 #pragma unroll
       for (int u = 0; u < 4; ++u) xv[u] = bload<typename T::X>(rx, (p + u * kBlock < n) ? (uint32_t)c[u] * sX : kOOB);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) a.fma(T::cvt(v[u]), (typename T::Acc)xv[u]);
+      for (int u = 0; u < 4; ++u) a.fma((A)T::cvt(v[u]), (A)xv[u]);
     }
   }
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) a.merge_xor(m);
   if (lane_id() == 0) {
     scratch[wave_id()] = a.s;
-    if constexpr (T::kComp) scratch[4 + wave_id()] = a.c;
+    if constexpr (kC) scratch[4 + wave_id()] = a.c;
   }
   __syncthreads();
-  typename T::Acc r = 0;
+  A r = 0;
   if (threadIdx.x == 0) {
-    Accum<typename T::Acc, T::kComp> t;
+    Accum<A, kC> t;
     for (int w = 0; w < kBlock / kWave; ++w) {
       t.add(scratch[w]);
-      if constexpr (T::kComp) t.add(scratch[4 + w]);
+      if constexpr (kC) t.add(scratch[4 + w]);
     }
     r = t.value();
   }
@@ -218,87 +223,155 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmv(const RP *__restrict__ rowp
 }
 
 // ---------------------------------------------------------------------------------------------
-// SELL-C-sigma (also ELL).  grid = n_ovf_long workgroups + ceil((n_ovf_vec + nchunks)/4).
+// Column-sliced stream SpMV (SELL-C-sigma, SELL-C-R, ELL)
+// grid = nlong_pad workgroups (one long row each, nlong_pad a multiple of 8) + 8 * wg_per_xcd
+// workgroups of 4 wave tasks; workgroup nlong_pad + 8j + q takes tasks 4j.. of XCD queue q.
 // ---------------------------------------------------------------------------------------------
-template <int VT, int C>
-__global__ __launch_bounds__(kBlock) void k_sell_spmv(const v4i *__restrict__ gcol,
-                                                      const typename VT_<VT>::G *__restrict__ gval,
-                                                      const int2 *__restrict__ rinfo, const long *__restrict__ cbase,
-                                                      const int *__restrict__ cwidth, int nchunks, int n_rows,
-                                                      const long *__restrict__ optr, const int *__restrict__ ocol,
-                                                      const typename VT_<VT>::V *__restrict__ oval,
-                                                      const int *__restrict__ orow, int n_olong, int n_ovec,
-                                                      const typename VT_<VT>::X *__restrict__ x, int n_cols,
-                                                      typename VT_<VT>::X *__restrict__ y,
-                                                      typename VT_<VT>::Acc alpha, typename VT_<VT>::Acc beta) {
-  using T = VT_<VT>;
-  using Acc = typename T::Acc;
-  constexpr uint32_t sX = sizeof(typename T::X), sG = sizeof(typename T::G);
-  __shared__ Acc lds[8];
-  const rsrc_t rx = make_rsrc(x, (uint64_t)n_cols * sX);
-  if ((int)blockIdx.x < n_olong) {
-    const int o = blockIdx.x;
-    const long p0 = optr[o];
-    Acc r = row_sum_block<VT>(ocol + p0, oval + p0, optr[o + 1] - p0, rx, lds);
-    if (threadIdx.x == 0) store_y(y, orow[o], alpha, beta, r);
-    return;
+struct SliceArgs {
+  const int *col;
+  const void *val;
+  const uint32_t *meta;
+  const int4 *tasks;
+  const long *lptr;
+  const int *lcol;
+  const void *lval;
+  const int *lrow;
+  double *part;
+  const void *x;
+  void *y;
+  long sbase[kXcds];
+  int qoff[kXcds + 1];
+  int n_rows, n_cols, P, nlong, nlong_pad;
+  double alpha, beta;
+};
+
+template <typename X>
+__device__ __forceinline__ void slice_store(const SliceArgs &a, bool direct, int s, int row, double sum) {
+  if (direct) {
+    X *y = static_cast<X *>(a.y);
+    double r = a.alpha * sum;
+    if (a.beta != 0.0) r += a.beta * (double)y[row];
+    y[row] = (X)r;
+  } else {
+    a.part[(long)s * a.n_rows + row] = sum;
   }
-  const int t = __builtin_amdgcn_readfirstlane(((int)blockIdx.x - n_olong) * (kBlock / kWave) + wave_id());
+}
+
+// The K elements a lane owns in a task stored lane-interleaved: with g = 16 / sizeof(E)
+// elements per 128-bit word, word b of lane l sits at word index b * L + l (L = lanes in
+// the task), so each of the K / g loads of the wave reads one contiguous block.
+template <typename E, int K>
+__device__ __forceinline__ void load_lane_run(rsrc_t r, int L, E (&out)[K]) {
+  constexpr int NB = K * (int)sizeof(E) / 16;
+  static_assert(NB * 16 == K * (int)sizeof(E), "run must be a multiple of 16 bytes");
   const int lane = lane_id();
-  if (t < n_ovec) {
-    const int o = n_olong + t;
-    const long p0 = optr[o];
-    Accum<Acc, T::kComp> a;
-    row_sum_wave<VT>(ocol + p0, oval + p0, optr[o + 1] - p0, rx, a);
+  v4i raw[NB];
 #pragma unroll
-    for (int m = 32; m > 0; m >>= 1) a.merge_xor(m);
-    if (lane == 0) store_y(y, orow[o], alpha, beta, a.value());
+  for (int b = 0; b < NB; ++b) raw[b] = bload<v4i, kAuxNT>(r, lane < L ? 16u * (uint32_t)(b * L + lane) : kOOB);
+  __builtin_memcpy(out, raw, sizeof(out));
+}
+
+template <int VT, bool DIRECT>
+__global__ __launch_bounds__(kBlock) void k_slice_spmv(const SliceArgs a) {
+  using T = VT_<VT>;
+  using V = typename T::V;
+  using X = typename T::X;
+  constexpr uint32_t sV = sizeof(V), sX = sizeof(X);
+  constexpr int K = slice_lane_entries(VT);
+  __shared__ double red[8];
+  const rsrc_t rx = make_rsrc(a.x, (uint64_t)a.n_cols * sX);
+  const int g = blockIdx.x;
+  if (g < a.nlong_pad) {
+    if (g >= a.nlong) return;
+    const long p0 = a.lptr[g], p1 = a.lptr[g + 1];
+    const double r = row_sum_block<VT, double, false>(a.lcol + p0, static_cast<const V *>(a.lval) + p0, p1 - p0,
+                                                      rx, red);
+    const int row = a.lrow[g];
+    if (threadIdx.x == 0) slice_store<X>(a, DIRECT, 0, row, r);
+    if (!DIRECT)
+      for (int s = 1 + (int)threadIdx.x; s < a.P; s += kBlock) a.part[(long)s * a.n_rows + row] = 0.0;
     return;
   }
-  const int ch = t - n_ovec;
-  if (ch >= nchunks) return;
-  constexpr int H = kWave / C;              // lanes per row
-  const int l = lane % C, h = lane / C;
-  const long base = cbase[ch];
-  const int W = cwidth[ch];
-  const rsrc_t rc = make_rsrc(gcol + base, (uint64_t)W * C * 16), rv = make_rsrc(gval + base, (uint64_t)W * C * sG);
-  const int gr = ch * C + l;
-  const int2 ri = gr < n_rows ? rinfo[gr] : make_int2(-1, -1);
-  const int len = ri.y;                     // -1: padding row or row moved to the overflow
-  Accum<Acc, T::kComp> a;
-  constexpr int U = (VT == SPOPS_VALUE_F64) ? 2 : 4;
-  for (int q0 = h; q0 < W; q0 += H * U) {
-    v4i cc[U];
-    typename T::G vv[U];
+  const int gg = g - a.nlong_pad;
+  const int q = gg % kXcds;
+  const int t = __builtin_amdgcn_readfirstlane(a.qoff[q] + (gg / kXcds) * (kBlock / kWave) + wave_id());
+  if (t >= a.qoff[q + 1]) return;
+  // the task is wave-uniform: keep it in scalar registers so that every buffer resource
+  // built from it is scalar too
+  const int4 tk = a.tasks[t];
+  const int r0 = __builtin_amdgcn_readfirstlane(tk.x), ne = __builtin_amdgcn_readfirstlane(tk.z);
+  const int tw = __builtin_amdgcn_readfirstlane(tk.w);
+  const int s = (tw >> 16) & 0xff, kind = tw >> 24;
+  const long eb = a.sbase[s] + __builtin_amdgcn_readfirstlane(tk.y);
+  const int *col = a.col + eb;
+  const V *val = static_cast<const V *>(a.val) + eb;
+  const int lane = lane_id();
+  if (kind == 1) {                          // one long segment: the wave strides over it
+    Accum<double, false> acc;
+    row_sum_wave<VT, double, false>(col, val, ne, rx, acc);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int q = q0 + u * H;
-      const bool act = 4 * q < len;
-      const uint32_t slot = (uint32_t)(q * C + l);
-      cc[u] = bload<v4i, kAuxNT>(rc, act ? slot * 16u : kOOB);
-      if constexpr (sG == 32) {
-        typedef double d2 __attribute__((ext_vector_type(2)));
-        const d2 lo = bload<d2, kAuxNT>(rv, act ? slot * 32u : kOOB);
-        const d2 hi = bload<d2, kAuxNT>(rv, act ? slot * 32u + 16u : kOOB);
-        vv[u] = typename T::G{lo.x, lo.y, hi.x, hi.y};
-      } else {
-        vv[u] = bload<typename T::G, kAuxNT>(rv, act ? slot * sG : kOOB);
-      }
-    }
-    typename T::X xv[U][4];
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        xv[u][e] = bload<typename T::X>(rx, (4 * (q0 + u * H) + e < len) ? (uint32_t)cc[u][e] * sX : kOOB);
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) a.fma(T::cvt(vv[u][e]), (Acc)xv[u][e]);
+    for (int m = 32; m > 0; m >>= 1) acc.merge_xor(m);
+    if (lane == 0) slice_store<X>(a, DIRECT, s, r0, acc.value());
+    return;
   }
+  // lane l: entries K l .. K l + K - 1 (dummy entries have column -1: their gather is out
+  // of range and returns 0; lanes past the task load zeros and gather x[0], times 0)
+  const uint32_t mw = a.meta[(long)t * kWave + lane];
+  const int L = (ne + K - 1) / K;            // lanes holding entries; the range is L * K long
+  const rsrc_t rc = make_rsrc(col, (uint64_t)L * K * 4), rv = make_rsrc(val, (uint64_t)L * K * sV);
+  int c[K];
+  V v[K];
+  load_lane_run<int, K>(rc, L, c);
+  load_lane_run<V, K>(rv, L, v);
+  X xv[K];
 #pragma unroll
-  for (int m = C; m < kWave; m <<= 1) a.merge_xor(m);
-  if (h == 0 && len >= 0) store_y(y, ri.x, alpha, beta, a.value());
+  for (int i = 0; i < K; ++i) xv[i] = bload<X>(rx, (uint32_t)c[i] * sX);
+  // local pass: pieces that start and end inside the lane are stored at once; the piece
+  // before the lane's first row start (carry) and the one after its last (tail) wait.
+  const uint32_t hbits = mw & 0xffffu;
+  int row = r0 + (int)(mw >> 16) - 1;
+  double cur = 0.0, carry = 0.0;
+  bool seen = false;
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    if (hbits & (1u << i)) {
+      if (seen) slice_store<X>(a, DIRECT, s, row, cur);
+      else carry = cur;
+      cur = 0.0;
+      ++row;
+      seen = true;
+    }
+    cur = __builtin_fma((double)T::cvt(v[i]), (double)xv[i], cur);
+  }
+  if (!seen) carry = cur;
+  // W_l = carry_l + (a row starts in lane l ? 0 : W_{l+1}): reverse segmented scan
+  double W = carry;
+  bool stop = seen;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const double w2 = __shfl_down(W, d, kWave);
+    const int f2 = __shfl_down((int)stop, d, kWave);
+    if (lane + d < kWave && !stop) {
+      W += w2;
+      stop = f2 != 0;
+    }
+  }
+  double next = __shfl_down(W, 1, kWave);
+  if (lane == kWave - 1) next = 0.0;
+  if (seen) slice_store<X>(a, DIRECT, s, row, cur + next);
+}
+
+// y = alpha * (sum over slices of part) + beta * y, slices added in order 0..P-1.
+template <typename X>
+__global__ __launch_bounds__(kBlock) void k_slice_reduce(const double *__restrict__ part, int P, int n_rows,
+                                                        X *__restrict__ y, double alpha, double beta) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n_rows) return;
+  double sum = 0.0;
+  for (int s = 0; s < P; ++s) sum += __builtin_nontemporal_load(part + (long)s * n_rows + r);
+  double o = alpha * sum;
+  if (beta != 0.0) o += beta * (double)y[r];
+  y[r] = (X)o;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -427,18 +500,7 @@ void build_csr_plan(const std::vector<int64_t> &rp, int64_t n_rows, std::vector<
   }
 }
 
-// Decide which rows of a device chunk leave for the overflow CSR: rows longer than
-// max(kSellOvfMin, 4 x (median length of the chunk's rows)).
-static void sell_split_chunk(const int64_t *len, int m, std::vector<char> &out) {
-  std::vector<int64_t> s(len, len + m);
-  std::nth_element(s.begin(), s.begin() + m / 2, s.end());
-  int64_t med = s[m / 2];
-  int64_t thr = std::max<int64_t>(kSellOvfMin, 4 * std::max<int64_t>(med, 4));
-  out.assign(m, 0);
-  for (int l = 0; l < m; ++l) out[l] = len[l] > thr;
-}
-
-// Row access for the SELL builder: entries of the row at sorted position g.
+// Row access for the sliced builder: entries of the row at SELL position g.
 struct RowSource {
   virtual ~RowSource() = default;
   virtual int64_t length(int64_t g) const = 0;
@@ -446,88 +508,235 @@ struct RowSource {
   virtual void entry(int64_t g, int64_t k, int64_t &col, double &val) const = 0;
 };
 
-void build_sell_layout(spops_matrix_s &M, const RowSource &src, int vt, int Cd) {
-  const int64_t n = M.n_rows;
-  const int64_t nch = (n + Cd - 1) / Cd;
-  std::vector<int2> rinfo(n);
-  std::vector<long> cbase(nch);
-  std::vector<int> cwidth(nch);
-  std::vector<int64_t> ovf_rows;                       // sorted positions moved to overflow
-  std::vector<char> ovf_flag;
-  int64_t groups = 0;
-  std::vector<int64_t> lens(Cd);
-  for (int64_t c = 0; c < nch; ++c) {
-    int m = (int)std::min<int64_t>(Cd, n - c * Cd);
-    for (int l = 0; l < m; ++l) lens[l] = src.length(c * Cd + l);
-    sell_split_chunk(lens.data(), m, ovf_flag);
-    int64_t w = 0;
-    for (int l = 0; l < m; ++l) {
-      int64_t g = c * Cd + l;
-      if (ovf_flag[l]) { ovf_rows.push_back(g); rinfo[g] = make_int2((int)src.orig_row(g), -1); }
-      else { rinfo[g] = make_int2((int)src.orig_row(g), (int)lens[l]); w = std::max(w, (lens[l] + 3) / 4); }
-    }
-    cbase[c] = groups;
-    cwidth[c] = (int)w;
-    groups += w * Cd;
-  }
-  // group arrays
-  const size_t vb = value_bytes(vt);
-  std::vector<int32_t> gcol((size_t)groups * 4, 0);
-  std::vector<uint8_t> gval((size_t)groups * 4 * vb, 0);
-  auto put_val = [&](size_t slot, double v) {
-    if (vt == SPOPS_VALUE_F64) std::memcpy(&gval[slot * 8], &v, 8);
-    else if (vt == SPOPS_VALUE_F32) { float f = (float)v; std::memcpy(&gval[slot * 4], &f, 4); }
-    else { uint16_t h = double_to_bf16_rne(v); std::memcpy(&gval[slot * 2], &h, 2); }
-  };
+int64_t g_slice_bytes = kSliceBytesDefault;
 
-  for (int64_t c = 0; c < nch; ++c) {
-    int m = (int)std::min<int64_t>(Cd, n - c * Cd);
-    for (int l = 0; l < m; ++l) {
-      int64_t g = c * Cd + l;
-      int len = rinfo[g].y;
-      for (int64_t k = 0; k < len; ++k) {
-        int64_t col; double v;
-        src.entry(g, k, col, v);
-        size_t slot = (size_t)(cbase[c] + (k / 4) * Cd + l) * 4 + (k % 4);
-        gcol[slot] = (int32_t)col;
-        put_val(slot, v);
+void build_slice_layout(spops_matrix_s &M, const RowSource &src, int vt) {
+  auto &S = M.sell;
+  const int64_t n = M.n_rows, m = M.n_cols;
+  std::vector<int64_t> pos(n);                          // original row -> SELL position
+  for (int64_t g = 0; g < n; ++g) pos[src.orig_row(g)] = g;
+  std::vector<int64_t> len(n);
+  for (int64_t r = 0; r < n; ++r) len[r] = src.length(pos[r]);
+  // slices: P from the x footprint of one XCD's rows (128-byte lines of x touched by an
+  // eighth of the rows, the largest eighth), column cuts at equal entry counts
+  const int64_t sx = (int64_t)xy_bytes(vt);
+  int64_t foot = 0;
+  if (m * sx > g_slice_bytes) {
+    const int64_t per_line = 128 / sx, nlines = (m + per_line - 1) / per_line;
+    std::vector<uint64_t> seen((size_t)(nlines + 63) / 64);
+    int64_t tot = 0;
+    for (int64_t r = 0; r < n; ++r) tot += len[r] > kCsrLongRow ? 0 : len[r];
+    int64_t acc = 0, r = 0;
+    for (int part = 0; part < kXcds; ++part) {
+      std::fill(seen.begin(), seen.end(), 0ull);
+      int64_t lines = 0;
+      for (; r < n && acc * kXcds < (part + 1) * tot; ++r) {
+        if (len[r] > kCsrLongRow) continue;
+        for (int64_t k = 0; k < len[r]; ++k) {
+          int64_t c; double v;
+          src.entry(pos[r], k, c, v);
+          const int64_t ln = c / per_line;
+          const uint64_t bit = 1ull << (ln & 63);
+          if (!(seen[(size_t)(ln >> 6)] & bit)) { seen[(size_t)(ln >> 6)] |= bit; ++lines; }
+        }
+        acc += len[r];
+      }
+      foot = std::max(foot, lines * 128);
+    }
+  }
+  int P = 1;
+  while (P < kXcds && foot > (int64_t)P * g_slice_bytes) P *= 2;
+  std::vector<int64_t> cut(P + 1, m);
+  cut[0] = 0;
+  if (P > 1) {
+    std::vector<int64_t> hist(m + 1, 0);
+    int64_t tot = 0;
+    for (int64_t r = 0; r < n; ++r) {
+      if (len[r] > kCsrLongRow) continue;
+      for (int64_t k = 0; k < len[r]; ++k) {
+        int64_t c; double v;
+        src.entry(pos[r], k, c, v);
+        ++hist[c];
+      }
+      tot += len[r];
+    }
+    int64_t acc = 0, sl = 1;
+    for (int64_t c = 0; c < m && sl < P; ++c) {
+      acc += hist[c];
+      while (sl < P && acc * P >= sl * tot) cut[sl++] = c + 1;
+    }
+    for (int i = 1; i <= P; ++i) cut[i] = std::max(cut[i], cut[i - 1]);
+  }
+  auto slice_of = [&](int64_t c) {
+    return (int)(std::upper_bound(cut.begin() + 1, cut.end(), c) - (cut.begin() + 1));
+  };
+  // segments: entries of each row bucketed per slice (temporary, slice-major, row order)
+  std::vector<int32_t> seg((size_t)P * n, 0);
+  std::vector<int64_t> lrows;
+  for (int64_t r = 0; r < n; ++r) {
+    if (len[r] > kCsrLongRow) { lrows.push_back(r); continue; }
+    for (int64_t k = 0; k < len[r]; ++k) {
+      int64_t c; double v;
+      src.entry(pos[r], k, c, v);
+      ++seg[(size_t)slice_of(c) * n + r];
+    }
+  }
+  std::vector<int64_t> seg_off((size_t)P * n + 1, 0);
+  for (size_t i = 0; i < (size_t)P * n; ++i) seg_off[i + 1] = seg_off[i] + seg[i];
+  const int64_t total = seg_off[(size_t)P * n];
+  std::vector<int32_t> tcol((size_t)total);
+  std::vector<double> tval((size_t)total);
+  {
+    std::vector<int64_t> fill(seg_off.begin(), seg_off.end() - 1);
+    for (int64_t r = 0; r < n; ++r) {
+      if (len[r] > kCsrLongRow) continue;
+      for (int64_t k = 0; k < len[r]; ++k) {
+        int64_t c; double v;
+        src.entry(pos[r], k, c, v);
+        const int64_t at = fill[(size_t)slice_of(c) * n + r]++;
+        tcol[at] = (int32_t)c;
+        tval[at] = v;
       }
     }
   }
-  // overflow CSR: workgroup rows (> kCsrLongRow) first, then wave rows
-  std::vector<int64_t> olong, ovec;
-  for (int64_t g : ovf_rows) (src.length(g) > kCsrLongRow ? olong : ovec).push_back(g);
-  std::vector<long> optr(1, 0);
-  std::vector<int32_t> ocol, orow;
-  std::vector<uint8_t> oval;
-  for (auto *lst : {&olong, &ovec})
-    for (int64_t g : *lst) {
-      int64_t len = src.length(g);
-      for (int64_t k = 0; k < len; ++k) {
-        int64_t col; double v;
-        src.entry(g, k, col, v);
-        ocol.push_back((int32_t)col);
-        pack_values(oval, vt, v);
+  // tasks and the final entry order: every non-long row has a segment of >= 1 entry in every
+  // slice (a dummy with column -1 when empty); each task's range is padded with dummies to a
+  // multiple of K entries (so task ranges start 16-byte aligned) and lane-interleaved.
+  const int K = slice_lane_entries(vt);
+  std::vector<int32_t> col;
+  std::vector<double> dval;
+  col.reserve((size_t)(total + total / 8 + 64));
+  dval.reserve(col.capacity());
+  std::vector<std::vector<int4>> stasks(P);
+  std::vector<std::vector<int64_t>> sents(P);
+  std::vector<std::vector<uint32_t>> smeta(P);          // 64 words per task
+  for (int s = 0; s < P; ++s) {
+    S.sbase[s] = (int64_t)col.size();
+    const int64_t s0 = S.sbase[s];
+    int64_t r0 = 0, e0 = 0;
+    int rows = 0, ne = 0;
+    uint32_t lanemeta[kWave];
+    auto pad = [&]() {
+      while (col.size() % K) { col.push_back(-1); dval.push_back(0.0); }
+    };
+    // lane-interleave the task's entries (see load_lane_run): element i of lane l goes to
+    // ((i / g) * L + l) * g + i % g, g = elements per 16 bytes (4 for columns, 16 / sV values)
+    auto interleave = [&](int64_t at, int L) {
+      const int gc = 4, gv = 16 / (int)value_bytes(vt);
+      std::vector<int32_t> c2((size_t)L * K);
+      std::vector<double> v2((size_t)L * K);
+      for (int l = 0; l < L; ++l)
+        for (int i = 0; i < K; ++i) {
+          const size_t from = (size_t)at + (size_t)l * K + i;
+          c2[(size_t)((i / gc) * L + l) * gc + i % gc] = col[from];
+          v2[(size_t)((i / gv) * L + l) * gv + i % gv] = dval[from];
+        }
+      std::copy(c2.begin(), c2.end(), col.begin() + at);
+      std::copy(v2.begin(), v2.end(), dval.begin() + at);
+    };
+    auto close = [&]() {
+      if (rows > 0) {
+        stasks[s].push_back(make_int4((int)r0, (int)(e0 - s0), ne, s << 16));
+        sents[s].push_back(ne);
+        // lane words: row-start bits and rows started in earlier lanes
+        uint32_t before = 0;
+        for (int l = 0; l < kWave; ++l) {
+          const uint32_t bits = lanemeta[l];
+          smeta[s].push_back(bits | (before << 16));
+          before += (uint32_t)__builtin_popcount(bits);
+        }
+        pad();
+        interleave(e0, (ne + K - 1) / K);
       }
-      optr.push_back(optr.back() + len);
-      orow.push_back((int32_t)src.orig_row(g));
+      rows = 0;
+      ne = 0;
+    };
+    auto append = [&](int64_t r, int64_t l) {
+      const int64_t so = seg_off[(size_t)s * n + r];
+      col.insert(col.end(), tcol.begin() + so, tcol.begin() + so + l);
+      dval.insert(dval.end(), tval.begin() + so, tval.begin() + so + l);
+    };
+    for (int64_t r = 0; r < n; ++r) {
+      if (len[r] > kCsrLongRow) { close(); continue; }
+      const int64_t l = seg[(size_t)s * n + r];
+      if (l > kWave * K) {                              // a wave task of its own
+        close();
+        stasks[s].push_back(make_int4((int)r, (int)((int64_t)col.size() - s0), (int)l, (s << 16) | (1 << 24)));
+        sents[s].push_back(l);
+        smeta[s].insert(smeta[s].end(), kWave, 0u);
+        append(r, l);
+        pad();
+        continue;
+      }
+      const int64_t le = l > 0 ? l : 1;
+      if (ne + le > kWave * K) close();
+      if (rows == 0) {
+        r0 = r;
+        e0 = (int64_t)col.size();
+        for (auto &w : lanemeta) w = 0;
+      }
+      lanemeta[ne / K] |= 1u << (ne % K);              // the row starts at entry ne
+      ++rows;
+      if (l > 0) append(r, l);
+      else { col.push_back(-1); dval.push_back(0.0); }
+      ne += (int)le;
     }
-  M.sell.gcol.upload(gcol);
-  M.sell.gval.upload(gval);
-  M.sell.rinfo.upload(rinfo);
-  M.sell.cbase.upload(cbase);
-  M.sell.cwidth.upload(cwidth);
-  M.sell.optr.upload(optr);
-  M.sell.ocol.upload(ocol);
-  M.sell.oval.upload(oval);
-  M.sell.orow.upload(orow);
-  M.sell.nchunks = nch;
-  M.sell.Cd = Cd;
-  M.sell.n_olong = (int)olong.size();
-  M.sell.n_ovec = (int)ovec.size();
-  M.slots = groups * 4 + (int64_t)ocol.size();
-  M.dev_bytes = M.sell.gcol.bytes + M.sell.gval.bytes + M.sell.rinfo.bytes + M.sell.cbase.bytes + M.sell.cwidth.bytes +
-                M.sell.optr.bytes + M.sell.ocol.bytes + M.sell.oval.bytes + M.sell.orow.bytes;
+    close();
+    SPOPS_REQUIRE((int64_t)col.size() - s0 < (int64_t)INT32_MAX, SPOPS_ERROR_LIMIT,
+                  "a column slice holds >= 2^31 entries");
+  }
+  // XCD queues: queue q serves slice q % P, part q / P of its tasks (equal entry counts)
+  std::vector<int4> tasks;
+  std::vector<uint32_t> meta;
+  const int parts = kXcds / P;
+  for (int q = 0; q < kXcds; ++q) {
+    S.qoff[q] = (int)tasks.size();
+    const int s = q % P, k = q / P;
+    int64_t tot = 0;
+    for (int64_t e : sents[s]) tot += e + 16;           // + a fixed cost per task
+    int64_t acc = 0;
+    for (size_t i = 0; i < stasks[s].size(); ++i) {
+      const int64_t part = std::min<int64_t>(parts - 1, acc * parts / std::max<int64_t>(tot, 1));
+      acc += sents[s][i] + 16;
+      if (part == k) {
+        tasks.push_back(stasks[s][i]);
+        meta.insert(meta.end(), smeta[s].begin() + (int64_t)i * kWave, smeta[s].begin() + (int64_t)(i + 1) * kWave);
+      }
+    }
+  }
+  S.qoff[kXcds] = (int)tasks.size();
+  // long rows
+  std::vector<long> lptr(1, 0);
+  std::vector<int32_t> lcol, lrow;
+  std::vector<uint8_t> lval;
+  for (int64_t r : lrows) {
+    for (int64_t k = 0; k < len[r]; ++k) {
+      int64_t c; double v;
+      src.entry(pos[r], k, c, v);
+      lcol.push_back((int32_t)c);
+      pack_values(lval, vt, v);
+    }
+    lptr.push_back(lptr.back() + len[r]);
+    lrow.push_back((int32_t)r);
+  }
+  std::vector<uint8_t> val;
+  val.reserve(dval.size() * value_bytes(vt));
+  for (double v : dval) pack_values(val, vt, v);
+  S.col.upload(col);
+  S.val.upload(val);
+  S.meta.upload(meta);
+  S.tasks.upload(tasks);
+  S.lptr.upload(lptr);
+  S.lcol.upload(lcol);
+  S.lval.upload(lval);
+  S.lrow.upload(lrow);
+  if (P > 1) S.part.alloc((size_t)P * n * sizeof(double));
+  S.P = P;
+  S.nlong = (int)lrows.size();
+  S.ntasks = (int)tasks.size();
+  M.slots = (int64_t)col.size() + (int64_t)lcol.size();
+  M.dev_bytes = S.col.bytes + S.val.bytes + S.meta.bytes + S.tasks.bytes + S.lptr.bytes + S.lcol.bytes +
+                S.lval.bytes + S.lrow.bytes + S.part.bytes;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -553,23 +762,42 @@ static void launch_spmv_vt(const spops_matrix_s &M, double alpha, const void *x,
                          P.longs.as<int>(), P.nlong);
   } else if (M.format == SPOPS_FORMAT_SELL || M.format == SPOPS_FORMAT_ELL) {
     const auto &S = M.sell;
-    int grid = S.n_olong + (int)((S.n_ovec + S.nchunks + 3) / 4);
-    if (grid == 0) return;
-#define SPOPS_SELL_LAUNCH(CC)                                                                                   \
-  hipLaunchKernelGGL((k_sell_spmv<VT, CC>), dim3(grid), dim3(kBlock), 0, s, S.gcol.as<v4i>(),                  \
-                     S.gval.as<typename T::G>(), S.rinfo.as<int2>(), S.cbase.as<long>(), S.cwidth.as<int>(),   \
-                     (int)S.nchunks, (int)M.n_rows, S.optr.as<long>(), S.ocol.as<int>(),                       \
-                     S.oval.as<typename T::V>(), S.orow.as<int>(), S.n_olong, S.n_ovec, xx, (int)M.n_cols, yy,     \
-                     (Acc)alpha,                                                                             \
-                     (Acc)beta)
-    switch (S.Cd) {
-      case 4: SPOPS_SELL_LAUNCH(4); break;
-      case 8: SPOPS_SELL_LAUNCH(8); break;
-      case 16: SPOPS_SELL_LAUNCH(16); break;
-      case 32: SPOPS_SELL_LAUNCH(32); break;
-      default: SPOPS_SELL_LAUNCH(64); break;
+    SliceArgs sa;
+    sa.col = S.col.as<int>();
+    sa.val = S.val.p;
+    sa.meta = S.meta.as<uint32_t>();
+    sa.tasks = S.tasks.as<int4>();
+    sa.lptr = S.lptr.as<long>();
+    sa.lcol = S.lcol.as<int>();
+    sa.lval = S.lval.p;
+    sa.lrow = S.lrow.as<int>();
+    sa.part = S.part.as<double>();
+    sa.x = x;
+    sa.y = y;
+    int wg = 0;
+    for (int q = 0; q < kXcds; ++q) {
+      sa.sbase[q] = (long)S.sbase[q];
+      sa.qoff[q] = S.qoff[q];
+      wg = std::max(wg, (S.qoff[q + 1] - S.qoff[q] + kBlock / kWave - 1) / (kBlock / kWave));
     }
-#undef SPOPS_SELL_LAUNCH
+    sa.qoff[kXcds] = S.qoff[kXcds];
+    sa.n_rows = (int)M.n_rows;
+    sa.n_cols = (int)M.n_cols;
+    sa.P = S.P;
+    sa.nlong = S.nlong;
+    sa.nlong_pad = (S.nlong + kXcds - 1) / kXcds * kXcds;
+    sa.alpha = alpha;
+    sa.beta = beta;
+    const int grid = sa.nlong_pad + kXcds * wg;
+    if (M.n_rows == 0) return;
+    if (S.P == 1) {
+      if (grid > 0) hipLaunchKernelGGL((k_slice_spmv<VT, true>), dim3(grid), dim3(kBlock), 0, s, sa);
+    } else {
+      if (grid > 0) hipLaunchKernelGGL((k_slice_spmv<VT, false>), dim3(grid), dim3(kBlock), 0, s, sa);
+      SPOPS_HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL((k_slice_reduce<typename T::X>), dim3((unsigned)((M.n_rows + kBlock - 1) / kBlock)),
+                         dim3(kBlock), 0, s, S.part.as<double>(), S.P, (int)M.n_rows, yy, alpha, beta);
+    }
   } else {
     const auto &B = M.bsr;
     int grid = (int)((B.mb + 3) / 4);
@@ -629,11 +857,6 @@ struct CsrRows : RowSource {
     val = host_value(vals, in_vt, j);
   }
 };
-
-int device_chunk_height(int64_t C) {
-  if (C == 4 || C == 8 || C == 16 || C == 32 || C == 64) return (int)C;
-  return 32;
-}
 
 // the reference's per-slice order: sort by (nnz, row) descending inside each sigma-slice
 std::vector<int64_t> sell_reorder(const std::vector<int64_t> &rp, int64_t n, int64_t sigma) {
@@ -708,9 +931,8 @@ static int sell_create_common(spops_matrix *A, int64_t n_rows, int64_t n_cols, i
   M->index_type = index_type;
   M->n_rows = n_rows; M->n_cols = n_cols; M->nnz = nnz;
   std::vector<int64_t> perm = sell_reorder(rp, n_rows, sigma);
-  int Cd = format == SPOPS_FORMAT_ELL ? 32 : device_chunk_height(C);
   CsrRows src(rp, ci, perm, values, in_vt);
-  build_sell_layout(*M, src, vt, Cd);
+  build_slice_layout(*M, src, vt);
   M->sell.C = C;
   M->sell.sigma = sigma;
   M->ref_nchunks = (n_rows + C - 1) / C;
@@ -812,7 +1034,7 @@ extern "C" int spops_sell_create_from_chunks(spops_matrix *A, int64_t n_rows, in
     M->index_type = index_type;
     M->n_rows = n_rows; M->n_cols = n_cols; M->nnz = nnz;
     ChunkRows src(C, perm, clen, lens, chunk_col, chunk_val, index_type, in_vt);
-    build_sell_layout(*M, src, vt, device_chunk_height(C));
+    build_slice_layout(*M, src, vt);
     M->sell.C = C;
     M->sell.sigma = 0;
     M->ref_nchunks = nchunks;
@@ -904,11 +1126,21 @@ extern "C" int spops_matrix_info(spops_matrix A, int64_t info[16]) {
     info[15] = A->index_type;
     if (A->format == SPOPS_FORMAT_CSR) { info[8] = A->csr.nlong; info[9] = A->csr.ntasks; }
     if (A->format == SPOPS_FORMAT_SELL || A->format == SPOPS_FORMAT_ELL) {
-      info[8] = A->sell.n_olong + A->sell.n_ovec; info[9] = A->sell.nchunks + A->sell.n_ovec;
-      info[10] = A->sell.C; info[11] = A->sell.sigma; info[14] = A->ref_nchunks;
+      info[8] = A->sell.nlong; info[9] = A->sell.ntasks;
+      info[10] = A->sell.C; info[11] = A->sell.sigma; info[12] = A->sell.P; info[14] = A->ref_nchunks;
       if (A->format == SPOPS_FORMAT_ELL) info[13] = A->ell_width;
     }
     if (A->format == SPOPS_FORMAT_BSR) { info[12] = A->bsr.bR; info[13] = A->bsr.bC; info[14] = A->bsr.nnzb; }
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_set_tuning(int key, int64_t value, int64_t *old) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(key == SPOPS_TUNE_SLICE_BYTES, SPOPS_ERROR_INVALID_VALUE, "unknown tuning key %d", key);
+    SPOPS_REQUIRE(value >= 64, SPOPS_ERROR_INVALID_VALUE, "slice bytes must be >= 64");
+    if (old) *old = g_slice_bytes;
+    g_slice_bytes = value;
     return SPOPS_SUCCESS;
   });
 }

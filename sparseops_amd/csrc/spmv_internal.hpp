@@ -9,9 +9,14 @@ namespace spops {
 
 constexpr int kCsrStreamNnz = 512;     // entries staged per wave stream block
 constexpr int64_t kCsrLongRow = 2048;  // longer rows: one workgroup each
-constexpr int64_t kSellOvfMin = 64;    // rows shorter than this never leave their chunk
 constexpr int kBsrMaxBlock = 256;      // bR * bC limit of the wave-per-block-row kernel
 constexpr int64_t kRowSegment = (int64_t)1 << 24;   // entries per buffer resource of a long row
+
+// Column-sliced stream layout (SELL-C-sigma, SELL-C-R, ELL)
+constexpr int kXcds = 8;                  // gfx950: workgroup g runs on XCD g % 8
+constexpr int64_t kSliceBytesDefault = (int64_t)2 << 20;   // x bytes per slice (one XCD's L2 share)
+// entries per lane of a multi-row task: 16 (8 for FP64 values, which need twice the registers)
+constexpr int slice_lane_entries(int vt) { return vt == SPOPS_VALUE_F64 ? 8 : 16; }
 
 struct CsrDev {
   DevBuf rowptr, col, val, tasks, longs;
@@ -19,13 +24,29 @@ struct CsrDev {
   int ntasks = 0, nlong = 0;
 };
 
-struct SellDev {
-  DevBuf gcol, gval, rinfo, cbase, cwidth;   // chunked 4-entry groups
-  DevBuf optr, ocol, oval, orow;             // overflow rows (CSR)
-  int64_t nchunks = 0;
-  int Cd = 32;                               // device chunk height
-  int n_olong = 0, n_ovec = 0;
-  int64_t C = 0, sigma = 0;                  // the reference parameters
+// The columns are cut into P slices of about equal entry count; slice s is served by the
+// XCDs q with q % P == s, so each XCD gathers from one slice of x only and that slice stays
+// in its L2.  Inside a slice every row contributes a segment (its entries in the slice,
+// or one dummy entry with column -1 when it has none), stored packed in row order.  A
+// wave task covers consecutive rows and up to 64 K entries (K = slice_lane_entries): lane
+// l loads entries K l .. K l + K - 1 with 128-bit loads and sums the row pieces it holds;
+// a segmented scan across lanes joins rows that span lanes.  Per task and lane one word
+// tells which of the lane's entries start a row (low K bits) and how many rows start in
+// earlier lanes (high 16 bits), so a row's index is r0 + that count + its rank in the
+// lane.  Task entry ranges start 16-byte aligned (dummy entries pad the gaps).  Segments
+// longer than 64 K entries are wave tasks of their own; rows longer than kCsrLongRow are
+// workgroup tasks over all of x.  Each slice writes one double partial per row; a second
+// pass sums the P partials (P == 1: tasks write y directly).
+struct SliceDev {
+  DevBuf col, val;            // entries, slice-major, row order inside a slice
+  DevBuf meta;                // uint32 [task][64]: row-start bits | rows started before << 16
+  DevBuf tasks;               // int4 {r0, e0 (inside the slice), ne, s << 16 | kind << 24}
+  DevBuf lptr, lcol, lval, lrow;   // rows longer than kCsrLongRow: one workgroup each
+  DevBuf part;                // double [P][n_rows] (P > 1)
+  int P = 1, nlong = 0, ntasks = 0;
+  int64_t sbase[kXcds] = {0};
+  int qoff[kXcds + 1] = {0};
+  int64_t C = 0, sigma = 0;   // the reference parameters
   std::vector<int32_t> ref_reorder, ref_chunk_len;
 };
 
@@ -41,10 +62,11 @@ struct spops_matrix_s {
   int64_t n_rows = 0, n_cols = 0, nnz = 0;
   int64_t dev_bytes = 0, algo_bytes = 0, slots = 0, ref_nchunks = 0, ell_width = 0;
   spops::CsrDev csr;
-  spops::SellDev sell;
+  spops::SliceDev sell;
   spops::BsrDev bsr;
 };
 
 namespace spops {
+extern int64_t g_slice_bytes;   // x bytes per slice for matrices created from now on
 void launch_spmv(const spops_matrix_s &M, double alpha, const void *x, double beta, void *y, hipStream_t s);
 }

@@ -79,6 +79,11 @@ class SpMatrix:
         L.call("spops_spmv", self._h, float(alpha), x.data_ptr(), float(beta), y.data_ptr(), s)
         return y
 
+    @property
+    def slices(self) -> int:
+        """Column slices of the SELL/ELL device layout."""
+        return self.info[12] if self.format in ("sell", "ell") else 1
+
     def layout(self, which: int) -> np.ndarray:
         n = self.n_rows if which == 0 else self.info[14]
         out = np.zeros(n, dtype=np.int32)
@@ -145,6 +150,23 @@ def sell_from_chunks(n_rows, n_cols, nnz, C, reorder, chunk_len, chunk_cols, chu
     L.call("spops_sell_create_from_chunks", ctypes.byref(h), n_rows, n_cols, nnz, C, nch, it,
            reorder.ctypes.data, chunk_len.ctypes.data, cp, vp, in_vt, _vt(value_type))
     return SpMatrix(h)
+
+
+class slice_bytes:
+    """Context manager (or plain call) setting the bytes of x per column slice of SELL/ELL
+    matrices created while it is active (library default 2 MiB, one XCD's share of L2)."""
+
+    def __init__(self, nbytes: int):
+        old = ctypes.c_int64(0)
+        L.call("spops_set_tuning", L.TUNE_SLICE_BYTES, int(nbytes), ctypes.byref(old))
+        self.old = old.value
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        L.call("spops_set_tuning", L.TUNE_SLICE_BYTES, self.old, None)
+        return False
 
 
 def SpOps(alpha: float, A: SpMatrix, x: torch.Tensor, beta: float, y: torch.Tensor):

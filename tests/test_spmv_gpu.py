@@ -166,9 +166,65 @@ def test_beta_zero_does_not_read_y(mats, dev):
     np.testing.assert_allclose(y, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
 
 
+def test_local_columns_stay_unsliced(dev):
+    """A banded matrix: each eighth of the rows touches a narrow band of x, so the layout
+    keeps one slice even when x is larger than the per-slice budget."""
+    n = 1 << 16
+    cols = np.sort((np.arange(n)[:, None] + np.arange(-3, 4)[None, :]) % n, axis=1)
+    A = Csr(n, n, (np.arange(n + 1) * 7).astype(np.int32), cols.reshape(-1).astype(np.int32),
+            np.random.default_rng(3).standard_normal(n * 7))
+    with S.slice_bytes(64 << 10):                  # x is 256 KiB in FP32
+        M = S.csr_to_sell_c_sigma(A, 1024, 32, "f32")
+        R = S.csr_to_sell_c_sigma(powerlaw(n, lmax=500, seed=2), 1024, 32, "f32")
+    assert M.slices == 1 and R.slices == 4
+    _check(A, M, uniform_x(n, 9), _y0(n), "f32", dev=dev)
+
+
 def test_empty_and_tiny(dev):
     for n in (1, 2, 63, 65):
         A = Csr(n, n, np.zeros(n + 1, np.int32), np.zeros(0, np.int32), np.zeros(0))
         for M in (S.csr(A), S.csr_to_sell_c_sigma(A, 32, 32), S.csr_to_ell(A), S.csr_to_bsr(A, 4, 4)):
             y = _run(M, np.ones(n), np.ones(n), dev, 1.0, 0.5)
             np.testing.assert_array_equal(y, 0.5)
+
+
+def _skewed_matrix(seed=9):
+    """Half of every row's entries in columns [0, 64), the rest spread over 40000 columns:
+    the equal-entry column cuts of the sliced layout crowd into the first 64 columns."""
+    rng = np.random.default_rng(seed)
+    n_rows, n_cols = 5000, 40000
+    lens = rng.integers(0, 24, n_rows)
+    lens[7], lens[8], lens[4999] = 2500, 300, 64
+    cols = []
+    for L in lens:
+        h = min(L // 2, 64)
+        a = rng.choice(64, size=h, replace=False)
+        b = 64 + rng.choice(n_cols - 64, size=L - h, replace=False)
+        cols.append(np.sort(np.concatenate([a, b])))
+    rp = np.zeros(n_rows + 1, np.int64)
+    np.cumsum(lens, out=rp[1:])
+    cols = np.concatenate(cols)
+    return Csr(n_rows, n_cols, rp.astype(np.int32), cols.astype(np.int32), rng.standard_normal(cols.size))
+
+
+@pytest.mark.parametrize("P", [1, 2, 4, 8])
+@pytest.mark.parametrize("vt", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("name", ["pl", "edge", "skew"])
+@pytest.mark.parametrize("fmt", ["sell", "ell"])
+def test_column_slices(mats, name, vt, P, fmt, dev):
+    """The XCD-sliced SELL/ELL layout at every slice count, including the partial-sum pass."""
+    A = mats[name] if name != "skew" else _skewed_matrix()
+    # the slice count follows the x bytes (in 128-byte lines) one eighth of the rows touches;
+    # these matrices have random columns, so an eighth of the rows touches all of x
+    xb = -(-A.n_cols * (8 if vt == "f64" else 4) // 128) * 128
+    with S.slice_bytes(-(-xb // P)):
+        M = S.csr_to_sell_c_sigma(A, 256, 32, vt) if fmt == "sell" else S.csr_to_ell(A, vt)
+    assert M.slices == P
+    x = uniform_x(A.n_cols, 8)
+    _check(A, M, x, _y0(A.n_rows), vt, dev=dev)
+    # beta == 0 must not read y, also through the partial-sum pass
+    y = _run(M, x, np.full(A.n_rows, np.nan), dev, 1.0, 0.0)
+    assert np.all(np.isfinite(y))
+    # repeated calls reuse the partial buffer: same result bit for bit
+    y2 = _run(M, x, np.full(A.n_rows, np.nan), dev, 1.0, 0.0)
+    np.testing.assert_array_equal(y, y2)
