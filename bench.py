@@ -5,8 +5,13 @@
 A step is one y = A*x of that matrix with A, x, y resident in HBM.  The headline line is the
 FP32-value SELL-C-sigma (C = 32, sigma = 1024) SpMV, in achieved GB/s of algorithmic bytes
 ((n+1)*4 + nnz*(4+4) + n*4 (x) + n*4 (y) + n*4 (reorder) + chunks*4); the FP64 CSR drop-in,
-FP64 SELL and BF16 SELL run on the same matrix and are reported beside it with their error
-against an FP64 product (scipy, float64).
+FP64 SELL, BF16 SELL and FP32 CSR run on the same matrix and are reported beside it with
+their error against an FP64 product (scipy, float64).  The roofline object covers one SpMV
+of the headline variant (its two launches: the column-slice kernel and the slice-sum pass),
+timed with HIP events on the launch stream; its HBM traffic comes from the committed
+rocprofv3 counter summary of the same command.  A secondary LeSpGEMM measurement (C = A*A,
+R-MAT, MinHash-LSH clustering, GFLOP/s) is reported with the reference's OpenMP pipeline
+timed on the same host.
 
 Prints ONE JSON line (rank 0, N = 1).  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -87,6 +92,62 @@ def cpu_baseline(A, seconds, threads):
     return json.loads(out.stdout.strip().splitlines()[-1])
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmv_sell_f32_powerlaw_n2097152_hbm_traffic.txt")
+
+
+def committed_traffic():
+    """HBM bytes per SpMV from the committed counter summary: per kernel FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) scaled by the copy-kernel calibration factors."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    vals = {}
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_spmv"):
+                vals["bytes"] = float(line.split("=")[1])
+    return vals.get("bytes")
+
+
+def lespgemm_measure(scale, iterations, cpu_threads, cpu):
+    """LeSpGEMM C = A*A on R-MAT(scale, edge factor 16), k=64, bands=16, seed=7,
+    cluster_size=8, hc_v=0: GPU pipeline and, unless cpu is False, the reference's own
+    OpenMP pipeline (kernel 3) on the same matrix."""
+    import torch
+    from sparseops_amd import lespgemm as LG
+    from sparseops_amd.gen import rmat
+    from sparseops_amd.io import write_csr_bin
+    A = rmat(scale, 16, seed=0)
+    r = LG.lespgemm(A, A, iterations=iterations)
+    gpu_ms = sum(r["runs_ms"]) / len(r["runs_ms"])
+    out = {"workload": f"R-MAT scale {scale} (n={A.n_rows}, nnz={A.nnz}), C = A*A, kernel 3, cluster_size 8, "
+                       f"k=64, bands=16, seed=7, hc_v=0",
+           "flop": int(r["flop"]), "gpu_ms": round(gpu_ms, 3), "gpu_gflops": round(r["flop"] / gpu_ms / 1e6, 2),
+           "gpu_genpairs_ms": round(r["times"]["genpairs_ms"], 2), "gpu_hc_ms": round(r["times"]["hc_ms"], 2),
+           "gpu_conversion_ms": round(r["times"]["conversion_ms"], 2), "nnz_C": int(len(r["C_col"])),
+           "clusters": int(r["nclusters"])}
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if cpu and os.path.exists(exe):
+        with tempfile.TemporaryDirectory(dir="/tmp") as d:
+            path = os.path.join(d, "A.csr")
+            write_csr_bin(path, A)
+            env = dict(os.environ, OMP_NUM_THREADS=str(cpu_threads), OMP_PLACES="cores", OMP_PROC_BIND="spread")
+            p = subprocess.run([exe, "bench_lespgemm", path, path, str(cpu_threads), "3", "64", "16", "7", "8", "3"],
+                               capture_output=True, text=True, env=env, timeout=900)
+        if p.returncode == 0:
+            c = json.loads(p.stdout.strip().splitlines()[-1])
+            out["cpu_reference"] = {"gflops": round(c["gflops"], 3), "lespgemm_ms": c["lespgemm_ms"],
+                                    "genpairs_ms": c["genpairs_ms"], "hc_ms": c["hc_ms"],
+                                    "conversion_ms": c["conversion_ms"], "threads": c["threads"],
+                                    "kind": "reference", "iterations": 3}
+            out["gpu_over_cpu"] = round(out["gpu_gflops"] / max(c["gflops"], 1e-12), 2)
+            # cluster-C column counts of both pipelines (equal when clusterings and structure match)
+            out["nnzc_gpu"], out["nnzc_cpu"] = int(r["stats"]["nnzc"]), int(c["nnzc"])
+        else:
+            log(f"cpu lespgemm failed: {p.stderr[-400:]}")
+    torch.cuda.synchronize()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +159,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variants", default="sell_f32,csr_f64,sell_f64,sell_bf16,csr_f32",
                     help="comma list of <fmt>_<value type>; the first one is the headline")
+    ap.add_argument("--lespgemm-scale", type=int, default=16, help="R-MAT scale of the LeSpGEMM line (0: skip)")
+    ap.add_argument("--lespgemm-iters", type=int, default=5)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -143,7 +206,9 @@ def main():
         yh = y.double().cpu().numpy()
         r = {"wall_ms": wall_ms, "kernel_ms": kern_ms, "algo_bytes": M.algo_bytes,
              "gbs": M.algo_bytes / (wall_ms * 1e-3) / 1e9, "kernel_gbs": M.algo_bytes / (kern_ms * 1e-3) / 1e9,
-             "device_bytes": M.device_bytes, "build_s": time.time() - t}
+             "device_bytes": M.device_bytes, "build_s": time.time() - t,
+             "kernels": (["k_slice_spmv", "k_slice_reduce"] if M.format in ("sell", "ell") and M.slices > 1
+                         else ["k_slice_spmv"] if M.format in ("sell", "ell") else ["k_csr_spmv"])}
         r.update(errors(yh, y64, absrow))
         res[name] = r
         log(f"{name}: {r['kernel_ms'] * 1e3:.1f} us/kernel, {r['kernel_gbs']:.0f} GB/s, "
@@ -164,6 +229,13 @@ def main():
                              f"{c['seconds_per_iter'] * 1e3:.2f} ms each"}
         log(f"cpu baseline done in {time.time() - t:.1f}s: {cpu}")
 
+    traffic = committed_traffic() if head_name == "sell_f32" and args.n == 1 << 21 else None
+    lg = None
+    if args.lespgemm_scale > 0:
+        t = time.time()
+        lg = lespgemm_measure(args.lespgemm_scale, args.lespgemm_iters, args.cpu_threads, not args.no_cpu)
+        log(f"lespgemm done in {time.time() - t:.1f}s: {lg}")
+
     out = {
         "metric": METRIC,
         "value": round(head["gbs"], 1),
@@ -182,7 +254,9 @@ def main():
                    "n": A.n_rows, "nnz": A.nnz, "C": 32, "sigma": 1024},
         "pct_hbm_peak": round(100.0 * head["gbs"] / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(head["kernel_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(head["kernel_gbs"] / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(head["kernel_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algo_bytes": head["algo_bytes"], "kernel_us": round(head["kernel_ms"] * 1e3, 2),
+                     "kernels": head.get("kernels")},
         "cpu_baseline": cpu,
         "rel_err_vs_fp64": {k: {e: v[e] for e in ("max_rel_err", "rms_rel_err", "normwise_rel_err")}
                             for k, v in res.items()},
@@ -190,6 +264,7 @@ def main():
                          "pct_hbm_peak": round(100 * v["kernel_gbs"] / HBM_PEAK_GBS, 2)} for k, v in res.items()},
         "fp32_sell_over_fp64_csr": (round(res["csr_f64"]["kernel_ms"] / res["sell_f32"]["kernel_ms"], 3)
                                     if "csr_f64" in res and "sell_f32" in res else None),
+        "lespgemm": lg,
     }
     print(json.dumps(out), flush=True)
 

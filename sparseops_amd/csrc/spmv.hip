@@ -261,14 +261,38 @@ __device__ __forceinline__ void slice_store(const SliceArgs &a, bool direct, int
 // elements per 128-bit word, word b of lane l sits at word index b * L + l (L = lanes in
 // the task), so each of the K / g loads of the wave reads one contiguous block.
 template <typename E, int K>
-__device__ __forceinline__ void load_lane_run(rsrc_t r, int L, E (&out)[K]) {
+__device__ __forceinline__ void load_lane_run(rsrc_t r, int L, int l, E (&out)[K]) {
   constexpr int NB = K * (int)sizeof(E) / 16;
   static_assert(NB * 16 == K * (int)sizeof(E), "run must be a multiple of 16 bytes");
-  const int lane = lane_id();
   v4i raw[NB];
 #pragma unroll
-  for (int b = 0; b < NB; ++b) raw[b] = bload<v4i, kAuxNT>(r, lane < L ? 16u * (uint32_t)(b * L + lane) : kOOB);
+  for (int b = 0; b < NB; ++b) raw[b] = bload<v4i, kAuxNT>(r, l < L ? 16u * (uint32_t)(b * L + l) : kOOB);
   __builtin_memcpy(out, raw, sizeof(out));
+}
+
+// DPP lane moves (bound_ctrl: lanes without a source read 0; rows outside row_mask keep 0)
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ int dpp(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, true);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const int lo = dpp<CTRL, ROWS>((int)(uint32_t)b), hi = dpp<CTRL, ROWS>((int)(uint32_t)(b >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+constexpr int kDppRowShr = 0x110, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143, kDppWaveShr1 = 0x138;
+
+// One step of an inclusive segmented scan towards higher lanes: a lane whose range has no
+// segment start yet adds the value arriving from below and takes over its start flag.
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ void seg_scan_step(double &w, int &stop) {
+  const double w2 = dpp<CTRL, ROWS>(w);
+  const int f2 = dpp<CTRL, ROWS>(stop);
+  if (!stop) {
+    w += w2;
+    stop = f2;
+  }
 }
 
 template <int VT, bool DIRECT>
@@ -279,6 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_spmv(const SliceArgs a) {
   constexpr uint32_t sV = sizeof(V), sX = sizeof(X);
   constexpr int K = slice_lane_entries(VT);
   __shared__ double red[8];
+  __shared__ double rsum[kBlock / kWave][kSliceTaskRows];
   const rsrc_t rx = make_rsrc(a.x, (uint64_t)a.n_cols * sX);
   const int g = blockIdx.x;
   if (g < a.nlong_pad) {
@@ -301,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_spmv(const SliceArgs a) {
   const int4 tk = a.tasks[t];
   const int r0 = __builtin_amdgcn_readfirstlane(tk.x), ne = __builtin_amdgcn_readfirstlane(tk.z);
   const int tw = __builtin_amdgcn_readfirstlane(tk.w);
-  const int s = (tw >> 16) & 0xff, kind = tw >> 24;
+  const int R = tw & 0xffff, s = (tw >> 16) & 0xff, kind = tw >> 24;
   const long eb = a.sbase[s] + __builtin_amdgcn_readfirstlane(tk.y);
   const int *col = a.col + eb;
   const V *val = static_cast<const V *>(a.val) + eb;
@@ -316,26 +341,31 @@ __global__ __launch_bounds__(kBlock) void k_slice_spmv(const SliceArgs a) {
   }
   // lane l: entries K l .. K l + K - 1 (dummy entries have column -1: their gather is out
   // of range and returns 0; lanes past the task load zeros and gather x[0], times 0)
-  const uint32_t mw = a.meta[(long)t * kWave + lane];
-  const int L = (ne + K - 1) / K;            // lanes holding entries; the range is L * K long
+  // Lane p holds the task's entry run ll = 63 - p, so that the pieces of a row spanning
+  // several runs flow towards higher lanes (the direction of the DPP scan below).
+  const int ll = kWave - 1 - lane;
+  const uint32_t mw = a.meta[(long)t * kWave + ll];
+  const int L = (ne + K - 1) / K;            // runs holding entries; the range is L * K long
   const rsrc_t rc = make_rsrc(col, (uint64_t)L * K * 4), rv = make_rsrc(val, (uint64_t)L * K * sV);
   int c[K];
   V v[K];
-  load_lane_run<int, K>(rc, L, c);
-  load_lane_run<V, K>(rv, L, v);
+  load_lane_run<int, K>(rc, L, ll, c);
+  load_lane_run<V, K>(rv, L, ll, v);
   X xv[K];
 #pragma unroll
   for (int i = 0; i < K; ++i) xv[i] = bload<X>(rx, (uint32_t)c[i] * sX);
-  // local pass: pieces that start and end inside the lane are stored at once; the piece
-  // before the lane's first row start (carry) and the one after its last (tail) wait.
+  // local pass: pieces that start and end inside the lane go to the task's row sums in LDS
+  // at once; the piece before the lane's first row start (carry) and the one after its
+  // last (tail) wait for the scan.
+  double *rs = rsum[wave_id()];
   const uint32_t hbits = mw & 0xffffu;
-  int row = r0 + (int)(mw >> 16) - 1;
+  int row = (int)(mw >> 16) - 1;             // task-relative
   double cur = 0.0, carry = 0.0;
   bool seen = false;
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     if (hbits & (1u << i)) {
-      if (seen) slice_store<X>(a, DIRECT, s, row, cur);
+      if (seen) rs[row] = cur;
       else carry = cur;
       cur = 0.0;
       ++row;
@@ -344,21 +374,22 @@ __global__ __launch_bounds__(kBlock) void k_slice_spmv(const SliceArgs a) {
     cur = __builtin_fma((double)T::cvt(v[i]), (double)xv[i], cur);
   }
   if (!seen) carry = cur;
-  // W_l = carry_l + (a row starts in lane l ? 0 : W_{l+1}): reverse segmented scan
+  // W_p = carry_p + (a row starts in lane p ? 0 : W_{p-1}); the row whose last start is in
+  // lane p ends with cur_p + W_{p-1}
   double W = carry;
-  bool stop = seen;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const double w2 = __shfl_down(W, d, kWave);
-    const int f2 = __shfl_down((int)stop, d, kWave);
-    if (lane + d < kWave && !stop) {
-      W += w2;
-      stop = f2 != 0;
-    }
-  }
-  double next = __shfl_down(W, 1, kWave);
-  if (lane == kWave - 1) next = 0.0;
-  if (seen) slice_store<X>(a, DIRECT, s, row, cur + next);
+  int stop = seen ? 1 : 0;
+  seg_scan_step<kDppRowShr + 1>(W, stop);
+  seg_scan_step<kDppRowShr + 2>(W, stop);
+  seg_scan_step<kDppRowShr + 4>(W, stop);
+  seg_scan_step<kDppRowShr + 8>(W, stop);
+  seg_scan_step<kDppRowBcast15, 0xa>(W, stop);
+  seg_scan_step<kDppRowBcast31, 0xc>(W, stop);
+  const double prev = dpp<kDppWaveShr1>(W);
+  if (seen) rs[row] = cur + prev;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int j = lane; j < R; j += kWave) slice_store<X>(a, DIRECT, s, r0 + j, rs[j]);
 }
 
 // y = alpha * (sum over slices of part) + beta * y, slices added in order 0..P-1.
@@ -636,7 +667,7 @@ void build_slice_layout(spops_matrix_s &M, const RowSource &src, int vt) {
     };
     auto close = [&]() {
       if (rows > 0) {
-        stasks[s].push_back(make_int4((int)r0, (int)(e0 - s0), ne, s << 16));
+        stasks[s].push_back(make_int4((int)r0, (int)(e0 - s0), ne, rows | (s << 16)));
         sents[s].push_back(ne);
         // lane words: row-start bits and rows started in earlier lanes
         uint32_t before = 0;
@@ -669,7 +700,7 @@ void build_slice_layout(spops_matrix_s &M, const RowSource &src, int vt) {
         continue;
       }
       const int64_t le = l > 0 ? l : 1;
-      if (ne + le > kWave * K) close();
+      if (ne + le > kWave * K || rows == kSliceTaskRows) close();
       if (rows == 0) {
         r0 = r;
         e0 = (int64_t)col.size();

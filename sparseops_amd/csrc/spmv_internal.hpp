@@ -15,6 +15,7 @@ constexpr int64_t kRowSegment = (int64_t)1 << 24;   // entries per buffer resour
 // Column-sliced stream layout (SELL-C-sigma, SELL-C-R, ELL)
 constexpr int kXcds = 8;                  // gfx950: workgroup g runs on XCD g % 8
 constexpr int64_t kSliceBytesDefault = (int64_t)2 << 20;   // x bytes per slice (one XCD's L2 share)
+constexpr int kSliceTaskRows = 256;       // rows of a multi-row wave task (row sums staged in LDS)
 // entries per lane of a multi-row task: 16 (8 for FP64 values, which need twice the registers)
 constexpr int slice_lane_entries(int vt) { return vt == SPOPS_VALUE_F64 ? 8 : 16; }
 
@@ -30,7 +31,8 @@ struct CsrDev {
 // or one dummy entry with column -1 when it has none), stored packed in row order.  A
 // wave task covers consecutive rows and up to 64 K entries (K = slice_lane_entries): lane
 // l loads entries K l .. K l + K - 1 with 128-bit loads and sums the row pieces it holds;
-// a segmented scan across lanes joins rows that span lanes.  Per task and lane one word
+// a segmented scan across lanes joins rows that span lanes; row sums are staged in LDS and
+// written out coalesced (a task covers at most kSliceTaskRows rows).  Per task and lane one word
 // tells which of the lane's entries start a row (low K bits) and how many rows start in
 // earlier lanes (high 16 bits), so a row's index is r0 + that count + its rank in the
 // lane.  Task entry ranges start 16-byte aligned (dummy entries pad the gaps).  Segments
@@ -40,7 +42,7 @@ struct CsrDev {
 struct SliceDev {
   DevBuf col, val;            // entries, slice-major, row order inside a slice
   DevBuf meta;                // uint32 [task][64]: row-start bits | rows started before << 16
-  DevBuf tasks;               // int4 {r0, e0 (inside the slice), ne, s << 16 | kind << 24}
+  DevBuf tasks;               // int4 {r0, e0 (inside the slice), ne, rows | s << 16 | kind << 24}
   DevBuf lptr, lcol, lval, lrow;   // rows longer than kCsrLongRow: one workgroup each
   DevBuf part;                // double [P][n_rows] (P > 1)
   int P = 1, nlong = 0, ntasks = 0;
