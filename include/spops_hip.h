@@ -218,9 +218,10 @@ int spops_cluster_destroy(spops_cluster A);
 
 /* C = A * B, cluster-wise with the hybrid accumulator (LeSpGEMM_VLength kernel 3,
  * src/spgemm.cpp:545-680).  B on the device: int64 rowptr/col, values of A's value type.
- * C's columns ascend inside each cluster.  stats (may be NULL): [1]=nnzc [2]=nnzv
- * [3]=LDS-path clusters [4]=HBM-path clusters [5..8]=setup/symbolic/malloc/numeric in us
- * [9]=cluster-wise products. */
+ * C's columns ascend inside each cluster.  B's rows need not be sorted (an unsorted B is
+ * sorted into a temporary copy).  stats (may be NULL): [0]=1 if B was sorted into a copy
+ * [1]=nnzc [2]=nnzv [3]=small clusters (one LDS block each) [4]=large clusters (chunked LDS
+ * accumulator) [5..8]=setup/symbolic/malloc/numeric in us [9]=cluster-wise products. */
 int spops_lespgemm(spops_cluster A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *d_browptr,
                    const int64_t *d_bcol, const void *d_bval, spops_cluster *C, int64_t stats[16], void *stream);
 

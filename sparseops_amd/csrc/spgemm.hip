@@ -22,9 +22,11 @@
 //   * per-cluster work flop_c = sum over its columns of nnz(B row) bins the clusters:
 //       small  (flop_c <= kSmallFlop, output block fits LDS): one workgroup, LDS hash
 //              table of columns, LDS bitonic sort, LDS f64 accumulator, coalesced store;
-//       large  : persistent workgroups, each owning a column bitmap in HBM; the bitmap
-//              gives the sorted column list and O(1) column ranks (word prefix counts),
-//              values accumulate with HBM f64 atomics into the zeroed output block.
+//       large  : one workgroup per cluster, all in LDS: bitmap passes over the B rows give
+//              the count / min / max and then C's sorted columns; the numeric pass takes
+//              those columns in chunks that fit an LDS accumulator (LDS hash column -> rank)
+//              and reads only the B entries inside each chunk's column range (B rows are
+//              sorted; an unsorted B is sorted into a copy first).
 //   * symbolic (counts, min/max column) and numeric (columns + values) are separate
 //     passes; C's offsets come from a device scan in between.
 #include <algorithm>
@@ -43,13 +45,6 @@ inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per
 __global__ __launch_bounds__(kBlock) void k_narrow(const int64_t *__restrict__ in, int64_t n, int32_t *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i < n) out[i] = (int32_t)in[i];
-}
-
-// Loads that bypass the per-CU vector L1, for words other waves of the workgroup updated
-// with L2 atomics or stores in the same kernel.
-template <typename T>
-__device__ __forceinline__ T ld_l2(const T *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -284,141 +279,260 @@ __global__ __launch_bounds__(kBlock) void k_num_small(const int32_t *__restrict_
 }
 
 // ---------------------------------------------------------------------------------------------
-// large clusters: persistent workgroups, one HBM column bitmap per workgroup
+// large clusters: one workgroup each, everything in LDS.
+//   symbolic: an LDS bitmap of kLargeTileBits columns per pass over the cluster's B rows
+//             gives the count and the min / max column; a second kernel (emit) builds the
+//             same bitmaps over [min, max] and writes C's sorted columns (word popcount
+//             prefix = rank).
+//   numeric:  C's columns of the cluster are taken in chunks of m <= kLargeAccEntries / csz;
+//             a chunk's columns go into an LDS hash (column -> rank inside the chunk) and
+//             its csz-wide values accumulate in LDS; each B row contributes only its entries
+//             inside the chunk's column range (binary search, B rows are sorted).
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ void bitmap_build(uint32_t *bm, int64_t t0, int64_t t1, const int32_t *__restrict__ acol,
-                                             const int64_t *__restrict__ brp, const int32_t *__restrict__ bcol,
-                                             int *cnt, int *mn, int *mx) {
-  for_each_product(t0, t1, acol, brp, [&](int64_t, int64_t k) {
-    const int32_t key = bcol[k];
-    const uint32_t bit = 1u << (key & 31);
-    const uint32_t old = atomicOr(&bm[key >> 5], bit);
-    if (!(old & bit)) {
-      if (cnt) atomicAdd(cnt, 1);
-      atomicMin(mn, key);
-      atomicMax(mx, key);
+
+// [k0, k1) narrowed to the entries of a sorted B row with columns in [lo, hi]
+__device__ __forceinline__ void row_range(const int32_t *__restrict__ bcol, int64_t &k0, int64_t &k1, int32_t lo,
+                                          int32_t hi) {
+  int64_t a = k0, b = k1;
+  while (a < b) {                            // first column >= lo
+    const int64_t m = (a + b) >> 1;
+    if (bcol[m] < lo) a = m + 1; else b = m;
+  }
+  int64_t c = a, d = k1;
+  while (c < d) {                            // first column > hi
+    const int64_t m = (c + d) >> 1;
+    if (bcol[m] <= hi) c = m + 1; else d = m;
+  }
+  k0 = a;
+  k1 = c;
+}
+
+// set the bits of the cluster's product columns inside [lo, lo + kLargeTileBits) (all of
+// B's columns when whole_rows)
+__device__ __forceinline__ void tile_bits(uint32_t *bm, int64_t t0, int64_t t1, const int32_t *__restrict__ acol,
+                                          const int64_t *__restrict__ brp, const int32_t *__restrict__ bcol, int32_t lo,
+                                          int32_t hi, bool whole_rows) {
+  for (int64_t t = t0 + wave_id(); t < t1; t += kBlock / kWave) {
+    const int32_t b = acol[t];
+    int64_t k0 = brp[b], k1 = brp[b + 1];
+    if (!whole_rows) row_range(bcol, k0, k1, lo, hi);
+    for (int64_t k = k0 + lane_id(); k < k1; k += kWave) {
+      const int32_t off = bcol[k] - lo;
+      atomicOr(&bm[off >> 5], 1u << (off & 31));
     }
-  });
+  }
 }
 
-__device__ __forceinline__ void bitmap_clear(uint32_t *bm, int mn, int mx) {
-  if (mx < mn) return;
-  for (int w = (mn >> 5) + (int)threadIdx.x; w <= (mx >> 5); w += kBlock) bm[w] = 0u;
-}
-
-__global__ __launch_bounds__(kBlock) void k_sym_large(const int32_t *__restrict__ list, int64_t nlist,
-                                                      unsigned int *__restrict__ next, uint32_t *__restrict__ bitmaps,
-                                                      int64_t words, const int64_t *__restrict__ arp,
+__global__ __launch_bounds__(kBlock) void k_sym_large(const int32_t *__restrict__ list, const int64_t *__restrict__ arp,
                                                       const int32_t *__restrict__ acol, const int64_t *__restrict__ brp,
-                                                      const int32_t *__restrict__ bcol, int64_t *__restrict__ nz,
-                                                      int64_t *__restrict__ cmin, int64_t *__restrict__ cmax) {
+                                                      const int32_t *__restrict__ bcol, int32_t b_cols,
+                                                      int64_t *__restrict__ nz, int64_t *__restrict__ cmin,
+                                                      int64_t *__restrict__ cmax) {
+  __shared__ uint32_t bm[kLargeTileBits / 32];
   __shared__ int cnt, mn, mx;
-  __shared__ int64_t item;
-  uint32_t *bm = bitmaps + (int64_t)blockIdx.x * words;
-  for (;;) {
-    if (threadIdx.x == 0) {
-      item = (int64_t)atomicAdd(next, 1u);
-      cnt = 0; mn = INT32_MAX; mx = -1;
+  const int64_t c = list[blockIdx.x];
+  const int64_t t0 = arp[c], t1 = arp[c + 1];
+  if (threadIdx.x == 0) { cnt = 0; mn = INT32_MAX; mx = -1; }
+  const bool whole = b_cols <= kLargeTileBits;
+  for (int32_t lo = 0; lo < b_cols; lo += kLargeTileBits) {
+    const int32_t hi = (int32_t)min((int64_t)lo + kLargeTileBits, (int64_t)b_cols) - 1;
+    for (int w = threadIdx.x; w < kLargeTileBits / 32; w += kBlock) bm[w] = 0u;
+    __syncthreads();
+    tile_bits(bm, t0, t1, acol, brp, bcol, lo, hi, whole);
+    __syncthreads();
+    int n = 0, first = INT32_MAX, last = -1;
+    for (int w = threadIdx.x; w < kLargeTileBits / 32; w += kBlock) {
+      const uint32_t x = bm[w];
+      if (x) {
+        n += __popc(x);
+        first = min(first, lo + w * 32 + __ffs(x) - 1);
+        last = max(last, lo + w * 32 + 31 - __clz(x));
+      }
+    }
+    if (n) {
+      atomicAdd(&cnt, n);
+      atomicMin(&mn, first);
+      atomicMax(&mx, last);
     }
     __syncthreads();
-    const int64_t it = item;
-    if (it >= nlist) break;
-    const int64_t c = list[it];
-    bitmap_build(bm, arp[c], arp[c + 1], acol, brp, bcol, &cnt, &mn, &mx);
+  }
+  if (threadIdx.x == 0) {
+    nz[c] = cnt;
+    cmin[c] = cnt ? mn : 0;
+    cmax[c] = cnt ? mx : 0;
+  }
+}
+
+// C's sorted columns of each large cluster
+__global__ __launch_bounds__(kBlock) void k_emit_large(const int32_t *__restrict__ list, const int64_t *__restrict__ arp,
+                                                       const int32_t *__restrict__ acol, const int64_t *__restrict__ brp,
+                                                       const int32_t *__restrict__ bcol, const int64_t *__restrict__ crp,
+                                                       const int64_t *__restrict__ cmin, const int64_t *__restrict__ cmax,
+                                                       int32_t *__restrict__ ccol) {
+  __shared__ uint32_t bm[kLargeTileBits / 32];
+  __shared__ int32_t wsum[kBlock / kWave];
+  const int64_t c = list[blockIdx.x];
+  const int64_t t0 = arp[c], t1 = arp[c + 1];
+  const int32_t cmn = (int32_t)cmin[c], cmx = (int32_t)cmax[c];
+  if (crp[c + 1] == crp[c]) return;
+  int64_t out = crp[c];
+  // each thread owns kLargeTileBits / 32 / kBlock consecutive words of the tile
+  constexpr int WPT = kLargeTileBits / 32 / kBlock;
+  for (int32_t lo = cmn; lo <= cmx; lo += kLargeTileBits) {
+    const int32_t hi = (int32_t)min((int64_t)lo + kLargeTileBits - 1, (int64_t)cmx);
+    for (int w = threadIdx.x; w < kLargeTileBits / 32; w += kBlock) bm[w] = 0u;
     __syncthreads();
-    const int lo = mn, hi = mx, n = cnt;
-    if (threadIdx.x == 0) {
-      nz[c] = n;
-      cmin[c] = n ? lo : 0;
-      cmax[c] = n ? hi : 0;
+    tile_bits(bm, t0, t1, acol, brp, bcol, lo, hi, cmx - cmn < kLargeTileBits);
+    __syncthreads();
+    uint32_t word[WPT];
+    int32_t mine = 0;
+#pragma unroll
+    for (int i = 0; i < WPT; ++i) {
+      word[i] = bm[threadIdx.x * WPT + i];
+      mine += __popc(word[i]);
     }
-    bitmap_clear(bm, lo, hi);
+    int32_t inc = mine;                       // block exclusive scan of per-thread counts
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int32_t o = __shfl_up(inc, d, kWave);
+      if (lane_id() >= d) inc += o;
+    }
+    if (lane_id() == kWave - 1) wsum[wave_id()] = inc;
+    __syncthreads();
+    int32_t before = 0, total = 0;
+    for (int q = 0; q < kBlock / kWave; ++q) {
+      if (q < wave_id()) before += wsum[q];
+      total += wsum[q];
+    }
+    int64_t r = out + before + inc - mine;
+#pragma unroll
+    for (int i = 0; i < WPT; ++i) {
+      uint32_t bits = word[i];
+      while (bits) {
+        const int b = __ffs(bits) - 1;
+        ccol[r++] = lo + (int32_t)(threadIdx.x * WPT + i) * 32 + b;
+        bits &= bits - 1;
+      }
+    }
+    out += total;
     __syncthreads();
   }
 }
 
 template <typename V>
-__global__ __launch_bounds__(kBlock) void k_num_large(const int32_t *__restrict__ list, int64_t nlist,
-                                                      unsigned int *__restrict__ next, uint32_t *__restrict__ bitmaps,
-                                                      int32_t *__restrict__ prefixes, int64_t words,
-                                                      const int64_t *__restrict__ arp, const int64_t *__restrict__ arpv,
-                                                      const int32_t *__restrict__ csz_a, const int32_t *__restrict__ acol,
-                                                      const V *__restrict__ aval, const int64_t *__restrict__ brp,
-                                                      const int32_t *__restrict__ bcol, const V *__restrict__ bval,
-                                                      const int64_t *__restrict__ crp, const int64_t *__restrict__ crpv,
-                                                      const int64_t *__restrict__ cmin, const int64_t *__restrict__ cmax,
-                                                      int32_t *__restrict__ ccol, V *__restrict__ cval) {
-  __shared__ int mn, mx;
-  __shared__ int64_t item;
-  __shared__ int32_t wsum[kBlock / kWave];
-  uint32_t *bm = bitmaps + (int64_t)blockIdx.x * words;
-  int32_t *pre = prefixes + (int64_t)blockIdx.x * words;
-  for (;;) {
-    if (threadIdx.x == 0) {
-      item = (int64_t)atomicAdd(next, 1u);
-      mn = INT32_MAX; mx = -1;
+__global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__restrict__ list, const int64_t *__restrict__ arp,
+                                                           const int64_t *__restrict__ arpv, const int32_t *__restrict__ csz_a,
+                                                           const int32_t *__restrict__ acol, const V *__restrict__ aval,
+                                                           const int64_t *__restrict__ brp, const int32_t *__restrict__ bcol,
+                                                           const V *__restrict__ bval, const int64_t *__restrict__ crp,
+                                                           const int64_t *__restrict__ crpv, const int32_t *__restrict__ ccol,
+                                                           V *__restrict__ cval) {
+  __shared__ V acc[kLargeAccEntries];
+  __shared__ int32_t hkey[kLargeHash];
+  __shared__ int16_t hval[kLargeHash];
+  __shared__ int32_t cursor[kLargeCursors];   // per A column: B entries consumed by earlier chunks
+  constexpr int W = kLargeBlock / kWave;
+  const int64_t c = list[blockIdx.x];
+  const int csz = csz_a[c];
+  const int64_t a0 = arp[c], a1 = arp[c + 1], av0 = arpv[c];
+  const int64_t c0 = crp[c], nzc = crp[c + 1] - c0, cv0 = crpv[c];
+  const int M = min(kLargeAccEntries / csz, kLargeHash / 2);
+  const bool cursors = a1 - a0 <= kLargeCursors;
+  for (int i = threadIdx.x; i < kLargeCursors; i += kLargeBlock) cursor[i] = 0;
+  const int lane = lane_id();
+  for (int64_t j0 = 0; j0 < nzc; j0 += M) {
+    const int m = (int)min((int64_t)M, nzc - j0);
+    const int32_t lo = ccol[c0 + j0], hi = ccol[c0 + j0 + m - 1];
+    for (int i = threadIdx.x; i < kLargeHash; i += kLargeBlock) hkey[i] = -1;
+    for (int i = threadIdx.x; i < m * csz; i += kLargeBlock) acc[i] = V(0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += kLargeBlock) {
+      const int32_t key = ccol[c0 + j0 + i];
+      uint32_t h = hslot(key, kLargeHash - 1);
+      while (atomicCAS(&hkey[h], -1, key) != -1) h = (h + 1) & (kLargeHash - 1);
+      hval[h] = (int16_t)i;
     }
     __syncthreads();
-    const int64_t it = item;
-    if (it >= nlist) break;
-    const int64_t c = list[it];
-    const int64_t a0 = arp[c], a1 = arp[c + 1];
-    bitmap_build(bm, a0, a1, acol, brp, bcol, nullptr, &mn, &mx);
-    __syncthreads();
-    const int w0 = (int)(cmin[c] >> 5), w1 = (int)(cmax[c] >> 5);
-    // exclusive prefix of popcounts over words [w0, w1], in chunks of kBlock words
-    int32_t carry = 0;
-    for (int base = w0; base <= w1; base += kBlock) {
-      const int w = base + (int)threadIdx.x;
-      const uint32_t word = (w <= w1) ? ld_l2(&bm[w]) : 0u;
-      const int32_t pc = __popc(word);
-      int32_t inc = pc;
-#pragma unroll
-      for (int d = 1; d < kWave; d <<= 1) {
-        const int32_t o = __shfl_up(inc, d, kWave);
-        if (lane_id() >= d) inc += o;
+    for (int64_t t = a0 + wave_id(); t < a1; t += W) {
+      const int32_t b = acol[t];
+      const int64_t kb = brp[b], ke = brp[b + 1];
+      int64_t k = kb;
+      if (cursors) {
+        k = kb + cursor[t - a0];
+      } else {
+        int64_t k1 = ke;
+        row_range(bcol, k, k1, lo, hi);
       }
-      if (lane_id() == kWave - 1) wsum[wave_id()] = inc;
-      __syncthreads();
-      int32_t wofs = 0, tot = 0;
-      for (int q = 0; q < kBlock / kWave; ++q) {
-        if (q < wave_id()) wofs += wsum[q];
-        tot += wsum[q];
-      }
-      if (w <= w1) {
-        const int32_t excl = carry + wofs + inc - pc;
-        __hip_atomic_store(&pre[w], excl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t bits = word;
-        int32_t r = excl;
-        while (bits) {                            // ascending columns of this word
-          const int b = __ffs(bits) - 1;
-          ccol[crp[c] + r] = w * 32 + b;
-          bits &= bits - 1;
-          ++r;
-        }
-      }
-      carry += tot;
-      __syncthreads();
-    }
-    __syncthreads();
-    const int csz = csz_a[c];
-    const int64_t av0 = arpv[c], cbase = crpv[c];
-    for_each_product(a0, a1, acol, brp, [&](int64_t t, int64_t k) {
-      const int32_t key = bcol[k];
-      const int w = key >> 5;
-      const int64_t r = ld_l2(&pre[w]) + __popc(ld_l2(&bm[w]) & ((1u << (key & 31)) - 1u));
-      const V b = bval[k];
+      if (k >= ke) continue;
       const V *a = aval + av0 + (t - a0) * csz;
-      V *dst = cval + cbase + r * csz;
-      for (int l = 0; l < csz; ++l) {
-        const V av = a[l];
-        if (av != V(0)) atomicAdd(&dst[l], av * b);
+      // the row's entries from k on, 64 at a time, up to the first column past hi (sorted row)
+      for (;;) {
+        const int64_t kk = k + lane;
+        const int32_t key = kk < ke ? bcol[kk] : INT32_MAX;
+        const bool in = key <= hi;
+        const int n = __builtin_popcountll(__ballot(in));
+        int h = -1;
+        if (in) {                              // every such column is one of the chunk's
+          uint32_t p = hslot(key, kLargeHash - 1);
+          for (int probe = 0; probe < kLargeHash; ++probe, p = (p + 1) & (kLargeHash - 1))
+            if (hkey[p] == key) { h = (int)p; break; }
+        }
+        if (h >= 0) {
+          const V bv = bval[kk];
+          const int r = hval[h];
+          for (int l0 = 0; l0 < csz; l0 += kMaxClusterRows) {    // one pass unless csz > 16
+            V *dst = acc + r * csz + l0;
+#pragma unroll
+            for (int l = 0; l < kMaxClusterRows; ++l) {
+              if (l0 + l < csz) {
+                const V av = a[l0 + l];
+                if (av != V(0)) atomicAdd(&dst[l], av * bv);
+              }
+            }
+          }
+        }
+        k += n;
+        if (n < kWave) break;
       }
-    });
+      if (cursors && lane == 0) cursor[t - a0] = (int32_t)(k - kb);
+    }
     __syncthreads();
-    bitmap_clear(bm, (int)cmin[c], (int)cmax[c]);
+    V *out = cval + cv0 + j0 * csz;
+    for (int i = threadIdx.x; i < m * csz; i += kLargeBlock) out[i] = acc[i];
     __syncthreads();
   }
+}
+
+// flag = 1 when some row of B has columns out of ascending order
+__global__ __launch_bounds__(kBlock) void k_rows_unsorted(const int64_t *__restrict__ brp, int64_t b_rows,
+                                                          const int32_t *__restrict__ bcol, int *__restrict__ flag) {
+  const int64_t r = (int64_t)blockIdx.x * (kBlock / kWave) + wave_id();
+  if (r >= b_rows) return;
+  const int64_t k0 = brp[r], k1 = brp[r + 1];
+  bool bad = false;
+  for (int64_t k = k0 + 1 + lane_id(); k < k1; k += kWave) bad |= bcol[k] < bcol[k - 1];
+  if (__ballot(bad) && lane_id() == 0) atomicOr(flag, 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_row_keys(const int64_t *__restrict__ brp, int64_t b_rows,
+                                                     const int32_t *__restrict__ bcol, uint64_t *__restrict__ keys,
+                                                     uint64_t *__restrict__ idx) {
+  const int64_t r = (int64_t)blockIdx.x * (kBlock / kWave) + wave_id();
+  if (r >= b_rows) return;
+  for (int64_t k = brp[r] + lane_id(); k < brp[r + 1]; k += kWave) {
+    keys[k] = ((uint64_t)r << 32) | (uint32_t)bcol[k];
+    idx[k] = (uint64_t)k;
+  }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_row_sorted_fill(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ idx,
+                                                            int64_t n, const V *__restrict__ val, int32_t *__restrict__ scol,
+                                                            V *__restrict__ sval) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  scol[i] = (int32_t)(uint32_t)keys[i];
+  sval[i] = val[idx[i]];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -582,28 +696,51 @@ struct Timer {
   }
 };
 
-int persistent_slots(int64_t words) {
-  int dev = 0, cus = 0;
-  SPOPS_HIP_CHECK(hipGetDevice(&dev));
-  SPOPS_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  int64_t slots = (int64_t)std::max(cus, 1) * kLargeSlotsPerCU;
-  const int64_t budget = kLargeScratchBytes / std::max<int64_t>(words * 8, 1);   // bitmap + prefix per slot
-  return (int)std::max<int64_t>(1, std::min(slots, budget));
-}
-
 }  // namespace
 
 void lespgemm(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *brp64,
               const int64_t *bcol64, const void *bval, spops_cluster_s &C, int64_t stats[kSpgemmStats], hipStream_t s) {
   const int vt = A.value_type;
   const int64_t nc = A.rows;
+  const size_t vb = value_bytes(vt);
   Timer tm(s);
-  // B with int32 columns
-  DevBuf bcol32;
+  SPOPS_REQUIRE(b_cols < (int64_t)INT32_MAX, SPOPS_ERROR_LIMIT, "B must have < 2^31 columns");
+  // B with int32 columns, rows sorted (the large-cluster kernels binary-search B rows)
+  DevBuf bcol32, bsorted_val, flag;
   bcol32.alloc((size_t)std::max<int64_t>(b_nnz, 1) * 4);
-  if (b_nnz > 0) hipLaunchKernelGGL(k_narrow, dim3(nblk(std::max<int64_t>(b_nnz, 1), kBlock)), dim3(kBlock), 0, s, bcol64, b_nnz,
-                     bcol32.as<int32_t>());
+  flag.alloc(16);
+  SPOPS_HIP_CHECK(hipMemsetAsync(flag.p, 0, 16, s));
+  if (b_nnz > 0) {
+    hipLaunchKernelGGL(k_narrow, dim3(nblk(b_nnz, kBlock)), dim3(kBlock), 0, s, bcol64, b_nnz, bcol32.as<int32_t>());
+    hipLaunchKernelGGL(k_rows_unsorted, dim3(nblk(b_rows, kBlock / kWave)), dim3(kBlock), 0, s, brp64, b_rows,
+                       bcol32.as<int32_t>(), flag.as<int>());
+  }
   SPOPS_HIP_CHECK(hipGetLastError());
+  int unsorted = 0;
+  SPOPS_HIP_CHECK(hipMemcpyAsync(&unsorted, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  SPOPS_HIP_CHECK(hipStreamSynchronize(s));
+  if (unsorted) {
+    SPOPS_REQUIRE(b_rows < ((int64_t)1 << 31), SPOPS_ERROR_LIMIT, "B must have < 2^31 rows");
+    DevBuf keys, idx;
+    keys.alloc((size_t)b_nnz * 8);
+    idx.alloc((size_t)b_nnz * 8);
+    hipLaunchKernelGGL(k_row_keys, dim3(nblk(b_rows, kBlock / kWave)), dim3(kBlock), 0, s, brp64, b_rows,
+                       bcol32.as<int32_t>(), keys.as<uint64_t>(), idx.as<uint64_t>());
+    SPOPS_HIP_CHECK(hipGetLastError());
+    radix_sort_u64(keys.as<uint64_t>(), idx.p, 8, b_nnz, 0, 64, s);
+    bsorted_val.alloc((size_t)b_nnz * vb);
+    if (vt == SPOPS_VALUE_F64)
+      hipLaunchKernelGGL(k_row_sorted_fill<double>, dim3(nblk(b_nnz, kBlock)), dim3(kBlock), 0, s, keys.as<uint64_t>(),
+                         idx.as<uint64_t>(), b_nnz, static_cast<const double *>(bval), bcol32.as<int32_t>(),
+                         bsorted_val.as<double>());
+    else
+      hipLaunchKernelGGL(k_row_sorted_fill<float>, dim3(nblk(b_nnz, kBlock)), dim3(kBlock), 0, s, keys.as<uint64_t>(),
+                         idx.as<uint64_t>(), b_nnz, static_cast<const float *>(bval), bcol32.as<int32_t>(),
+                         bsorted_val.as<float>());
+    SPOPS_HIP_CHECK(hipGetLastError());
+    bval = bsorted_val.p;
+    SPOPS_HIP_CHECK(hipStreamSynchronize(s));
+  }
   // work estimate and bins
   DevBuf flop;
   flop.alloc((size_t)std::max<int64_t>(nc, 1) * 8);
@@ -626,9 +763,9 @@ void lespgemm(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t 
     if (hflop[c] <= kSmallFlop && hflop[c] * hcsz[c] <= kSmallAcc) small.push_back((int32_t)c);
     else large.push_back((int32_t)c);
   }
-  // large clusters in descending work so the persistent workgroups end together
+  // large clusters in descending work, so that the longest start first
   std::stable_sort(large.begin(), large.end(), [&](int32_t a, int32_t b) { return hflop[a] > hflop[b]; });
-  DevBuf dsmall, dlarge, nz, cmin, cmax, counter;
+  DevBuf dsmall, dlarge, nz, cmin, cmax;
   dsmall.upload(small);
   dlarge.upload(large);
   nz.alloc((size_t)std::max<int64_t>(nc, 1) * 8);
@@ -637,28 +774,16 @@ void lespgemm(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t 
   SPOPS_HIP_CHECK(hipMemsetAsync(nz.p, 0, (size_t)std::max<int64_t>(nc, 1) * 8, s));
   SPOPS_HIP_CHECK(hipMemsetAsync(cmin.p, 0, (size_t)std::max<int64_t>(nc, 1) * 8, s));
   SPOPS_HIP_CHECK(hipMemsetAsync(cmax.p, 0, (size_t)std::max<int64_t>(nc, 1) * 8, s));
-  counter.alloc(16);
-  const int64_t words = (b_cols + 31) / 32 + 1;
-  const int slots = large.empty() ? 0 : persistent_slots(words);
-  DevBuf bitmaps, prefixes;
-  if (slots) {
-    bitmaps.alloc((size_t)slots * words * 4);
-    prefixes.alloc((size_t)slots * words * 4);
-    SPOPS_HIP_CHECK(hipMemsetAsync(bitmaps.p, 0, (size_t)slots * words * 4, s));
-  }
   double ms_setup = tm.stop();
   // ---- symbolic
   if (!small.empty())
     hipLaunchKernelGGL(k_sym_small, dim3((unsigned)small.size()), dim3(kBlock), 0, s, dsmall.as<int32_t>(),
                        (int64_t)small.size(), A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64, bcol32.as<int32_t>(),
                        nz.as<int64_t>(), cmin.as<int64_t>(), cmax.as<int64_t>());
-  if (slots) {
-    SPOPS_HIP_CHECK(hipMemsetAsync(counter.p, 0, 16, s));
-    hipLaunchKernelGGL(k_sym_large, dim3((unsigned)slots), dim3(kBlock), 0, s, dlarge.as<int32_t>(), (int64_t)large.size(),
-                       counter.as<unsigned int>(), bitmaps.as<uint32_t>(), words, A.rowptr.as<int64_t>(),
-                       A.colids.as<int32_t>(), brp64, bcol32.as<int32_t>(), nz.as<int64_t>(), cmin.as<int64_t>(),
-                       cmax.as<int64_t>());
-  }
+  if (!large.empty())
+    hipLaunchKernelGGL(k_sym_large, dim3((unsigned)large.size()), dim3(kBlock), 0, s, dlarge.as<int32_t>(),
+                       A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64, bcol32.as<int32_t>(), (int32_t)b_cols,
+                       nz.as<int64_t>(), cmin.as<int64_t>(), cmax.as<int64_t>());
   SPOPS_HIP_CHECK(hipGetLastError());
   C.csr_rows = A.csr_rows;
   C.rows = nc;
@@ -682,54 +807,42 @@ void lespgemm(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t 
   C.nnzc = nnzc;
   C.nnzv = nnzv;
   double ms_symbolic = tm.stop();
-  // ---- allocate output (the reference's "malloc" phase)
+  // ---- allocate output (the reference's "malloc" phase); every entry is written below
   C.colids.alloc((size_t)std::max<int64_t>(nnzc, 1) * 4);
-  C.values.alloc((size_t)std::max<int64_t>(nnzv, 1) * value_bytes(vt));
-  SPOPS_HIP_CHECK(hipMemsetAsync(C.values.p, 0, (size_t)std::max<int64_t>(nnzv, 1) * value_bytes(vt), s));
+  C.values.alloc((size_t)std::max<int64_t>(nnzv, 1) * vb);
   C.cluster_sz.alloc((size_t)std::max<int64_t>(nc, 1) * 4);
   SPOPS_HIP_CHECK(hipMemcpyAsync(C.cluster_sz.p, A.cluster_sz.p, (size_t)nc * 4, hipMemcpyDeviceToDevice, s));
   double ms_malloc = tm.stop();
-  // ---- numeric
-  if (vt == SPOPS_VALUE_F64) {
+  // ---- numeric (large clusters first get their sorted columns)
+  if (!large.empty())
+    hipLaunchKernelGGL(k_emit_large, dim3((unsigned)large.size()), dim3(kBlock), 0, s, dlarge.as<int32_t>(),
+                       A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64, bcol32.as<int32_t>(),
+                       C.rowptr.as<int64_t>(), cmin.as<int64_t>(), cmax.as<int64_t>(), C.colids.as<int32_t>());
+  SPOPS_HIP_CHECK(hipGetLastError());
+  auto numeric = [&](auto tag) {
+    using V = decltype(tag);
     if (!small.empty())
-      hipLaunchKernelGGL(k_num_small<double>, dim3((unsigned)small.size()), dim3(kBlock), 0, s, dsmall.as<int32_t>(),
+      hipLaunchKernelGGL(k_num_small<V>, dim3((unsigned)small.size()), dim3(kBlock), 0, s, dsmall.as<int32_t>(),
                          (int64_t)small.size(), A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(),
-                         A.cluster_sz.as<int32_t>(), A.colids.as<int32_t>(), A.values.as<double>(), brp64,
-                         bcol32.as<int32_t>(), static_cast<const double *>(bval), C.rowptr.as<int64_t>(),
-                         C.rowptr_val.as<int64_t>(), C.colids.as<int32_t>(), C.values.as<double>());
-    if (slots) {
-      SPOPS_HIP_CHECK(hipMemsetAsync(counter.p, 0, 16, s));
-      hipLaunchKernelGGL(k_num_large<double>, dim3((unsigned)slots), dim3(kBlock), 0, s, dlarge.as<int32_t>(),
-                         (int64_t)large.size(), counter.as<unsigned int>(), bitmaps.as<uint32_t>(), prefixes.as<int32_t>(),
-                         words, A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(), A.cluster_sz.as<int32_t>(),
-                         A.colids.as<int32_t>(), A.values.as<double>(), brp64, bcol32.as<int32_t>(),
-                         static_cast<const double *>(bval), C.rowptr.as<int64_t>(), C.rowptr_val.as<int64_t>(),
-                         cmin.as<int64_t>(), cmax.as<int64_t>(), C.colids.as<int32_t>(), C.values.as<double>());
-    }
-  } else {
-    if (!small.empty())
-      hipLaunchKernelGGL(k_num_small<float>, dim3((unsigned)small.size()), dim3(kBlock), 0, s, dsmall.as<int32_t>(),
-                         (int64_t)small.size(), A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(),
-                         A.cluster_sz.as<int32_t>(), A.colids.as<int32_t>(), A.values.as<float>(), brp64,
-                         bcol32.as<int32_t>(), static_cast<const float *>(bval), C.rowptr.as<int64_t>(),
-                         C.rowptr_val.as<int64_t>(), C.colids.as<int32_t>(), C.values.as<float>());
-    if (slots) {
-      SPOPS_HIP_CHECK(hipMemsetAsync(counter.p, 0, 16, s));
-      hipLaunchKernelGGL(k_num_large<float>, dim3((unsigned)slots), dim3(kBlock), 0, s, dlarge.as<int32_t>(),
-                         (int64_t)large.size(), counter.as<unsigned int>(), bitmaps.as<uint32_t>(), prefixes.as<int32_t>(),
-                         words, A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(), A.cluster_sz.as<int32_t>(),
-                         A.colids.as<int32_t>(), A.values.as<float>(), brp64, bcol32.as<int32_t>(),
-                         static_cast<const float *>(bval), C.rowptr.as<int64_t>(), C.rowptr_val.as<int64_t>(),
-                         cmin.as<int64_t>(), cmax.as<int64_t>(), C.colids.as<int32_t>(), C.values.as<float>());
-    }
-  }
+                         A.cluster_sz.as<int32_t>(), A.colids.as<int32_t>(), A.values.as<V>(), brp64,
+                         bcol32.as<int32_t>(), static_cast<const V *>(bval), C.rowptr.as<int64_t>(),
+                         C.rowptr_val.as<int64_t>(), C.colids.as<int32_t>(), C.values.as<V>());
+    if (!large.empty())
+      hipLaunchKernelGGL(k_num_large<V>, dim3((unsigned)large.size()), dim3(kLargeBlock), 0, s, dlarge.as<int32_t>(),
+                         A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(), A.cluster_sz.as<int32_t>(),
+                         A.colids.as<int32_t>(), A.values.as<V>(), brp64, bcol32.as<int32_t>(),
+                         static_cast<const V *>(bval), C.rowptr.as<int64_t>(), C.rowptr_val.as<int64_t>(),
+                         C.colids.as<int32_t>(), C.values.as<V>());
+  };
+  if (vt == SPOPS_VALUE_F64) numeric(double(0));
+  else numeric(float(0));
   SPOPS_HIP_CHECK(hipGetLastError());
   double ms_numeric = tm.stop();
   // per-cluster min/max kept for the classification report
   C.cmin = std::move(cmin);
   C.cmax = std::move(cmax);
   C.nzc = std::move(nz);
-  stats[0] = 0;
+  stats[0] = unsorted;
   stats[1] = nnzc;
   stats[2] = nnzv;
   stats[3] = (int64_t)small.size();

@@ -8,8 +8,12 @@ namespace spops {
 constexpr int kSmallFlop = 1024;                 // products of a cluster handled in LDS
 constexpr int kSmallTable = 2048;                // LDS hash slots (load factor <= 1/2)
 constexpr int kSmallAcc = 4096;                  // LDS accumulator entries (nz * csz)
-constexpr int kLargeSlotsPerCU = 4;              // persistent workgroups per CU (large bin)
-constexpr int64_t kLargeScratchBytes = (int64_t)8 << 30;   // bitmaps + rank prefixes
+constexpr int kLargeTileBits = 1 << 18;         // columns per LDS bitmap pass (32 KiB)
+constexpr int kLargeBlock = 512;                 // threads of a large-cluster numeric workgroup
+constexpr int kLargeAccEntries = 14336;          // LDS accumulator entries of a numeric chunk
+constexpr int kLargeHash = 4096;                 // LDS hash slots of a chunk's columns
+constexpr int kLargeCursors = 2048;              // A columns whose B-row position is kept in LDS
+constexpr int kMaxClusterRows = 16;              // A values of a column held in registers per pass
 constexpr int kSpgemmStats = 16;
 
 }  // namespace spops

@@ -107,7 +107,7 @@ class ClusterMatrix:
                dB.val.data_ptr(), ctypes.byref(h), stats, L.current_stream())
         C = ClusterMatrix(h, self.value_type)
         s = list(stats)
-        C.stats = {"nnzc": s[1], "nnzv": s[2], "small_clusters": s[3], "large_clusters": s[4],
+        C.stats = {"b_rows_sorted_copy": s[0], "nnzc": s[1], "nnzv": s[2], "small_clusters": s[3], "large_clusters": s[4],
                    "setup_ms": s[5] / 1e3, "symbolic_ms": s[6] / 1e3, "malloc_ms": s[7] / 1e3,
                    "numeric_ms": s[8] / 1e3, "cluster_products": s[9]}
         return C

@@ -130,3 +130,62 @@ def test_lespgemm_error_paths(dev):
         LG.lespgemm(A, rmat(7, 4, seed=1), device=dev)
     with pytest.raises(ValueError):
         LG.lespgemm(A, A, k=10, bands=3, device=dev)
+
+
+def _random_b(rows, cols, per_row, seed, shuffle=False):
+    """B with `per_row` distinct uniform columns per row, values U(0,1]; rows sorted unless
+    shuffle (then each row's entries are in random order)."""
+    rng = np.random.default_rng(seed)
+    col = np.concatenate([np.sort(rng.choice(cols, size=per_row, replace=False)) for _ in range(rows)])
+    val = 1.0 - rng.random(col.size)
+    if shuffle:                                   # the same entries, each row in another order
+        p = np.concatenate([r * per_row + np.random.default_rng(seed + 1 + r).permutation(per_row)
+                            for r in range(rows)])
+        col, val = col[p], val[p]
+    rp = (np.arange(rows + 1) * per_row).astype(np.int64)
+    return Csr(rows, cols, rp, col.astype(np.int64), val)
+
+
+def _check_against_oracle(A, B, out):
+    want = orc.lespgemm(A, B, use_long_double=True)
+    np.testing.assert_array_equal(out["perm"], want["perm"])
+    np.testing.assert_array_equal(out["C_rowptr"], want["C_rowptr"])
+    np.testing.assert_array_equal(out["C_col"], want["C_col"])
+    np.testing.assert_array_equal(out["cmin"], want["cmin"])
+    np.testing.assert_array_equal(out["cmax"], want["cmax"])
+    rel = np.abs(out["C_val"] - want["C_val"]) / np.abs(want["C_val"])
+    assert rel.max() <= 1e-12, rel.max()
+
+
+def test_lespgemm_wide_b_several_bitmap_tiles(dev):
+    """B with 600,000 columns: the large-cluster symbolic and column emission run several
+    LDS bitmap tiles of 2^18 columns."""
+    A = rmat(10, 16, seed=3)
+    B = _random_b(A.n_cols, 600_000, 48, seed=4)
+    out = LG.lespgemm(A, B, device=dev)
+    assert out["stats"]["large_clusters"] > 0 and int(out["cmax"].max()) >= 2 * (1 << 18)
+    _check_against_oracle(A, B, out)
+
+
+def test_lespgemm_chunked_accumulator(dev):
+    """Clusters whose C block has more than 1024 columns: the numeric pass runs several LDS
+    accumulator chunks per cluster."""
+    A = rmat(11, 16, seed=5)
+    B = _random_b(A.n_cols, A.n_cols, 96, seed=6)
+    out = LG.lespgemm(A, B, device=dev)
+    assert int(out["nz"].max()) > 1024
+    _check_against_oracle(A, B, out)
+
+
+def test_lespgemm_unsorted_b_rows(dev):
+    """B rows in random order give the same C (B is sorted into a copy first)."""
+    A = rmat(10, 16, seed=7)
+    Bs = _random_b(A.n_cols, A.n_cols, 24, seed=8)
+    Bu = _random_b(A.n_cols, A.n_cols, 24, seed=8, shuffle=True)
+    o1 = LG.lespgemm(A, Bs, device=dev)
+    o2 = LG.lespgemm(A, Bu, device=dev)
+    assert o1["stats"]["b_rows_sorted_copy"] == 0 and o2["stats"]["b_rows_sorted_copy"] == 1
+    np.testing.assert_array_equal(o1["C_rowptr"], o2["C_rowptr"])
+    np.testing.assert_array_equal(o1["C_col"], o2["C_col"])
+    np.testing.assert_allclose(o2["C_val"], o1["C_val"], rtol=1e-12, atol=0)
+    _check_against_oracle(A, Bs, o1)
