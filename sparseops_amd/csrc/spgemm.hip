@@ -431,6 +431,7 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
   __shared__ int32_t hkey[kLargeHash];
   __shared__ int16_t hval[kLargeHash];
   __shared__ int32_t cursor[kLargeCursors];   // per A column: B entries consumed by earlier chunks
+  __shared__ int32_t nextcol[kLargeCursors];  // per A column: column at the cursor (INT32_MAX: done)
   constexpr int W = kLargeBlock / kWave;
   const int64_t c = list[blockIdx.x];
   const int csz = csz_a[c];
@@ -438,7 +439,12 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
   const int64_t c0 = crp[c], nzc = crp[c + 1] - c0, cv0 = crpv[c];
   const int M = min(kLargeAccEntries / csz, kLargeHash / 2);
   const bool cursors = a1 - a0 <= kLargeCursors;
-  for (int i = threadIdx.x; i < kLargeCursors; i += kLargeBlock) cursor[i] = 0;
+  if (cursors)
+    for (int64_t t = a0 + threadIdx.x; t < a1; t += kLargeBlock) {
+      const int32_t b = acol[t];
+      cursor[t - a0] = 0;
+      nextcol[t - a0] = brp[b] < brp[b + 1] ? bcol[brp[b]] : INT32_MAX;
+    }
   const int lane = lane_id();
   for (int64_t j0 = 0; j0 < nzc; j0 += M) {
     const int m = (int)min((int64_t)M, nzc - j0);
@@ -453,48 +459,71 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
       hval[h] = (int16_t)i;
     }
     __syncthreads();
-    for (int64_t t = a0 + wave_id(); t < a1; t += W) {
-      const int32_t b = acol[t];
-      const int64_t kb = brp[b], ke = brp[b + 1];
-      int64_t k = kb;
-      if (cursors) {
-        k = kb + cursor[t - a0];
-      } else {
-        int64_t k1 = ke;
-        row_range(bcol, k, k1, lo, hi);
-      }
-      if (k >= ke) continue;
-      const V *a = aval + av0 + (t - a0) * csz;
-      // the row's entries from k on, 64 at a time, up to the first column past hi (sorted row)
-      for (;;) {
-        const int64_t kk = k + lane;
-        const int32_t key = kk < ke ? bcol[kk] : INT32_MAX;
-        const bool in = key <= hi;
-        const int n = __builtin_popcountll(__ballot(in));
-        int h = -1;
-        if (in) {                              // every such column is one of the chunk's
-          uint32_t p = hslot(key, kLargeHash - 1);
-          for (int probe = 0; probe < kLargeHash; ++probe, p = (p + 1) & (kLargeHash - 1))
-            if (hkey[p] == key) { h = (int)p; break; }
+    // lane groups of kLargeGroupLanes: each group scans one B row (most rows contribute a few
+    // entries to a chunk, so a whole wave per row would idle)
+    constexpr int GL = kLargeGroupLanes, G = kWave / GL;
+    const int grp = lane / GL, sub = lane % GL;
+    const uint64_t gmask = ((GL == 64) ? ~0ull : ((1ull << GL) - 1)) << (grp * GL);
+    for (int64_t base = a0 + (int64_t)wave_id() * G; base < a1; base += (int64_t)W * G) {
+      // every lane of the wave takes part in the ballots below, groups without a row idle
+      const int64_t t = base + grp;
+      bool live = t < a1 && !(cursors && nextcol[t - a0] > hi);
+      int64_t kb = 0, ke = 0, k = 0;
+      const V *a = nullptr;
+      if (live) {
+        const int32_t b = acol[t];
+        kb = brp[b];
+        ke = brp[b + 1];
+        k = kb;
+        if (cursors) {
+          k = kb + cursor[t - a0];
+        } else {
+          int64_t k1 = ke;
+          row_range(bcol, k, k1, lo, hi);
         }
-        if (h >= 0) {
-          const V bv = bval[kk];
-          const int r = hval[h];
-          for (int l0 = 0; l0 < csz; l0 += kMaxClusterRows) {    // one pass unless csz > 16
-            V *dst = acc + r * csz + l0;
+        live = k < ke;
+        a = aval + av0 + (t - a0) * csz;
+      }
+      const int64_t kstart = k;
+      int32_t nxt = INT32_MAX;
+      for (int l0 = 0; l0 < csz; l0 += kMaxClusterRows) {      // one pass unless csz > 16
+        V ar[kMaxClusterRows];
 #pragma unroll
-            for (int l = 0; l < kMaxClusterRows; ++l) {
-              if (l0 + l < csz) {
-                const V av = a[l0 + l];
-                if (av != V(0)) atomicAdd(&dst[l], av * bv);
-              }
+        for (int l = 0; l < kMaxClusterRows; ++l) ar[l] = (live && l0 + l < csz) ? a[l0 + l] : V(0);
+        k = kstart;
+        bool going = live;
+        while (__ballot(going)) {
+          const int64_t kk = k + sub;
+          const int32_t key = (going && kk < ke) ? bcol[kk] : INT32_MAX;
+          const bool in = going && key <= hi;
+          const int n = __builtin_popcountll(__ballot(in) & gmask);
+          int h = -1;
+          if (in) {                            // every such column is one of the chunk's
+            uint32_t p = hslot(key, kLargeHash - 1);
+            for (int probe = 0; probe < kLargeHash; ++probe, p = (p + 1) & (kLargeHash - 1))
+              if (hkey[p] == key) { h = (int)p; break; }
+          }
+          if (h >= 0) {
+            const V bv = bval[kk];
+            V *dst = acc + (int)hval[h] * csz + l0;
+#pragma unroll
+            for (int l = 0; l < kMaxClusterRows; ++l)
+              if (ar[l] != V(0)) atomicAdd(&dst[l], ar[l] * bv);
+          }
+          const int32_t first_out = __shfl(key, grp * GL + (n < GL ? n : 0), kWave);
+          if (going) {
+            k += n;
+            if (n < GL) {
+              nxt = first_out;                 // first column past hi (INT32_MAX past the row)
+              going = false;
             }
           }
         }
-        k += n;
-        if (n < kWave) break;
       }
-      if (cursors && lane == 0) cursor[t - a0] = (int32_t)(k - kb);
+      if (cursors && live && sub == 0) {
+        cursor[t - a0] = (int32_t)(k - kb);
+        nextcol[t - a0] = nxt;
+      }
     }
     __syncthreads();
     V *out = cval + cv0 + j0 * csz;
