@@ -128,9 +128,27 @@ int spops_bsr_create_from_blocks(spops_matrix *A, int64_t n_rows, int64_t n_cols
  * follows the value type (see top).  Asynchronous on `stream`. */
 int spops_spmv(spops_matrix A, double alpha, const void *d_x, double beta, void *d_y, void *stream);
 
-/* Y = A*X (+ beta*Y) for k right-hand sides, X row-major (n_cols x k), Y (n_rows x k),
- * CSR and SELL matrices.  x/y element types as for spops_spmv. */
+/* ---- multi-RHS and differentiable forms (CSR matrices) ------------------------------------
+ * No reference counterpart as entry points: SpOps_csr (src/spmv_csr.cpp:165-185) applied to k
+ * right-hand sides, its gradient with respect to the values (SDDMM on A's pattern) and the
+ * transpose of csr_transpose (include/sparse_conversion.h:2067-2108) with the entry
+ * permutation kept.  X row-major (n_cols x k), Y row-major (n_rows x k).  Accumulation is FP64
+ * when the matrix or X is FP64, otherwise FP32.  Other formats return NOT_SUPPORTED. */
+/* Y = alpha*A*X + beta*Y with X, Y of the matrix's x/y type (as for spops_spmv) */
 int spops_spmm(spops_matrix A, int64_t k, double alpha, const void *d_X, double beta, void *d_Y, void *stream);
+/* the same with explicit X and Y element types (SPOPS_VALUE_F64 / F32 / BF16) */
+int spops_spmm_typed(spops_matrix A, int64_t k, double alpha, const void *d_X, int x_type, double beta, void *d_Y,
+                     int y_type, void *stream);
+/* out[e] = sum_c G[row_e, c] * X[col_e, c] for every entry e of A (A's values unused);
+ * G row-major (n_rows x k), X row-major (n_cols x k), out[nnz] */
+int spops_sddmm(spops_matrix A, int64_t k, const void *d_G, int g_type, const void *d_X, int x_type, void *d_out,
+                int out_type, void *stream);
+/* AT = A^T as a new CSR handle; AT's entry e is A's entry perm[e] (kept inside AT), rows of
+ * AT hold A's entries in A's row order */
+int spops_csr_transpose(spops_matrix A, spops_matrix *AT);
+/* overwrite the stored values from a device array (cast to the stored type).  For a handle
+ * made by spops_csr_transpose the array is in the ORIGINAL matrix's entry order. */
+int spops_csr_set_values(spops_matrix A, const void *d_values, int values_type, void *stream);
 
 /* info[0]=format info[1]=n_rows info[2]=n_cols info[3]=nnz info[4]=value_type
  * info[5]=device bytes owned info[6]=algorithmic bytes of one SpMV (no padding)

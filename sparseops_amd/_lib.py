@@ -55,6 +55,10 @@ _SIGNATURES = [
      [c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_int, c_int]),
     ("spops_spmv", c_int, [c_vp, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
     ("spops_spmm", c_int, [c_vp, c_i64, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
+    ("spops_spmm_typed", c_int, [c_vp, c_i64, c_dbl, c_vp, c_int, c_dbl, c_vp, c_int, c_vp]),
+    ("spops_sddmm", c_int, [c_vp, c_i64, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_vp]),
+    ("spops_csr_transpose", c_int, [c_vp, c_vp]),
+    ("spops_csr_set_values", c_int, [c_vp, c_vp, c_int, c_vp]),
     ("spops_matrix_info", c_int, [c_vp, c_vp]),
     ("spops_matrix_layout", c_int, [c_vp, c_int, c_vp]),
     ("spops_set_tuning", c_int, [c_int, c_i64, c_vp]),

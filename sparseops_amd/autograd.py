@@ -1,0 +1,67 @@
+"""Differentiable SpMM Y = A(vals) X with hand-written backward kernels (torch.autograd).
+
+The pattern of A is fixed and lives on the GPU; its values are a tensor that may require
+gradients.  Forward and backward call the C ABI of libsparseops_hip.so:
+  Y     = A X                      spops_spmm_typed on A
+  dX    = A^T G                    spops_spmm_typed on a cached transposed handle whose values
+                                   follow vals through the entry permutation
+  dvals = SDDMM(G, X) on A's pattern   (dvals[e] = <G[row_e, :], X[col_e, :]>, spops_sddmm)
+Accumulation is FP64 when the values or X are FP64, otherwise FP32 (BF16 X / G are widened).
+The transpose semantics are the reference's csr_transpose (include/sparse_conversion.h:2067-2108)
+without its duplicate merging: the pattern has no duplicates.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import spmv as S
+from .gen import Csr
+
+
+class SparsePattern:
+    """A CSR pattern on the device with values of `value_type` (f64, f32 or bf16 storage)
+    and its transpose, built once."""
+
+    def __init__(self, A: Csr, value_type="f32"):
+        self.A = S.csr(A, value_type)
+        self.AT = self.A.transpose()
+        self.n_rows, self.n_cols, self.nnz = A.n_rows, A.n_cols, A.nnz
+
+    def close(self):
+        self.A.close()
+        self.AT.close()
+
+
+class _SpMM(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, vals: torch.Tensor, X: torch.Tensor, pat: SparsePattern, out_dtype):
+        X = X.contiguous()
+        pat.A.set_values(vals.detach())
+        Y = torch.empty(pat.n_rows, X.shape[1], dtype=out_dtype or X.dtype, device=X.device)
+        pat.A.spmm(X, Y)
+        ctx.save_for_backward(vals, X)
+        ctx.pat = pat
+        return Y
+
+    @staticmethod
+    def backward(ctx, G: torch.Tensor):
+        vals, X = ctx.saved_tensors
+        pat = ctx.pat
+        G = G.contiguous()
+        dvals = dX = None
+        if ctx.needs_input_grad[1]:
+            pat.AT.set_values(vals.detach())
+            dX = torch.empty_like(X)
+            pat.AT.spmm(G, dX)
+        if ctx.needs_input_grad[0]:
+            dvals = torch.empty(pat.nnz, dtype=vals.dtype, device=vals.device)
+            pat.A.sddmm(G, X, dvals)
+        return dvals, dX, None, None
+
+
+def spmm(pat: SparsePattern, vals: torch.Tensor, X: torch.Tensor, out_dtype=None) -> torch.Tensor:
+    """Y = A(vals) X, differentiable in vals and X.  out_dtype defaults to X's dtype
+    (pass torch.float32 to keep FP32 outputs for BF16 X)."""
+    if vals.numel() != pat.nnz:
+        raise ValueError("vals must have nnz elements")
+    return _SpMM.apply(vals, X, pat, out_dtype)

@@ -1192,16 +1192,6 @@ extern "C" int spops_matrix_destroy(spops_matrix A) {
   return SPOPS_SUCCESS;
 }
 
-extern "C" int spops_spmm(spops_matrix A, int64_t k, double alpha, const void *d_X, double beta, void *d_Y,
-                          void *stream) {
-  return guarded([&]() {
-    SPOPS_REQUIRE(A != nullptr && k > 0, SPOPS_ERROR_INVALID_VALUE, "null matrix or k <= 0");
-    (void)alpha; (void)d_X; (void)beta; (void)d_Y; (void)stream;
-    throw_error(SPOPS_ERROR_NOT_SUPPORTED, "spops_spmm: not implemented for this format yet");
-    return SPOPS_ERROR_NOT_SUPPORTED;
-  });
-}
-
 extern "C" int spops_bsr_create_from_blocks(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t bR, int64_t bC,
                                             int64_t mb, int64_t nnzb, int index_type, const void *row_ptr,
                                             const void *block_colindex, const void *block_data, int in_vt, int vt) {

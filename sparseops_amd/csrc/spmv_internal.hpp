@@ -21,7 +21,8 @@ constexpr int slice_lane_entries(int vt) { return vt == SPOPS_VALUE_F64 ? 8 : 16
 
 struct CsrDev {
   DevBuf rowptr, col, val, tasks, longs;
-  bool rowptr64 = false;
+  DevBuf tperm;                 // transposed view: entry e holds the source matrix's entry tperm[e]
+  bool rowptr64 = false, tperm64 = false;
   int ntasks = 0, nlong = 0;
 };
 
@@ -70,5 +71,7 @@ struct spops_matrix_s {
 
 namespace spops {
 extern int64_t g_slice_bytes;   // x bytes per slice for matrices created from now on
+void build_csr_plan(const std::vector<int64_t> &rp, int64_t n_rows, std::vector<int2> &tasks, std::vector<int> &longs);
+int64_t spmv_algo_bytes(const spops_matrix_s &M, int64_t sI);
 void launch_spmv(const spops_matrix_s &M, double alpha, const void *x, double beta, void *y, hipStream_t s);
 }

@@ -22,6 +22,17 @@ XY_DTYPE = {L.VALUE_F64: torch.float64, L.VALUE_F32: torch.float32, L.VALUE_BF16
 FORMAT_NAMES = {L.FORMAT_CSR: "csr", L.FORMAT_SELL: "sell", L.FORMAT_ELL: "ell", L.FORMAT_BSR: "bsr"}
 
 
+TORCH_TYPES = {torch.float64: L.VALUE_F64, torch.float32: L.VALUE_F32, torch.bfloat16: L.VALUE_BF16}
+
+
+def _rhs_check(X: torch.Tensor, rows: int, name: str) -> int:
+    if X.dim() != 2 or X.shape[0] != rows or not X.is_contiguous() or not X.is_cuda:
+        raise ValueError(f"{name} must be a contiguous device tensor of shape ({rows}, k)")
+    if X.dtype not in TORCH_TYPES:
+        raise TypeError(f"{name} must be float64, float32 or bfloat16")
+    return int(X.shape[1])
+
+
 def _vt(v) -> int:
     return VALUE_TYPES[v] if isinstance(v, str) else int(v)
 
@@ -78,6 +89,44 @@ class SpMatrix:
         s = L.current_stream() if stream is None else stream
         L.call("spops_spmv", self._h, float(alpha), x.data_ptr(), float(beta), y.data_ptr(), s)
         return y
+
+    # ---- multi-RHS and differentiable forms (CSR) ----------------------------------------
+    def spmm(self, X: torch.Tensor, Y: torch.Tensor, alpha: float = 1.0, beta: float = 0.0, stream=None):
+        """Y = alpha*A*X + beta*Y; X (n_cols x k) and Y (n_rows x k) row-major device tensors
+        in FP64, FP32 or BF16 (independently)."""
+        k = _rhs_check(X, self.n_cols, "X")
+        if Y.dim() != 2 or Y.shape[0] != self.n_rows or Y.shape[1] != k or not Y.is_contiguous() or not Y.is_cuda:
+            raise ValueError("Y must be a contiguous device tensor of shape (n_rows, k)")
+        s = L.current_stream() if stream is None else stream
+        L.call("spops_spmm_typed", self._h, k, float(alpha), X.data_ptr(), TORCH_TYPES[X.dtype], float(beta),
+               Y.data_ptr(), TORCH_TYPES[Y.dtype], s)
+        return Y
+
+    def sddmm(self, G: torch.Tensor, X: torch.Tensor, out: torch.Tensor, stream=None):
+        """out[e] = <G[row_e, :], X[col_e, :]> over A's pattern (the gradient of the values)."""
+        k = _rhs_check(X, self.n_cols, "X")
+        if G.dim() != 2 or G.shape != (self.n_rows, k) or not G.is_contiguous() or not G.is_cuda:
+            raise ValueError("G must be a contiguous device tensor of shape (n_rows, k)")
+        if out.numel() != self.nnz or not out.is_contiguous() or not out.is_cuda:
+            raise ValueError("out must be a contiguous device tensor of nnz elements")
+        s = L.current_stream() if stream is None else stream
+        L.call("spops_sddmm", self._h, k, G.data_ptr(), TORCH_TYPES[G.dtype], X.data_ptr(), TORCH_TYPES[X.dtype],
+               out.data_ptr(), TORCH_TYPES[out.dtype], s)
+        return out
+
+    def transpose(self) -> "SpMatrix":
+        """A^T as a CSR handle whose values follow this matrix's (see set_values)."""
+        h = ctypes.c_void_p()
+        L.call("spops_csr_transpose", self._h, ctypes.byref(h))
+        return SpMatrix(h)
+
+    def set_values(self, vals: torch.Tensor, stream=None):
+        """Overwrite the stored values (cast to the stored type) from a device tensor of nnz
+        elements; for a transpose() handle, vals are in the original matrix's entry order."""
+        if vals.numel() != self.nnz or not vals.is_contiguous() or not vals.is_cuda:
+            raise ValueError("vals must be a contiguous device tensor of nnz elements")
+        s = L.current_stream() if stream is None else stream
+        L.call("spops_csr_set_values", self._h, vals.data_ptr(), TORCH_TYPES[vals.dtype], s)
 
     @property
     def slices(self) -> int:

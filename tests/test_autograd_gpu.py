@@ -1,0 +1,192 @@
+"""SpMM, SDDMM, transpose and the differentiable SpMM (through the C ABI) against the CPU
+oracle (orc_spmm_csr, orc_sddmm_csr, orc_csr_transpose) and FP64 finite differences.
+
+Tolerances: FP64 data, FP64 accumulation: 1e-13 relative to sum |a x| per output; FP32 /
+BF16 data with FP32 accumulation: 2e-6 of that scale against the exact product of the
+values as stored (BF16 inputs are rounded on the host first, so only the accumulation is
+measured)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_bind as orc
+from sparseops_amd import autograd as AG
+from sparseops_amd import spmv as S
+from sparseops_amd.gen import Csr, powerlaw, rmat, rmat_symnorm
+
+pytestmark = pytest.mark.gpu
+
+
+def _edge(seed=4):
+    """Rows of every SpMM path: empty, short, long (> 2048 entries)."""
+    rng = np.random.default_rng(seed)
+    n = 5000
+    lens = rng.integers(0, 30, n)
+    lens[::50] = 0
+    lens[7], lens[1500] = 2600, 4100
+    cols = [np.sort(rng.choice(n, size=L, replace=False)) for L in lens]
+    rp = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=rp[1:])
+    col = np.concatenate(cols)
+    return Csr(n, n, rp.astype(np.int32), col.astype(np.int32), rng.standard_normal(col.size))
+
+
+def _bf16(a):
+    return torch.tensor(a, dtype=torch.float64).to(torch.bfloat16).to(torch.float64).numpy()
+
+
+def _close(got, want, scale, tol):
+    err = np.abs(got - want) / (scale + 1e-300)
+    assert np.all(err <= tol), f"max scaled error {err.max():.3e} > {tol}"
+
+
+@pytest.fixture(scope="module")
+def mats():
+    return {"edge": _edge(), "pl": powerlaw(5000, lmax=800, seed=3), "rmat": rmat(11, 8, seed=2, index_dtype=np.int32)}
+
+
+@pytest.mark.parametrize("k", [1, 3, 8, 17, 64, 100])
+@pytest.mark.parametrize("name", ["edge", "pl", "rmat"])
+def test_spmm_f64(mats, name, k, dev):
+    A = mats[name]
+    rng = np.random.default_rng(k)
+    X = rng.standard_normal((A.n_cols, k))
+    Y0 = rng.standard_normal((A.n_rows, k))
+    M = S.csr(A, "f64")
+    Xt = torch.tensor(X, device=dev)
+    Yt = torch.tensor(Y0, device=dev)
+    M.spmm(Xt, Yt, 0.75, -0.5)
+    want = 0.75 * orc.spmm(A, X) - 0.5 * Y0
+    Aabs = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, np.abs(A.val))
+    scale = 0.75 * orc.spmm(Aabs, np.abs(X)) + 0.5 * np.abs(Y0)
+    _close(Yt.cpu().numpy(), want, scale, 1e-13)
+
+
+@pytest.mark.parametrize("xt,yt", [(torch.float32, torch.float32), (torch.bfloat16, torch.float32),
+                                   (torch.bfloat16, torch.bfloat16), (torch.float32, torch.float64)])
+def test_spmm_mixed_types(mats, xt, yt, dev):
+    A = mats["edge"]
+    k = 64
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((A.n_cols, k))
+    M = S.csr(A, "f32")
+    Xt = torch.tensor(X, device=dev).to(xt)
+    Yt = torch.zeros(A.n_rows, k, dtype=yt, device=dev)
+    M.spmm(Xt, Yt)
+    Xr = Xt.double().cpu().numpy()
+    Ar = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, A.val.astype(np.float32).astype(np.float64))
+    want = orc.spmm(Ar, Xr)
+    scale = orc.spmm(Csr(A.n_rows, A.n_cols, A.rowptr, A.col, np.abs(Ar.val)), np.abs(Xr))
+    tol = 2e-6 if yt != torch.bfloat16 else 2 ** -8 + 2e-6      # BF16 output: its own rounding
+    _close(Yt.double().cpu().numpy(), want, scale, tol)
+
+
+@pytest.mark.parametrize("k", [1, 16, 64, 70])
+@pytest.mark.parametrize("gt", [torch.float64, torch.float32, torch.bfloat16])
+def test_sddmm(mats, k, gt, dev):
+    A = mats["rmat"]
+    rng = np.random.default_rng(k + 1)
+    G = torch.tensor(rng.standard_normal((A.n_rows, k)), device=dev).to(gt)
+    X = torch.tensor(rng.standard_normal((A.n_cols, k)), device=dev).to(gt)
+    M = S.csr(A, "f64")
+    out = torch.zeros(A.nnz, dtype=torch.float64 if gt == torch.float64 else torch.float32, device=dev)
+    M.sddmm(G, X, out)
+    Gn, Xn = G.double().cpu().numpy(), X.double().cpu().numpy()
+    want = orc.sddmm(A, Gn, Xn)
+    scale = orc.sddmm(A, np.abs(Gn), np.abs(Xn))
+    _close(out.double().cpu().numpy(), want, scale, 1e-13 if gt == torch.float64 else 2e-6)
+
+
+@pytest.mark.parametrize("name", ["edge", "rmat"])
+def test_transpose_and_value_updates(mats, name, dev):
+    A = mats[name]
+    M = S.csr(A, "f64")
+    T = M.transpose()
+    trp, tc, tv, tp = orc.transpose(A)
+    assert (T.n_rows, T.n_cols, T.nnz) == (A.n_cols, A.n_rows, A.nnz)
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((A.n_rows, 5))
+    Y = torch.zeros(A.n_cols, 5, dtype=torch.float64, device=dev)
+    T.spmm(torch.tensor(X, device=dev), Y)
+    At = Csr(A.n_cols, A.n_rows, trp, tc, tv)
+    np.testing.assert_allclose(Y.cpu().numpy(), orc.spmm(At, X), rtol=1e-12, atol=1e-12)
+    # new values (in A's entry order) reach both handles
+    v2 = rng.standard_normal(A.nnz)
+    vt = torch.tensor(v2, device=dev)
+    M.set_values(vt)
+    T.set_values(vt)
+    Y.zero_()
+    T.spmm(torch.tensor(X, device=dev), Y)
+    np.testing.assert_allclose(Y.cpu().numpy(), orc.spmm(Csr(A.n_cols, A.n_rows, trp, tc, v2[tp]), X), rtol=1e-12,
+                               atol=1e-12)
+    X2 = rng.standard_normal((A.n_cols, 5))
+    Y2 = torch.zeros(A.n_rows, 5, dtype=torch.float64, device=dev)
+    M.spmm(torch.tensor(X2, device=dev), Y2)
+    np.testing.assert_allclose(Y2.cpu().numpy(), orc.spmm(Csr(A.n_rows, A.n_cols, A.rowptr, A.col, v2), X2),
+                               rtol=1e-12, atol=1e-12)
+
+
+def test_gradcheck_fp64_finite_differences(dev):
+    """torch.autograd.gradcheck: analytic dvals and dX against FP64 central differences."""
+    A = rmat(6, 4, seed=11, index_dtype=np.int32)
+    pat = AG.SparsePattern(A, "f64")
+    rng = np.random.default_rng(0)
+    vals = torch.tensor(rng.standard_normal(A.nnz), device=dev, requires_grad=True)
+    X = torch.tensor(rng.standard_normal((A.n_cols, 3)), device=dev, requires_grad=True)
+    assert torch.autograd.gradcheck(lambda v, x: AG.spmm(pat, v, x), (vals, X), eps=1e-6, atol=1e-9, rtol=1e-7)
+
+
+def test_backward_matches_cpu_reference_bf16(dev):
+    """The layer's form: FP32 symmetric-normalised values, BF16 X and G, FP32 accumulation.
+    dX = A^T G and dvals = SDDMM(G, X) against the CPU oracle on the same rounded inputs."""
+    A = rmat_symnorm(12, 16, seed=1)
+    pat = AG.SparsePattern(A, "f32")
+    rng = np.random.default_rng(2)
+    vals = torch.tensor(A.val, device=dev, requires_grad=True)
+    X = torch.tensor(rng.standard_normal((A.n_cols, 64)), device=dev).to(torch.bfloat16).requires_grad_()
+    Y = AG.spmm(pat, vals, X, out_dtype=torch.float32)
+    G = torch.tensor(rng.standard_normal((A.n_rows, 64)), device=dev).to(torch.bfloat16)
+    Y.backward(G.float())
+    Xn, Gn = X.detach().double().cpu().numpy(), G.double().cpu().numpy()
+    A64 = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, A.val.astype(np.float64))
+    absA = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, np.abs(A64.val))
+    # forward
+    _close(Y.detach().double().cpu().numpy(), orc.spmm(A64, Xn), orc.spmm(absA, np.abs(Xn)), 2e-6)
+    # dX = A^T G (rounded to BF16 on output: X's dtype)
+    trp, tc, tv, tp = orc.transpose(A64)
+    At = Csr(A.n_cols, A.n_rows, trp, tc, tv)
+    want = orc.spmm(At, Gn)
+    scale = orc.spmm(Csr(A.n_cols, A.n_rows, trp, tc, np.abs(tv)), np.abs(Gn))
+    assert X.grad.dtype == torch.bfloat16
+    _close(X.grad.double().cpu().numpy(), want, scale, 2 ** -8 + 2e-6)
+    # dvals
+    _close(vals.grad.double().cpu().numpy(), orc.sddmm(A64, Gn, Xn), orc.sddmm(A64, np.abs(Gn), np.abs(Xn)), 2e-6)
+
+
+@pytest.mark.parametrize("k", [1, 5, 8, 33])
+def test_spmm_writes_nothing_past_y(k, dev):
+    """Y sits inside a larger buffer whose tail must stay untouched (rows past n_rows in the
+    last workgroup write nothing)."""
+    A = rmat(9, 4, seed=6, index_dtype=np.int32)
+    M = S.csr(A, "f64")
+    buf = torch.full((A.n_rows * k + 4096,), 7.0, dtype=torch.float64, device=dev)
+    Y = buf[:A.n_rows * k].view(A.n_rows, k)
+    X = torch.ones(A.n_cols, k, dtype=torch.float64, device=dev)
+    M.spmm(X, Y)
+    torch.cuda.synchronize()
+    assert torch.all(buf[A.n_rows * k:] == 7.0)
+    want = np.repeat(np.add.reduceat(np.r_[A.val, 0.0], np.r_[A.rowptr[:-1]])[:, None], k, axis=1)
+    want[np.diff(A.rowptr) == 0] = 0.0
+    np.testing.assert_allclose(Y.cpu().numpy(), want, rtol=1e-12, atol=1e-12)
+
+
+def test_spmm_rejects_other_formats_and_shapes(mats, dev):
+    A = mats["pl"]
+    M = S.csr_to_sell_c_sigma(A, 1024, 32, "f32")
+    X = torch.zeros(A.n_cols, 4, device=dev)
+    Y = torch.zeros(A.n_rows, 4, device=dev)
+    with pytest.raises(S.L.SpopsError):
+        M.spmm(X, Y)
+    C = S.csr(A, "f32")
+    with pytest.raises(ValueError):
+        C.spmm(X[:-1], Y)
