@@ -11,7 +11,8 @@ of the headline variant (its two launches: the column-slice kernel and the slice
 timed with HIP events on the launch stream; its HBM traffic comes from the committed
 rocprofv3 counter summary of the same command.  A secondary LeSpGEMM measurement (C = A*A,
 R-MAT, MinHash-LSH clustering, GFLOP/s) is reported with the reference's OpenMP pipeline
-timed on the same host.
+timed on the same host; so are the differentiable SpMM layer (R-MAT scale 20, BF16 X / G,
+forward and backward) and the batched SpMM (512 band matrices, one HIP-graph replay per step).
 
 Prints ONE JSON line (rank 0, N = 1).  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -148,6 +149,162 @@ def lespgemm_measure(scale, iterations, cpu_threads, cpu):
     return out
 
 
+def autograd_measure(scale, k, steps, cpu):
+    """The differentiable SpMM layer Y = A(vals) X: R-MAT(scale, edge factor 32) with symmetric
+    normalised FP32 values, X and G of width k in BF16, FP32 accumulation.  Forward and backward
+    (dX = A^T G and dvals = SDDMM(G, X)) timed with HIP events; the CPU leg is the oracle's
+    single-thread CSR SpMM on the first 1/64 of the rows, scaled to the whole matrix."""
+    import ctypes
+    import numpy as np
+    import torch
+    from sparseops_amd import _lib as L
+    from sparseops_amd import autograd as AG
+    from sparseops_amd.gen import rmat_symnorm
+    dev = torch.device("cuda:0")
+    A = rmat_symnorm(scale, 32, seed=0)
+    pat = AG.SparsePattern(A, "f32")
+    rng = np.random.default_rng(0)
+    vals = torch.tensor(A.val, device=dev, requires_grad=True)
+    X = torch.tensor(rng.standard_normal((A.n_cols, k), dtype=np.float32), device=dev).to(torch.bfloat16)
+    X.requires_grad_()
+    G = torch.tensor(rng.standard_normal((A.n_rows, k), dtype=np.float32), device=dev).to(torch.bfloat16).float()
+    stream = torch.cuda.current_stream().cuda_stream
+    ev = [ctypes.c_void_p() for _ in range(3)]
+    for e in ev:
+        L.call("spops_event_create", ctypes.byref(e))
+    fw = bw = 0.0
+    ms = ctypes.c_float()
+    for it in range(steps + 2):
+        vals.grad = None
+        X.grad = None
+        torch.cuda.synchronize()
+        L.call("spops_event_record", ev[0], stream)
+        Y = AG.spmm(pat, vals, X, out_dtype=torch.float32)
+        L.call("spops_event_record", ev[1], stream)
+        Y.backward(G)
+        L.call("spops_event_record", ev[2], stream)
+        torch.cuda.synchronize()
+        if it >= 2:
+            L.call("spops_event_elapsed_ms", ctypes.byref(ms), ev[0], ev[1])
+            fw += ms.value
+            L.call("spops_event_elapsed_ms", ctypes.byref(ms), ev[1], ev[2])
+            bw += ms.value
+    for e in ev:
+        L.call("spops_event_destroy", e)
+    fw, bw = fw / steps, bw / steps
+    n, nnz = A.n_rows, A.nnz
+    b_fwd = nnz * 8 + (n + 1) * 4 + n * k * 2 + n * k * 4
+    b_bwd = (nnz * 8 + (n + 1) * 4 + n * k * 4 + n * k * 2) + (nnz * 4 + (n + 1) * 4 + n * k * 4 + n * k * 2 + nnz * 4)
+    out = {"workload": f"Y = A X, R-MAT scale {scale} edge factor 32 (n={n}, nnz={nnz}), symmetric-normalised FP32 "
+                       f"values, X/G BF16 width {k}, FP32 accumulation; backward dX = A^T G, dvals = SDDMM(G, X)",
+           "forward_ms": round(fw, 3), "backward_ms": round(bw, 3),
+           "forward_gbs": round(b_fwd / fw / 1e6, 1), "backward_gbs": round(b_bwd / bw / 1e6, 1),
+           "forward_gflops": round(2 * nnz * k / fw / 1e6, 1),
+           "algo_bytes_forward": b_fwd, "algo_bytes_backward": b_bwd}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_bind as orc
+        from sparseops_amd.gen import Csr
+        r1 = max(1, n // 64)
+        sub = Csr(r1, A.n_cols, A.rowptr[:r1 + 1].astype(np.int64), A.col[:A.rowptr[r1]].astype(np.int64),
+                  A.val[:A.rowptr[r1]].astype(np.float64))
+        Xh = X.detach().double().cpu().numpy()
+        t = time.perf_counter()
+        orc.spmm(sub, Xh)
+        dt = (time.perf_counter() - t) * n / r1
+        out["cpu_baseline"] = {"forward_ms": round(dt * 1e3, 1), "kind": "port", "cores": 1,
+                               "sample": f"oracle CSR SpMM (FP64, one thread) on rows 0..{r1 - 1}, scaled by n/{r1}"}
+        out["gpu_over_cpu_forward"] = round(dt * 1e3 / fw, 1)
+    pat.close()
+    return out
+
+
+def batch_measure(count, steps, warmup, cpu, cpu_seconds):
+    """Batched SpMM: `count` band matrices (n log-uniform in [1k, 64k], half-width U{1..64},
+    0-8 off-band entries per row, k U{1..32}, FP32 N(0,1) values and X), formats chosen by the
+    autotuner, one persistent launch per step replayed from a captured HIP graph and timed with
+    HIP events on the replay stream.  The CPU leg is the oracle's single-thread CSR SpMM over
+    whole matrices of the batch, in order, for about cpu_seconds, scaled by flop."""
+    import ctypes
+    import numpy as np
+    import torch
+    from sparseops_amd import _lib as L
+    from sparseops_amd.batch import BatchSpMM
+    from sparseops_amd.gen import Csr, band_batch
+    t = time.time()
+    mats, ks = band_batch(count, seed=0)
+    gen_s = time.time() - t
+    t = time.time()
+    B = BatchSpMM(mats, ks, tune=True)
+    create_s = time.time() - t
+    X, Y = B.new_x(), B.new_y()
+    X.copy_(torch.randn(X.numel(), generator=torch.Generator().manual_seed(0)).to(X.device))
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    B.capture(X, Y, s.cuda_stream)
+    ev = [ctypes.c_void_p() for _ in range(2)]
+    for e in ev:
+        L.call("spops_event_create", ctypes.byref(e))
+    for _ in range(warmup):
+        B.replay(s.cuda_stream)
+    s.synchronize()
+    L.call("spops_event_record", ev[0], s.cuda_stream)
+    for _ in range(steps):
+        B.replay(s.cuda_stream)
+    L.call("spops_event_record", ev[1], s.cuda_stream)
+    s.synchronize()
+    ms = ctypes.c_float()
+    L.call("spops_event_elapsed_ms", ctypes.byref(ms), ev[0], ev[1])
+    for e in ev:
+        L.call("spops_event_destroy", e)
+    step_ms = ms.value / steps
+    fmts = B.formats()
+    out = {"workload": f"{count} band matrices (n log-uniform 1k-64k, half-width 1-64, 0-8 off-band entries per row), "
+                       f"k U{{1..32}}, FP32 values / X / accumulation, one persistent launch replayed from a HIP graph",
+           "matrices": count, "rows": int(sum(A.n_rows for A in mats)), "nnz": int(sum(A.nnz for A in mats)),
+           "flop": B.flop, "algo_bytes": B.algo_bytes, "ms_per_step": round(step_ms, 4),
+           "gbs": round(B.algo_bytes / step_ms / 1e6, 1),
+           "pct_hbm_peak": round(100 * B.algo_bytes / step_ms / 1e6 / HBM_PEAK_GBS, 2),
+           "gflops": round(B.flop / step_ms / 1e6, 1), "tasks": B.ntasks,
+           "formats": {f: fmts.count(f) for f in set(fmts)}, "generate_s": round(gen_s, 1),
+           "create_and_tune_s": round(create_s, 1), "kernel": "k_batch_spmm"}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as orc
+    Xh, Yh = X.double().cpu(), Y.double().cpu()
+    worst = 0.0
+    for i in range(0, count, max(1, count // 16)):
+        A, k = mats[i], ks[i]
+        o, oy = int(B.x_off[i]), int(B.y_off[i])
+        Xi = Xh[o:o + A.n_cols * k].view(A.n_cols, k).numpy()
+        A64 = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, A.val.astype(np.float64))
+        want = orc.spmm(A64, Xi)
+        scale = orc.spmm(Csr(A.n_rows, A.n_cols, A.rowptr, A.col, np.abs(A64.val)), np.abs(Xi))
+        got = Yh[oy:oy + A.n_rows * k].view(A.n_rows, k).numpy()
+        worst = max(worst, float(np.max(np.abs(got - want) / (scale + 1e-30))))
+    out["max_scaled_err_sampled"] = worst
+    if cpu:
+        t = time.perf_counter()
+        done_flop, used = 0, 0
+        for i in range(count):
+            A, k = mats[i], ks[i]
+            A64 = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, A.val.astype(np.float64))
+            Xi = np.random.default_rng(i).standard_normal((A.n_cols, k))
+            t0 = time.perf_counter()
+            orc.spmm(A64, Xi)
+            used += time.perf_counter() - t0
+            done_flop += 2 * A.nnz * k
+            if time.perf_counter() - t > cpu_seconds:
+                break
+        cpu_ms = B.flop / done_flop * used * 1e3
+        out["cpu_baseline"] = {"ms_per_step": round(cpu_ms, 1), "gflops": round(done_flop / used / 1e9, 3),
+                               "kind": "port", "cores": 1,
+                               "sample": f"oracle CSR SpMM (FP64, one thread) over matrices 0..{i} of the batch "
+                                         f"({done_flop:.3e} flop), scaled by flop to the whole batch"}
+        out["gpu_over_cpu"] = round(cpu_ms / step_ms, 1)
+    B.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +318,8 @@ def main():
                     help="comma list of <fmt>_<value type>; the first one is the headline")
     ap.add_argument("--lespgemm-scale", type=int, default=16, help="R-MAT scale of the LeSpGEMM line (0: skip)")
     ap.add_argument("--lespgemm-iters", type=int, default=5)
+    ap.add_argument("--autograd-scale", type=int, default=20, help="R-MAT scale of the autograd SpMM line (0: skip)")
+    ap.add_argument("--batch-count", type=int, default=512, help="matrices of the batched SpMM line (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -236,6 +395,19 @@ def main():
         lg = lespgemm_measure(args.lespgemm_scale, args.lespgemm_iters, args.cpu_threads, not args.no_cpu)
         log(f"lespgemm done in {time.time() - t:.1f}s: {lg}")
 
+    ag = None
+    if args.autograd_scale > 0:
+        t = time.time()
+        ag = autograd_measure(args.autograd_scale, 64, 5, not args.no_cpu)
+        log(f"autograd spmm done in {time.time() - t:.1f}s: {ag}")
+
+    bt = None
+    if args.batch_count > 0:
+        t = time.time()
+        bt = batch_measure(args.batch_count, max(5, args.steps // 10), max(2, args.warmup // 5), not args.no_cpu,
+                           args.cpu_seconds)
+        log(f"batched spmm done in {time.time() - t:.1f}s: {bt}")
+
     out = {
         "metric": METRIC,
         "value": round(head["gbs"], 1),
@@ -265,6 +437,8 @@ def main():
         "fp32_sell_over_fp64_csr": (round(res["csr_f64"]["kernel_ms"] / res["sell_f32"]["kernel_ms"], 3)
                                     if "csr_f64" in res and "sell_f32" in res else None),
         "lespgemm": lg,
+        "autograd_spmm": ag,
+        "batch_spmm": bt,
     }
     print(json.dumps(out), flush=True)
 

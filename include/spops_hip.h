@@ -128,6 +128,32 @@ int spops_bsr_create_from_blocks(spops_matrix *A, int64_t n_rows, int64_t n_cols
  * follows the value type (see top).  Asynchronous on `stream`. */
 int spops_spmv(spops_matrix A, double alpha, const void *d_x, double beta, void *d_y, void *stream);
 
+/* ---- batched SpMM ---------------------------------------------------------------------------
+ * No reference counterpart as an entry point: many independent matrices, each with its own
+ * number of right-hand sides k_i, multiplied in one persistent launch (Y_i = alpha A_i X_i +
+ * beta Y_i; each product is SpOps_csr, src/spmv_csr.cpp:165-185, on k_i columns).  FP32
+ * values, X and Y.  X_i (n_cols_i x k_i) and Y_i (n_rows_i x k_i) are row-major blocks of one
+ * packed X and one packed Y buffer at the offsets spops_batch_plan reports.  Per matrix an
+ * autotuner picks the format (CSR lane groups or SELL-64), sigma and lane-group width:
+ * tune = 0 uses the feature rule, tune = 1 times the candidates per feature class at
+ * creation. */
+typedef struct spops_batch_s *spops_batch;
+int spops_batch_create(spops_batch *out, int64_t nmat, const int64_t *n_rows, const int64_t *n_cols, const int64_t *k,
+                       int index_type, const void *const *rowptr, const void *const *col, const void *const *values,
+                       int in_value_type, int tune);
+/* packed lengths (floats) of X and Y, 2*sum(nnz_i*k_i), algorithmic bytes, tasks */
+int spops_batch_info(spops_batch B, int64_t *x_len, int64_t *y_len, int64_t *flop, int64_t *algo_bytes,
+                     int64_t *ntasks);
+/* per matrix: X / Y offsets (floats), format (0 CSR lane groups, 1 SELL-64), lane-group width,
+ * sigma, feature class; any pointer may be NULL */
+int spops_batch_plan(spops_batch B, int64_t *x_off, int64_t *y_off, int32_t *fmt, int32_t *gs, int32_t *sigma,
+                     int32_t *cls);
+int spops_batch_spmm(spops_batch B, float alpha, const float *d_X, float beta, float *d_Y, void *stream);
+/* capture the launch on `stream` into a HIP graph kept in B; replays reuse X, Y, alpha, beta */
+int spops_batch_capture(spops_batch B, float alpha, const float *d_X, float beta, float *d_Y, void *stream);
+int spops_batch_replay(spops_batch B, void *stream);
+int spops_batch_destroy(spops_batch B);
+
 /* ---- multi-RHS and differentiable forms (CSR matrices) ------------------------------------
  * No reference counterpart as entry points: SpOps_csr (src/spmv_csr.cpp:165-185) applied to k
  * right-hand sides, its gradient with respect to the values (SDDMM on A's pattern) and the

@@ -153,22 +153,35 @@ def rmat_symnorm(scale: int = 20, edge_factor: int = 32, seed: int = 0,
 
 def band_matrix(n: int, half_width: int, extra_max: int, rng, dtype=np.float32,
                 index_dtype=np.int32) -> Csr:
-    """Band |i-j| <= half_width plus U{0..extra_max} uniform off-band entries per row."""
+    """Band |i-j| <= half_width plus U{0..extra_max} uniform off-band entries per row
+    (duplicates and extras inside the band merged).  Built without a global sort: per row,
+    the sorted extras left of the band, the band, then the extras right of it."""
+    w = int(half_width)
     i = np.arange(n, dtype=np.int64)
-    offs = np.arange(-half_width, half_width + 1, dtype=np.int64)
-    bc = i[:, None] + offs[None, :]
-    ok = (bc >= 0) & (bc < n)
-    brow = np.broadcast_to(i[:, None], bc.shape)[ok]
-    bcol = bc[ok]
+    lo = np.maximum(i - w, 0)
+    blen = np.minimum(i + w, n - 1) - lo + 1
     extra = rng.integers(0, extra_max + 1, n)
     erow = np.repeat(i, extra)
     ecol = rng.integers(0, n, erow.size, dtype=np.int64)
-    rows = np.concatenate([brow, erow])
-    cols = np.concatenate([bcol, ecol])
-    key = np.unique(rows * n + cols)
-    rows, cols = key // n, key % n
-    vals = rng.standard_normal(rows.size).astype(dtype)
-    return _finish(n, n, rows, cols, vals, index_dtype)
+    keep = np.abs(ecol - erow) > w
+    key = np.unique(erow[keep] * n + ecol[keep])
+    erow, ecol = key // n, key % n
+    left = ecol < erow
+    nleft = np.bincount(erow[left], minlength=n).astype(np.int64)
+    nex = np.bincount(erow, minlength=n).astype(np.int64)
+    counts = blen + nex
+    rowptr = np.zeros(n + 1, np.int64)
+    np.cumsum(counts, out=rowptr[1:])
+    nnz = int(rowptr[-1])
+    # band entries: position p of row r holds column p - (rowptr[r] + nleft[r]) + lo[r]
+    cols = np.arange(nnz, dtype=np.int64) + np.repeat(lo - rowptr[:-1] - nleft, counts)
+    # extras: rank inside the row's sorted extras
+    estart = np.zeros(n + 1, np.int64)
+    np.cumsum(nex, out=estart[1:])
+    rank = np.arange(erow.size, dtype=np.int64) - estart[erow]
+    cols[rowptr[erow] + np.where(left, rank, blen[erow] + rank)] = ecol
+    vals = rng.standard_normal(nnz, dtype=np.float32 if dtype == np.float32 else np.float64).astype(dtype, copy=False)
+    return Csr(n, n, rowptr.astype(index_dtype), cols.astype(index_dtype), vals)
 
 
 def band_batch(count: int = 512, nmin: int = 1000, nmax: int = 64000, wmax: int = 64,
