@@ -273,13 +273,12 @@ def batch_measure(count, steps, warmup, cpu, cpu_seconds):
     Xh, Yh = X.double().cpu(), Y.double().cpu()
     worst = 0.0
     for i in range(0, count, max(1, count // 16)):
-        A, k = mats[i], ks[i]
-        o, oy = int(B.x_off[i]), int(B.y_off[i])
-        Xi = Xh[o:o + A.n_cols * k].view(A.n_cols, k).numpy()
+        A = mats[i]
+        Xi = B.x_view(Xh, i).numpy()
         A64 = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, A.val.astype(np.float64))
         want = orc.spmm(A64, Xi)
         scale = orc.spmm(Csr(A.n_rows, A.n_cols, A.rowptr, A.col, np.abs(A64.val)), np.abs(Xi))
-        got = Yh[oy:oy + A.n_rows * k].view(A.n_rows, k).numpy()
+        got = B.y_view(Yh, i).numpy()
         worst = max(worst, float(np.max(np.abs(got - want) / (scale + 1e-30))))
     out["max_scaled_err_sampled"] = worst
     if cpu:

@@ -132,11 +132,12 @@ int spops_spmv(spops_matrix A, double alpha, const void *d_x, double beta, void 
  * No reference counterpart as an entry point: many independent matrices, each with its own
  * number of right-hand sides k_i, multiplied in one persistent launch (Y_i = alpha A_i X_i +
  * beta Y_i; each product is SpOps_csr, src/spmv_csr.cpp:165-185, on k_i columns).  FP32
- * values, X and Y.  X_i (n_cols_i x k_i) and Y_i (n_rows_i x k_i) are row-major blocks of one
- * packed X and one packed Y buffer at the offsets spops_batch_plan reports.  Per matrix an
- * autotuner picks the format (CSR lane groups or SELL-64), sigma and lane-group width:
- * tune = 0 uses the feature rule, tune = 1 times the candidates per feature class at
- * creation. */
+ * values, X and Y.  X_i (n_cols_i x ld_i) and Y_i (n_rows_i x ld_i) are row-major blocks of one
+ * packed X and one packed Y buffer (16-byte aligned) at the offsets spops_batch_plan reports;
+ * ld_i >= k_i is the row stride (columns past k_i are padding).  Per matrix an autotuner picks
+ * the format (row chunks with {col, value} pairs, or CSR lane groups), the row ordering and
+ * the lanes per row: tune = 0 uses the default rule, tune = 1 times the candidates per
+ * feature class at creation. */
 typedef struct spops_batch_s *spops_batch;
 int spops_batch_create(spops_batch *out, int64_t nmat, const int64_t *n_rows, const int64_t *n_cols, const int64_t *k,
                        int index_type, const void *const *rowptr, const void *const *col, const void *const *values,
@@ -144,10 +145,11 @@ int spops_batch_create(spops_batch *out, int64_t nmat, const int64_t *n_rows, co
 /* packed lengths (floats) of X and Y, 2*sum(nnz_i*k_i), algorithmic bytes, tasks */
 int spops_batch_info(spops_batch B, int64_t *x_len, int64_t *y_len, int64_t *flop, int64_t *algo_bytes,
                      int64_t *ntasks);
-/* per matrix: X / Y offsets (floats), format (0 CSR lane groups, 1 SELL-64), lane-group width,
- * sigma, feature class; any pointer may be NULL */
-int spops_batch_plan(spops_batch B, int64_t *x_off, int64_t *y_off, int32_t *fmt, int32_t *gs, int32_t *sigma,
-                     int32_t *cls);
+/* per matrix: X / Y offsets (floats), row stride ld, format (0 CSR lane groups, 1 row
+ * chunks), lanes per row, sigma (rows ordered by length inside windows of sigma rows; 1 = no
+ * reordering), feature class; any pointer may be NULL */
+int spops_batch_plan(spops_batch B, int64_t *x_off, int64_t *y_off, int32_t *ld, int32_t *fmt, int32_t *lanes,
+                     int32_t *sigma, int32_t *cls);
 int spops_batch_spmm(spops_batch B, float alpha, const float *d_X, float beta, float *d_Y, void *stream);
 /* capture the launch on `stream` into a HIP graph kept in B; replays reuse X, Y, alpha, beta */
 int spops_batch_capture(spops_batch B, float alpha, const float *d_X, float beta, float *d_Y, void *stream);

@@ -61,7 +61,7 @@ _SIGNATURES = [
     ("spops_csr_set_values", c_int, [c_vp, c_vp, c_int, c_vp]),
     ("spops_batch_create", c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_int]),
     ("spops_batch_info", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
-    ("spops_batch_plan", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    ("spops_batch_plan", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("spops_batch_spmm", c_int, [c_vp, ctypes.c_float, c_vp, ctypes.c_float, c_vp, c_vp]),
     ("spops_batch_capture", c_int, [c_vp, ctypes.c_float, c_vp, ctypes.c_float, c_vp, c_vp]),
     ("spops_batch_replay", c_int, [c_vp, c_vp]),

@@ -3,7 +3,7 @@ captured HIP graph), with the per-matrix format / tile autotuner of the library.
 
     B = BatchSpMM(mats, ks, tune=True)        # mats: list of Csr, ks: RHS count per matrix
     X = B.new_x(); Y = B.new_y()              # packed device buffers
-    B.x_view(X, i)                            # (n_cols_i, k_i) view of matrix i's block
+    B.x_view(X, i)                            # (n_cols_i, k_i) view of matrix i's block (row stride ld_i)
     B.spmm(X, Y)                              # Y_i = A_i X_i for every i, one launch
     B.capture(X, Y, stream); B.replay(stream) # the same launch as a HIP graph
 """
@@ -17,7 +17,7 @@ import torch
 from . import _lib as L
 from .gen import Csr
 
-FORMATS = {0: "csr_groups", 1: "sell64"}
+FORMATS = {0: "csr_groups", 1: "chunks"}
 
 
 class BatchSpMM:
@@ -50,12 +50,13 @@ class BatchSpMM:
         self.x_len, self.y_len, self.flop, self.algo_bytes, self.ntasks = [v.value for v in vals]
         self.x_off = np.zeros(n, np.int64)
         self.y_off = np.zeros(n, np.int64)
+        self.ld = np.zeros(n, np.int32)
         self.fmt = np.zeros(n, np.int32)
-        self.gs = np.zeros(n, np.int32)
+        self.lanes = np.zeros(n, np.int32)
         self.sigma = np.zeros(n, np.int32)
         self.cls = np.zeros(n, np.int32)
-        L.call("spops_batch_plan", h, self.x_off.ctypes.data, self.y_off.ctypes.data, self.fmt.ctypes.data,
-               self.gs.ctypes.data, self.sigma.ctypes.data, self.cls.ctypes.data)
+        L.call("spops_batch_plan", h, self.x_off.ctypes.data, self.y_off.ctypes.data, self.ld.ctypes.data,
+               self.fmt.ctypes.data, self.lanes.ctypes.data, self.sigma.ctypes.data, self.cls.ctypes.data)
 
     def new_x(self, device="cuda:0"):
         return torch.zeros(max(self.x_len, 1), dtype=torch.float32, device=device)
@@ -64,17 +65,19 @@ class BatchSpMM:
         return torch.zeros(max(self.y_len, 1), dtype=torch.float32, device=device)
 
     def x_view(self, X: torch.Tensor, i: int) -> torch.Tensor:
-        o, k = int(self.x_off[i]), self.ks[i]
-        return X[o:o + self.shapes[i][1] * k].view(self.shapes[i][1], k)
+        o, ld, n = int(self.x_off[i]), int(self.ld[i]), self.shapes[i][1]
+        return X[o:o + n * ld].view(n, ld)[:, :self.ks[i]]
 
     def y_view(self, Y: torch.Tensor, i: int) -> torch.Tensor:
-        o, k = int(self.y_off[i]), self.ks[i]
-        return Y[o:o + self.shapes[i][0] * k].view(self.shapes[i][0], k)
+        o, ld, n = int(self.y_off[i]), int(self.ld[i]), self.shapes[i][0]
+        return Y[o:o + n * ld].view(n, ld)[:, :self.ks[i]]
 
     def _check(self, X, Y):
         for T, n in ((X, self.x_len), (Y, self.y_len)):
-            if T.dtype != torch.float32 or not T.is_cuda or not T.is_contiguous() or T.numel() < n:
-                raise ValueError("X / Y must be contiguous float32 device buffers of x_len / y_len elements")
+            if T.dtype != torch.float32 or not T.is_cuda or not T.is_contiguous() or T.numel() < n \
+                    or T.data_ptr() % 16:
+                raise ValueError("X / Y must be contiguous, 16-byte aligned float32 device buffers of x_len / y_len "
+                                 "elements")
 
     def spmm(self, X: torch.Tensor, Y: torch.Tensor, alpha: float = 1.0, beta: float = 0.0, stream=None):
         self._check(X, Y)

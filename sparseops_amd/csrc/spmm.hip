@@ -7,12 +7,16 @@
 //
 // X, Y, G in FP64, FP32 or BF16; accumulation in FP64 when the matrix or X is FP64, else
 // FP32.  Kernels:
-//   k_spmm_rows : one row per lane group of GS lanes (GS = next power of two >= k, capped at
-//                 64; 64 / GS rows per wave), lanes over the k columns, eight row entries
-//                 in flight (their X rows are independent loads).  Rows longer than
-//                 kCsrLongRow entries are left to
-//   k_spmm_long : one workgroup per long row, the 4 waves split its entries, LDS reduction.
-//   k_sddmm     : one wave per row, lanes over the k columns; per entry a wave reduction.
+//   k_spmm_rows    : one row per group of L lanes (64 / L rows per wave), each lane covering
+//                    VEC consecutive columns with one 16-byte load per entry when k and the
+//                    alignment of X and Y allow it (else one column per lane), eight row
+//                    entries in flight; rows ordered by length inside windows of
+//                    kSpmmWindow rows.  Rows longer than kSpmmLongRow entries are left to
+//   k_spmm_seg     : one workgroup per segment of kSpmmSegment entries of a long row, its
+//                    partial row of sums to a scratch buffer, then
+//   k_spmm_seg_sum : one workgroup per long row adds its segments in order (deterministic).
+//   k_sddmm        : entry-parallel, a group of L lanes per entry with 16-byte loads of the
+//                    G and X rows, cross-lane sum.
 #include <algorithm>
 #include <memory>
 #include <numeric>
@@ -51,25 +55,41 @@ template <> struct Ty<SPOPS_VALUE_BF16> {
 template <int VC, int XC>
 using AccT = typename std::conditional<VC == SPOPS_VALUE_F64 || XC == SPOPS_VALUE_F64, double, float>::type;
 
-template <int VC, int XC, int YC, typename RP, int GS>
+// VEC consecutive elements of one row of X, Y or G as one aligned vector access
+template <int C, int VEC> struct Vec {
+  using T = typename Ty<C>::T;
+  struct alignas(VEC * sizeof(T)) V { T e[VEC]; };
+};
+
+// Rows of at most kSpmmLongRow entries, taken in `order` (by length inside windows of rows,
+// so the rows of a wave have about the same length).  A row is served by a group of L lanes
+// (64 / L rows per wave); lane q of the group covers columns c .. c + VEC - 1 with
+// c = cb + q VEC, loading them as one vector per entry.  U entries of the row are in flight.
+template <int VC, int XC, int YC, typename RP, int VEC>
 __global__ __launch_bounds__(kBlock) void k_spmm_rows(const RP *__restrict__ rowptr, const int *__restrict__ col,
-                                                      const typename Ty<VC>::T *__restrict__ val, int64_t n_rows, int k,
+                                                      const typename Ty<VC>::T *__restrict__ val,
+                                                      const int *__restrict__ order, int64_t n_short, int k, int L,
                                                       const typename Ty<XC>::T *__restrict__ X,
                                                       typename Ty<YC>::T *__restrict__ Y, double alpha, double beta) {
   using Acc = AccT<VC, XC>;
-  constexpr int R = kWave / GS, U = 8;
-  const int lane = lane_id(), g = lane / GS, c0 = lane % GS;
-  const int64_t row = ((int64_t)blockIdx.x * (kBlock / kWave) + wave_id()) * R + g;
-  if (row >= n_rows) return;                        // lane groups past the last row
+  using XV = typename Vec<XC, VEC>::V;
+  using YV = typename Vec<YC, VEC>::V;
+  constexpr int U = 8;
+  const int lane = lane_id(), R = kWave / L, g = lane / L, q = lane % L;
+  const int64_t slot = ((int64_t)blockIdx.x * (kBlock / kWave) + wave_id()) * R + g;
+  if (slot >= n_short) return;                      // lane groups past the last row
+  const int64_t row = order[slot];
   const int64_t p0 = (int64_t)rowptr[row], p1 = (int64_t)rowptr[row + 1];
-  if (p1 - p0 > kCsrLongRow) return;                // a long row: k_spmm_long writes it
-  for (int cb = 0; cb < k; cb += GS) {
-    const int c = cb + c0;
-    const bool cv = c < k;
-    Acc acc = 0;
+  for (int cb = 0; cb < k; cb += L * VEC) {
+    const int c = cb + q * VEC;
+    const bool cv = c < k;                          // k % VEC == 0: a vector is all in or all out
+    Acc acc[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) acc[e] = 0;
     for (int64_t p = p0; p < p1; p += U) {
       int cc[U];
-      Acc vv[U], xv[U];
+      Acc vv[U];
+      XV xv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const bool ok = p + u < p1;
@@ -77,102 +97,182 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rows(const RP *__restrict__ row
         vv[u] = ok ? (Acc)Ty<VC>::ld(val[p + u]) : Acc(0);
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-        xv[u] = (cv && p + u < p1) ? (Acc)Ty<XC>::ld(X[(int64_t)cc[u] * k + c]) : Acc(0);
+      for (int u = 0; u < U; ++u) {
+        XV t{};
+        if (cv && p + u < p1) t = *reinterpret_cast<const XV *>(X + (int64_t)cc[u] * k + c);
+        xv[u] = t;
+      }
 #pragma unroll
-      for (int u = 0; u < U; ++u) acc = vv[u] * xv[u] + acc;
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) acc[e] = vv[u] * (Acc)Ty<XC>::ld(xv[u].e[e]) + acc[e];
     }
     if (cv) {
-      double o = alpha * (double)acc;
-      if (beta != 0.0) o += beta * Ty<YC>::ld(Y[row * k + c]);
-      Y[row * k + c] = Ty<YC>::st(o);
+      YV *dst = reinterpret_cast<YV *>(Y + row * k + c);
+      YV o;
+      if (beta != 0.0) {
+        const YV old = *dst;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) o.e[e] = Ty<YC>::st(alpha * (double)acc[e] + beta * Ty<YC>::ld(old.e[e]));
+      } else {
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) o.e[e] = Ty<YC>::st(alpha * (double)acc[e]);
+      }
+      *dst = o;
     }
   }
 }
 
-template <int VC, int XC, int YC, typename RP>
-__global__ __launch_bounds__(kBlock) void k_spmm_long(const RP *__restrict__ rowptr, const int *__restrict__ col,
-                                                      const typename Ty<VC>::T *__restrict__ val,
-                                                      const int *__restrict__ long_rows, int k,
-                                                      const typename Ty<XC>::T *__restrict__ X,
-                                                      typename Ty<YC>::T *__restrict__ Y, double alpha, double beta) {
+// Long rows, pass 1: one workgroup per segment of kSpmmSegment entries of a long row.  The
+// wave's 64 / L entry slots and the 4 waves each take every (4 * 64 / L)-th entry; the slots
+// are summed with cross-lane adds, the waves through LDS, and the segment's row of partial
+// sums is written to part[segment * k ...].
+template <int VC, int XC, typename RP, int VEC>
+__global__ __launch_bounds__(kBlock) void k_spmm_seg(const RP *__restrict__ rowptr, const int *__restrict__ col,
+                                                     const typename Ty<VC>::T *__restrict__ val,
+                                                     const int *__restrict__ long_rows, const int2 *__restrict__ segs,
+                                                     int k, int L, const typename Ty<XC>::T *__restrict__ X,
+                                                     AccT<VC, XC> *__restrict__ part) {
   using Acc = AccT<VC, XC>;
-  constexpr int W = kBlock / kWave, U = 8;
-  __shared__ Acc red[W][kWave];
-  const int64_t row = long_rows[blockIdx.x];
-  const int64_t p0 = (int64_t)rowptr[row], p1 = (int64_t)rowptr[row + 1];
-  const int lane = lane_id(), w = wave_id();
-  for (int cb = 0; cb < k; cb += kWave) {
-    const int c = cb + lane;
+  using XV = typename Vec<XC, VEC>::V;
+  constexpr int W = kBlock / kWave, U = 4;
+  __shared__ Acc red[W][kWave * VEC];
+  const int2 sg = segs[blockIdx.x];
+  const int64_t row = long_rows[sg.x];
+  const int64_t q0 = (int64_t)rowptr[row] + sg.y;
+  const int64_t q1 = std::min<int64_t>((int64_t)rowptr[row + 1], q0 + kSpmmSegment);
+  const int lane = lane_id(), w = wave_id(), E = kWave / L, es = lane / L, q = lane % L;
+  const int64_t stride = (int64_t)W * E;
+  for (int cb = 0; cb < k; cb += L * VEC) {
+    const int c = cb + q * VEC;
     const bool cv = c < k;
-    Acc acc = 0;
-    for (int64_t p = p0 + (int64_t)w * U; p < p1; p += (int64_t)W * U) {
+    Acc acc[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) acc[e] = 0;
+    for (int64_t p = q0 + (int64_t)w * E + es; p < q1; p += stride * U) {
       int cc[U];
-      Acc vv[U], xv[U];
+      Acc vv[U];
+      XV xv[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const bool ok = p + u < p1;
-        cc[u] = ok ? col[p + u] : 0;
-        vv[u] = ok ? (Acc)Ty<VC>::ld(val[p + u]) : Acc(0);
+        const bool ok = p + u * stride < q1;
+        cc[u] = ok ? col[p + u * stride] : 0;
+        vv[u] = ok ? (Acc)Ty<VC>::ld(val[p + u * stride]) : Acc(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        XV t{};
+        if (cv && p + u * stride < q1) t = *reinterpret_cast<const XV *>(X + (int64_t)cc[u] * k + c);
+        xv[u] = t;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
-        xv[u] = (cv && p + u < p1) ? (Acc)Ty<XC>::ld(X[(int64_t)cc[u] * k + c]) : Acc(0);
 #pragma unroll
-      for (int u = 0; u < U; ++u) acc = vv[u] * xv[u] + acc;
+        for (int e = 0; e < VEC; ++e) acc[e] = vv[u] * (Acc)Ty<XC>::ld(xv[u].e[e]) + acc[e];
     }
-    red[w][lane] = acc;
+    for (int m = L; m < kWave; m <<= 1)
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) acc[e] += __shfl_xor(acc[e], m, kWave);
+    if (es == 0)
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) red[w][q * VEC + e] = acc[e];
     __syncthreads();
-    if (w == 0 && cv) {
-      Acc s = 0;
-      for (int q = 0; q < W; ++q) s += red[q][lane];
-      double o = alpha * (double)s;
-      if (beta != 0.0) o += beta * Ty<YC>::ld(Y[row * k + c]);
-      Y[row * k + c] = Ty<YC>::st(o);
+    if (w == 0 && es == 0 && cv) {
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) {
+        Acc t = red[0][q * VEC + e];
+#pragma unroll
+        for (int v = 1; v < W; ++v) t += red[v][q * VEC + e];
+        part[(int64_t)blockIdx.x * k + c + e] = t;
+      }
     }
     __syncthreads();
   }
 }
 
-// out[e] = sum_c G[row_e, c] * X[col_e, c]; one wave per row
-template <int GC, int XC, int OC, typename RP>
-__global__ __launch_bounds__(kBlock) void k_sddmm(const RP *__restrict__ rowptr, const int *__restrict__ col,
-                                                  int64_t n_rows, int k, const typename Ty<GC>::T *__restrict__ G,
+// Long rows, pass 2: one workgroup per long row sums its segments in order (deterministic)
+// and writes y = alpha * sum + beta * y.
+template <typename Acc, int YC>
+__global__ __launch_bounds__(kBlock) void k_spmm_seg_sum(const int *__restrict__ long_rows,
+                                                         const int *__restrict__ segoff, const Acc *__restrict__ part,
+                                                         int k, typename Ty<YC>::T *__restrict__ Y, double alpha,
+                                                         double beta) {
+  const int li = blockIdx.x;
+  const int64_t row = long_rows[li];
+  const int s0 = segoff[li], s1 = segoff[li + 1];
+  for (int c = threadIdx.x; c < k; c += kBlock) {
+    Acc t = 0;
+    for (int sg = s0; sg < s1; ++sg) t += part[(int64_t)sg * k + c];
+    double o = alpha * (double)t;
+    if (beta != 0.0) o += beta * Ty<YC>::ld(Y[row * k + c]);
+    Y[row * k + c] = Ty<YC>::st(o);
+  }
+}
+
+// out[e] = sum_c G[row_e, c] * X[col_e, c], entry-parallel: a group of L lanes per entry
+// (64 / L entries per wave step, U steps per wave), lane q of the group covering columns
+// c .. c + VEC - 1 (c = cb + q VEC) with one vector load from G's row and one from X's row;
+// the group is summed with cross-lane adds.  erow holds each entry's row.
+template <int GC, int XC, int OC, int VEC>
+__global__ __launch_bounds__(kBlock) void k_sddmm(const int *__restrict__ erow, const int *__restrict__ col,
+                                                  int64_t nnz, int k, int L, const typename Ty<GC>::T *__restrict__ G,
                                                   const typename Ty<XC>::T *__restrict__ X,
                                                   typename Ty<OC>::T *__restrict__ out) {
   using Acc = AccT<GC, XC>;
-  constexpr int U = 8;
-  const int64_t row = (int64_t)blockIdx.x * (kBlock / kWave) + wave_id();
-  if (row >= n_rows) return;
-  const int lane = lane_id();
-  const int64_t p0 = (int64_t)rowptr[row], p1 = (int64_t)rowptr[row + 1];
-  for (int64_t p = p0; p < p1; p += U) {
-    int cc[U];
+  using GV = typename Vec<GC, VEC>::V;
+  using XV = typename Vec<XC, VEC>::V;
+  constexpr int U = 4;
+  const int lane = lane_id(), E = kWave / L, es = lane / L, q = lane % L;
+  const int64_t base = ((int64_t)blockIdx.x * (kBlock / kWave) + wave_id()) * E * U + es;
+  int rr[U], cc[U];
+  bool ok[U];
+  Acc part[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) cc[u] = p + u < p1 ? col[p + u] : 0;
-    Acc part[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) part[u] = 0;
-    for (int cb = 0; cb < k; cb += kWave) {
-      const int c = cb + lane;
-      const bool cv = c < k;
-      const Acc gv = cv ? (Acc)Ty<GC>::ld(G[row * k + c]) : Acc(0);
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        part[u] += (cv && p + u < p1) ? gv * (Acc)Ty<XC>::ld(X[(int64_t)cc[u] * k + c]) : Acc(0);
-    }
+  for (int u = 0; u < U; ++u) {
+    const int64_t p = base + (int64_t)u * E;
+    ok[u] = p < nnz;
+    rr[u] = ok[u] ? erow[p] : 0;
+    cc[u] = ok[u] ? col[p] : 0;
+    part[u] = 0;
+  }
+  for (int cb = 0; cb < k; cb += L * VEC) {
+    const int c = cb + q * VEC;
+    const bool cv = c < k;
+    GV gv[U];
+    XV xv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-#pragma unroll
-      for (int m = 32; m > 0; m >>= 1) part[u] += __shfl_xor(part[u], m, kWave);
+      GV a{};
+      XV b{};
+      if (cv && ok[u]) {
+        a = *reinterpret_cast<const GV *>(G + (int64_t)rr[u] * k + c);
+        b = *reinterpret_cast<const XV *>(X + (int64_t)cc[u] * k + c);
+      }
+      gv[u] = a;
+      xv[u] = b;
     }
-    if (lane < U && p + lane < p1) {
-      Acc v = part[0];
 #pragma unroll
-      for (int u = 1; u < U; ++u) v = lane == u ? part[u] : v;
-      out[p + lane] = Ty<OC>::st((double)v);
-    }
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) part[u] += (Acc)Ty<GC>::ld(gv[u].e[e]) * (Acc)Ty<XC>::ld(xv[u].e[e]);
   }
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    for (int m = 1; m < L; m <<= 1) part[u] += __shfl_xor(part[u], m, kWave);
+  if (q == 0) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (ok[u]) out[base + (int64_t)u * E] = Ty<OC>::st((double)part[u]);
+  }
+}
+
+// erow[e] = row of entry e; one wave per row
+template <typename RP>
+__global__ __launch_bounds__(kBlock) void k_entry_rows(const RP *__restrict__ rowptr, int64_t n_rows,
+                                                       int *__restrict__ erow) {
+  const int64_t row = (int64_t)blockIdx.x * (kBlock / kWave) + wave_id();
+  if (row >= n_rows) return;
+  for (int64_t p = (int64_t)rowptr[row] + lane_id(); p < (int64_t)rowptr[row + 1]; p += kWave) erow[p] = (int)row;
 }
 
 template <int SC, int DC, typename IDX>
@@ -198,36 +298,53 @@ void require_type(int t, const char *what) {
 }
 
 // ---- dispatch on (value, x, y) types and rowptr width -----------------------------------
-template <int VC, int XC, int YC, typename RP>
-void spmm_launch(const spops_matrix_s &A, int k, double alpha, const void *X, double beta, void *Y, hipStream_t s) {
+inline bool aligned_to(const void *p, size_t a) { return ((uintptr_t)p % a) == 0; }
+
+// lanes per row (or entry) for k columns, VEC columns per lane: a power of two, at most 64
+inline int lanes_for(int k, int vec) {
+  const int need = (k + vec - 1) / vec;
+  int L = 1;
+  while (L < need && L < kWave) L <<= 1;
+  return L;
+}
+
+template <int VC, int XC, int YC, typename RP, int VEC>
+void spmm_launch_vec(const spops_matrix_s &A, int k, double alpha, const void *X, double beta, void *Y,
+                     hipStream_t s) {
+  using Acc = AccT<VC, XC>;
   const auto &C = A.csr;
   const RP *rp = C.rowptr.as<RP>();
   const auto *v = C.val.as<typename Ty<VC>::T>();
   const auto *x = static_cast<const typename Ty<XC>::T *>(X);
   auto *y = static_cast<typename Ty<YC>::T *>(Y);
-  const int64_t n = A.n_rows;
-  if (n > 0) {
-    const int GS = k <= 4 ? 4 : k <= 8 ? 8 : k <= 16 ? 16 : k <= 32 ? 32 : 64;
-    const int64_t rows_per_block = (int64_t)(kBlock / kWave) * (kWave / GS);
-    const unsigned grid = nblk(n, (int)rows_per_block);
-#define SPOPS_SPMM_ROWS(G)                                                                                    \
-  hipLaunchKernelGGL((k_spmm_rows<VC, XC, YC, RP, G>), dim3(grid), dim3(kBlock), 0, s, rp, C.col.as<int>(), v, n, k, \
-                     x, y, alpha, beta)
-    switch (GS) {
-      case 4: SPOPS_SPMM_ROWS(4); break;
-      case 8: SPOPS_SPMM_ROWS(8); break;
-      case 16: SPOPS_SPMM_ROWS(16); break;
-      case 32: SPOPS_SPMM_ROWS(32); break;
-      default: SPOPS_SPMM_ROWS(64); break;
-    }
-#undef SPOPS_SPMM_ROWS
+  const int L = lanes_for(k, VEC);
+  if (C.nshort > 0) {
+    const int64_t rows_per_block = (int64_t)(kBlock / kWave) * (kWave / L);
+    hipLaunchKernelGGL((k_spmm_rows<VC, XC, YC, RP, VEC>), dim3(nblk(C.nshort, (int)rows_per_block)), dim3(kBlock), 0,
+                       s, rp, C.col.as<int>(), v, C.sorder.as<int>(), (int64_t)C.nshort, k, L, x, y, alpha, beta);
     SPOPS_HIP_CHECK(hipGetLastError());
   }
-  if (C.nlong > 0) {
-    hipLaunchKernelGGL((k_spmm_long<VC, XC, YC, RP>), dim3((unsigned)C.nlong), dim3(kBlock), 0, s, rp,
-                       C.col.as<int>(), v, C.longs.as<int>(), k, x, y, alpha, beta);
+  if (C.nslong > 0) {
+    const size_t need = (size_t)C.nlseg * (size_t)k * sizeof(Acc);
+    if (C.spmm_part.bytes < need) C.spmm_part.alloc(need);
+    hipLaunchKernelGGL((k_spmm_seg<VC, XC, RP, VEC>), dim3((unsigned)C.nlseg), dim3(kBlock), 0, s, rp, C.col.as<int>(),
+                       v, C.slongs.as<int>(), C.lsegs.as<int2>(), k, L, x, C.spmm_part.as<Acc>());
+    SPOPS_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL((k_spmm_seg_sum<Acc, YC>), dim3((unsigned)C.nslong), dim3(kBlock), 0, s, C.slongs.as<int>(),
+                       C.lsegoff.as<int>(), C.spmm_part.as<Acc>(), k, y, alpha, beta);
     SPOPS_HIP_CHECK(hipGetLastError());
   }
+}
+
+// One vector access covers 16 bytes of X when k is a multiple of that many elements and X
+// and Y are aligned for it; otherwise one element per lane.
+template <int VC, int XC, int YC, typename RP>
+void spmm_launch(const spops_matrix_s &A, int k, double alpha, const void *X, double beta, void *Y, hipStream_t s) {
+  constexpr int V = 16 / (int)sizeof(typename Ty<XC>::T);
+  if (k % V == 0 && aligned_to(X, 16) && aligned_to(Y, V * sizeof(typename Ty<YC>::T)))
+    spmm_launch_vec<VC, XC, YC, RP, V>(A, k, alpha, X, beta, Y, s);
+  else
+    spmm_launch_vec<VC, XC, YC, RP, 1>(A, k, alpha, X, beta, Y, s);
 }
 
 template <int VC, int XC, typename RP>
@@ -259,21 +376,41 @@ void spmm_v(const spops_matrix_s &A, int k, double alpha, const void *X, int xt,
   }
 }
 
+template <typename RP>
+void build_entry_rows(const spops_matrix_s &A, hipStream_t s) {
+  const auto &C = A.csr;
+  C.erow.alloc((size_t)std::max<int64_t>(A.nnz, 1) * 4);
+  hipLaunchKernelGGL((k_entry_rows<RP>), dim3(nblk(A.n_rows, kBlock / kWave)), dim3(kBlock), 0, s, C.rowptr.as<RP>(),
+                     A.n_rows, C.erow.as<int>());
+  SPOPS_HIP_CHECK(hipGetLastError());
+  SPOPS_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+template <int GC, int XC, int OC, int VEC>
+void sddmm_launch_vec(const spops_matrix_s &A, int k, const void *G, const void *X, void *out, hipStream_t s) {
+  constexpr int U = 4;
+  const int L = lanes_for(k, VEC);
+  const int64_t per_block = (int64_t)(kBlock / kWave) * (kWave / L) * U;
+  hipLaunchKernelGGL((k_sddmm<GC, XC, OC, VEC>), dim3(nblk(A.nnz, (int)per_block)), dim3(kBlock), 0, s,
+                     A.csr.erow.as<int>(), A.csr.col.as<int>(), A.nnz, k, L,
+                     static_cast<const typename Ty<GC>::T *>(G), static_cast<const typename Ty<XC>::T *>(X),
+                     static_cast<typename Ty<OC>::T *>(out));
+  SPOPS_HIP_CHECK(hipGetLastError());
+}
+
 template <int GC, int XC, int OC>
 void sddmm_launch(const spops_matrix_s &A, int k, const void *G, const void *X, void *out, hipStream_t s) {
-  const int64_t n = A.n_rows;
-  if (n == 0) return;
-  const unsigned grid = nblk(n, kBlock / kWave);
-  const auto *g = static_cast<const typename Ty<GC>::T *>(G);
-  const auto *x = static_cast<const typename Ty<XC>::T *>(X);
-  auto *o = static_cast<typename Ty<OC>::T *>(out);
-  if (A.csr.rowptr64)
-    hipLaunchKernelGGL((k_sddmm<GC, XC, OC, long>), dim3(grid), dim3(kBlock), 0, s, A.csr.rowptr.as<long>(),
-                       A.csr.col.as<int>(), n, k, g, x, o);
+  if (A.nnz == 0) return;
+  SPOPS_REQUIRE(A.nnz < (int64_t)INT32_MAX * 16, SPOPS_ERROR_LIMIT, "SDDMM grid limit");
+  if (A.csr.erow.p == nullptr) {
+    if (A.csr.rowptr64) build_entry_rows<long>(A, s);
+    else build_entry_rows<int>(A, s);
+  }
+  constexpr int V = 16 / (int)sizeof(typename Ty<XC>::T);
+  if (k % V == 0 && aligned_to(X, 16) && aligned_to(G, V * sizeof(typename Ty<GC>::T)))
+    sddmm_launch_vec<GC, XC, OC, V>(A, k, G, X, out, s);
   else
-    hipLaunchKernelGGL((k_sddmm<GC, XC, OC, int>), dim3(grid), dim3(kBlock), 0, s, A.csr.rowptr.as<int>(),
-                       A.csr.col.as<int>(), n, k, g, x, o);
-  SPOPS_HIP_CHECK(hipGetLastError());
+    sddmm_launch_vec<GC, XC, OC, 1>(A, k, G, X, out, s);
 }
 
 template <int GC, int XC>
@@ -448,13 +585,7 @@ extern "C" int spops_csr_transpose(spops_matrix A, spops_matrix *AT) {
     // values follow A's through the permutation
     convert_values(A->csr.val.p, A->value_type, P.tperm.p, P.tperm64, nnz, P.val.p, A->value_type, nullptr);
     SPOPS_HIP_CHECK(hipDeviceSynchronize());
-    std::vector<int2> tasks;
-    std::vector<int> longs;
-    build_csr_plan(trp, m, tasks, longs);
-    P.tasks.upload(tasks);
-    P.longs.upload(longs);
-    P.ntasks = (int)tasks.size();
-    P.nlong = (int)longs.size();
+    upload_csr_plan(P, trp, m);
     T->slots = nnz;
     T->dev_bytes = P.rowptr.bytes + P.col.bytes + P.val.bytes + P.tasks.bytes + P.longs.bytes + P.tperm.bytes;
     T->algo_bytes = spmv_algo_bytes(*T, A->index_type == SPOPS_INDEX_I32 ? 4 : 8);

@@ -531,6 +531,35 @@ void build_csr_plan(const std::vector<int64_t> &rp, int64_t n_rows, std::vector<
   }
 }
 
+void upload_csr_plan(CsrDev &P, const std::vector<int64_t> &rp, int64_t n_rows) {
+  std::vector<int2> tasks, segs;
+  std::vector<int> longs, order, slongs, segoff{0};
+  build_csr_plan(rp, n_rows, tasks, longs);
+  for (int64_t w0 = 0; w0 < n_rows; w0 += kSpmmWindow) {
+    const size_t at = order.size();
+    for (int64_t r = w0; r < std::min(n_rows, w0 + kSpmmWindow); ++r) {
+      const int64_t len = rp[r + 1] - rp[r];
+      if (len <= kSpmmLongRow) { order.push_back((int)r); continue; }
+      slongs.push_back((int)r);
+      for (int64_t off = 0; off < len; off += kSpmmSegment) segs.push_back(make_int2((int)slongs.size() - 1, (int)off));
+      segoff.push_back((int)segs.size());
+    }
+    std::stable_sort(order.begin() + at, order.end(),
+                     [&](int a, int b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
+  }
+  P.tasks.upload(tasks);
+  P.longs.upload(longs);
+  P.sorder.upload(order);
+  P.slongs.upload(slongs);
+  P.lsegs.upload(segs);
+  P.lsegoff.upload(segoff);
+  P.ntasks = (int)tasks.size();
+  P.nlong = (int)longs.size();
+  P.nshort = (int)order.size();
+  P.nslong = (int)slongs.size();
+  P.nlseg = (int)segs.size();
+}
+
 // Row access for the sliced builder: entries of the row at SELL position g.
 struct RowSource {
   virtual ~RowSource() = default;
@@ -938,13 +967,7 @@ extern "C" int spops_csr_create(spops_matrix *A, int64_t n_rows, int64_t n_cols,
     v.reserve((size_t)nnz * value_bytes(vt));
     for (int64_t j = 0; j < nnz; ++j) pack_values(v, vt, host_value(values, in_vt, j));
     P.val.upload(v);
-    std::vector<int2> tasks;
-    std::vector<int> longs;
-    build_csr_plan(rp, n_rows, tasks, longs);
-    P.tasks.upload(tasks);
-    P.longs.upload(longs);
-    P.ntasks = (int)tasks.size();
-    P.nlong = (int)longs.size();
+    upload_csr_plan(P, rp, n_rows);
     M->slots = nnz;
     M->dev_bytes = P.rowptr.bytes + P.col.bytes + P.val.bytes + P.tasks.bytes + P.longs.bytes;
     M->algo_bytes = spmv_algo_bytes(*M, index_type == SPOPS_INDEX_I32 ? 4 : 8);

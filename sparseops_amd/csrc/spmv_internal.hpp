@@ -9,6 +9,9 @@ namespace spops {
 
 constexpr int kCsrStreamNnz = 512;     // entries staged per wave stream block
 constexpr int64_t kCsrLongRow = 2048;  // longer rows: one workgroup each
+constexpr int64_t kSpmmLongRow = 256;  // SpMM: longer rows are summed in segments
+constexpr int64_t kSpmmSegment = 512;  // SpMM: entries per segment of a long row
+constexpr int64_t kSpmmWindow = 4096;  // SpMM: short rows are ordered by length inside windows of this many rows
 constexpr int kBsrMaxBlock = 256;      // bR * bC limit of the wave-per-block-row kernel
 constexpr int64_t kRowSegment = (int64_t)1 << 24;   // entries per buffer resource of a long row
 
@@ -22,8 +25,14 @@ constexpr int slice_lane_entries(int vt) { return vt == SPOPS_VALUE_F64 ? 8 : 16
 struct CsrDev {
   DevBuf rowptr, col, val, tasks, longs;
   DevBuf tperm;                 // transposed view: entry e holds the source matrix's entry tperm[e]
+  DevBuf sorder;                // SpMM: the rows of at most kSpmmLongRow entries, by length inside windows
+  DevBuf slongs;                // SpMM: the longer rows
+  DevBuf lsegs;                 // SpMM: segments of the long rows, int2 {index in slongs, entry offset}
+  DevBuf lsegoff;               // SpMM: first segment of each long row (nslong + 1 ints)
+  mutable DevBuf spmm_part;     // SpMM: partial sums of the segments (grown on demand)
+  mutable DevBuf erow;          // SDDMM: row of every entry (built on first use)
   bool rowptr64 = false, tperm64 = false;
-  int ntasks = 0, nlong = 0;
+  int ntasks = 0, nlong = 0, nshort = 0, nslong = 0, nlseg = 0;
 };
 
 // The columns are cut into P slices of about equal entry count; slice s is served by the
@@ -71,6 +80,9 @@ struct spops_matrix_s {
 
 namespace spops {
 extern int64_t g_slice_bytes;   // x bytes per slice for matrices created from now on
+// Builds and uploads the CSR plan of a matrix: SpMV row tasks and long rows (> kCsrLongRow
+// entries); for SpMM the short-row order and the segments of the rows > kSpmmLongRow.
+void upload_csr_plan(CsrDev &P, const std::vector<int64_t> &rp, int64_t n_rows);
 void build_csr_plan(const std::vector<int64_t> &rp, int64_t n_rows, std::vector<int2> &tasks, std::vector<int> &longs);
 int64_t spmv_algo_bytes(const spops_matrix_s &M, int64_t sI);
 void launch_spmv(const spops_matrix_s &M, double alpha, const void *x, double beta, void *y, hipStream_t s);

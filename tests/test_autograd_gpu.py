@@ -190,3 +190,32 @@ def test_spmm_rejects_other_formats_and_shapes(mats, dev):
     C = S.csr(A, "f32")
     with pytest.raises(ValueError):
         C.spmm(X[:-1], Y)
+
+
+@pytest.mark.parametrize("offset", [0, 1, 2])
+def test_spmm_sddmm_unaligned_views(mats, offset, dev):
+    """X, Y and G as views at an element offset: the 16-byte vector path is used only when
+    the pointers allow it; every offset gives the same products."""
+    A = mats["edge"]
+    k = 32
+    rng = np.random.default_rng(offset)
+    Xn = rng.standard_normal((A.n_cols, k)).astype(np.float32)
+    Gn = rng.standard_normal((A.n_rows, k)).astype(np.float32)
+    xb = torch.zeros(A.n_cols * k + 8, device=dev)
+    xb[offset:offset + A.n_cols * k] = torch.tensor(Xn.ravel(), device=dev)
+    X = xb[offset:offset + A.n_cols * k].view(A.n_cols, k)
+    yb = torch.zeros(A.n_rows * k + 8, device=dev)
+    Y = yb[offset:offset + A.n_rows * k].view(A.n_rows, k)
+    gb = torch.zeros(A.n_rows * k + 8, device=dev)
+    gb[offset:offset + A.n_rows * k] = torch.tensor(Gn.ravel(), device=dev)
+    G = gb[offset:offset + A.n_rows * k].view(A.n_rows, k)
+    M = S.csr(A, "f32")
+    M.spmm(X, Y)
+    out = torch.zeros(A.nnz, device=dev)
+    M.sddmm(G, X, out)
+    Ar = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, A.val.astype(np.float32).astype(np.float64))
+    absA = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, np.abs(Ar.val))
+    X64, G64 = Xn.astype(np.float64), Gn.astype(np.float64)
+    _close(Y.double().cpu().numpy(), orc.spmm(Ar, X64), orc.spmm(absA, np.abs(X64)), 2e-6)
+    _close(out.double().cpu().numpy(), orc.sddmm(A, G64, X64), orc.sddmm(A, np.abs(G64), np.abs(X64)), 2e-6)
+    assert torch.all(yb[:offset] == 0) and torch.all(yb[offset + A.n_rows * k:] == 0)

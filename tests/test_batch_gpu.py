@@ -47,14 +47,13 @@ def _check(B, mats, X, Y, alpha=1.0, beta=0.0, Y0=None):
         if A.n_rows == 0:
             continue
         k = B.ks[i]
-        o, oy = int(B.x_off[i]), int(B.y_off[i])
-        Xi = Xh[o:o + A.n_cols * k].view(A.n_cols, k).numpy()
-        got = Yh[oy:oy + A.n_rows * k].view(A.n_rows, k).numpy()
+        Xi = B.x_view(Xh, i).numpy()
+        got = B.y_view(Yh, i).numpy()
         A64 = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, A.val.astype(np.float64))
         want = alpha * orc.spmm(A64, Xi)
         scale = abs(alpha) * orc.spmm(Csr(A.n_rows, A.n_cols, A.rowptr, A.col, np.abs(A64.val)), np.abs(Xi))
         if Y0 is not None:
-            y0 = Y0[oy:oy + A.n_rows * k].view(A.n_rows, k).double().cpu().numpy()
+            y0 = B.y_view(Y0.double().cpu(), i).numpy()
             want = want + beta * y0
             scale = scale + abs(beta) * np.abs(y0)
         err = np.abs(got - want) / (scale + 1e-30)
@@ -65,11 +64,14 @@ def _check(B, mats, X, Y, alpha=1.0, beta=0.0, Y0=None):
 def test_batch_matches_oracle(batch, tune, dev):
     mats, ks = batch
     B = BatchSpMM(mats, ks, tune=tune)
-    assert B.x_len == sum(A.n_cols * k for A, k in zip(mats, ks))
-    assert B.y_len == sum(A.n_rows * k for A, k in zip(mats, ks))
     assert B.flop == sum(2 * A.nnz * k for A, k in zip(mats, ks))
-    if not tune:      # the cost rule: SELL-64 for k <= 2, CSR lane groups otherwise
-        assert [f == "sell64" for f in B.formats()] == [k <= 2 for k in ks]
+    # packed blocks: 16-byte aligned, row stride ld >= k, disjoint, inside x_len / y_len
+    for off, dim, total in ((B.x_off, 1, B.x_len), (B.y_off, 0, B.y_len)):
+        spans = sorted((int(off[i]), int(off[i]) + B.shapes[i][dim] * int(B.ld[i])) for i in range(B.n))
+        assert all(a % 4 == 0 for a, _ in spans) and all(B.ld[i] >= ks[i] for i in range(B.n))
+        assert all(spans[j][1] <= spans[j + 1][0] for j in range(len(spans) - 1)) and spans[-1][1] <= total
+    if not tune:      # the default rule: row chunks for k <= 64, CSR lane groups beyond
+        assert [f == "chunks" for f in B.formats()] == [k <= 64 for k in ks]
     X, Y = B.new_x(), B.new_y()
     _fill(B, X, 0)
     Y.fill_(float("nan"))
@@ -110,7 +112,8 @@ def test_batch_graph_replays(batch, dev):
         s.synchronize()
         B.spmm(X, Ye)
         torch.cuda.synchronize()
-        assert torch.equal(Y, Ye), f"replay {rep} differs from the eager launch"
+        assert all(torch.equal(B.y_view(Y, i), B.y_view(Ye, i)) for i in range(B.n)), \
+            f"replay {rep} differs from the eager launch"
         _check(B, mats, X, Y)
 
 
@@ -127,5 +130,7 @@ def test_batch_rejects(dev):
     B = BatchSpMM([A], [2])
     with pytest.raises(ValueError):
         B.spmm(torch.zeros(3, device=dev), B.new_y())
+    with pytest.raises(ValueError):                          # not 16-byte aligned
+        B.spmm(torch.zeros(B.x_len + 1, device=dev)[1:], B.new_y())
     with pytest.raises(Exception):
         B.replay(torch.cuda.current_stream().cuda_stream)     # nothing captured yet
