@@ -149,6 +149,136 @@ def lespgemm_measure(scale, iterations, cpu_threads, cpu):
     return out
 
 
+def lespgemm_panels_measure(scale, budget_gb, cpu_threads, cpu, cpu_share=0.02):
+    """LeSpGEMM C = A*A on R-MAT(scale, edge factor 16) (k=64, bands=16, seed=7,
+    cluster_size=8, hc_v=0, kernel 3) when C is too large for one piece: the plan (B
+    preparation + symbolic phase of every cluster) and the numeric phase in cluster panels
+    whose output stays under budget_gb, output buffers reused from panel to panel.  Timed
+    twice; the second (warm) pass is reported.  The CPU leg runs the reference's OpenMP
+    pipeline and its LeSpGEMM_VLength on two cluster ranges of about cpu_share of the work
+    each; the GPU computes the same ranges, and their checksums are compared."""
+    import numpy as np
+    import torch
+    from sparseops_amd import lespgemm as LG
+    from sparseops_amd.gen import rmat
+    from sparseops_amd.io import write_csr_bin
+    dev = torch.device("cuda:0")
+    t = time.time()
+    A = rmat(scale, 16, seed=0)
+    gen_s = time.time() - t
+    dA = LG.DeviceCsr(A, dev)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    pi, pj, ps = LG.lsh_candidate_pairs(dA, 64, 16, 7, constrain=True)
+    genpairs_ms = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter()
+    perm, offsets = LG.hierarchical_clustering(A, pi, pj, ps, 8, version=0)
+    hc_ms = (time.perf_counter() - t) * 1e3
+    t = time.perf_counter()
+    Acl = LG.ClusterMatrix.build(dA, perm, offsets)
+    torch.cuda.synchronize()
+    conv_ms = (time.perf_counter() - t) * 1e3
+    flop = LG.spgemm_flop(dA, dA)
+    budget = int(budget_gb * (1 << 30))
+    C = None
+    passes = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        plan = LG.SpgemmPlan(Acl, dA)
+        ranges = plan.panel_ranges(budget)
+        numeric_ms = 0.0
+        for c0, c1 in ranges:
+            C = plan.numeric(c0, c1, C)
+            numeric_ms += C.stats["symbolic_ms"] + C.stats["malloc_ms"] + C.stats["numeric_ms"]
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t) * 1e3
+        passes.append({"wall_ms": wall, "plan_ms": plan.setup_ms + plan.symbolic_ms, "panels_ms": numeric_ms})
+        last = plan
+        if len(passes) < 2:
+            plan.close()
+    plan = last
+    w = passes[-1]
+    out = {"workload": f"R-MAT scale {scale} (n={A.n_rows}, nnz={A.nnz}), C = A*A, kernel 3, cluster_size 8, k=64, "
+                       f"bands=16, seed=7, hc_v=0; output in cluster panels of <= {budget_gb} GiB",
+           "flop": int(flop), "clusters": int(plan.clusters), "nnzc": int(plan.nnzc), "nnzv": int(plan.nnzv),
+           "output_bytes": int(plan.nnzc * 4 + plan.nnzv * 8), "panels": len(ranges),
+           "gpu_ms": round(w["wall_ms"], 2), "gpu_gflops": round(flop / w["wall_ms"] / 1e6, 2),
+           "gpu_plan_ms": round(w["plan_ms"], 2), "gpu_panels_ms": round(w["panels_ms"], 2),
+           "gpu_genpairs_ms": round(genpairs_ms, 1), "gpu_hc_ms": round(hc_ms, 1),
+           "gpu_conversion_ms": round(conv_ms, 1), "generate_s": round(gen_s, 1), "pairs": int(len(pi))}
+    # per-cluster flop in get_spgemm_flop's sense (2 * sum over the cluster's rows of nnz(B_k))
+    blen = np.diff(A.rowptr.astype(np.int64))
+    rowf = np.zeros(A.n_rows, np.int64)
+    nzr = blen > 0
+    if A.nnz:
+        rowf[nzr] = np.add.reduceat(blen[A.col.astype(np.int64)], A.rowptr[:-1].astype(np.int64)[nzr])
+    csz = np.diff(offsets)
+    cf = 2 * np.add.reduceat(rowf[perm], offsets[:-1]) * (csz > 0)
+    cum = np.cumsum(cf)
+    tot = int(cum[-1])
+    picks = []
+    for start in (0.0, 0.5):
+        c0 = int(np.searchsorted(cum, start * tot, side="right"))
+        c1 = int(np.searchsorted(cum, cum[c0 - 1] + cpu_share * tot if c0 else cpu_share * tot)) + 1
+        picks.append((c0, min(c1, plan.clusters)))
+    gpu_pick = []
+    for c0, c1 in picks:
+        P = plan.numeric(c0, c1)
+        ms = P.stats["symbolic_ms"] + P.stats["malloc_ms"] + P.stats["numeric_ms"]
+        P2 = plan.numeric(c0, c1, P)
+        ms = P2.stats["symbolic_ms"] + P2.stats["malloc_ms"] + P2.stats["numeric_ms"]
+        gpu_pick.append({"c0": c0, "c1": c1, "flop": int(cf[c0:c1].sum()), "gpu_ms": round(ms, 3),
+                         "nnzc": int(P2.nnzc), "nnzv": int(P2.nnzv), **P2.checksum()})
+        P2.close()
+    out["sampled_panels"] = gpu_pick
+    fp_perm = int(np.sum(perm.astype(np.uint64) * (np.arange(len(perm), dtype=np.uint64) + np.uint64(1)),
+                         dtype=np.uint64))
+    fp_off = int(np.sum(offsets.astype(np.uint64) * (np.arange(len(offsets), dtype=np.uint64) + np.uint64(1)),
+                        dtype=np.uint64))
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if cpu and os.path.exists(exe):
+        with tempfile.TemporaryDirectory(dir="/tmp") as d:
+            path = os.path.join(d, "A.csr")
+            write_csr_bin(path, A)
+            env = dict(os.environ, OMP_NUM_THREADS=str(cpu_threads), OMP_PLACES="cores", OMP_PROC_BIND="spread")
+            rl = ",".join(f"{a}:{b}" for a, b in picks)
+            proc = subprocess.Popen([exe, "bench_lespgemm_panels", path, path, str(cpu_threads), "1", "64", "16", "7",
+                                     "8", "3", rl], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
+            lines = []
+            for line in proc.stdout:
+                lines.append(json.loads(line))
+                log(f"cpu lespgemm panels: {line.strip()}")
+            proc.wait(timeout=1200)
+        if proc.returncode == 0 and len(lines) == 1 + len(picks):
+            head, cp = lines[0], lines[1:]
+            cpu_ms = sum(c["lespgemm_ms"] for c in cp)
+            gpu_ms = sum(g["gpu_ms"] for g in gpu_pick)
+            pflop = sum(g["flop"] for g in gpu_pick)
+            match = all(c["nnzc"] == g["nnzc"] and c["nnzv"] == g["nnzv"] and int(c["colid_sum"]) == g["colid_sum"]
+                        and int(c["nonzeros"]) == g["nonzeros"]
+                        and abs(c["sum"] - g["sum"]) <= 1e-10 * max(c["sum_abs"], 1e-300) for c, g in zip(cp, gpu_pick))
+            out["cpu_reference"] = {"kind": "reference", "threads": head["threads"], "genpairs_ms": head["genpairs_ms"],
+                                    "hc_ms": head["hc_ms"], "conversion_ms": head["conversion_ms"],
+                                    "sampled_lespgemm_ms": round(cpu_ms, 2),
+                                    "sampled_gflops": round(pflop / cpu_ms / 1e6, 3),
+                                    "sample": f"LeSpGEMM_VLength kernel 3 on cluster ranges {rl} "
+                                              f"({pflop:.3e} of {tot:.3e} flop)"}
+            out["gpu_over_cpu_same_panels"] = round(cpu_ms / max(gpu_ms, 1e-9), 2)
+            out["clustering_matches_reference"] = (head["clusters"] == plan.clusters
+                                                   and head["perm_fingerprint"] == fp_perm
+                                                   and head["offsets_fingerprint"] == fp_off)
+            out["sampled_panels_match_reference"] = bool(match)
+        else:
+            log(f"cpu lespgemm panels failed: rc={proc.returncode} {proc.stderr.read()[-400:]}")
+    if C is not None:
+        C.close()
+    plan.close()
+    Acl.close()
+    torch.cuda.synchronize()
+    return out
+
+
 def autograd_measure(scale, k, steps, cpu):
     """The differentiable SpMM layer Y = A(vals) X: R-MAT(scale, edge factor 32) with symmetric
     normalised FP32 values, X and G of width k in BF16, FP32 accumulation.  Forward and backward
@@ -316,6 +446,9 @@ def main():
     ap.add_argument("--variants", default="sell_f32,csr_f64,sell_f64,sell_bf16,csr_f32",
                     help="comma list of <fmt>_<value type>; the first one is the headline")
     ap.add_argument("--lespgemm-scale", type=int, default=16, help="R-MAT scale of the LeSpGEMM line (0: skip)")
+    ap.add_argument("--lespgemm-panels-scale", type=int, default=20,
+                    help="R-MAT scale of the panelled LeSpGEMM line (0: skip)")
+    ap.add_argument("--lespgemm-budget-gb", type=float, default=32.0, help="output bytes per LeSpGEMM panel")
     ap.add_argument("--lespgemm-iters", type=int, default=5)
     ap.add_argument("--autograd-scale", type=int, default=20, help="R-MAT scale of the autograd SpMM line (0: skip)")
     ap.add_argument("--batch-count", type=int, default=512, help="matrices of the batched SpMM line (0: skip)")
@@ -394,6 +527,13 @@ def main():
         lg = lespgemm_measure(args.lespgemm_scale, args.lespgemm_iters, args.cpu_threads, not args.no_cpu)
         log(f"lespgemm done in {time.time() - t:.1f}s: {lg}")
 
+    lp = None
+    if args.lespgemm_panels_scale > 0:
+        t = time.time()
+        lp = lespgemm_panels_measure(args.lespgemm_panels_scale, args.lespgemm_budget_gb, args.cpu_threads,
+                                     not args.no_cpu)
+        log(f"lespgemm panels done in {time.time() - t:.1f}s: {lp}")
+
     ag = None
     if args.autograd_scale > 0:
         t = time.time()
@@ -436,6 +576,7 @@ def main():
         "fp32_sell_over_fp64_csr": (round(res["csr_f64"]["kernel_ms"] / res["sell_f32"]["kernel_ms"], 3)
                                     if "csr_f64" in res and "sell_f32" in res else None),
         "lespgemm": lg,
+        "lespgemm_panels": lp,
         "autograd_spmm": ag,
         "batch_spmm": bt,
     }

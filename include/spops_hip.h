@@ -185,11 +185,16 @@ int spops_csr_set_values(spops_matrix A, const void *d_values, int values_type, 
  * (BSR) or max row width (ELL) info[14]=nchunks (reference layout) info[15]=index_type */
 int spops_matrix_info(spops_matrix A, int64_t info[16]);
 
-/* Layout tuning for matrices created afterwards (process-wide; no reference counterpart).
- *   SPOPS_TUNE_SLICE_BYTES: bytes of x per column slice of the SELL/ELL layout (default
- *   2 MiB, one XCD's share of L2); the slice count is the smallest power of two <= 8 with
- *   n_cols * sizeof(x) / slices <= this.  Returns the previous value in *old (may be null). */
+/* Tuning (process-wide; no reference counterpart).  Returns the previous value in *old (may
+ * be null).
+ *   SPOPS_TUNE_SLICE_BYTES: bytes of x per column slice of the SELL/ELL layout of matrices
+ *   created afterwards (default 2 MiB, one XCD's share of L2); the slice count is the
+ *   smallest power of two <= 8 with n_cols * sizeof(x) / slices <= this.
+ *   SPOPS_TUNE_DENSE_PRODUCTS: LeSpGEMM clusters with at least this many products (default
+ *   2^20) accumulate in a dense block in device memory (many workgroups per cluster) instead
+ *   of one workgroup's LDS chunks; applies to plans created afterwards. */
 #define SPOPS_TUNE_SLICE_BYTES 0
+#define SPOPS_TUNE_DENSE_PRODUCTS 1
 int spops_set_tuning(int key, int64_t value, int64_t *old);
 
 /* Device copies of the layout arrays a format conversion produced, for parity checks:
@@ -270,6 +275,29 @@ int spops_cluster_destroy(spops_cluster A);
  * accumulator) [5..8]=setup/symbolic/malloc/numeric in us [9]=cluster-wise products. */
 int spops_lespgemm(spops_cluster A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *d_browptr,
                    const int64_t *d_bcol, const void *d_bval, spops_cluster *C, int64_t stats[16], void *stream);
+
+/* The same product in panels (cluster ranges), for outputs larger than memory: no reference
+ * counterpart as an entry point; each panel is what LeSpGEMM_VLength kernel 3
+ * (src/spgemm.cpp:545-659) computes for clusters [c0, c1).  plan_create prepares B and runs
+ * the symbolic phase for every cluster; A and the device arrays of B must outlive the plan.
+ * info = {clusters, nnzc, nnzv, products, small clusters, large clusters, setup us,
+ * symbolic us}.  plan_panels cuts [0, clusters) into ranges whose output (colids + values)
+ * stays under budget_bytes (a range holds at least one cluster); bounds needs clusters + 2
+ * entries.  plan_numeric computes one range into *C (created when *C is NULL, else its
+ * buffers are reused); C then has c1 - c0 clusters.  stats as spops_lespgemm (setup and
+ * symbolic count once, in the plan). */
+typedef struct spops_spgemm_plan_s *spops_spgemm_plan;
+int spops_lespgemm_plan_create(spops_cluster A, int64_t b_rows, int64_t b_cols, int64_t b_nnz,
+                               const int64_t *d_browptr, const int64_t *d_bcol, const void *d_bval,
+                               spops_spgemm_plan *plan, void *stream);
+int spops_lespgemm_plan_info(spops_spgemm_plan plan, int64_t info[8]);
+int spops_lespgemm_plan_panels(spops_spgemm_plan plan, int64_t budget_bytes, int64_t *npanels, int64_t *bounds);
+int spops_lespgemm_plan_numeric(spops_spgemm_plan plan, int64_t c0, int64_t c1, spops_cluster *C, int64_t stats[16],
+                                void *stream);
+int spops_lespgemm_plan_destroy(spops_spgemm_plan plan);
+/* checksums of a cluster matrix's values and column ids: out = {sum of values, sum of |values|,
+ * sum of column ids, number of nonzero values} (for comparing panels with another implementation) */
+int spops_cluster_checksum(spops_cluster C, double out[4], void *stream);
 
 /* vlength_cluster2csr + inverse permutation (include/sparse_conversion.h:1778-1862,
  * test/test_spgemm_hc_lsh.cpp:296-303): C in the original row order, exact zeros dropped,

@@ -15,12 +15,16 @@
 //   lespgemm <A.csr> <B.csr> <outdir> <k> <bands> <seed> <hc_v> <cluster_size> <kernel> <prec>
 //   bench_spmv <A.csr> <fmt csr|sell> <prec> <threads> <min_seconds> [sigma C]
 //   bench_lespgemm <A.csr> <B.csr> <threads> <iterations> <k> <bands> <seed> <cluster_size> <kernel>
+//   bench_lespgemm_panels <A.csr> <B.csr> <threads> <iterations> <k> <bands> <seed> <cluster_size> <kernel>
+//                         <c0:c1,c0:c1,...>   LeSpGEMM_VLength on cluster ranges of the clustered A
 #include "SpOps.h"
 #include "spgemm.h"
 #include "spgemm_MinHashLSH.h"
 #include "spgemm_utility.h"
 #include "sparse_conversion.h"
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -326,6 +330,102 @@ int bench_lespgemm(const HostCsr &ha, const HostCsr &hb, int iterations, int k, 
   return 0;
 }
 
+// Clusters [c0, c1) of a clustered matrix as a matrix of its own (arrays copied, pointers
+// rebased), so that LeSpGEMM_VLength computes that panel of the product.
+template <typename I, typename V>
+CSR_VlengthCluster<I, V> cluster_slice(const CSR_VlengthCluster<I, V> &A, I c0, I c1) {
+  CSR_VlengthCluster<I, V> S{};
+  S.rows = c1 - c0;
+  S.cols = A.cols;
+  S.cluster_sz = new_array<I>(S.rows > 0 ? S.rows : 1);
+  S.rowptr = new_array<I>(S.rows + 1);
+  S.rowptr_val = new_array<I>(S.rows + 1);
+  S.csr_rows = 0;
+  S.max_cluster_sz = 0;
+  for (I c = 0; c < S.rows; ++c) {
+    S.cluster_sz[c] = A.cluster_sz[c0 + c];
+    S.csr_rows += S.cluster_sz[c];
+    S.max_cluster_sz = std::max(S.max_cluster_sz, S.cluster_sz[c]);
+  }
+  for (I c = 0; c <= S.rows; ++c) {
+    S.rowptr[c] = A.rowptr[c0 + c] - A.rowptr[c0];
+    S.rowptr_val[c] = A.rowptr_val[c0 + c] - A.rowptr_val[c0];
+  }
+  S.nnzc = S.rowptr[S.rows];
+  S.nnzv = S.rowptr_val[S.rows];
+  S.colids = new_array<I>(S.nnzc > 0 ? S.nnzc : 1);
+  S.values = new_array<V>(S.nnzv > 0 ? S.nnzv : 1);
+  std::copy(A.colids + A.rowptr[c0], A.colids + A.rowptr[c1], S.colids);
+  std::copy(A.values + A.rowptr_val[c0], A.values + A.rowptr_val[c1], S.values);
+  return S;
+}
+
+int bench_lespgemm_panels(const HostCsr &ha, const HostCsr &hb, int iterations, int k, int bands, uint64_t seed,
+                          int64_t cluster_size, int kernel, const char *ranges) {
+  typedef int64_t I; typedef double V;
+  CSR_Matrix<I, V> A = to_ref_csr<I, V>(ha);
+  CSR_Matrix<I, V> B = to_ref_csr<I, V>(hb);
+  double t0 = omp_get_wtime();
+  std::vector<CandidatePair<I, V>> pairs = lsh_candidate_pairs_vector<I, V>(
+      A.row_offset, A.col_index, A.num_rows, k, bands, seed, true);
+  double t_pairs = omp_get_wtime() - t0;
+  std::vector<I> perm, offsets;
+  t0 = omp_get_wtime();
+  hierarchical_clustering_v0_fast<I, V>(A.row_offset, A.col_index, A.num_rows, pairs, (I)cluster_size, perm, offsets);
+  double t_hc = omp_get_wtime() - t0;
+  CSR_Matrix<I, V> Ar = csr_reorder_rows<I, V>(A, perm);
+  t0 = omp_get_wtime();
+  CSR_VlengthCluster<I, V> Acl = csr_to_vlength_cluster<I, V>(Ar, offsets);
+  double t_conv = omp_get_wtime() - t0;
+  // fingerprints of the clustering: sum of perm[i] * (i + 1) and offsets[i] * (i + 1), mod 2^64
+  uint64_t fp_perm = 0, fp_off = 0;
+  for (size_t i = 0; i < perm.size(); ++i) fp_perm += (uint64_t)perm[i] * (uint64_t)(i + 1);
+  for (size_t i = 0; i < offsets.size(); ++i) fp_off += (uint64_t)offsets[i] * (uint64_t)(i + 1);
+  std::printf("{\"threads\": %d, \"pairs\": %zu, \"clusters\": %zu, \"genpairs_ms\": %.3f, \"hc_ms\": %.3f, "
+              "\"conversion_ms\": %.3f, \"perm_fingerprint\": %llu, \"offsets_fingerprint\": %llu}\n",
+              Le_get_thread_num(), pairs.size(), offsets.size() - 1, t_pairs * 1e3, t_hc * 1e3, t_conv * 1e3,
+              (unsigned long long)fp_perm, (unsigned long long)fp_off);
+  std::fflush(stdout);
+  const char *p = ranges;
+  while (*p) {
+    char *end = nullptr;
+    const I c0 = std::strtoll(p, &end, 10);
+    if (*end != ':') { std::fprintf(stderr, "bad range list\n"); return 2; }
+    const I c1 = std::strtoll(end + 1, &end, 10);
+    p = *end == ',' ? end + 1 : end;
+    if (c0 < 0 || c1 < c0 || c1 > Acl.rows) { std::fprintf(stderr, "bad range %lld:%lld\n", (long long)c0, (long long)c1); return 2; }
+    CSR_VlengthCluster<I, V> S = cluster_slice(Acl, c0, c1);
+    long long products = 0;
+    for (I c = 0; c < S.rows; ++c)
+      for (I t = S.rowptr[c]; t < S.rowptr[c + 1]; ++t) products += B.row_offset[S.colids[t] + 1] - B.row_offset[S.colids[t]];
+    CSR_VlengthCluster<I, V> Ccl{};
+    Ccl.acc_flag = new_array<char>(S.rows > 0 ? S.rows : 1);
+    Ccl.min_ccol = new_array<I>(S.rows > 0 ? S.rows : 1);
+    Ccl.max_ccol = new_array<I>(S.rows > 0 ? S.rows : 1);
+    LeSpGEMM_VLength<false, I, V>(S, B, Ccl, kernel, -1.0);     // warm-up
+    double total = 0;
+    for (int it = 0; it < iterations; ++it) {
+      delete_vlength_cluster_matrix(Ccl);
+      t0 = omp_get_wtime();
+      LeSpGEMM_VLength<false, I, V>(S, B, Ccl, kernel, -1.0);
+      total += omp_get_wtime() - t0;
+    }
+    double sum = 0, sum_abs = 0, colsum = 0, nonzeros = 0;
+    for (I i = 0; i < Ccl.nnzv; ++i) { sum += Ccl.values[i]; sum_abs += std::fabs(Ccl.values[i]); nonzeros += Ccl.values[i] != 0.0; }
+    for (I i = 0; i < Ccl.nnzc; ++i) colsum += (double)Ccl.colids[i];
+    std::printf("{\"c0\": %lld, \"c1\": %lld, \"lespgemm_ms\": %.3f, \"products\": %lld, \"nnzc\": %lld, \"nnzv\": %lld, "
+                "\"sum\": %.17g, \"sum_abs\": %.17g, \"colid_sum\": %.17g, \"nonzeros\": %.17g}\n",
+                (long long)c0, (long long)c1, iterations ? total / iterations * 1e3 : 0.0, products,
+                (long long)Ccl.nnzc, (long long)Ccl.nnzv, sum, sum_abs, colsum, nonzeros);
+    std::fflush(stdout);
+    delete_vlength_cluster_matrix(Ccl);
+    delete_array(Ccl.acc_flag); delete_array(Ccl.min_ccol); delete_array(Ccl.max_ccol);
+    delete_vlength_cluster_matrix(S);
+  }
+  delete_vlength_cluster_matrix(Acl); delete_host_matrix(Ar); delete_host_matrix(A); delete_host_matrix(B);
+  return 0;
+}
+
 std::vector<double> read_f64(const char *path, size_t n) {
   std::vector<double> v(n);
   FILE *f = std::fopen(path, "rb");
@@ -376,6 +476,14 @@ int main(int argc, char **argv) {
     Le_set_thread_num(std::atoi(argv[4]));
     return bench_lespgemm(A, B, std::atoi(argv[5]), std::atoi(argv[6]), std::atoi(argv[7]),
                           std::strtoull(argv[8], nullptr, 10), std::atoll(argv[9]), std::atoi(argv[10]));
+  }
+  if (mode == "bench_lespgemm_panels" && argc >= 12) {
+    HostCsr A = read_csr_file(argv[2]);
+    HostCsr B = read_csr_file(argv[3]);
+    Le_set_thread_num(std::atoi(argv[4]));
+    return bench_lespgemm_panels(A, B, std::atoi(argv[5]), std::atoi(argv[6]), std::atoi(argv[7]),
+                                 std::strtoull(argv[8], nullptr, 10), std::atoll(argv[9]), std::atoi(argv[10]),
+                                 argv[11]);
   }
   std::fprintf(stderr, "bad arguments for mode %s\n", mode.c_str());
   return 2;

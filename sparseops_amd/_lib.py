@@ -18,7 +18,7 @@ SUCCESS = 0
 INDEX_I32, INDEX_I64 = 0, 1
 VALUE_F64, VALUE_F32, VALUE_BF16 = 0, 1, 2
 FORMAT_CSR, FORMAT_SELL, FORMAT_ELL, FORMAT_BSR = 0, 1, 2, 3
-TUNE_SLICE_BYTES = 0
+TUNE_SLICE_BYTES, TUNE_DENSE_PRODUCTS = 0, 1
 
 c_i64 = ctypes.c_int64
 c_int = ctypes.c_int
@@ -86,6 +86,12 @@ _SIGNATURES = [
     ("spops_cluster_download", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("spops_cluster_symbolic_download", c_int, [c_vp, c_vp, c_vp, c_vp]),
     ("spops_cluster_destroy", c_int, [c_vp]),
+    ("spops_lespgemm_plan_create", c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    ("spops_lespgemm_plan_info", c_int, [c_vp, c_vp]),
+    ("spops_lespgemm_plan_panels", c_int, [c_vp, c_i64, c_vp, c_vp]),
+    ("spops_lespgemm_plan_numeric", c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp]),
+    ("spops_lespgemm_plan_destroy", c_int, [c_vp]),
+    ("spops_cluster_checksum", c_int, [c_vp, c_vp, c_vp]),
     ("spops_lespgemm", c_int, [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("spops_cluster_to_csr", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("spops_spgemm_flop", c_int, [c_i64, c_vp, c_i64, c_vp, c_vp, c_vp]),

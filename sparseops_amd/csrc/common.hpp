@@ -169,6 +169,16 @@ __device__ __forceinline__ T bload(rsrc_t r, uint32_t off) {
   return out;
 }
 
+// Workgroups for `items` work items at `per` items per workgroup of kBlock threads.  One
+// dispatch holds fewer than 2^32 work items; a launch past that is refused (kernels with more
+// items than that stride over them with a capped grid instead).
+inline unsigned grid_blocks(int64_t items, int64_t per) {
+  const int64_t b = (items + per - 1) / per;
+  SPOPS_REQUIRE(b * kBlock < ((int64_t)1 << 32), SPOPS_ERROR_LIMIT,
+                "a launch of %lld workgroups would exceed 2^32 work items", (long long)b);
+  return (unsigned)b;
+}
+
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 __device__ __forceinline__ int wave_id() { return (int)(threadIdx.x >> 6); }
 

@@ -129,25 +129,28 @@ __global__ __launch_bounds__(kBlock) void k_unique_score(const uint64_t *__restr
                                                          const uint64_t *__restrict__ sigs, int k,
                                                          int64_t *__restrict__ pi, int64_t *__restrict__ pj,
                                                          S *__restrict__ score) {
-  const int64_t a = __builtin_amdgcn_readfirstlane((int)blockIdx.x) * (int64_t)(kBlock / kWave) + wave_id();
-  if (a >= m || !f[a]) return;
-  const uint64_t key = keys[a];
-  const int64_t i = (int64_t)(key >> 32), j = (int64_t)(key & 0xffffffffull);
-  int match = 0;
-  for (int d0 = 0; d0 < k; d0 += kWave) {
-    const int d = d0 + lane_id();
-    const bool eq = d < k && sigs[i * k + d] == sigs[j * k + d];
-    match += __popcll(__ballot(eq));
-  }
-  if (lane_id() == 0) {
-    const int64_t o = pos[a];
-    pi[o] = i;
-    pj[o] = j;
-    score[o] = (S)((double)match / (double)k);
+  // one wave per raw pair, the grid striding over them (there can be more than 2^26 pairs)
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / kWave);
+  for (int64_t a = (int64_t)blockIdx.x * (kBlock / kWave) + wave_id(); a < m; a += stride) {
+    if (!f[a]) continue;
+    const uint64_t key = keys[a];
+    const int64_t i = (int64_t)(key >> 32), j = (int64_t)(key & 0xffffffffull);
+    int match = 0;
+    for (int d0 = 0; d0 < k; d0 += kWave) {
+      const int d = d0 + lane_id();
+      const bool eq = d < k && sigs[i * k + d] == sigs[j * k + d];
+      match += __popcll(__ballot(eq));
+    }
+    if (lane_id() == 0) {
+      const int64_t o = pos[a];
+      pi[o] = i;
+      pj[o] = j;
+      score[o] = (S)((double)match / (double)k);
+    }
   }
 }
 
-inline unsigned blocks_for(int64_t n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
+inline unsigned blocks_for(int64_t n, int per_block) { return grid_blocks(n, per_block); }
 
 }  // namespace
 
@@ -251,7 +254,8 @@ void lsh_pairs_device(const uint64_t *sigs, int64_t n, int k, int bands, int con
   out.pi.alloc((size_t)nu * 8);
   out.pj.alloc((size_t)nu * 8);
   out.score.alloc((size_t)nu * (score_type == SPOPS_VALUE_F32 ? 4 : 8));
-  const unsigned grid = blocks_for(npairs_raw, kBlock / kWave);
+  const unsigned grid = std::min<unsigned>(blocks_for(std::min<int64_t>(npairs_raw, (int64_t)1 << 24), kBlock / kWave),
+                                           1u << 22);
   if (score_type == SPOPS_VALUE_F32)
     hipLaunchKernelGGL(k_unique_score<float>, dim3(grid), dim3(kBlock), 0, s, raw.as<uint64_t>(), flag.as<int64_t>(),
                        pos.as<int64_t>(), npairs_raw, sigs, k, out.pi.as<int64_t>(), out.pj.as<int64_t>(),

@@ -40,7 +40,7 @@ namespace spops {
 
 namespace {
 
-inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+inline unsigned nblk(int64_t n, int per) { return grid_blocks(n, per); }
 
 __global__ __launch_bounds__(kBlock) void k_narrow(const int64_t *__restrict__ in, int64_t n, int32_t *__restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -426,24 +426,30 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
                                                            const int64_t *__restrict__ brp, const int32_t *__restrict__ bcol,
                                                            const V *__restrict__ bval, const int64_t *__restrict__ crp,
                                                            const int64_t *__restrict__ crpv, const int32_t *__restrict__ ccol,
-                                                           V *__restrict__ cval) {
+                                                           V *__restrict__ cval, bool b32) {
   __shared__ V acc[kLargeAccEntries];
   __shared__ int32_t hkey[kLargeHash];
   __shared__ int16_t hval[kLargeHash];
-  __shared__ int32_t cursor[kLargeCursors];   // per A column: B entries consumed by earlier chunks
-  __shared__ int32_t nextcol[kLargeCursors];  // per A column: column at the cursor (INT32_MAX: done)
+  // per A column (when the cluster has at most kLargeCursors and B's offsets fit 32 bits):
+  // the next unread entry of its B row, the row's end, and the column at that entry
+  // (INT32_MAX: the row is done), so a chunk reads only B entries and skips rows at once
+  __shared__ int32_t rpos[kLargeCursors];
+  __shared__ int32_t rend[kLargeCursors];
+  __shared__ int32_t nextcol[kLargeCursors];
   constexpr int W = kLargeBlock / kWave;
   const int64_t c = list[blockIdx.x];
   const int csz = csz_a[c];
   const int64_t a0 = arp[c], a1 = arp[c + 1], av0 = arpv[c];
   const int64_t c0 = crp[c], nzc = crp[c + 1] - c0, cv0 = crpv[c];
   const int M = min(kLargeAccEntries / csz, kLargeHash / 2);
-  const bool cursors = a1 - a0 <= kLargeCursors;
+  const bool cursors = a1 - a0 <= kLargeCursors && b32;
   if (cursors)
     for (int64_t t = a0 + threadIdx.x; t < a1; t += kLargeBlock) {
       const int32_t b = acol[t];
-      cursor[t - a0] = 0;
-      nextcol[t - a0] = brp[b] < brp[b + 1] ? bcol[brp[b]] : INT32_MAX;
+      const int32_t kb = (int32_t)brp[b], ke = (int32_t)brp[b + 1];
+      rpos[t - a0] = kb;
+      rend[t - a0] = ke;
+      nextcol[t - a0] = kb < ke ? bcol[kb] : INT32_MAX;
     }
   const int lane = lane_id();
   for (int64_t j0 = 0; j0 < nzc; j0 += M) {
@@ -468,16 +474,16 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
       // every lane of the wave takes part in the ballots below, groups without a row idle
       const int64_t t = base + grp;
       bool live = t < a1 && !(cursors && nextcol[t - a0] > hi);
-      int64_t kb = 0, ke = 0, k = 0;
+      int64_t ke = 0, k = 0;
       const V *a = nullptr;
       if (live) {
-        const int32_t b = acol[t];
-        kb = brp[b];
-        ke = brp[b + 1];
-        k = kb;
         if (cursors) {
-          k = kb + cursor[t - a0];
+          k = rpos[t - a0];
+          ke = rend[t - a0];
         } else {
+          const int32_t b = acol[t];
+          k = brp[b];
+          ke = brp[b + 1];
           int64_t k1 = ke;
           row_range(bcol, k, k1, lo, hi);
         }
@@ -494,7 +500,9 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
         bool going = live;
         while (__ballot(going)) {
           const int64_t kk = k + sub;
-          const int32_t key = (going && kk < ke) ? bcol[kk] : INT32_MAX;
+          const bool have = going && kk < ke;
+          const int32_t key = have ? bcol[kk] : INT32_MAX;
+          const V bv = have ? bval[kk] : V(0);
           const bool in = going && key <= hi;
           const int n = __builtin_popcountll(__ballot(in) & gmask);
           int h = -1;
@@ -504,7 +512,6 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
               if (hkey[p] == key) { h = (int)p; break; }
           }
           if (h >= 0) {
-            const V bv = bval[kk];
             V *dst = acc + (int)hval[h] * csz + l0;
 #pragma unroll
             for (int l = 0; l < kMaxClusterRows; ++l)
@@ -521,7 +528,7 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
         }
       }
       if (cursors && live && sub == 0) {
-        cursor[t - a0] = (int32_t)(k - kb);
+        rpos[t - a0] = (int32_t)k;
         nextcol[t - a0] = nxt;
       }
     }
@@ -727,47 +734,146 @@ struct Timer {
 
 }  // namespace
 
-void lespgemm(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *brp64,
-              const int64_t *bcol64, const void *bval, spops_cluster_s &C, int64_t stats[kSpgemmStats], hipStream_t s) {
+// ---- dense accumulation for clusters with very many products (R-MAT hubs: tens of
+// thousands of A columns, C rows covering most columns).  The cluster's rows of C accumulate
+// in a dense block D[l * range + (col - cmin)] (l < csz, range = cmax - cmin + 1) in device
+// memory: one workgroup per kDenseTaskCols consecutive A columns of a cluster, 16 lanes per B
+// row, atomic adds; then the block is gathered at the cluster's sorted output columns.
+struct DenseTask {
+  int64_t cluster, t0, t1, off;   // A columns [t0, t1) of the cluster, D at scratch + off
+};
+
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_dense_products(const DenseTask *__restrict__ tasks,
+                                                           const int64_t *__restrict__ arp,
+                                                           const int64_t *__restrict__ arpv,
+                                                           const int32_t *__restrict__ csz_a,
+                                                           const int32_t *__restrict__ acol, const V *__restrict__ aval,
+                                                           const int64_t *__restrict__ brp,
+                                                           const int32_t *__restrict__ bcol, const V *__restrict__ bval,
+                                                           const int64_t *__restrict__ cmin,
+                                                           const int64_t *__restrict__ cmax, V *__restrict__ scratch) {
+  const DenseTask tk = tasks[blockIdx.x];
+  const int64_t c = tk.cluster;
+  const int csz = csz_a[c];
+  const int64_t lo = cmin[c], range = cmax[c] - lo + 1;
+  V *D = scratch + tk.off;
+  const int64_t a0 = arp[c], av0 = arpv[c];
+  constexpr int GL = 16, GPB = kBlock / GL;
+  const int g = threadIdx.x / GL, sub = threadIdx.x % GL;
+  for (int64_t t = tk.t0 + g; t < tk.t1; t += GPB) {
+    const int32_t b = acol[t];
+    const V *a = aval + av0 + (t - a0) * csz;
+    const int64_t k0 = brp[b], k1 = brp[b + 1];
+    for (int l0 = 0; l0 < csz; l0 += kMaxClusterRows) {
+      V ar[kMaxClusterRows];
+#pragma unroll
+      for (int l = 0; l < kMaxClusterRows; ++l) ar[l] = l0 + l < csz ? a[l0 + l] : V(0);
+      for (int64_t k = k0 + sub; k < k1; k += GL) {
+        const int64_t col = (int64_t)bcol[k] - lo;
+        const V bv = bval[k];
+#pragma unroll
+        for (int l = 0; l < kMaxClusterRows; ++l)
+          if (ar[l] != V(0)) atomicAdd(&D[(int64_t)(l0 + l) * range + col], ar[l] * bv);
+      }
+    }
+  }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_dense_gather(const int32_t *__restrict__ list,
+                                                         const int64_t *__restrict__ offs,
+                                                         const int32_t *__restrict__ csz_a,
+                                                         const int64_t *__restrict__ cmin,
+                                                         const int64_t *__restrict__ cmax, const V *__restrict__ scratch,
+                                                         const int64_t *__restrict__ crp,
+                                                         const int64_t *__restrict__ crpv,
+                                                         const int32_t *__restrict__ ccol, V *__restrict__ cval) {
+  const int64_t c = list[blockIdx.y];
+  const int csz = csz_a[c];
+  const int64_t lo = cmin[c], range = cmax[c] - lo + 1;
+  const V *D = scratch + offs[blockIdx.y];
+  const int64_t j0 = crp[c], nzc = crp[c + 1] - j0, cv0 = crpv[c];
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < nzc; j += (int64_t)gridDim.x * kBlock) {
+    const int64_t col = (int64_t)ccol[j0 + j] - lo;
+    for (int l = 0; l < csz; ++l) cval[cv0 + j * csz + l] = D[(int64_t)l * range + col];
+  }
+}
+
+// per-block partial checksums of a cluster matrix: {sum v, sum |v|, sum colids, nonzeros}
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_cluster_checksum(const V *__restrict__ val, int64_t nnzv,
+                                                             const int32_t *__restrict__ colids, int64_t nnzc,
+                                                             double *__restrict__ part) {
+  __shared__ double red[4][kBlock];
+  double sv = 0, sa = 0, sc = 0, nzv = 0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nnzv; i += stride) {
+    const double v = (double)val[i];
+    sv += v;
+    sa += fabs(v);
+    nzv += v != 0.0 ? 1.0 : 0.0;
+  }
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nnzc; i += stride) sc += (double)colids[i];
+  red[0][threadIdx.x] = sv;
+  red[1][threadIdx.x] = sa;
+  red[2][threadIdx.x] = sc;
+  red[3][threadIdx.x] = nzv;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *brp64,
+                        const int64_t *bcol64, const void *bval, spops_spgemm_plan_s &P, hipStream_t s) {
   const int vt = A.value_type;
   const int64_t nc = A.rows;
   const size_t vb = value_bytes(vt);
   Timer tm(s);
   SPOPS_REQUIRE(b_cols < (int64_t)INT32_MAX, SPOPS_ERROR_LIMIT, "B must have < 2^31 columns");
-  // B with int32 columns, rows sorted (the large-cluster kernels binary-search B rows)
-  DevBuf bcol32, bsorted_val, flag;
-  bcol32.alloc((size_t)std::max<int64_t>(b_nnz, 1) * 4);
+  P.A = &A;
+  P.b_rows = b_rows;
+  P.b_cols = b_cols;
+  P.b_nnz = b_nnz;
+  P.brp64 = brp64;
+  P.bval = bval;
+  // B with int32 columns, rows sorted (the large-cluster kernels walk B rows in column order)
+  DevBuf flag;
+  P.bcol32.alloc((size_t)std::max<int64_t>(b_nnz, 1) * 4);
   flag.alloc(16);
   SPOPS_HIP_CHECK(hipMemsetAsync(flag.p, 0, 16, s));
   if (b_nnz > 0) {
-    hipLaunchKernelGGL(k_narrow, dim3(nblk(b_nnz, kBlock)), dim3(kBlock), 0, s, bcol64, b_nnz, bcol32.as<int32_t>());
+    hipLaunchKernelGGL(k_narrow, dim3(nblk(b_nnz, kBlock)), dim3(kBlock), 0, s, bcol64, b_nnz, P.bcol32.as<int32_t>());
     hipLaunchKernelGGL(k_rows_unsorted, dim3(nblk(b_rows, kBlock / kWave)), dim3(kBlock), 0, s, brp64, b_rows,
-                       bcol32.as<int32_t>(), flag.as<int>());
+                       P.bcol32.as<int32_t>(), flag.as<int>());
   }
   SPOPS_HIP_CHECK(hipGetLastError());
-  int unsorted = 0;
-  SPOPS_HIP_CHECK(hipMemcpyAsync(&unsorted, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  SPOPS_HIP_CHECK(hipMemcpyAsync(&P.unsorted, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
   SPOPS_HIP_CHECK(hipStreamSynchronize(s));
-  if (unsorted) {
+  if (P.unsorted) {
     SPOPS_REQUIRE(b_rows < ((int64_t)1 << 31), SPOPS_ERROR_LIMIT, "B must have < 2^31 rows");
     DevBuf keys, idx;
     keys.alloc((size_t)b_nnz * 8);
     idx.alloc((size_t)b_nnz * 8);
     hipLaunchKernelGGL(k_row_keys, dim3(nblk(b_rows, kBlock / kWave)), dim3(kBlock), 0, s, brp64, b_rows,
-                       bcol32.as<int32_t>(), keys.as<uint64_t>(), idx.as<uint64_t>());
+                       P.bcol32.as<int32_t>(), keys.as<uint64_t>(), idx.as<uint64_t>());
     SPOPS_HIP_CHECK(hipGetLastError());
     radix_sort_u64(keys.as<uint64_t>(), idx.p, 8, b_nnz, 0, 64, s);
-    bsorted_val.alloc((size_t)b_nnz * vb);
+    P.bsorted_val.alloc((size_t)b_nnz * vb);
     if (vt == SPOPS_VALUE_F64)
       hipLaunchKernelGGL(k_row_sorted_fill<double>, dim3(nblk(b_nnz, kBlock)), dim3(kBlock), 0, s, keys.as<uint64_t>(),
-                         idx.as<uint64_t>(), b_nnz, static_cast<const double *>(bval), bcol32.as<int32_t>(),
-                         bsorted_val.as<double>());
+                         idx.as<uint64_t>(), b_nnz, static_cast<const double *>(bval), P.bcol32.as<int32_t>(),
+                         P.bsorted_val.as<double>());
     else
       hipLaunchKernelGGL(k_row_sorted_fill<float>, dim3(nblk(b_nnz, kBlock)), dim3(kBlock), 0, s, keys.as<uint64_t>(),
-                         idx.as<uint64_t>(), b_nnz, static_cast<const float *>(bval), bcol32.as<int32_t>(),
-                         bsorted_val.as<float>());
+                         idx.as<uint64_t>(), b_nnz, static_cast<const float *>(bval), P.bcol32.as<int32_t>(),
+                         P.bsorted_val.as<float>());
     SPOPS_HIP_CHECK(hipGetLastError());
-    bval = bsorted_val.p;
+    P.bval = P.bsorted_val.p;
     SPOPS_HIP_CHECK(hipStreamSynchronize(s));
   }
   // work estimate and bins
@@ -777,110 +883,238 @@ void lespgemm(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t 
     hipLaunchKernelGGL(k_cluster_flop, dim3(nblk(nc, kBlock / kWave)), dim3(kBlock), 0, s, A.rowptr.as<int64_t>(),
                        A.colids.as<int32_t>(), brp64, nc, flop.as<int64_t>());
   SPOPS_HIP_CHECK(hipGetLastError());
-  std::vector<int64_t> hflop(nc);
-  std::vector<int32_t> hcsz(nc);
+  P.hflop.assign(nc, 0);
+  P.hcsz.assign(nc, 0);
   if (nc > 0) {
-    SPOPS_HIP_CHECK(hipMemcpyAsync(hflop.data(), flop.p, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
-    SPOPS_HIP_CHECK(hipMemcpyAsync(hcsz.data(), A.cluster_sz.p, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
+    SPOPS_HIP_CHECK(hipMemcpyAsync(P.hflop.data(), flop.p, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
+    SPOPS_HIP_CHECK(hipMemcpyAsync(P.hcsz.data(), A.cluster_sz.p, (size_t)nc * 4, hipMemcpyDeviceToHost, s));
   }
   SPOPS_HIP_CHECK(hipStreamSynchronize(s));
-  std::vector<int32_t> small, large;
-  int64_t total_cluster_flop = 0;
+  P.small.clear();
+  P.large.clear();
+  P.products = 0;
   for (int64_t c = 0; c < nc; ++c) {
-    total_cluster_flop += hflop[c];
-    if (hflop[c] == 0) continue;
-    if (hflop[c] <= kSmallFlop && hflop[c] * hcsz[c] <= kSmallAcc) small.push_back((int32_t)c);
-    else large.push_back((int32_t)c);
+    P.products += P.hflop[c];
+    if (P.hflop[c] == 0) continue;
+    if (P.hflop[c] <= kSmallFlop && P.hflop[c] * P.hcsz[c] <= kSmallAcc) P.small.push_back((int32_t)c);
+    else P.large.push_back((int32_t)c);
   }
   // large clusters in descending work, so that the longest start first
-  std::stable_sort(large.begin(), large.end(), [&](int32_t a, int32_t b) { return hflop[a] > hflop[b]; });
-  DevBuf dsmall, dlarge, nz, cmin, cmax;
-  dsmall.upload(small);
-  dlarge.upload(large);
-  nz.alloc((size_t)std::max<int64_t>(nc, 1) * 8);
-  cmin.alloc((size_t)std::max<int64_t>(nc, 1) * 8);
-  cmax.alloc((size_t)std::max<int64_t>(nc, 1) * 8);
-  SPOPS_HIP_CHECK(hipMemsetAsync(nz.p, 0, (size_t)std::max<int64_t>(nc, 1) * 8, s));
-  SPOPS_HIP_CHECK(hipMemsetAsync(cmin.p, 0, (size_t)std::max<int64_t>(nc, 1) * 8, s));
-  SPOPS_HIP_CHECK(hipMemsetAsync(cmax.p, 0, (size_t)std::max<int64_t>(nc, 1) * 8, s));
-  double ms_setup = tm.stop();
-  // ---- symbolic
-  if (!small.empty())
-    hipLaunchKernelGGL(k_sym_small, dim3((unsigned)small.size()), dim3(kBlock), 0, s, dsmall.as<int32_t>(),
-                       (int64_t)small.size(), A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64, bcol32.as<int32_t>(),
-                       nz.as<int64_t>(), cmin.as<int64_t>(), cmax.as<int64_t>());
-  if (!large.empty())
-    hipLaunchKernelGGL(k_sym_large, dim3((unsigned)large.size()), dim3(kBlock), 0, s, dlarge.as<int32_t>(),
-                       A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64, bcol32.as<int32_t>(), (int32_t)b_cols,
-                       nz.as<int64_t>(), cmin.as<int64_t>(), cmax.as<int64_t>());
+  std::stable_sort(P.large.begin(), P.large.end(), [&](int32_t a, int32_t b) { return P.hflop[a] > P.hflop[b]; });
+  P.dsmall.upload(P.small);
+  P.dlarge.upload(P.large);
+  const size_t nb = (size_t)std::max<int64_t>(nc, 1) * 8;
+  P.nz.alloc(nb);
+  P.cmin.alloc(nb);
+  P.cmax.alloc(nb);
+  SPOPS_HIP_CHECK(hipMemsetAsync(P.nz.p, 0, nb, s));
+  SPOPS_HIP_CHECK(hipMemsetAsync(P.cmin.p, 0, nb, s));
+  SPOPS_HIP_CHECK(hipMemsetAsync(P.cmax.p, 0, nb, s));
+  P.ms_setup = tm.stop();
+  // ---- symbolic, every cluster
+  if (!P.small.empty())
+    hipLaunchKernelGGL(k_sym_small, dim3((unsigned)P.small.size()), dim3(kBlock), 0, s, P.dsmall.as<int32_t>(),
+                       (int64_t)P.small.size(), A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64,
+                       P.bcol32.as<int32_t>(), P.nz.as<int64_t>(), P.cmin.as<int64_t>(), P.cmax.as<int64_t>());
+  if (!P.large.empty())
+    hipLaunchKernelGGL(k_sym_large, dim3((unsigned)P.large.size()), dim3(kBlock), 0, s, P.dlarge.as<int32_t>(),
+                       A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64, P.bcol32.as<int32_t>(), (int32_t)b_cols,
+                       P.nz.as<int64_t>(), P.cmin.as<int64_t>(), P.cmax.as<int64_t>());
   SPOPS_HIP_CHECK(hipGetLastError());
-  C.csr_rows = A.csr_rows;
-  C.rows = nc;
-  C.cols = b_cols;
+  P.hnz.assign(nc, 0);
+  if (nc > 0) SPOPS_HIP_CHECK(hipMemcpyAsync(P.hnz.data(), P.nz.p, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
+  SPOPS_HIP_CHECK(hipStreamSynchronize(s));
+  P.nnzc = P.nnzv = 0;
+  for (int64_t c = 0; c < nc; ++c) {
+    P.nnzc += P.hnz[c];
+    P.nnzv += P.hnz[c] * P.hcsz[c];
+  }
+  // clusters with very many products accumulate densely (see k_dense_products)
+  P.hcmin.assign(nc, 0);
+  P.hcmax.assign(nc, 0);
+  P.harp.assign(nc + 1, 0);
+  if (nc > 0) {
+    SPOPS_HIP_CHECK(hipMemcpyAsync(P.hcmin.data(), P.cmin.p, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
+    SPOPS_HIP_CHECK(hipMemcpyAsync(P.hcmax.data(), P.cmax.p, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
+    SPOPS_HIP_CHECK(hipMemcpyAsync(P.harp.data(), A.rowptr.p, (size_t)(nc + 1) * 8, hipMemcpyDeviceToHost, s));
+  }
+  SPOPS_HIP_CHECK(hipStreamSynchronize(s));
+  P.dense.assign(nc, 0);
+  for (int32_t c : P.large) {
+    const int64_t bytes = (P.hcmax[c] - P.hcmin[c] + 1) * P.hcsz[c] * (int64_t)vb;
+    P.dense[c] = P.hflop[c] >= g_dense_products && P.hnz[c] > 0 && bytes <= kDenseScratch;
+  }
+  P.ms_symbolic = tm.stop();
+}
+
+namespace {
+void ensure(DevBuf &b, size_t bytes) {
+  if (b.bytes < bytes || b.p == nullptr) b.alloc(bytes);
+}
+}  // namespace
+
+void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_cluster_s &C,
+                         int64_t stats[kSpgemmStats], hipStream_t s) {
+  const spops_cluster_s &A = *P.A;
+  const int vt = A.value_type;
+  const int64_t nc = A.rows;
+  SPOPS_REQUIRE(0 <= c0 && c0 <= c1 && c1 <= nc, SPOPS_ERROR_INVALID_VALUE, "cluster range [%lld, %lld) outside [0, %lld)",
+                (long long)c0, (long long)c1, (long long)nc);
+  const int64_t np = c1 - c0;
+  const size_t vb = value_bytes(vt);
+  const bool whole = c0 == 0 && c1 == nc;
+  Timer tm(s);
+  // the panel's clusters with work, in the plan's bin order
+  std::vector<int32_t> psmall, plarge, pregular, pdense;
+  DevBuf dsmall, dlarge, dregular;
+  if (!whole) {
+    for (int32_t c : P.small) if (c >= c0 && c < c1) psmall.push_back(c);
+    for (int32_t c : P.large) if (c >= c0 && c < c1) plarge.push_back(c);
+    dsmall.upload(psmall);
+    dlarge.upload(plarge);
+  }
+  const std::vector<int32_t> &small = whole ? P.small : psmall;
+  const std::vector<int32_t> &large = whole ? P.large : plarge;
+  const int32_t *lsmall = whole ? P.dsmall.as<int32_t>() : dsmall.as<int32_t>();
+  const int32_t *llarge = whole ? P.dlarge.as<int32_t>() : dlarge.as<int32_t>();
+  for (int32_t c : large) (P.dense[c] ? pdense : pregular).push_back(c);
+  const bool split = !pdense.empty();
+  if (split) dregular.upload(pregular);
+  const std::vector<int32_t> &regular = split ? pregular : large;
+  const int32_t *lregular = split ? dregular.as<int32_t>() : llarge;
+  int64_t nnzc = 0, nnzv = 0, csr_rows = 0;
+  for (int64_t c = c0; c < c1; ++c) {
+    nnzc += P.hnz[c];
+    nnzv += P.hnz[c] * P.hcsz[c];
+    csr_rows += P.hcsz[c];
+  }
+  C.csr_rows = whole ? A.csr_rows : csr_rows;
+  C.rows = np;
+  C.cols = P.b_cols;
   C.value_type = vt;
   C.max_cluster_sz = A.max_cluster_sz;
-  C.rowptr.alloc((size_t)(nc + 1) * 8);
-  C.rowptr_val.alloc((size_t)(nc + 1) * 8);
-  int64_t nnzc = 0, nnzv = 0;
-  {
-    DevBuf nzp;
-    nzp.alloc((size_t)(nc + 1) * 8);
-    SPOPS_HIP_CHECK(hipMemsetAsync(nzp.p, 0, (size_t)(nc + 1) * 8, s));
-    SPOPS_HIP_CHECK(hipMemcpyAsync(nzp.p, nz.p, (size_t)nc * 8, hipMemcpyDeviceToDevice, s));
-    exclusive_scan_i64(nzp.as<int64_t>(), C.rowptr.as<int64_t>(), nc + 1, s, &nnzc);
-    if (nc > 0)
-      hipLaunchKernelGGL(k_mul_csz, dim3(nblk(nc, kBlock)), dim3(kBlock), 0, s, nz.as<int64_t>(),
-                         A.cluster_sz.as<int32_t>(), nc, nzp.as<int64_t>());
-    exclusive_scan_i64(nzp.as<int64_t>(), C.rowptr_val.as<int64_t>(), nc + 1, s, &nnzv);
-  }
   C.nnzc = nnzc;
   C.nnzv = nnzv;
-  double ms_symbolic = tm.stop();
+  ensure(C.rowptr, (size_t)(np + 1) * 8);
+  ensure(C.rowptr_val, (size_t)(np + 1) * 8);
+  {
+    DevBuf nzp;
+    nzp.alloc((size_t)(np + 1) * 8);
+    SPOPS_HIP_CHECK(hipMemsetAsync(nzp.p, 0, (size_t)(np + 1) * 8, s));
+    if (np > 0)
+      SPOPS_HIP_CHECK(hipMemcpyAsync(nzp.p, P.nz.as<int64_t>() + c0, (size_t)np * 8, hipMemcpyDeviceToDevice, s));
+    int64_t t = 0;
+    exclusive_scan_i64(nzp.as<int64_t>(), C.rowptr.as<int64_t>(), np + 1, s, &t);
+    if (np > 0)
+      hipLaunchKernelGGL(k_mul_csz, dim3(nblk(np, kBlock)), dim3(kBlock), 0, s, P.nz.as<int64_t>() + c0,
+                         A.cluster_sz.as<int32_t>() + c0, np, nzp.as<int64_t>());
+    exclusive_scan_i64(nzp.as<int64_t>(), C.rowptr_val.as<int64_t>(), np + 1, s, &t);
+  }
+  // per-cluster symbolic results and sizes of the panel
+  ensure(C.cluster_sz, (size_t)std::max<int64_t>(np, 1) * 4);
+  ensure(C.nzc, (size_t)std::max<int64_t>(np, 1) * 8);
+  ensure(C.cmin, (size_t)std::max<int64_t>(np, 1) * 8);
+  ensure(C.cmax, (size_t)std::max<int64_t>(np, 1) * 8);
+  if (np > 0) {
+    SPOPS_HIP_CHECK(hipMemcpyAsync(C.cluster_sz.p, A.cluster_sz.as<int32_t>() + c0, (size_t)np * 4,
+                                   hipMemcpyDeviceToDevice, s));
+    SPOPS_HIP_CHECK(hipMemcpyAsync(C.nzc.p, P.nz.as<int64_t>() + c0, (size_t)np * 8, hipMemcpyDeviceToDevice, s));
+    SPOPS_HIP_CHECK(hipMemcpyAsync(C.cmin.p, P.cmin.as<int64_t>() + c0, (size_t)np * 8, hipMemcpyDeviceToDevice, s));
+    SPOPS_HIP_CHECK(hipMemcpyAsync(C.cmax.p, P.cmax.as<int64_t>() + c0, (size_t)np * 8, hipMemcpyDeviceToDevice, s));
+  }
+  double ms_scan = tm.stop();
   // ---- allocate output (the reference's "malloc" phase); every entry is written below
-  C.colids.alloc((size_t)std::max<int64_t>(nnzc, 1) * 4);
-  C.values.alloc((size_t)std::max<int64_t>(nnzv, 1) * vb);
-  C.cluster_sz.alloc((size_t)std::max<int64_t>(nc, 1) * 4);
-  SPOPS_HIP_CHECK(hipMemcpyAsync(C.cluster_sz.p, A.cluster_sz.p, (size_t)nc * 4, hipMemcpyDeviceToDevice, s));
+  ensure(C.colids, (size_t)std::max<int64_t>(nnzc, 1) * 4);
+  ensure(C.values, (size_t)std::max<int64_t>(nnzv, 1) * vb);
   double ms_malloc = tm.stop();
-  // ---- numeric (large clusters first get their sorted columns)
+  // ---- numeric.  The kernels index the output's cluster pointers by the global cluster id c,
+  // so they get the panel's arrays shifted by c0 (only ids in [c0, c1) are touched).
+  int64_t *crp = C.rowptr.as<int64_t>() - c0;
+  int64_t *crpv = C.rowptr_val.as<int64_t>() - c0;
   if (!large.empty())
-    hipLaunchKernelGGL(k_emit_large, dim3((unsigned)large.size()), dim3(kBlock), 0, s, dlarge.as<int32_t>(),
-                       A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64, bcol32.as<int32_t>(),
-                       C.rowptr.as<int64_t>(), cmin.as<int64_t>(), cmax.as<int64_t>(), C.colids.as<int32_t>());
+    hipLaunchKernelGGL(k_emit_large, dim3((unsigned)large.size()), dim3(kBlock), 0, s, llarge, A.rowptr.as<int64_t>(),
+                       A.colids.as<int32_t>(), P.brp64, P.bcol32.as<int32_t>(), crp, P.cmin.as<int64_t>(),
+                       P.cmax.as<int64_t>(), C.colids.as<int32_t>());
   SPOPS_HIP_CHECK(hipGetLastError());
   auto numeric = [&](auto tag) {
     using V = decltype(tag);
     if (!small.empty())
-      hipLaunchKernelGGL(k_num_small<V>, dim3((unsigned)small.size()), dim3(kBlock), 0, s, dsmall.as<int32_t>(),
+      hipLaunchKernelGGL(k_num_small<V>, dim3((unsigned)small.size()), dim3(kBlock), 0, s, lsmall,
                          (int64_t)small.size(), A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(),
-                         A.cluster_sz.as<int32_t>(), A.colids.as<int32_t>(), A.values.as<V>(), brp64,
-                         bcol32.as<int32_t>(), static_cast<const V *>(bval), C.rowptr.as<int64_t>(),
-                         C.rowptr_val.as<int64_t>(), C.colids.as<int32_t>(), C.values.as<V>());
-    if (!large.empty())
-      hipLaunchKernelGGL(k_num_large<V>, dim3((unsigned)large.size()), dim3(kLargeBlock), 0, s, dlarge.as<int32_t>(),
+                         A.cluster_sz.as<int32_t>(), A.colids.as<int32_t>(), A.values.as<V>(), P.brp64,
+                         P.bcol32.as<int32_t>(), static_cast<const V *>(P.bval), crp, crpv, C.colids.as<int32_t>(),
+                         C.values.as<V>());
+    if (!regular.empty())
+      hipLaunchKernelGGL(k_num_large<V>, dim3((unsigned)regular.size()), dim3(kLargeBlock), 0, s, lregular,
                          A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(), A.cluster_sz.as<int32_t>(),
-                         A.colids.as<int32_t>(), A.values.as<V>(), brp64, bcol32.as<int32_t>(),
-                         static_cast<const V *>(bval), C.rowptr.as<int64_t>(), C.rowptr_val.as<int64_t>(),
-                         C.colids.as<int32_t>(), C.values.as<V>());
+                         A.colids.as<int32_t>(), A.values.as<V>(), P.brp64, P.bcol32.as<int32_t>(),
+                         static_cast<const V *>(P.bval), crp, crpv, C.colids.as<int32_t>(), C.values.as<V>(),
+                         P.b_nnz < (int64_t)INT32_MAX);
+    // dense clusters in batches whose accumulators fit kDenseScratch
+    size_t i = 0;
+    while (i < pdense.size()) {
+      std::vector<DenseTask> tasks;
+      std::vector<int32_t> blist;
+      std::vector<int64_t> boff;
+      int64_t used = 0, maxnz = 0;
+      while (i < pdense.size()) {
+        const int32_t c = pdense[i];
+        const int64_t n = (P.hcmax[c] - P.hcmin[c] + 1) * P.hcsz[c];
+        if (!blist.empty() && (used + n) * (int64_t)sizeof(V) > kDenseScratch) break;
+        blist.push_back(c);
+        boff.push_back(used);
+        for (int64_t t = P.harp[c]; t < P.harp[c + 1]; t += kDenseTaskCols)
+          tasks.push_back(DenseTask{c, t, std::min<int64_t>(t + kDenseTaskCols, P.harp[c + 1]), used});
+        used += n;
+        maxnz = std::max(maxnz, P.hnz[c]);
+        ++i;
+      }
+      ensure(P.scratch, (size_t)std::max<int64_t>(used, 1) * sizeof(V));
+      DevBuf dtasks, dlist, doff;
+      dtasks.upload(tasks);
+      dlist.upload(blist);
+      doff.upload(boff);
+      SPOPS_HIP_CHECK(hipMemsetAsync(P.scratch.p, 0, (size_t)used * sizeof(V), s));
+      if (!tasks.empty())
+        hipLaunchKernelGGL(k_dense_products<V>, dim3((unsigned)tasks.size()), dim3(kBlock), 0, s,
+                           dtasks.as<DenseTask>(), A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(),
+                           A.cluster_sz.as<int32_t>(), A.colids.as<int32_t>(), A.values.as<V>(), P.brp64,
+                           P.bcol32.as<int32_t>(), static_cast<const V *>(P.bval), P.cmin.as<int64_t>(),
+                           P.cmax.as<int64_t>(), P.scratch.as<V>());
+      SPOPS_HIP_CHECK(hipGetLastError());
+      const unsigned gx = (unsigned)std::min<int64_t>(std::max<int64_t>((maxnz + kBlock - 1) / kBlock, 1), 1024);
+      hipLaunchKernelGGL(k_dense_gather<V>, dim3(gx, (unsigned)blist.size()), dim3(kBlock), 0, s, dlist.as<int32_t>(),
+                         doff.as<int64_t>(), A.cluster_sz.as<int32_t>(), P.cmin.as<int64_t>(), P.cmax.as<int64_t>(),
+                         P.scratch.as<V>(), crp, crpv, C.colids.as<int32_t>(), C.values.as<V>());
+      SPOPS_HIP_CHECK(hipGetLastError());
+      SPOPS_HIP_CHECK(hipStreamSynchronize(s));    // the batch's task lists are freed next
+    }
   };
   if (vt == SPOPS_VALUE_F64) numeric(double(0));
   else numeric(float(0));
   SPOPS_HIP_CHECK(hipGetLastError());
   double ms_numeric = tm.stop();
-  // per-cluster min/max kept for the classification report
-  C.cmin = std::move(cmin);
-  C.cmax = std::move(cmax);
-  C.nzc = std::move(nz);
-  stats[0] = unsorted;
+  int64_t products = 0;
+  for (int64_t c = c0; c < c1; ++c) products += P.hflop[c];
+  stats[0] = P.unsorted;
   stats[1] = nnzc;
   stats[2] = nnzv;
   stats[3] = (int64_t)small.size();
   stats[4] = (int64_t)large.size();
-  stats[5] = (int64_t)(ms_setup * 1e3);
-  stats[6] = (int64_t)(ms_symbolic * 1e3);
+  stats[10] = (int64_t)pdense.size();
+  stats[5] = (int64_t)((whole ? P.ms_setup : 0.0) * 1e3);
+  stats[6] = (int64_t)((whole ? P.ms_symbolic : 0.0) * 1e3 + ms_scan * 1e3);
   stats[7] = (int64_t)(ms_malloc * 1e3);
   stats[8] = (int64_t)(ms_numeric * 1e3);
-  stats[9] = total_cluster_flop;
+  stats[9] = products;
+}
+
+void lespgemm(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *brp64,
+              const int64_t *bcol64, const void *bval, spops_cluster_s &C, int64_t stats[kSpgemmStats], hipStream_t s) {
+  spops_spgemm_plan_s P;
+  spgemm_plan_create(A, b_rows, b_cols, b_nnz, brp64, bcol64, bval, P, s);
+  spgemm_plan_numeric(P, 0, A.rows, C, stats, s);
+  SPOPS_HIP_CHECK(hipStreamSynchronize(s));    // the plan's buffers are freed on return
 }
 
 void cluster_to_csr(const spops_cluster_s &C, const spops_cluster_s &A, int64_t &nnz, DevBuf &rowptr, DevBuf &col,
@@ -925,7 +1159,7 @@ void cluster_to_csr(const spops_cluster_s &C, const spops_cluster_s &A, int64_t 
 using namespace spops;
 
 namespace {
-unsigned nblocks(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+unsigned nblocks(int64_t n, int per) { return grid_blocks(n, per); }
 }  // namespace
 
 extern "C" int spops_cluster_create(spops_cluster *out, int64_t n_rows, int64_t n_cols, int64_t nnz,
@@ -1030,6 +1264,110 @@ extern "C" int spops_lespgemm(spops_cluster A, int64_t b_rows, int64_t b_cols, i
     *C_out = C.release();
     return SPOPS_SUCCESS;
   });
+}
+
+extern "C" int spops_lespgemm_plan_create(spops_cluster A, int64_t b_rows, int64_t b_cols, int64_t b_nnz,
+                                          const int64_t *d_browptr, const int64_t *d_bcol, const void *d_bval,
+                                          spops_spgemm_plan *out, void *stream) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(A && out, SPOPS_ERROR_INVALID_VALUE, "null argument");
+    SPOPS_REQUIRE(A->cols == b_rows, SPOPS_ERROR_INVALID_VALUE, "A.cols (%lld) != B.rows (%lld)", (long long)A->cols,
+                  (long long)b_rows);
+    SPOPS_REQUIRE(b_cols < ((int64_t)1 << 31), SPOPS_ERROR_LIMIT, "B columns must be < 2^31");
+    SPOPS_REQUIRE(b_rows == 0 || (d_browptr && (b_nnz == 0 || (d_bcol && d_bval))), SPOPS_ERROR_INVALID_VALUE,
+                  "null B array");
+    auto P = std::make_unique<spops_spgemm_plan_s>();
+    spgemm_plan_create(*A, b_rows, b_cols, b_nnz, d_browptr, d_bcol, d_bval, *P, static_cast<hipStream_t>(stream));
+    *out = P.release();
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_lespgemm_plan_info(spops_spgemm_plan P, int64_t info[8]) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(P && info, SPOPS_ERROR_INVALID_VALUE, "null argument");
+    info[0] = P->A->rows;
+    info[1] = P->nnzc;
+    info[2] = P->nnzv;
+    info[3] = P->products;
+    info[4] = (int64_t)P->small.size();
+    info[5] = (int64_t)P->large.size();
+    info[6] = (int64_t)(P->ms_setup * 1e3);
+    info[7] = (int64_t)(P->ms_symbolic * 1e3);
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_lespgemm_plan_panels(spops_spgemm_plan P, int64_t budget_bytes, int64_t *npanels,
+                                          int64_t *bounds) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(P && npanels && bounds, SPOPS_ERROR_INVALID_VALUE, "null argument");
+    SPOPS_REQUIRE(budget_bytes > 0, SPOPS_ERROR_INVALID_VALUE, "budget must be positive");
+    const int64_t nc = P->A->rows, vb = (int64_t)value_bytes(P->A->value_type);
+    int64_t n = 0, acc = 0;
+    bounds[0] = 0;
+    for (int64_t c = 0; c < nc; ++c) {
+      const int64_t b = P->hnz[c] * (4 + P->hcsz[c] * vb) + 32;
+      if (acc > 0 && acc + b > budget_bytes) {
+        bounds[++n] = c;
+        acc = 0;
+      }
+      acc += b;
+    }
+    bounds[++n] = nc;
+    if (nc == 0) n = 1;
+    *npanels = n;
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_lespgemm_plan_numeric(spops_spgemm_plan P, int64_t c0, int64_t c1, spops_cluster *C_inout,
+                                           int64_t stats[16], void *stream) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(P && C_inout, SPOPS_ERROR_INVALID_VALUE, "null argument");
+    std::unique_ptr<spops_cluster_s> fresh;
+    spops_cluster_s *C = *C_inout;
+    if (!C) {
+      fresh = std::make_unique<spops_cluster_s>();
+      C = fresh.get();
+    }
+    int64_t st[kSpgemmStats] = {0};
+    spgemm_plan_numeric(*P, c0, c1, *C, st, static_cast<hipStream_t>(stream));
+    if (stats)
+      for (int i = 0; i < 16; ++i) stats[i] = i < kSpgemmStats ? st[i] : 0;
+    if (fresh) *C_inout = fresh.release();
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_cluster_checksum(spops_cluster C, double out[4], void *stream) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(C && out, SPOPS_ERROR_INVALID_VALUE, "null argument");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    constexpr int grid = 1024;
+    DevBuf part;
+    part.alloc((size_t)grid * 4 * 8);
+    if (C->value_type == SPOPS_VALUE_F64)
+      hipLaunchKernelGGL(k_cluster_checksum<double>, dim3(grid), dim3(kBlock), 0, s, C->values.as<double>(), C->nnzv,
+                         C->colids.as<int32_t>(), C->nnzc, part.as<double>());
+    else
+      hipLaunchKernelGGL(k_cluster_checksum<float>, dim3(grid), dim3(kBlock), 0, s, C->values.as<float>(), C->nnzv,
+                         C->colids.as<int32_t>(), C->nnzc, part.as<double>());
+    SPOPS_HIP_CHECK(hipGetLastError());
+    std::vector<double> h((size_t)grid * 4);
+    SPOPS_HIP_CHECK(hipMemcpyAsync(h.data(), part.p, h.size() * 8, hipMemcpyDeviceToHost, s));
+    SPOPS_HIP_CHECK(hipStreamSynchronize(s));
+    for (int q = 0; q < 4; ++q) out[q] = 0;
+    for (int g = 0; g < grid; ++g)
+      for (int q = 0; q < 4; ++q) out[q] += h[(size_t)g * 4 + q];
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_lespgemm_plan_destroy(spops_spgemm_plan P) {
+  if (P) (void)hipDeviceSynchronize();
+  delete P;
+  return SPOPS_SUCCESS;
 }
 
 extern "C" int spops_cluster_to_csr(spops_cluster C, spops_cluster A, int64_t *nnz, int64_t **d_rowptr, int64_t **d_col,

@@ -10,12 +10,14 @@ constexpr int kSmallTable = 2048;                // LDS hash slots (load factor 
 constexpr int kSmallAcc = 4096;                  // LDS accumulator entries (nz * csz)
 constexpr int kLargeTileBits = 1 << 18;         // columns per LDS bitmap pass (32 KiB)
 constexpr int kLargeBlock = 512;                 // threads of a large-cluster numeric workgroup
-constexpr int kLargeAccEntries = 13824;          // LDS accumulator entries of a numeric chunk
-constexpr int kLargeHash = 4096;                 // LDS hash slots of a chunk's columns
+constexpr int kLargeAccEntries = 4608;           // LDS accumulator entries of a numeric chunk
+constexpr int kLargeHash = 2048;                 // LDS hash slots of a chunk's columns
 constexpr int kLargeCursors = 2048;              // A columns whose B-row position is kept in LDS
 constexpr int kLargeGroupLanes = 8;              // lanes scanning one B row in the numeric pass
 constexpr int kMaxClusterRows = 16;              // A values of a column held in registers per pass
 constexpr int kSpgemmStats = 16;
+constexpr int64_t kDenseScratch = (int64_t)1 << 30;    // bytes of dense accumulators live at once
+constexpr int kDenseTaskCols = 32;                      // A columns per dense-product workgroup
 
 }  // namespace spops
 
@@ -33,7 +35,37 @@ struct spops_cluster_s {
   spops::DevBuf nzc, cmin, cmax;
 };
 
+// A LeSpGEMM product C = A * B prepared once and computed in cluster ranges ("panels"): B
+// with int32 columns and sorted rows, the per-cluster work, the small / large bins and the
+// symbolic results (nz, min / max column) of every cluster.  A and B must outlive the plan.
+struct spops_spgemm_plan_s {
+  const spops_cluster_s *A = nullptr;
+  int64_t b_rows = 0, b_cols = 0, b_nnz = 0;
+  const int64_t *brp64 = nullptr;
+  const void *bval = nullptr;              // B's values, or the sorted copy below
+  spops::DevBuf bcol32, bsorted_val;
+  int unsorted = 0;
+  std::vector<int64_t> hflop, hnz;         // per cluster: products, distinct columns of C
+  std::vector<int32_t> hcsz;
+  std::vector<int32_t> small, large;       // clusters with work, by bin (large by decreasing work)
+  std::vector<char> dense;                 // per cluster: accumulated in a dense row block in memory
+  std::vector<int64_t> harp;               // A's cluster column pointers (host copy)
+  std::vector<int64_t> hcmin, hcmax;       // per cluster: column range of C
+  spops::DevBuf scratch;                   // dense accumulators (kDenseScratch at most)
+  spops::DevBuf dsmall, dlarge;
+  spops::DevBuf nz, cmin, cmax;            // int64 [rows]
+  int64_t nnzc = 0, nnzv = 0, products = 0;
+  double ms_setup = 0, ms_symbolic = 0;
+};
+
 namespace spops {
+extern int64_t g_dense_products;   // products from which a cluster accumulates densely (tuning)
+void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *brp64,
+                        const int64_t *bcol64, const void *bval, spops_spgemm_plan_s &P, hipStream_t s);
+// C = rows [c0, c1) of the product as a cluster matrix of c1 - c0 clusters; C's buffers are
+// reused when large enough
+void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_cluster_s &C,
+                         int64_t stats[kSpgemmStats], hipStream_t s);
 void cluster_build(spops_cluster_s &A, int64_t n, int64_t n_cols, int64_t nnz, const int64_t *rowptr, const int64_t *col,
                    const void *values, int vt, const int64_t *h_perm, int64_t nc, const int64_t *h_offsets, hipStream_t s);
 void lespgemm(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *brp64,

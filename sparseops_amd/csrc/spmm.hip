@@ -284,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_convert_values(const typename Ty<SC>
   dst[i] = Ty<DC>::st(Ty<SC>::ld(src[perm ? (int64_t)perm[i] : i]));
 }
 
-inline unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+inline unsigned nblk(int64_t n, int per) { return grid_blocks(n, per); }
 
 void require_csr(const spops_matrix_s *A) {
   SPOPS_REQUIRE(A != nullptr, SPOPS_ERROR_INVALID_VALUE, "null matrix");
@@ -401,7 +401,6 @@ void sddmm_launch_vec(const spops_matrix_s &A, int k, const void *G, const void 
 template <int GC, int XC, int OC>
 void sddmm_launch(const spops_matrix_s &A, int k, const void *G, const void *X, void *out, hipStream_t s) {
   if (A.nnz == 0) return;
-  SPOPS_REQUIRE(A.nnz < (int64_t)INT32_MAX * 16, SPOPS_ERROR_LIMIT, "SDDMM grid limit");
   if (A.csr.erow.p == nullptr) {
     if (A.csr.rowptr64) build_entry_rows<long>(A, s);
     else build_entry_rows<int>(A, s);

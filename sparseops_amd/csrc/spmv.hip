@@ -569,6 +569,7 @@ struct RowSource {
 };
 
 int64_t g_slice_bytes = kSliceBytesDefault;
+int64_t g_dense_products = (int64_t)1 << 20;
 
 void build_slice_layout(spops_matrix_s &M, const RowSource &src, int vt) {
   auto &S = M.sell;
@@ -1191,10 +1192,17 @@ extern "C" int spops_matrix_info(spops_matrix A, int64_t info[16]) {
 
 extern "C" int spops_set_tuning(int key, int64_t value, int64_t *old) {
   return guarded([&]() {
-    SPOPS_REQUIRE(key == SPOPS_TUNE_SLICE_BYTES, SPOPS_ERROR_INVALID_VALUE, "unknown tuning key %d", key);
-    SPOPS_REQUIRE(value >= 64, SPOPS_ERROR_INVALID_VALUE, "slice bytes must be >= 64");
-    if (old) *old = g_slice_bytes;
-    g_slice_bytes = value;
+    if (key == SPOPS_TUNE_SLICE_BYTES) {
+      SPOPS_REQUIRE(value >= 64, SPOPS_ERROR_INVALID_VALUE, "slice bytes must be >= 64");
+      if (old) *old = g_slice_bytes;
+      g_slice_bytes = value;
+    } else if (key == SPOPS_TUNE_DENSE_PRODUCTS) {
+      SPOPS_REQUIRE(value >= 1, SPOPS_ERROR_INVALID_VALUE, "dense product threshold must be >= 1");
+      if (old) *old = g_dense_products;
+      g_dense_products = value;
+    } else {
+      throw_error(SPOPS_ERROR_INVALID_VALUE, "unknown tuning key %d", key);
+    }
     return SPOPS_SUCCESS;
   });
 }

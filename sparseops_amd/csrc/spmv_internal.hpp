@@ -79,7 +79,8 @@ struct spops_matrix_s {
 };
 
 namespace spops {
-extern int64_t g_slice_bytes;   // x bytes per slice for matrices created from now on
+extern int64_t g_slice_bytes;
+extern int64_t g_dense_products;   // LeSpGEMM: products of a cluster from which it accumulates densely   // x bytes per slice for matrices created from now on
 // Builds and uploads the CSR plan of a matrix: SpMV row tasks and long rows (> kCsrLongRow
 // entries); for SpMM the short-row order and the segments of the rows > kSpmmLongRow.
 void upload_csr_plan(CsrDev &P, const std::vector<int64_t> &rp, int64_t n_rows);

@@ -6,6 +6,10 @@
     C_cl    = A_cl.multiply(B)                                       # LeSpGEMM_VLength kernel 3
     C       = C_cl.to_csr(A_cl)                                      # vlength_cluster2csr + inverse permutation
 
+    plan    = SpgemmPlan(A_cl, B)                                    # the same product in cluster panels
+    for c0, c1 in plan.panel_ranges(budget_bytes):
+        P = plan.numeric(c0, c1, P)                                  # clusters [c0, c1) of C_cl
+
 Every step runs in libsparseops_hip.so; this module only moves arrays.
 """
 from __future__ import annotations
@@ -85,10 +89,19 @@ class ClusterMatrix:
     def __init__(self, handle, value_type):
         self._h = handle
         self.value_type = value_type
+        self.stats = None
+        self._refresh()
+
+    def _refresh(self):
         info = (c_i64 * 8)()
         L.call("spops_cluster_info", self._h, info)
         self.rows, self.csr_rows, self.cols, self.nnzc, self.nnzv, self.max_cluster_sz = list(info)[:6]
-        self.stats = None
+
+    def checksum(self):
+        """{sum of values, sum of |values|, sum of column ids, nonzero values}."""
+        out = (ctypes.c_double * 4)()
+        L.call("spops_cluster_checksum", self._h, out, L.current_stream())
+        return {"sum": out[0], "sum_abs": out[1], "colid_sum": int(round(out[2])), "nonzeros": int(round(out[3]))}
 
     @classmethod
     def build(cls, dA: DeviceCsr, perm, offsets):
@@ -106,10 +119,7 @@ class ClusterMatrix:
         L.call("spops_lespgemm", self._h, dB.n_rows, dB.n_cols, dB.nnz, dB.rowptr.data_ptr(), dB.col.data_ptr(),
                dB.val.data_ptr(), ctypes.byref(h), stats, L.current_stream())
         C = ClusterMatrix(h, self.value_type)
-        s = list(stats)
-        C.stats = {"b_rows_sorted_copy": s[0], "nnzc": s[1], "nnzv": s[2], "small_clusters": s[3], "large_clusters": s[4],
-                   "setup_ms": s[5] / 1e3, "symbolic_ms": s[6] / 1e3, "malloc_ms": s[7] / 1e3,
-                   "numeric_ms": s[8] / 1e3, "cluster_products": s[9]}
+        C.stats = _stats(stats)
         return C
 
     def download(self):
@@ -150,6 +160,79 @@ class ClusterMatrix:
     def close(self):
         if self._h:
             L.lib.spops_cluster_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class dense_products:
+    """Context manager: LeSpGEMM plans created inside use a dense accumulator for clusters
+    with at least `products` products."""
+
+    def __init__(self, products: int):
+        self.products = int(products)
+
+    def __enter__(self):
+        old = c_i64()
+        L.call("spops_set_tuning", L.TUNE_DENSE_PRODUCTS, self.products, ctypes.byref(old))
+        self.old = old.value
+        return self
+
+    def __exit__(self, *exc):
+        L.call("spops_set_tuning", L.TUNE_DENSE_PRODUCTS, self.old, None)
+
+
+def _stats(stats):
+    s = list(stats)
+    return {"b_rows_sorted_copy": s[0], "nnzc": s[1], "nnzv": s[2], "small_clusters": s[3], "large_clusters": s[4],
+            "setup_ms": s[5] / 1e3, "symbolic_ms": s[6] / 1e3, "malloc_ms": s[7] / 1e3, "numeric_ms": s[8] / 1e3,
+            "cluster_products": s[9], "dense_clusters": s[10]}
+
+
+class SpgemmPlan:
+    """C = A_cl * B prepared once (B's sorted int32 copy, bins, symbolic phase of every
+    cluster), then computed in cluster ranges whose output fits a memory budget."""
+
+    def __init__(self, Acl: ClusterMatrix, dB: DeviceCsr):
+        if Acl.cols != dB.n_rows:
+            raise ValueError("Error: A.cols != B.rows")
+        self._keep = (Acl, dB)
+        self.value_type = Acl.value_type
+        h = ctypes.c_void_p()
+        L.call("spops_lespgemm_plan_create", Acl._h, dB.n_rows, dB.n_cols, dB.nnz, dB.rowptr.data_ptr(),
+               dB.col.data_ptr(), dB.val.data_ptr(), ctypes.byref(h), L.current_stream())
+        self._h = h
+        info = (c_i64 * 8)()
+        L.call("spops_lespgemm_plan_info", h, info)
+        (self.clusters, self.nnzc, self.nnzv, self.products, self.small_clusters,
+         self.large_clusters) = list(info)[:6]
+        self.setup_ms, self.symbolic_ms = info[6] / 1e3, info[7] / 1e3
+
+    def panel_ranges(self, budget_bytes: int):
+        bounds = np.zeros(self.clusters + 2, np.int64)
+        n = c_i64()
+        L.call("spops_lespgemm_plan_panels", self._h, int(budget_bytes), ctypes.byref(n), bounds.ctypes.data)
+        return [(int(bounds[i]), int(bounds[i + 1])) for i in range(n.value)]
+
+    def numeric(self, c0: int, c1: int, C: ClusterMatrix | None = None) -> ClusterMatrix:
+        """Clusters [c0, c1) of the product; C's buffers are reused when given."""
+        h = ctypes.c_void_p(C._h.value if C is not None else None)
+        stats = (c_i64 * 16)()
+        L.call("spops_lespgemm_plan_numeric", self._h, int(c0), int(c1), ctypes.byref(h), stats, L.current_stream())
+        if C is None:
+            C = ClusterMatrix(h, self.value_type)
+        else:
+            C._refresh()
+        C.stats = _stats(stats)
+        return C
+
+    def close(self):
+        if self._h:
+            L.lib.spops_lespgemm_plan_destroy(self._h)
             self._h = None
 
     def __del__(self):

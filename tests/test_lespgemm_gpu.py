@@ -189,3 +189,95 @@ def test_lespgemm_unsorted_b_rows(dev):
     np.testing.assert_array_equal(o1["C_col"], o2["C_col"])
     np.testing.assert_allclose(o2["C_val"], o1["C_val"], rtol=1e-12, atol=0)
     _check_against_oracle(A, Bs, o1)
+
+
+@pytest.mark.parametrize("scale,budget", [(12, 1 << 16), (12, 1 << 20), (10, 1)])
+def test_panels_match_the_whole_product(scale, budget, dev):
+    """The product computed in cluster panels (output buffers reused from panel to panel)
+    equals the whole product panel by panel: structure exactly, values to 1e-12 (the LDS
+    accumulators add in a run-dependent order); budget 1 gives one cluster per panel.
+    Checksums of the panels add up to the whole product's."""
+    A = rmat(scale, 16, seed=scale)
+    dA = LG.DeviceCsr(A, dev)
+    sig = orc.minhash(A, 64, 7)
+    perm, off = orc.hc(A, *orc.lsh_pairs(sig, 64, 16), cluster_size=8)
+    Acl = LG.ClusterMatrix.build(dA, perm, off)
+    whole = Acl.multiply(dA)
+    csz, rp, rpv, col, val = whole.download()
+    plan = LG.SpgemmPlan(Acl, dA)
+    assert (plan.clusters, plan.nnzc, plan.nnzv) == (Acl.rows, whole.nnzc, whole.nnzv)
+    ranges = plan.panel_ranges(budget)
+    assert ranges[0][0] == 0 and ranges[-1][1] == Acl.rows
+    assert all(a < b for a, b in ranges) and all(ranges[i][1] == ranges[i + 1][0] for i in range(len(ranges) - 1))
+    if budget == 1:
+        assert len(ranges) == Acl.rows
+    elif budget == 1 << 16:
+        assert len(ranges) > 4
+    sums = np.zeros(4)
+    P = None
+    for c0, c1 in ranges[:: max(1, len(ranges) // 40)] if budget == 1 else ranges:
+        P = plan.numeric(c0, c1, P)
+        pcsz, prp, prpv, pcol, pval = P.download()
+        np.testing.assert_array_equal(pcsz, csz[c0:c1])
+        np.testing.assert_array_equal(prp, rp[c0:c1 + 1] - rp[c0])
+        np.testing.assert_array_equal(prpv, rpv[c0:c1 + 1] - rpv[c0])
+        np.testing.assert_array_equal(pcol, col[rp[c0]:rp[c1]])
+        np.testing.assert_allclose(pval, val[rpv[c0]:rpv[c1]], rtol=1e-12, atol=0)
+        assert P.csr_rows == int(csz[c0:c1].sum())
+        ck = P.checksum()
+        assert ck["colid_sum"] == int(pcol.sum()) and ck["nonzeros"] == int(np.count_nonzero(pval))
+        np.testing.assert_allclose(ck["sum"], pval.sum(), rtol=1e-12)
+        sums += [ck["sum"], ck["sum_abs"], ck["colid_sum"], ck["nonzeros"]]
+    if budget != 1:
+        wk = whole.checksum()
+        np.testing.assert_allclose(sums[:2], [wk["sum"], wk["sum_abs"]], rtol=1e-12)
+        assert sums[2] == wk["colid_sum"] and sums[3] == wk["nonzeros"]
+    with pytest.raises(LG.L.SpopsError):
+        plan.numeric(0, Acl.rows + 1)
+    plan.close()
+
+
+def test_lsh_more_raw_pairs_than_one_wave_each_fits_in_a_launch(dev):
+    """2^20 identical rows: one bucket per band, each row paired with its next 16 (constrain),
+    16 bands -> 2.7e8 raw pairs before de-duplication, more waves than one launch can hold
+    (the scoring kernel strides).  Every one of the 16.8e6 unique pairs (i, i + d), d <= 16,
+    has score 1."""
+    n = 1 << 20
+    A = Csr(n, 1, np.arange(n + 1, dtype=np.int64), np.zeros(n, np.int64), np.ones(n))
+    pi, pj, sc = LG.lsh_candidate_pairs(LG.DeviceCsr(A, dev), 64, 16, 7, constrain=True)
+    d = np.arange(1, 17)
+    wi = np.repeat(np.arange(n, dtype=np.int64), 16)
+    wj = wi + np.tile(d, n)
+    keep = wj < n
+    np.testing.assert_array_equal(pi, wi[keep])
+    np.testing.assert_array_equal(pj, wj[keep])
+    assert np.all(sc == 1.0)
+
+
+@pytest.mark.parametrize("threshold,budget", [(500, 1 << 40), (2000, 1 << 18)])
+def test_dense_accumulator_clusters(threshold, budget, dev):
+    """Clusters with at least `threshold` products take the dense path (device-memory row
+    block, many workgroups per cluster, atomic adds, gather at the sorted columns): the
+    product equals the LDS-chunk path's, structure exactly and values to 1e-12, whole or in
+    panels."""
+    A = rmat(12, 16, seed=3)
+    dA = LG.DeviceCsr(A, dev)
+    sig = orc.minhash(A, 64, 7)
+    perm, off = orc.hc(A, *orc.lsh_pairs(sig, 64, 16), cluster_size=8)
+    Acl = LG.ClusterMatrix.build(dA, perm, off)
+    ref = Acl.multiply(dA)
+    csz, rp, rpv, col, val = ref.download()
+    assert ref.stats["dense_clusters"] == 0
+    with LG.dense_products(threshold):
+        plan = LG.SpgemmPlan(Acl, dA)
+    P = None
+    dense = 0
+    for c0, c1 in plan.panel_ranges(budget):
+        P = plan.numeric(c0, c1, P)
+        dense += P.stats["dense_clusters"]
+        pcsz, prp, prpv, pcol, pval = P.download()
+        np.testing.assert_array_equal(prp, rp[c0:c1 + 1] - rp[c0])
+        np.testing.assert_array_equal(pcol, col[rp[c0]:rp[c1]])
+        np.testing.assert_allclose(pval, val[rpv[c0]:rpv[c1]], rtol=1e-12, atol=0)
+    assert dense > 10
+    plan.close()
