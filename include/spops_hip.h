@@ -128,6 +128,25 @@ int spops_bsr_create_from_blocks(spops_matrix *A, int64_t n_rows, int64_t n_cols
  * follows the value type (see top).  Asynchronous on `stream`. */
 int spops_spmv(spops_matrix A, double alpha, const void *d_x, double beta, void *d_y, void *stream);
 
+/* ---- matrix features ------------------------------------------------------------------------
+ * The reference's feature extractor (MTX<I,V>, include/sparse_features.h:13-326;
+ * src/sparse_features.cpp:170-480, 609-850) for a general real CSR matrix on the device:
+ * out[SPOPS_NUM_FEATURES] in this order (the reference's names):
+ *   0 num_rows  1 num_cols  2 real_nnz  3 nnz_ratio  4 num_of_diags  5 nnz_closed2diag_ratio
+ *   6 nnz_lower  7 nnz_upper  8 nnz_diagonal
+ *   9..16  rows: NoneZero_Row_Ratio, min, max, ave, var, standard_dev, P-ratio, Gini
+ *   17..24 columns: the same
+ *   25 diagonal_dominant_ratio  26 max_value_offdiag  27 max_value_diagonal
+ *   28 row_variability  29 col_variability
+ *   30 number of tiles per dimension (256)  31 rows per tile  32 columns per tile
+ *   33..40 tiles: NE_ratio, min, max, ave, var, stand_dev, P-ratio, Gini
+ *   41..48 row blocks (RB): the same   49..56 column blocks (CB): the same
+ *   57 uniq_R  58 uniq_C  59 potReuseR  60 potReuseC
+ * Synchronous on `stream` (the host finishes the statistics). */
+#define SPOPS_NUM_FEATURES 61
+int spops_matrix_features(const void *d_rowptr, const void *d_col, const void *d_values, int index_type,
+                          int value_type, int64_t n_rows, int64_t n_cols, double *out, void *stream);
+
 /* ---- batched SpMM ---------------------------------------------------------------------------
  * No reference counterpart as an entry point: many independent matrices, each with its own
  * number of right-hand sides k_i, multiplied in one persistent launch (Y_i = alpha A_i X_i +

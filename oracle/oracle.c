@@ -897,3 +897,187 @@ void orc_csr_transpose(int64_t n_rows, int64_t n_cols, const int64_t *rowptr, co
     }
   free(cur);
 }
+
+/* ---- matrix features ------------------------------------------------------------------------
+ * The reference's feature extractor for a general real matrix given as CSR, entries taken in
+ * CSR order (the order MtxLoad reads a file written from that CSR):
+ *   per entry     src/sparse_features.cpp:336-467 (general branch): row / column counts,
+ *                 diagonal offsets, 256 x 256 tiles with the uneven split of :186-190 and
+ *                 :357-358, first occurrence of a row in a tile column and of a column in a
+ *                 tile row (Rows_flag / Cols_flag), Diag_Dom, lower / upper / diagonal counts,
+ *                 largest |value| off and on the diagonal, log10 extremes per row / column
+ *                 starting from 0.0;
+ *   statistics    CalculateFeatures :643-760, CalculateTilesFeatures :763-850 (tile variance
+ *                 and nonzero ratio start from -1.0), P_ratioAndGini :609-640,
+ *                 trapezoidalRule :89-100.
+ * The output order is the library's (spops_matrix_features).
+ * Parity: pinned against the reference's own MTX::FeaturesPrint output on seeded matrices
+ * (tests/golden/features_*.npz, printed to 6 significant digits) and, exactly, against the
+ * device implementation on larger ones. */
+static int cmp_i64(const void *a, const void *b) {
+  const int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+  return (x > y) - (x < y);
+}
+
+static void orc_p_ratio_gini(const int64_t *vec, int64_t length, int64_t nnz, double *p_ratio, double *gini) {
+  int64_t *v = (int64_t *)malloc(sizeof(int64_t) * (size_t)length);
+  memcpy(v, vec, sizeof(int64_t) * (size_t)length);
+  qsort(v, (size_t)length, sizeof(int64_t), cmp_i64);
+  int64_t quit = length, step = 0, count = 0;
+  double p = 0.0, q = 0.0;
+  do {
+    --quit;
+    ++count;
+    step += v[quit];
+    p = (double)count / (double)length;
+    q = (double)step / (double)nnz;
+  } while (p + q < 1.0);
+  *p_ratio = p;
+  /* cumulative sums, trapezoidal area with unit steps, area under the diagonal */
+  double area = 0.0, x_step = (double)length / (double)length;
+  int64_t cum = v[0], prev;
+  area += (double)(cum + 0) / 2.0;
+  for (int64_t i = 0; i + 1 < length; ++i) {
+    prev = cum;
+    cum += v[i + 1];
+    area += (double)(prev + cum) / 2.0 * x_step;
+  }
+  const double total = (double)nnz * (double)length / 2.0;
+  *gini = (total - area) / total;
+  free(v);
+}
+
+static void orc_count_stats(const int64_t *n, int64_t len, int64_t nnz, double *o) {
+  int64_t mn = 100000000, mx = 0, nz = 0;
+  const double ave = (double)nnz / (double)len;
+  double var = 0.0;
+  for (int64_t i = 0; i < len; ++i) {
+    if (n[i] < mn) mn = n[i];
+    if (n[i] > mx) mx = n[i];
+    if (n[i]) ++nz;
+    const double d = (double)n[i] - ave;
+    var += d * d;
+  }
+  var /= (double)len;
+  o[0] = (double)nz / (double)len;
+  o[1] = (double)mn;
+  o[2] = (double)mx;
+  o[3] = ave;
+  o[4] = var;
+  o[5] = sqrt(var);
+  orc_p_ratio_gini(n, len, nnz, &o[6], &o[7]);
+}
+
+static void orc_tile_stats(const int64_t *n, int64_t len, int64_t nnz, double ave, double *o) {
+  double nzr = -1.0, var = -1.0;
+  int64_t mn = 100000000, mx = 0;
+  for (int64_t i = 0; i < len; ++i) {
+    if (n[i]) nzr += 1.0;
+    const double d = (double)n[i] - ave;
+    var += d * d;
+    if (n[i] < mn) mn = n[i];
+    if (n[i] > mx) mx = n[i];
+  }
+  nzr /= (double)len;
+  var /= (double)len;
+  o[0] = nzr;
+  o[1] = (double)mn;
+  o[2] = (double)mx;
+  o[3] = ave;
+  o[4] = var;
+  o[5] = sqrt(var > 0.0 ? var : 0.0);
+  orc_p_ratio_gini(n, len, nnz, &o[6], &o[7]);
+}
+
+void orc_features(int64_t n_rows, int64_t n_cols, const int64_t *rowptr, const int64_t *col, const double *val,
+                  double *out) {
+  enum { T = 256 };
+  const int64_t nnz = rowptr[n_rows] - rowptr[0];
+  const int64_t ndiag = n_rows + n_cols - 1, close_thr = ndiag / 20;
+  const int64_t rb_num = n_rows / T, rb_mod = n_rows % T, rb_thr = rb_mod * (rb_num + 1);
+  const int64_t cb_num = n_cols / T, cb_mod = n_cols % T, cb_thr = cb_mod * (cb_num + 1);
+  int64_t *rowlen = (int64_t *)calloc((size_t)n_rows, 8), *collen = (int64_t *)calloc((size_t)n_cols, 8);
+  int64_t *offcnt = (int64_t *)calloc((size_t)ndiag, 8);
+  int64_t *tiles = (int64_t *)calloc(T * T, 8), *rbc = (int64_t *)calloc(T, 8), *cbc = (int64_t *)calloc(T, 8);
+  int64_t *urb = (int64_t *)calloc(T * T, 8), *ucb = (int64_t *)calloc(T * T, 8);
+  unsigned char *rflag = (unsigned char *)calloc((size_t)n_rows * T, 1), *cflag = (unsigned char *)calloc((size_t)n_cols * T, 1);
+  double *dom = (double *)calloc((size_t)n_rows, 8);
+  double *rmax = (double *)calloc((size_t)n_rows, 8), *rmin = (double *)calloc((size_t)n_rows, 8);
+  double *cmax = (double *)calloc((size_t)n_cols, 8), *cmin = (double *)calloc((size_t)n_cols, 8);
+  int64_t lower = 0, upper = 0, diag = 0;
+  double maxoff = -1.7976931348623157e308, maxdiag = -1.7976931348623157e308;
+  for (int64_t r = 0; r < n_rows; ++r)
+    for (int64_t p = rowptr[r]; p < rowptr[r + 1]; ++p) {
+      const int64_t c = col[p];
+      const double a = fabs(val[p]);
+      rowlen[r]++;
+      collen[c]++;
+      offcnt[c - r + n_rows - 1]++;
+      const int64_t tr = r < rb_thr ? r / (rb_num + 1) : rb_mod + (r - rb_thr) / rb_num;
+      const int64_t tc = c < cb_thr ? c / (cb_num + 1) : cb_mod + (c - cb_thr) / cb_num;
+      tiles[tr * T + tc]++;
+      rbc[tr]++;
+      cbc[tc]++;
+      if (!rflag[r * T + tc]) { urb[tr * T + tc]++; rflag[r * T + tc] = 1; }
+      if (!cflag[c * T + tr]) { ucb[tr * T + tc]++; cflag[c * T + tr] = 1; }
+      if (r == c) {
+        diag++;
+        maxdiag = maxdiag > a ? maxdiag : a;
+        dom[r] += a;
+      } else {
+        if (r > c) lower++; else upper++;
+        maxoff = maxoff > a ? maxoff : a;
+        dom[r] -= a;
+      }
+      const double lg = log10(a);
+      if (rmax[r] < lg) rmax[r] = lg;
+      if (a > 0.0 && rmin[r] > lg) rmin[r] = lg;
+      if (cmax[c] < lg) cmax[c] = lg;
+      if (a > 0.0 && cmin[c] > lg) cmin[c] = lg;
+    }
+  int64_t ndiags = 0, close = 0;
+  for (int64_t d = 0; d < ndiag; ++d)
+    if (offcnt[d]) {
+      ndiags++;
+      const int64_t off = d - (n_rows - 1);
+      if ((off < 0 ? -off : off) <= close_thr) close += offcnt[d];
+    }
+  out[0] = (double)n_rows;
+  out[1] = (double)n_cols;
+  out[2] = (double)nnz;
+  out[3] = (double)nnz / ((double)n_rows * (double)n_cols);
+  out[4] = (double)ndiags;
+  out[5] = (double)close / (double)nnz;
+  out[6] = (double)lower;
+  out[7] = (double)upper;
+  out[8] = (double)diag;
+  orc_count_stats(rowlen, n_rows, nnz, out + 9);
+  orc_count_stats(collen, n_cols, nnz, out + 17);
+  int64_t dominance = 0;
+  double rv = -1.0, cv = -1.0;
+  for (int64_t r = 0; r < n_rows; ++r) {
+    if (dom[r] > 0) dominance++;
+    if (rmin[r] != 0.0) { const double t = rmax[r] - rmin[r]; rv = rv > t ? rv : t; }
+  }
+  for (int64_t c = 0; c < n_cols; ++c)
+    if (cmin[c] != 0.0) { const double t = cmax[c] - cmin[c]; cv = cv > t ? cv : t; }
+  out[25] = (double)dominance / (double)(n_rows < n_cols ? n_rows : n_cols);
+  out[26] = maxoff;
+  out[27] = maxdiag;
+  out[28] = rv;
+  out[29] = cv;
+  out[30] = (double)T;
+  out[31] = (double)rb_num;
+  out[32] = (double)cb_num;
+  orc_tile_stats(tiles, T * T, nnz, (double)nnz / ((double)T * (double)T), out + 33);
+  orc_tile_stats(rbc, T, nnz, (double)nnz / (double)T, out + 41);
+  orc_tile_stats(cbc, T, nnz, (double)nnz / (double)T, out + 49);
+  int64_t su = 0, sv = 0;
+  for (int i = 0; i < T * T; ++i) { su += urb[i]; sv += ucb[i]; }
+  out[57] = (double)su / (double)nnz;
+  out[58] = (double)sv / (double)nnz;
+  out[59] = (double)su / (double)n_rows;
+  out[60] = (double)sv / (double)n_cols;
+  free(rowlen); free(collen); free(offcnt); free(tiles); free(rbc); free(cbc); free(urb); free(ucb);
+  free(rflag); free(cflag); free(dom); free(rmax); free(rmin); free(cmax); free(cmin);
+}

@@ -93,6 +93,11 @@ void orc_sddmm_csr(int64_t n_rows, const int64_t *rowptr, const int64_t *col, in
 void orc_csr_transpose(int64_t n_rows, int64_t n_cols, const int64_t *rowptr, const int64_t *col,
                        const double *val, int64_t *t_rowptr, int64_t *t_col, double *t_val, int64_t *t_perm);
 
+/* ---- matrix features (MTX<I,V>, src/sparse_features.cpp) ---- */
+#define ORC_NUM_FEATURES 61
+void orc_features(int64_t n_rows, int64_t n_cols, const int64_t *rowptr, const int64_t *col, const double *val,
+                  double *out);
+
 #ifdef __cplusplus
 }
 #endif

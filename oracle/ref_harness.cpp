@@ -17,11 +17,14 @@
 //   bench_lespgemm <A.csr> <B.csr> <threads> <iterations> <k> <bands> <seed> <cluster_size> <kernel>
 //   bench_lespgemm_panels <A.csr> <B.csr> <threads> <iterations> <k> <bands> <seed> <cluster_size> <kernel>
 //                         <c0:c1,c0:c1,...>   LeSpGEMM_VLength on cluster ranges of the clustered A
+//   features <A.mtx>      the reference's feature extractor (MTX<int,double>: MtxLoad,
+//                         CalculateFeatures, FeaturesPrint) on a Matrix Market file
 #include "SpOps.h"
 #include "spgemm.h"
 #include "spgemm_MinHashLSH.h"
 #include "spgemm_utility.h"
 #include "sparse_conversion.h"
+#include "sparse_features.h"
 
 #include <algorithm>
 #include <cmath>
@@ -476,6 +479,14 @@ int main(int argc, char **argv) {
     Le_set_thread_num(std::atoi(argv[4]));
     return bench_lespgemm(A, B, std::atoi(argv[5]), std::atoi(argv[6]), std::atoi(argv[7]),
                           std::strtoull(argv[8], nullptr, 10), std::atoll(argv[9]), std::atoi(argv[10]));
+  }
+  if (mode == "features" && argc >= 3) {
+    MTX<int, double> m(0);
+    if (!m.MtxLoad(argv[2])) return 2;
+    m.CalculateFeatures();
+    m.FeaturesPrint();
+    m.ExtraFeaturesPrint();
+    return 0;
   }
   if (mode == "bench_lespgemm_panels" && argc >= 12) {
     HostCsr A = read_csr_file(argv[2]);

@@ -59,6 +59,7 @@ _SIGNATURES = [
     ("spops_sddmm", c_int, [c_vp, c_i64, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_vp]),
     ("spops_csr_transpose", c_int, [c_vp, c_vp]),
     ("spops_csr_set_values", c_int, [c_vp, c_vp, c_int, c_vp]),
+    ("spops_matrix_features", c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_i64, c_i64, c_vp, c_vp]),
     ("spops_batch_create", c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_int]),
     ("spops_batch_info", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("spops_batch_plan", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),

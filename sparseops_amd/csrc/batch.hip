@@ -24,10 +24,10 @@
 // counter, so the launch is graph-safe).  Queues list matrices by decreasing cost, so the
 // last tasks are the small ones.
 //
-// The autotuner (batch_autotune): features per matrix (k, mean row length, coefficient of
-// variation of row lengths) place it in a class; for each class present the candidate plans
-// are timed on the class's largest matrix, once, at creation, and the winner is used for the
-// class.
+// The autotuner (batch_autotune): the reference's matrix features (average row length, row
+// Gini coefficient, share of entries near the diagonal; src/sparse_features.cpp:609-760)
+// and k place each matrix in a class; for each class present the candidate plans are timed
+// on the class's largest matrix, once, at creation, and the winner is used for the class.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -451,24 +451,18 @@ void launch_batch(const BatchDev &B, const float *X, float *Y, float alpha, floa
   SPOPS_HIP_CHECK(hipGetLastError());
 }
 
-// matrix class: k bucket (1, 2, 3-4, 5-8, 9-16, 17-32, 33-64, > 64) x row-length regime (mean
-// row length <= 16 / > 16) x irregularity (coefficient of variation of row lengths <= 0.25 /
-// > 0.25)
+// matrix class from the reference's features (row_features_host): k bucket (1, 2, 3-4, 5-8,
+// 9-16, 17-32, 33-64, > 64) x ave_nnz_each_row > 16 x Gini_coeff_row > 0.2 (irregular row
+// lengths) x nnz_closed2diag_ratio > 0.5 (entries near the diagonal: X gathers stay local)
 int matrix_class(const HostMat &M) {
   const int kb = M.k <= 1 ? 0 : M.k <= 2 ? 1 : M.k <= 4 ? 2 : M.k <= 8 ? 3 : M.k <= 16 ? 4 : M.k <= 32 ? 5
                : M.k <= 64 ? 6 : 7;
-  double mean = 0, sq = 0;
-  for (int64_t r = 0; r < M.n_rows; ++r) {
-    const double l = (double)(M.rp[r + 1] - M.rp[r]);
-    mean += l;
-    sq += l * l;
-  }
-  mean /= std::max<int64_t>(M.n_rows, 1);
-  const double var = sq / std::max<int64_t>(M.n_rows, 1) - mean * mean;
-  const double cv = mean > 0 ? std::sqrt(std::max(var, 0.0)) / mean : 0.0;
-  return kb * 4 + (mean > 16 ? 2 : 0) + (cv > 0.25 ? 1 : 0);
+  if (M.n_rows == 0 || M.n_cols == 0) return kb * 8;
+  double f[3];
+  row_features_host(M.rp.data(), M.ci.data(), M.n_rows, M.n_cols, f);
+  return kb * 8 + (f[0] > 16 ? 4 : 0) + (f[1] > 0.2 ? 2 : 0) + (f[2] > 0.5 ? 1 : 0);
 }
-constexpr int kClasses = 32;
+constexpr int kClasses = 64;
 
 double time_plan(const HostMat &M, const MatPlan &P) {
   std::vector<HostMat> one{M};

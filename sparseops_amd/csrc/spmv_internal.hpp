@@ -80,6 +80,9 @@ struct spops_matrix_s {
 
 namespace spops {
 extern int64_t g_slice_bytes;
+// The reference's ave_nnz_each_row, Gini_coeff_row and nnz_closed2diag_ratio of a host CSR
+// (src/sparse_features.cpp:609-760): the features the batched SpMM autotuner classifies by.
+void row_features_host(const int64_t *rowptr, const int64_t *col, int64_t n_rows, int64_t n_cols, double out[3]);
 extern int64_t g_dense_products;   // LeSpGEMM: products of a cluster from which it accumulates densely   // x bytes per slice for matrices created from now on
 // Builds and uploads the CSR plan of a matrix: SpMV row tasks and long rows (> kCsrLongRow
 // entries); for SpMM the short-row order and the segments of the rows > kSpmmLongRow.

@@ -132,6 +132,21 @@ def rmat(scale: int = 20, edge_factor: int = 16, a=0.57, b=0.19, c=0.19, d=0.05,
     return _finish(n, n, rows, cols, vals, index_dtype)
 
 
+def random_rect(n_rows: int, n_cols: int, per_row_max: int, seed: int = 0, index_dtype=np.int64) -> Csr:
+    """Rectangular matrix: U{0..per_row_max} distinct uniform columns per row, values
+    N(0,1) times 10^U(-3,3) (magnitudes on both sides of 1), every tenth row with an
+    explicit zero."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, per_row_max + 1, n_rows)
+    lens = np.minimum(lens, n_cols)
+    rows, cols = _unique_cols_per_row(rng, n_rows, n_cols, lens)
+    vals = rng.standard_normal(rows.size) * 10.0 ** rng.uniform(-3, 3, rows.size)
+    first = np.r_[0, np.cumsum(lens)[:-1]]
+    z = first[(np.arange(n_rows) % 10 == 0) & (lens > 0)]
+    vals[z] = 0.0
+    return _finish(n_rows, n_cols, rows, cols, vals, index_dtype)
+
+
 def transpose(A: Csr) -> Csr:
     """Exact transpose (CSC of A read as CSR); value multiset is unchanged."""
     rows = np.repeat(np.arange(A.n_rows, dtype=np.int64), A.row_lengths())

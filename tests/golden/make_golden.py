@@ -20,8 +20,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
-from sparseops_amd.gen import laplacian_2d, powerlaw, rmat, transpose, uniform_x  # noqa: E402
-from sparseops_amd.io import write_csr_bin  # noqa: E402
+from sparseops_amd.features import NAMES as FEATURE_NAMES  # noqa: E402
+from sparseops_amd.gen import laplacian_2d, powerlaw, random_rect, rmat, transpose, uniform_x  # noqa: E402
+from sparseops_amd.io import write_csr_bin, write_mtx  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 OUT = os.path.dirname(os.path.abspath(__file__))
@@ -105,6 +106,43 @@ def lespgemm_case(name, A, B, k=64, bands=16, seed=7, hc_v=0, cs=8, prec=64, ful
     print("wrote", name, "pairs", out["npairs"], "clusters", len(out["offsets"]) - 1, "nnz(C)", out["C_rowptr"][-1])
 
 
+# labels MTX::FeaturesPrint / ExtraFeaturesPrint use (include/sparse_features.h:44-150) for
+# the library's feature names, and the factor they are printed with (percentages x100)
+PERCENT = {"nnz_ratio", "nnz_closed2diag_ratio", "NoneZero_Row_Ratio", "NoneZero_Col_Ratio",
+           "diagonal_dominant_ratio"}
+PRINTED = {"real_nnz": "real_nnz", "uniq_R": "uniq_R", "uniq_C": "uniq_C"}
+
+
+def features_case(name, A):
+    """The reference's feature extractor on A written as a general Matrix Market file; the
+    printed values (6 significant digits) as a vector in the library's order (NaN: not
+    printed)."""
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "A.mtx")
+        write_mtx(path, A)
+        text = subprocess.check_output([HARNESS, "features", path], text=True)
+    got = {}
+    for line in text.splitlines():
+        for sep in (":", "="):
+            if sep in line:
+                k, v = line.split(sep, 1)
+                k, v = k.strip(), v.strip()
+                if k == "Tile inner size":
+                    a, b = v.split("*")
+                    got["tile_rows"], got["tile_cols"] = float(a), float(b)
+                else:
+                    try:
+                        got[k] = float(v.rstrip("%"))
+                    except ValueError:
+                        pass
+                break
+    vec = np.full(len(FEATURE_NAMES), np.nan)
+    for i, n in enumerate(FEATURE_NAMES):
+        if n in got:
+            vec[i] = got[n] / 100.0 if n in PERCENT else got[n]
+    np.savez(os.path.join(OUT, f"{name}.npz"), features=vec, input_digest=digest(A.rowptr, A.col, A.val))
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: run make -C oracle ref (needs the reference sources)")
@@ -121,6 +159,11 @@ def main():
     lespgemm_case("lsh_rmat10_f32", R10, R10, prec=32, full_pairs=False)
     lespgemm_case("lsh_rmat10_k128", R10, R10, k=128, bands=32, seed=42, full_pairs=False)
     lespgemm_case("lsh_rmat12", R12, R12, full_pairs=False)
+    features_case("features_pl3000", powerlaw(3000, lmax=300, seed=2))
+    features_case("features_lap40", laplacian_2d(40))
+    features_case("features_rmat10", rmat(10, 16, seed=10))
+    features_case("features_rect700x1900", random_rect(700, 1900, 20, seed=1))
+    features_case("features_rect100x60", random_rect(100, 60, 12, seed=4))
 
 
 if __name__ == "__main__":

@@ -255,3 +255,10 @@ def transpose(A):
     lib.orc_csr_transpose(i64(A.n_rows), i64(A.n_cols), P(_i64(A.rowptr)), P(_i64(A.col)), P(_f64(A.val)), P(trp),
                           P(tc), P(tv), P(tp))
     return trp, tc, tv, tp
+
+
+def features(A):
+    """The reference's matrix features (61, spops_matrix_features' order) of a general CSR."""
+    out = np.zeros(61)
+    lib.orc_features(i64(A.n_rows), i64(A.n_cols), P(_i64(A.rowptr)), P(_i64(A.col)), P(_f64(A.val)), P(out))
+    return out

@@ -15,6 +15,7 @@ import pytest
 import scipy.sparse as sp
 
 import oracle_bind as orc
+from sparseops_amd import gen
 from sparseops_amd.gen import Csr, laplacian_2d, powerlaw, rmat, transpose, uniform_x
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -311,3 +312,64 @@ def test_spmm_sddmm_transpose_dense():
     T = sp.csr_matrix((tv, tc, trp), shape=(A.n_cols, A.n_rows)).toarray()
     np.testing.assert_array_equal(T, D.T)
     np.testing.assert_array_equal(tv, A.val[tp])
+
+
+FEATURE_CASES = {
+    "features_pl3000": lambda: gen.powerlaw(3000, lmax=300, seed=2),
+    "features_lap40": lambda: gen.laplacian_2d(40),
+    "features_rmat10": lambda: gen.rmat(10, 16, seed=10),
+    "features_rect700x1900": lambda: gen.random_rect(700, 1900, 20, seed=1),
+    "features_rect100x60": lambda: gen.random_rect(100, 60, 12, seed=4),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FEATURE_CASES))
+def test_features_match_reference(name):
+    """orc_features against the reference's own MTX feature extractor (printed with 6
+    significant digits): square and rectangular, fewer rows than tiles, explicit zeros,
+    magnitudes on both sides of 1."""
+    A = FEATURE_CASES[name]()
+    g = np.load(os.path.join(GOLD, f"{name}.npz"))
+    h = hashlib.sha256()
+    for a in (A.rowptr, A.col, A.val):
+        h.update(np.ascontiguousarray(a).tobytes())
+    assert h.hexdigest() == str(g["input_digest"])
+    want = g["features"]
+    got = orc.features(A)
+    printed = ~np.isnan(want)
+    assert printed.sum() == 61
+    np.testing.assert_allclose(got[printed], want[printed], rtol=6e-6, atol=6e-7)
+
+
+def test_features_closed_forms():
+    """Pins the formulas that the printed values round: an n x n diagonal matrix has one
+    diagonal, every row and column one entry (variance 0, Gini 0, P-ratio 1/2), full
+    dominance, and tiles whose counts are known exactly; a matrix whose row i holds i + 1
+    entries has the Gini of the discrete triangle."""
+    n = 1024
+    A = gen.Csr(n, n, np.arange(n + 1), np.arange(n), np.full(n, 2.0))
+    f = orc.features(A)
+    assert f[4] == 1 and f[5] == 1.0 and f[8] == n and f[6] == f[7] == 0
+    for base in (9, 17):
+        assert f[base] == 1.0 and f[base + 1] == f[base + 2] == 1 and f[base + 3] == 1.0 and f[base + 4] == 0.0
+        assert f[base + 6] == 0.5 and f[base + 7] == 0.0     # equal counts: area under the line = n^2 / 2
+    assert f[25] == 1.0 and f[27] == 2.0 and f[26] == -np.finfo(np.float64).max
+    assert f[28] == -1.0 and f[29] == -1.0                                      # no |v| < 1
+    assert f[31] == f[32] == 4                                                  # 1024 / 256 rows per tile
+    # 256 diagonal tiles of 4 entries: the tile variance starts from -1.0 as in the reference
+    ave = n / 65536
+    var = (-1.0 + 256 * (4 - ave) ** 2 + (65536 - 256) * ave ** 2) / 65536
+    np.testing.assert_allclose(f[37], var, rtol=1e-12)
+    assert f[33] == (256 - 1) / 65536 and f[41] == (256 - 1) / 256
+    assert f[57] == 1.0 and f[58] == 1.0 and f[59] == 1.0 and f[60] == 1.0
+    # row i holds columns 0..i: sorted counts 1..m, cumulative c_k = k (k + 1) / 2; the
+    # trapezoid gives sum_k c_k - c_m / 2, the total area nnz * m / 2
+    m = 300
+    rp = np.r_[0, np.cumsum(np.arange(1, m + 1))]
+    cols = np.concatenate([np.arange(i + 1) for i in range(m)])
+    f = orc.features(gen.Csr(m, m, rp, cols, np.ones(cols.size)))
+    c = np.arange(1, m + 1) * np.arange(2, m + 2) // 2
+    area = c.sum() - c[-1] / 2
+    total = c[-1] * m / 2
+    np.testing.assert_allclose(f[16], (total - area) / total, rtol=1e-14)
+    assert f[10] == 1 and f[11] == m and f[4] == m
