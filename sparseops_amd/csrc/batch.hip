@@ -17,12 +17,12 @@
 //                  accesses for VEC = 4);
 //   kFmtCsrGroup : CSR, a lane group of GS lanes per row (GS = 4..64, a power of two >= k),
 //                  lanes over the k columns, eight row entries in flight (any k).
-// Schedule: each matrix's chunks (rows) are cut into 8 pieces of equal entry count, piece q
-// going to the task queue of XCD q, so every XCD's L2 holds one contiguous eighth of each
-// matrix's X band; task t of the launch is entry t / 8 of queue t % 8, and the persistent
-// workgroup g (on XCD g % 8, the grid a multiple of 8) takes tasks g, g + grid, ... (no
-// counter, so the launch is graph-safe).  Queues list matrices by decreasing cost, so the
-// last tasks are the small ones.
+// Schedule: whole matrices go to the task queues of the 8 XCDs, largest first to the least
+// loaded queue, so that a matrix's X (band and scattered entries alike) is gathered through
+// one L2; task t of the launch is entry t / 8 of queue t % 8, and the persistent workgroup g
+// (on XCD g % 8, the grid a multiple of 8) takes tasks g, g + grid, ... (no counter, so the
+// launch is graph-safe).  Queues list matrices by decreasing cost, so the last tasks are the
+// small ones.
 //
 // The autotuner (batch_autotune): the reference's matrix features (average row length, row
 // Gini coefficient, share of entries near the diagonal; src/sparse_features.cpp:609-760)
@@ -266,21 +266,6 @@ MatPlan default_plan(int64_t k) {
 
 int64_t plan_ld(const MatPlan &p, int64_t k) { return p.fmt == kFmtChunk ? (k + p.vec - 1) / p.vec * p.vec : k; }
 
-// Cuts units 0..n-1 with the given costs into `parts` contiguous pieces of about equal cost.
-std::vector<int64_t> equal_cuts(const std::vector<int64_t> &cost, int parts) {
-  const int64_t n = (int64_t)cost.size();
-  int64_t tot = 0;
-  for (int64_t c : cost) tot += c;
-  std::vector<int64_t> cut(parts + 1, n);
-  cut[0] = 0;
-  int64_t acc = 0, i = 0;
-  for (int p = 1; p < parts; ++p) {
-    while (i < n && (acc + cost[i]) * parts <= tot * p) acc += cost[i++];
-    cut[p] = i;
-  }
-  return cut;
-}
-
 // Device arrays for a set of matrices with the given plans.
 void build_batch(const std::vector<HostMat> &ms, const std::vector<MatPlan> &plans, BatchDev &B) {
   const size_t nm = ms.size();
@@ -289,7 +274,7 @@ void build_batch(const std::vector<HostMat> &ms, const std::vector<MatPlan> &pla
   std::vector<float> val;
   std::vector<int2> ent;
   std::vector<int64_t> cbase;
-  std::vector<std::vector<std::vector<int4>>> mq(nm, std::vector<std::vector<int4>>(kXcds));  // per matrix, per XCD
+  std::vector<std::vector<int4>> mq(nm);        // per matrix: its tasks in row order
   std::vector<int64_t> mcost(nm, 0);
   int64_t x = 0, y = 0, flop = 0, algo = 0;
   for (size_t i = 0; i < nm; ++i) {
@@ -321,17 +306,13 @@ void build_batch(const std::vector<HostMat> &ms, const std::vector<MatPlan> &pla
       b.e = (int64_t)col.size();
       for (int64_t r = 0; r <= M.n_rows; ++r) rowptr.push_back((int32_t)M.rp[r]);
       for (int64_t j = 0; j < nnz; ++j) { col.push_back((int32_t)M.ci[j]); val.push_back(M.v[j]); }
-      std::vector<int64_t> rc(M.n_rows);
-      for (int64_t r = 0; r < M.n_rows; ++r) rc[r] = M.rp[r + 1] - M.rp[r] + 1;
-      const std::vector<int64_t> cut = equal_cuts(rc, kXcds);
-      for (int q = 0; q < kXcds; ++q) {
-        int64_t r = cut[q];
-        while (r < cut[q + 1]) {               // runs of rows with about kBatchTaskNnz entries
-          int64_t r1 = r, acc = 0;
-          while (r1 < cut[q + 1] && (acc == 0 || acc + rc[r1] <= kBatchTaskNnz) && r1 - r < 4096) acc += rc[r1++];
-          mq[i][q].push_back(make_int4((int)i, (int)r, (int)(r1 - r), 0));
-          r = r1;
-        }
+      int64_t r = 0;
+      while (r < M.n_rows) {                   // runs of rows with about kBatchTaskNnz entries
+        int64_t r1 = r, acc = 0;
+        while (r1 < M.n_rows && (acc == 0 || acc + (M.rp[r1 + 1] - M.rp[r1] + 1) <= kBatchTaskNnz) && r1 - r < 4096)
+          acc += M.rp[r1 + 1] - M.rp[r1] + 1, ++r1;
+        mq[i].push_back(make_int4((int)i, (int)r, (int)(r1 - r), 0));
+        r = r1;
       }
     } else {
       const int R = kWave / P.lanes;
@@ -391,27 +372,29 @@ void build_batch(const std::vector<HostMat> &ms, const std::vector<MatPlan> &pla
           }
         }
       }
-      const std::vector<int64_t> cut = equal_cuts(ccost, kXcds);
-      for (int q = 0; q < kXcds; ++q) {
-        int64_t c = cut[q];
-        while (c < cut[q + 1]) {
-          int64_t c1 = c, acc = 0;
-          while (c1 < cut[q + 1] && (acc == 0 || acc + ccost[c1] <= kBatchTaskEntries)) acc += ccost[c1++];
-          mq[i][q].push_back(make_int4((int)i, (int)c, (int)(c1 - c), 1));
-          c = c1;
-        }
+      int64_t c = 0;
+      while (c < nch) {
+        int64_t c1 = c, acc = 0;
+        while (c1 < nch && (acc == 0 || acc + ccost[c1] <= kBatchTaskEntries)) acc += ccost[c1++];
+        mq[i].push_back(make_int4((int)i, (int)c, (int)(c1 - c), 1));
+        c = c1;
       }
     }
   }
   SPOPS_REQUIRE(col.size() < (size_t)INT32_MAX && ent.size() < ((size_t)1 << 40), SPOPS_ERROR_LIMIT,
                 "a batch holds < 2^31 CSR entries");
-  // queues: matrices by decreasing cost, then interleaved t = 8 i + q
+  // queues: whole matrices to XCDs, largest first to the least loaded queue (so a matrix's X
+  // is gathered through one L2), then interleaved t = 8 i + q
   std::vector<size_t> mo(nm);
   std::iota(mo.begin(), mo.end(), 0);
   std::stable_sort(mo.begin(), mo.end(), [&](size_t a, size_t c) { return mcost[a] > mcost[c]; });
   std::vector<std::vector<int4>> queue(kXcds);
-  for (size_t i : mo)
-    for (int q = 0; q < kXcds; ++q) queue[q].insert(queue[q].end(), mq[i][q].begin(), mq[i][q].end());
+  std::vector<int64_t> load(kXcds, 0);
+  for (size_t i : mo) {
+    const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    queue[q].insert(queue[q].end(), mq[i].begin(), mq[i].end());
+    load[q] += mcost[i];
+  }
   size_t qlen = 0;
   for (auto &qq : queue) qlen = std::max(qlen, qq.size());
   std::vector<int4> ht(qlen * kXcds, make_int4(0, 0, 0, 0));
