@@ -14,7 +14,9 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <iostream>
+#include <string>
 #include <new>
 #include <numeric>
 #include <stdexcept>
@@ -308,6 +310,38 @@ template <typename I, typename V>
 CSR_Matrix<I, V> csr_reorder_rows(const CSR_Matrix<I, V> &csr, const std::vector<I> &row_permutation) {
   if ((I)row_permutation.size() < csr.num_rows) throw std::invalid_argument("permutation shorter than num_rows");
   return csr_reorder_rows(csr, row_permutation.data());
+}
+
+/* A row permutation file (include/sparse_conversion.h:2012-2052): one 0-based row per line,
+ * or a Matrix Market file whose header and size line are skipped.  With num_rows > 0 the
+ * list is padded with (size, size+1, ...) or cut to num_rows, as the reference does. */
+template <typename I>
+std::vector<I> read_row_permutation(const char *filename, I num_rows = 0) {
+  std::vector<I> rows;
+  std::ifstream file(filename);
+  if (!file.is_open()) {
+    std::cerr << "Error: Cannot open permutation file: " << filename << std::endl;
+    throw std::runtime_error("Cannot open permutation file");
+  }
+  std::string first;
+  std::getline(file, first);
+  I v;
+  if (first.find("%%MatrixMarket") != std::string::npos) {
+    std::string line;
+    while (std::getline(file, line))
+      if (line.empty() || line[0] != '%') break;
+  } else {
+    file.clear();
+    file.seekg(0, std::ios::beg);
+  }
+  while (file >> v) rows.push_back(v);
+  if (num_rows > 0) {
+    if ((I)rows.size() < num_rows)
+      for (I i = (I)rows.size(); i < num_rows; ++i) rows.push_back(i);
+    else if ((I)rows.size() > num_rows)
+      rows.resize(num_rows);
+  }
+  return rows;
 }
 
 /* Cluster format of an already reordered CSR (offsets delimit the clusters).  Built on the

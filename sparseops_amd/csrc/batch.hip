@@ -652,7 +652,14 @@ extern "C" int spops_batch_capture(spops_batch B, float alpha, const float *d_X,
                   "X and Y must be 16-byte aligned");
     const hipStream_t s = static_cast<hipStream_t>(stream);
     SPOPS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    launch_batch(D, d_X, d_Y, alpha, beta, s);
+    try {
+      launch_batch(D, d_X, d_Y, alpha, beta, s);
+    } catch (...) {                 // leave the stream out of capture mode
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
     SPOPS_HIP_CHECK(hipStreamEndCapture(s, &D.graph));
     SPOPS_HIP_CHECK(hipGraphInstantiate(&D.exec, D.graph, nullptr, nullptr, 0));
     return SPOPS_SUCCESS;

@@ -14,6 +14,8 @@ int64 col[nnz]; float64 val[nnz]; little-endian.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 from .gen import Csr
@@ -102,3 +104,35 @@ def sum_duplicates(A: Csr) -> Csr:
     rowptr = np.zeros(A.n_rows + 1, dtype=np.int64)
     np.cumsum(counts, out=rowptr[1:])
     return Csr(A.n_rows, A.n_cols, rowptr.astype(A.rowptr.dtype), c.astype(A.col.dtype), vals)
+
+
+def write_clustering(out_dir: str, name: str, perm, offsets) -> tuple[str, str]:
+    """A row clustering as record_permutation_lsh writes it (test/record_permutation_lsh.cpp:
+    160-181): <out_dir>/<name>.perm and <name>.offsets, one integer per line."""
+    os.makedirs(out_dir, exist_ok=True)
+    pp, po = os.path.join(out_dir, f"{name}.perm"), os.path.join(out_dir, f"{name}.offsets")
+    for path, arr in ((pp, perm), (po, offsets)):
+        with open(path, "w") as f:
+            f.write("".join(f"{int(v)}\n" for v in np.asarray(arr, dtype=np.int64)))
+    return pp, po
+
+
+def read_clustering(lsh_dir: str, name: str, n_rows: int):
+    """(perm, offsets) of <lsh_dir>/<name>.perm / .offsets; the permutation padded with
+    (size, size+1, ...) or cut to n_rows, as include/sparse_conversion.h:2012-2052 does (a
+    Matrix Market header and size line, if present, are skipped)."""
+    with open(os.path.join(lsh_dir, f"{name}.perm")) as f:
+        lines = f.read().splitlines()
+    if lines and "%%MatrixMarket" in lines[0]:
+        i = 1
+        while i < len(lines) and lines[i].startswith("%"):
+            i += 1
+        lines = lines[i + 1:]
+    perm = np.array([int(t) for ln in lines for t in ln.split()], dtype=np.int64)
+    if n_rows > 0:
+        if perm.size < n_rows:
+            perm = np.concatenate([perm, np.arange(perm.size, n_rows, dtype=np.int64)])
+        perm = perm[:n_rows]
+    with open(os.path.join(lsh_dir, f"{name}.offsets")) as f:
+        offsets = np.array([int(t) for t in f.read().split()], dtype=np.int64)
+    return perm, offsets

@@ -88,3 +88,19 @@ def test_library_hc_matches_oracle_on_other_inputs():
             got = hierarchical_clustering(A, pi, pj, ps, cs, version)
             np.testing.assert_array_equal(got[0], want[0])
             np.testing.assert_array_equal(got[1], want[1])
+
+
+def test_clustering_files_round_trip(tmp_path):
+    """.perm / .offsets as record_permutation_lsh writes them; the reader pads a short
+    permutation with (size, size+1, ...), cuts a long one, and skips a Matrix Market header."""
+    from sparseops_amd.io import read_clustering, write_clustering
+    perm = np.array([3, 0, 2, 1])
+    pp, po = write_clustering(str(tmp_path), "m", perm, [0, 2, 4])
+    assert open(pp).read() == "3\n0\n2\n1\n" and open(po).read() == "0\n2\n4\n"
+    p, o = read_clustering(str(tmp_path), "m", 4)
+    assert p.tolist() == [3, 0, 2, 1] and o.tolist() == [0, 2, 4]
+    assert read_clustering(str(tmp_path), "m", 6)[0].tolist() == [3, 0, 2, 1, 4, 5]
+    assert read_clustering(str(tmp_path), "m", 2)[0].tolist() == [3, 0]
+    with open(pp, "w") as f:
+        f.write("%%MatrixMarket matrix array integer general\n% c\n4 1\n1\n0\n3\n2\n")
+    assert read_clustering(str(tmp_path), "m", 4)[0].tolist() == [1, 0, 3, 2]
