@@ -14,7 +14,7 @@ R-MAT, MinHash-LSH clustering, GFLOP/s) is reported with the reference's OpenMP 
 timed on the same host; so are the differentiable SpMM layer (R-MAT scale 20, BF16 X / G,
 forward and backward) and the batched SpMM (512 band matrices, one HIP-graph replay per step).
 
-Prints ONE JSON line (rank 0, N = 1).  --gpus N > 1 exits: this build runs on one GPU.
+Prints ONE JSON line (rank 0, N = 1).  --gpus N > 1 exits: the benchmark runs on one GPU.
 """
 from __future__ import annotations
 
@@ -456,8 +456,8 @@ def main():
                     help="only run a device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md, not run.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the sharded multi-GPU form is a design, not run here.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

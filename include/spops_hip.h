@@ -37,7 +37,7 @@ extern "C" {
 #define SPOPS_ERROR_INVALID_VALUE   1   /* bad argument (shape, type code, null pointer)  */
 #define SPOPS_ERROR_OUT_OF_MEMORY   2   /* device or host allocation failed               */
 #define SPOPS_ERROR_HIP             3   /* a HIP runtime call failed                        */
-#define SPOPS_ERROR_NOT_SUPPORTED   4   /* valid request this build does not implement      */
+#define SPOPS_ERROR_NOT_SUPPORTED   4   /* valid request the library does not implement    */
 #define SPOPS_ERROR_LIMIT           5   /* exceeds a documented size limit                  */
 
 #define SPOPS_INDEX_I32  0

@@ -53,7 +53,7 @@ def test_saved_clustering_round_trip_and_errors(tmp_path):
     write_mtx(mtx, A)
     p = _run("record_permutation_lsh", mtx, f"--out-dir={tmp_path}/none", "--save=0")
     assert p.returncode == 0 and "Save disabled" in p.stdout and not os.path.exists(f"{tmp_path}/none/r8.perm")
-    # a clustering written from Python is what the driver reads (identity permutation, singletons)
+    # a clustering written from Python is what test_spgemm_lsh reads (identity permutation, singletons)
     write_clustering(str(tmp_path / "py"), "r8", np.arange(A.n_rows), np.arange(A.n_rows + 1))
     p = _run("test_spgemm_lsh", mtx, mtx, f"--lsh-order-dir={tmp_path}/py", "--iterations=1")
     assert p.returncode == 0, p.stderr
