@@ -10,9 +10,9 @@ constexpr int kSmallTable = 2048;                // LDS hash slots (load factor 
 constexpr int kSmallAcc = 4096;                  // LDS accumulator entries (nz * csz)
 constexpr int kLargeTileBits = 1 << 18;         // columns per LDS bitmap pass (32 KiB)
 constexpr int kLargeBlock = 512;                 // threads of a large-cluster numeric workgroup
-constexpr int kLargeAccEntries = 4608;           // LDS accumulator entries of a numeric chunk
-constexpr int kLargeHash = 2048;                 // LDS hash slots of a chunk's columns
-constexpr int kLargeCursors = 2048;              // A columns whose B-row position is kept in LDS
+constexpr int kLargeAccEntries = 2304;           // LDS accumulator entries of a numeric chunk
+constexpr int kLargeHash = 1024;                 // LDS hash slots of a chunk's columns
+constexpr int kLargeCursors = 1024;              // A columns whose B-row position is kept in LDS
 constexpr int kLargeGroupLanes = 8;              // lanes scanning one B row in the numeric pass
 constexpr int kMaxClusterRows = 16;              // A values of a column held in registers per pass
 constexpr int kSpgemmStats = 16;
