@@ -6,10 +6,12 @@ A step is one y = A*x of that matrix with A, x, y resident in HBM.  The headline
 FP32-value SELL-C-sigma (C = 32, sigma = 1024) SpMV, in achieved GB/s of algorithmic bytes
 ((n+1)*4 + nnz*(4+4) + n*4 (x) + n*4 (y) + n*4 (reorder) + chunks*4); the FP64 CSR drop-in,
 FP64 SELL, BF16 SELL and FP32 CSR run on the same matrix and are reported beside it with
-their error against an FP64 product (scipy, float64).  The roofline object covers one SpMV
-of the headline variant (its two launches: the column-slice kernel and the slice-sum pass),
-timed with HIP events on the launch stream; its HBM traffic comes from the committed
-rocprofv3 counter summary of the same command.  A secondary LeSpGEMM measurement (C = A*A,
+their error against an FP64 product (scipy, float64).  The SELL handles of this matrix use
+the swept device layout (row blocks summed in FP64 in LDS, each block's entries sorted by
+column).  The roofline object covers one SpMV of the headline variant (its two launches:
+the swept kernel and the pass that adds the two partial planes), timed with HIP events on
+the launch stream; its HBM traffic comes from the committed rocprofv3 counter summary of
+the same command.  A secondary LeSpGEMM measurement (C = A*A,
 R-MAT, MinHash-LSH clustering, GFLOP/s) is reported with the reference's OpenMP pipeline
 timed on the same host; so are the differentiable SpMM layer (R-MAT scale 20, BF16 X / G,
 forward and backward) and the batched SpMM (512 band matrices, one HIP-graph replay per step).
@@ -497,9 +499,10 @@ def main():
         yh = y.double().cpu().numpy()
         r = {"wall_ms": wall_ms, "kernel_ms": kern_ms, "algo_bytes": M.algo_bytes,
              "gbs": M.algo_bytes / (wall_ms * 1e-3) / 1e9, "kernel_gbs": M.algo_bytes / (kern_ms * 1e-3) / 1e9,
-             "device_bytes": M.device_bytes, "build_s": time.time() - t,
-             "kernels": (["k_slice_spmv", "k_slice_reduce"] if M.format in ("sell", "ell") and M.slices > 1
-                         else ["k_slice_spmv"] if M.format in ("sell", "ell") else ["k_csr_spmv"])}
+             "device_bytes": M.device_bytes, "build_s": time.time() - t, "sweep": M.sweep,
+             "kernels": (["k_csr_spmv"] if M.format not in ("sell", "ell")
+                         else (["k_sweep_spmv", "k_sweep_reduce"] if M.sweep[1] > 1 else ["k_sweep_spmv"])
+                         if M.sweep else (["k_slice_spmv", "k_slice_reduce"] if M.slices > 1 else ["k_slice_spmv"]))}
         r.update(errors(yh, y64, absrow))
         res[name] = r
         log(f"{name}: {r['kernel_ms'] * 1e3:.1f} us/kernel, {r['kernel_gbs']:.0f} GB/s, "
@@ -572,9 +575,12 @@ def main():
         "rel_err_vs_fp64": {k: {e: v[e] for e in ("max_rel_err", "rms_rel_err", "normwise_rel_err")}
                             for k, v in res.items()},
         "variants": {k: {"gbs": round(v["gbs"], 1), "kernel_us": round(v["kernel_ms"] * 1e3, 2),
-                         "pct_hbm_peak": round(100 * v["kernel_gbs"] / HBM_PEAK_GBS, 2)} for k, v in res.items()},
+                         "pct_hbm_peak": round(100 * v["kernel_gbs"] / HBM_PEAK_GBS, 2), "kernels": v["kernels"],
+                         "sweep": v["sweep"]} for k, v in res.items()},
         "fp32_sell_over_fp64_csr": (round(res["csr_f64"]["kernel_ms"] / res["sell_f32"]["kernel_ms"], 3)
                                     if "csr_f64" in res and "sell_f32" in res else None),
+        "fp32_sell_over_fp64_sell": (round(res["sell_f64"]["kernel_ms"] / res["sell_f32"]["kernel_ms"], 3)
+                                     if "sell_f64" in res and "sell_f32" in res else None),
         "lespgemm": lg,
         "lespgemm_panels": lp,
         "autograd_spmm": ag,

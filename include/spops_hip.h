@@ -200,7 +200,8 @@ int spops_csr_set_values(spops_matrix A, const void *d_values, int values_type, 
 /* info[0]=format info[1]=n_rows info[2]=n_cols info[3]=nnz info[4]=value_type
  * info[5]=device bytes owned info[6]=algorithmic bytes of one SpMV (no padding)
  * info[7]=stored slots (incl. padding) info[8]=long rows info[9]=wave tasks
- * info[10]=C info[11]=sigma info[12]=bR (BSR) or column slices (SELL/ELL) info[13]=bC
+ * info[10]=C info[11]=sigma info[12]=bR (BSR) or column slices (SELL/ELL; 0 for the swept
+ * layout, whose info[9] counts entry groups) info[13]=bC
  * (BSR) or max row width (ELL) info[14]=nchunks (reference layout) info[15]=index_type */
 int spops_matrix_info(spops_matrix A, int64_t info[16]);
 
@@ -211,14 +212,30 @@ int spops_matrix_info(spops_matrix A, int64_t info[16]);
  *   smallest power of two <= 8 with n_cols * sizeof(x) / slices <= this.
  *   SPOPS_TUNE_DENSE_PRODUCTS: LeSpGEMM clusters with at least this many products (default
  *   2^20) accumulate in a dense block in device memory (many workgroups per cluster) instead
- *   of one workgroup's LDS chunks; applies to plans created afterwards. */
+ *   of one workgroup's LDS chunks; applies to plans created afterwards.
+ *   SPOPS_TUNE_SPMV_LAYOUT: device layout of SELL-C-sigma / SELL-C-R / ELL matrices created
+ *   afterwards: 0 (default) chosen per matrix, 1 the column-sliced stream, 2 the swept layout
+ *   (blocks of consecutive rows summed in FP64 in LDS, each block's entries sorted by column).
+ *   The default picks the swept layout when one XCD's rows gather from more than the slice
+ *   bytes of x and 16384 rows hold at least one entry per 128-byte line of x.
+ *   SPOPS_TUNE_SWEEP_WORKGROUPS: workgroups one swept-layout SpMV aims for (default 256, one
+ *   per CU).  With B the fewest row blocks the row limit allows, a block is shared by
+ *   H = this / B workgroups (1..16), and rows are cut into about this / H blocks of equal
+ *   entry counts; the H partial sums of a row are added in a fixed order.
+ *   SPOPS_TUNE_SWEEP_MAX_ROWS: rows of the largest row block of the swept layout, 1..20480
+ *   (default 20480: 160 KiB of FP64 sums in LDS).  Both apply to matrices created afterwards. */
 #define SPOPS_TUNE_SLICE_BYTES 0
 #define SPOPS_TUNE_DENSE_PRODUCTS 1
+#define SPOPS_TUNE_SPMV_LAYOUT 2
+#define SPOPS_TUNE_SWEEP_WORKGROUPS 3
+#define SPOPS_TUNE_SWEEP_MAX_ROWS 4
 int spops_set_tuning(int key, int64_t value, int64_t *old);
 
 /* Device copies of the layout arrays a format conversion produced, for parity checks:
  * which=0: reorder (int32, n_rows; SELL only); which=1: chunk_len in the reference's
- * sense (int32, nchunks; SELL only, rounded to the reference alignment 256/8/sizeof(V)).
+ * sense (int32, nchunks; SELL only, rounded to the reference alignment 256/8/sizeof(V));
+ * which=2: the swept layout's parameters (int64[4]: row bits of a block-local row index,
+ * workgroups per row block, row blocks, entry groups; zeros for the column-sliced layout).
  * `host_out` must hold the element count given by spops_matrix_info. */
 int spops_matrix_layout(spops_matrix A, int which, void *host_out);
 

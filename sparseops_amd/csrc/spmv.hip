@@ -406,6 +406,119 @@ __global__ __launch_bounds__(kBlock) void k_slice_reduce(const double *__restric
 }
 
 // ---------------------------------------------------------------------------------------------
+// Swept layout: one workgroup of kSweepBlock threads per (row block, share h of its groups).
+// ---------------------------------------------------------------------------------------------
+struct SweepArgs {
+  const uint32_t *word;
+  const void *val;
+  const int *gdesc, *wgrp, *rstart;
+  void *part;                 // H > 1: y type [H][n_rows], summed by k_sweep_reduce
+  const void *x;
+  void *y;
+  int n_rows, n_cols, rb, H;
+  double alpha, beta;
+};
+
+template <int VT>
+__global__ __launch_bounds__(kSweepBlock) void k_sweep_spmv(const SweepArgs a) {
+  using T = VT_<VT>;
+  using V = typename T::V;
+  using X = typename T::X;
+  constexpr uint32_t sV = sizeof(V), sX = sizeof(X);
+  constexpr int K = kSweepLaneEntries, NW = kSweepBlock / kWave;
+  // groups in flight per wave: two (four with the prefetch) when the values take one VGPR
+  constexpr int D = sV == 8 ? 1 : 2;
+  __shared__ double acc[kSweepMaxRows];
+  const int rb = a.rb;
+  const uint32_t rmask = (1u << rb) - 1u;
+  const int blk = (int)blockIdx.x / a.H, h = (int)blockIdx.x % a.H;
+  const int r0 = a.rstart[blk], rows = a.rstart[blk + 1] - r0;
+  for (int i = (int)threadIdx.x; i < rows; i += kSweepBlock) acc[i] = 0.0;
+  __syncthreads();
+  const int g0 = a.wgrp[blockIdx.x], g1 = a.wgrp[blockIdx.x + 1];
+  const int lane = lane_id();
+  const long cb_cols = 1l << (32 - rb);
+  // The 16 waves take D * 16 consecutive groups at a time (wave w: groups base + d * 16 + w),
+  // so the workgroup sweeps x in order.  Each wave loads the words and values of its next D
+  // groups after issuing the gathers of the current ones (loads return in issue order, so
+  // the gathers are consumed while the stream loads are still in flight).  Groups past g1
+  // load nothing and are skipped.  Padding slots (past `valid`) hold word 0 and value 0 and
+  // gather nothing (out-of-range offset: 0), so they add an exact +0.0 to acc[0], which never
+  // changes a sum that starts at +0.0.
+  auto load_group = [&](int g, uint32_t (&w)[K], V (&v)[K]) {
+    const bool in = g < g1;
+    const rsrc_t rw = make_rsrc(a.word + (long)g * kSweepGroup, in ? (uint64_t)kSweepGroup * 4 : 0);
+    const rsrc_t rv = make_rsrc(static_cast<const V *>(a.val) + (long)g * kSweepGroup, in ? (uint64_t)kSweepGroup * sV : 0);
+    load_lane_run<uint32_t, K>(rw, kWave, lane, w);
+    load_lane_run<V, K>(rv, kWave, lane, v);
+    return in ? __builtin_amdgcn_readfirstlane(a.gdesc[g]) : 0;
+  };
+  int base = __builtin_amdgcn_readfirstlane(g0);
+  const int wv = __builtin_amdgcn_readfirstlane(wave_id());
+  uint32_t w[D][K];
+  V v[D][K];
+  int d[D];
+#pragma unroll
+  for (int q = 0; q < D; ++q) d[q] = load_group(base + q * NW + wv, w[q], v[q]);
+  for (; base < g1; base += D * NW) {
+    X xv[D][K];
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      const int cb = d[q] & 0xffff, valid = d[q] >> 16;
+      const long c0 = (long)cb * cb_cols;
+      const long ncx = (long)a.n_cols - c0 < cb_cols ? (long)a.n_cols - c0 : cb_cols;
+      const rsrc_t rx = make_rsrc(static_cast<const X *>(a.x) + c0, (uint64_t)(ncx > 0 ? ncx : 0) * sX);
+#pragma unroll
+      for (int u = 0; u < K; ++u) xv[q][u] = bload<X>(rx, (u * kWave + lane < valid) ? (w[q][u] >> rb) * sX : kOOB);
+    }
+    uint32_t wn[D][K];
+    V vn[D][K];
+    int dn[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) dn[q] = load_group(base + (D + q) * NW + wv, wn[q], vn[q]);
+#pragma unroll
+    for (int q = 0; q < D; ++q)
+      if (d[q] != 0) {
+#pragma unroll
+        for (int u = 0; u < K; ++u) atomicAdd(&acc[w[q][u] & rmask], (double)T::cvt(v[q][u]) * (double)xv[q][u]);
+      }
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+#pragma unroll
+      for (int u = 0; u < K; ++u) { w[q][u] = wn[q][u]; v[q][u] = vn[q][u]; }
+      d[q] = dn[q];
+    }
+  }
+  __syncthreads();
+  if (a.H == 1) {
+    X *y = static_cast<X *>(a.y);
+    for (int i = (int)threadIdx.x; i < rows; i += kSweepBlock) {
+      double o = a.alpha * acc[i];
+      if (a.beta != 0.0) o += a.beta * (double)y[r0 + i];
+      y[r0 + i] = (X)o;
+    }
+  } else {
+    X *mine = static_cast<X *>(a.part) + (long)h * a.n_rows + r0;
+    for (int i = (int)threadIdx.x; i < rows; i += kSweepBlock) mine[i] = (X)acc[i];
+  }
+}
+
+// y = alpha * (sum of the H partial planes, in order 0..H-1, in double) + beta * y.  FP32
+// planes round each block share's FP64 sum once: with H shares the error stays within
+// (H + 1) / 2 units of FP32 rounding of the sum of |products|.
+template <typename X>
+__global__ __launch_bounds__(kBlock) void k_sweep_reduce(const X *__restrict__ part, int H, int n_rows,
+                                                        X *__restrict__ y, double alpha, double beta) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n_rows) return;
+  double sum = 0.0;
+  for (int h = 0; h < H; ++h) sum += (double)__builtin_nontemporal_load(part + (long)h * n_rows + r);
+  double o = alpha * sum;
+  if (beta != 0.0) o += beta * (double)y[r];
+  y[r] = (X)o;
+}
+
+// ---------------------------------------------------------------------------------------------
 // BSR: one wave per block row; products of the bR x bC block staged in LDS per lane slot.
 // ---------------------------------------------------------------------------------------------
 template <int VT>
@@ -570,6 +683,136 @@ struct RowSource {
 
 int64_t g_slice_bytes = kSliceBytesDefault;
 int64_t g_dense_products = (int64_t)1 << 20;
+int64_t g_sweep_mode = 0;
+int64_t g_sweep_workgroups = kSweepTargetWorkgroups;
+int64_t g_sweep_max_rows = kSweepMaxRows;
+
+// Stable LSD radix sort of idx[0..n) by key[idx] (keys < 2^bits), 11-bit digits.
+static void radix_sort_by(std::vector<int32_t> &idx, const std::vector<uint32_t> &key, int bits) {
+  const size_t n = idx.size();
+  std::vector<int32_t> tmp(n);
+  std::vector<size_t> cnt(2048 + 1);
+  for (int sh = 0; sh < bits; sh += 11) {
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (size_t i = 0; i < n; ++i) ++cnt[((key[idx[i]] >> sh) & 2047u) + 1];
+    for (int d = 0; d < 2048; ++d) cnt[d + 1] += cnt[d];
+    for (size_t i = 0; i < n; ++i) tmp[cnt[(key[idx[i]] >> sh) & 2047u]++] = idx[i];
+    idx.swap(tmp);
+  }
+}
+
+// The swept layout (see SweepDev).  pos[r] is the SELL position of original row r, len[r] its length.
+static void build_sweep_layout(spops_matrix_s &M, const RowSource &src, const std::vector<int64_t> &pos,
+                               const std::vector<int64_t> &len, int vt) {
+  auto &W = M.sweep;
+  const int64_t n = M.n_rows, m = M.n_cols;
+  // row blocks: H workgroups per block and about g_sweep_workgroups / H blocks of equal entry
+  // counts, each at most g_sweep_max_rows rows
+  const int64_t rmax = g_sweep_max_rows;
+  const int64_t bmin = std::max<int64_t>(1, (n + rmax - 1) / rmax);
+  const int H = (int)std::min<int64_t>(kSweepMaxSplit, std::max<int64_t>(1, g_sweep_workgroups / bmin));
+  const int64_t btarget = std::max<int64_t>(bmin, g_sweep_workgroups / H);
+  std::vector<int64_t> cum(n + 1, 0);
+  for (int64_t r = 0; r < n; ++r) cum[r + 1] = cum[r] + len[r];
+  // greedy cut: a block takes rows while it has at most `cap` entries and rmax rows (and at
+  // least one row); the smallest cap that needs at most btarget blocks, by bisection (the
+  // greedy count falls as cap grows, and cap = nnz needs bmin <= btarget blocks)
+  auto cut = [&](int64_t cap, std::vector<int32_t> *out) {
+    int64_t blocks = 0;
+    for (int64_t r = 0; r < n; ++blocks) {
+      const int64_t limit = std::min(n, r + rmax);
+      int64_t r1 = std::upper_bound(cum.begin() + r + 1, cum.begin() + limit + 1, cum[r] + cap) - cum.begin() - 1;
+      r1 = std::max(r1, r + 1);
+      if (out) out->push_back((int32_t)r1);
+      r = r1;
+    }
+    return blocks;
+  };
+  int64_t lo = std::max<int64_t>(1, cum[n] / btarget), hi = std::max<int64_t>(1, cum[n]);
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (cut(mid, nullptr) <= btarget) hi = mid;
+    else lo = mid + 1;
+  }
+  std::vector<int32_t> rstart{0};
+  cut(lo, &rstart);
+  if (n == 0) rstart.push_back(0);
+  const int64_t n_rb = (int64_t)rstart.size() - 1;
+  int64_t maxrows = 1;
+  for (int64_t b = 0; b < n_rb; ++b) maxrows = std::max<int64_t>(maxrows, rstart[b + 1] - rstart[b]);
+  int rb = 6;
+  while (((int64_t)1 << rb) < maxrows) ++rb;
+  const int cbits = 32 - rb;
+  SPOPS_REQUIRE((m >> cbits) < 0xffff, SPOPS_ERROR_LIMIT, "too many column blocks for the swept layout");
+  int colbits = 1;
+  while (colbits < 31 && ((int64_t)1 << colbits) < m) ++colbits;
+  const int gc = 4, gv = 16 / (int)value_bytes(vt);
+  std::vector<uint32_t> word;
+  std::vector<double> dval;
+  std::vector<int32_t> gdesc;
+  std::vector<int64_t> bg(n_rb + 1, 0);                            // first group of each row block
+  std::vector<int32_t> idx;
+  std::vector<uint32_t> bcol, bword;
+  std::vector<double> bval;
+  for (int64_t b = 0; b < n_rb; ++b) {
+    bcol.clear(); bword.clear(); bval.clear();
+    for (int64_t r = rstart[b]; r < rstart[b + 1]; ++r)
+      for (int64_t k = 0; k < len[r]; ++k) {
+        int64_t c; double v;
+        src.entry(pos[r], k, c, v);
+        bcol.push_back((uint32_t)c);
+        bword.push_back((uint32_t)(((uint64_t)c & (((uint64_t)1 << cbits) - 1)) << rb) | (uint32_t)(r - rstart[b]));
+        bval.push_back(v);
+      }
+    idx.resize(bcol.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    radix_sort_by(idx, bcol, colbits);                             // by column; rows ascend among equals
+    size_t i = 0;
+    while (i < idx.size()) {
+      const uint32_t cb = bcol[idx[i]] >> cbits;
+      size_t e = i;
+      while (e < idx.size() && (bcol[idx[e]] >> cbits) == cb) ++e;
+      for (size_t g0 = i; g0 < e; g0 += kSweepGroup) {
+        const int valid = (int)std::min<size_t>(kSweepGroup, e - g0);
+        gdesc.push_back((int32_t)cb | (valid << 16));
+        const size_t at = word.size();
+        word.resize(at + kSweepGroup, 0u);
+        dval.resize(at + kSweepGroup, 0.0);
+        for (int j = 0; j < valid; ++j) {
+          const int l = j % kWave, u = j / kWave;
+          const int32_t s = idx[g0 + j];
+          word[at + (size_t)((u / gc) * kWave + l) * gc + u % gc] = bword[s];
+          dval[at + (size_t)((u / gv) * kWave + l) * gv + u % gv] = bval[s];
+        }
+      }
+      i = e;
+    }
+    bg[b + 1] = (int64_t)gdesc.size();
+  }
+  const int64_t ngroups = (int64_t)gdesc.size();
+  SPOPS_REQUIRE(ngroups * kSweepGroup < (int64_t)INT32_MAX, SPOPS_ERROR_LIMIT,
+                "the swept layout holds >= 2^31 entry slots");
+  const int Hu = ngroups == 0 ? 1 : H;
+  std::vector<int32_t> wgrp((size_t)(n_rb * Hu + 1));
+  for (int64_t b = 0; b < n_rb; ++b)
+    for (int h = 0; h < Hu; ++h) wgrp[(size_t)(b * Hu + h)] = (int32_t)(bg[b] + (bg[b + 1] - bg[b]) * h / Hu);
+  wgrp[(size_t)(n_rb * Hu)] = (int32_t)ngroups;
+  std::vector<uint8_t> val;
+  val.reserve(dval.size() * value_bytes(vt));
+  for (double v : dval) pack_values(val, vt, v);
+  W.word.upload(word);
+  W.val.upload(val);
+  W.gdesc.upload(gdesc);
+  W.wgrp.upload(wgrp);
+  W.rstart.upload(rstart);
+  if (Hu > 1) W.part.alloc((size_t)Hu * (size_t)n * xy_bytes(vt));
+  W.rb = rb;
+  W.H = Hu;
+  W.n_rb = (int)n_rb;
+  W.ngroups = (int)ngroups;
+  M.slots = ngroups * kSweepGroup;
+  M.dev_bytes = W.word.bytes + W.val.bytes + W.gdesc.bytes + W.wgrp.bytes + W.rstart.bytes + W.part.bytes;
+}
 
 void build_slice_layout(spops_matrix_s &M, const RowSource &src, int vt) {
   auto &S = M.sell;
@@ -607,6 +850,15 @@ void build_slice_layout(spops_matrix_s &M, const RowSource &src, int vt) {
   }
   int P = 1;
   while (P < kXcds && foot > (int64_t)P * g_slice_bytes) P *= 2;
+  // Rows that gather from more x than one XCD's L2 holds: the swept layout, when a row block
+  // of kSweepRowsTypical rows holds at least one entry per 128-byte line of x (so the sorted
+  // sweep reads every line it fetches more than once on average).
+  if (g_sweep_mode == 2 || (g_sweep_mode == 0 && P > 1 && n > 0 &&
+                            (double)M.nnz * (double)std::min<int64_t>(n, kSweepRowsTypical) /
+                                    (double)n >= (double)((m * sx + 127) / 128))) {
+    build_sweep_layout(M, src, pos, len, vt);
+    return;
+  }
   std::vector<int64_t> cut(P + 1, m);
   cut[0] = 0;
   if (P > 1) {
@@ -821,6 +1073,31 @@ static void launch_spmv_vt(const spops_matrix_s &M, double alpha, const void *x,
       hipLaunchKernelGGL((k_csr_spmv<VT, int>), dim3(grid), dim3(kBlock), 0, s, P.rowptr.as<int>(), P.col.as<int>(),
                          P.val.as<typename T::V>(), xx, (int)M.n_cols, yy, (Acc)alpha, (Acc)beta, P.tasks.as<int2>(), P.ntasks,
                          P.longs.as<int>(), P.nlong);
+  } else if ((M.format == SPOPS_FORMAT_SELL || M.format == SPOPS_FORMAT_ELL) && M.sweep.H > 0) {
+    const auto &W = M.sweep;
+    if (M.n_rows == 0) return;
+    SweepArgs wa;
+    wa.word = W.word.as<uint32_t>();
+    wa.val = W.val.p;
+    wa.gdesc = W.gdesc.as<int>();
+    wa.wgrp = W.wgrp.as<int>();
+    wa.rstart = W.rstart.as<int>();
+    wa.part = W.part.p;
+    wa.x = x;
+    wa.y = y;
+    wa.n_rows = (int)M.n_rows;
+    wa.n_cols = (int)M.n_cols;
+    wa.rb = W.rb;
+    wa.H = W.H;
+    wa.alpha = alpha;
+    wa.beta = beta;
+    const dim3 grid((unsigned)(W.n_rb * W.H));
+    hipLaunchKernelGGL((k_sweep_spmv<VT>), grid, dim3(kSweepBlock), 0, s, wa);
+    if (W.H > 1) {
+      SPOPS_HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL((k_sweep_reduce<typename T::X>), dim3((unsigned)((M.n_rows + kBlock - 1) / kBlock)),
+                         dim3(kBlock), 0, s, W.part.as<typename T::X>(), W.H, (int)M.n_rows, yy, alpha, beta);
+    }
   } else if (M.format == SPOPS_FORMAT_SELL || M.format == SPOPS_FORMAT_ELL) {
     const auto &S = M.sell;
     SliceArgs sa;
@@ -1183,6 +1460,7 @@ extern "C" int spops_matrix_info(spops_matrix A, int64_t info[16]) {
     if (A->format == SPOPS_FORMAT_SELL || A->format == SPOPS_FORMAT_ELL) {
       info[8] = A->sell.nlong; info[9] = A->sell.ntasks;
       info[10] = A->sell.C; info[11] = A->sell.sigma; info[12] = A->sell.P; info[14] = A->ref_nchunks;
+      if (A->sweep.H > 0) { info[8] = 0; info[9] = A->sweep.ngroups; info[12] = 0; }
       if (A->format == SPOPS_FORMAT_ELL) info[13] = A->ell_width;
     }
     if (A->format == SPOPS_FORMAT_BSR) { info[12] = A->bsr.bR; info[13] = A->bsr.bC; info[14] = A->bsr.nnzb; }
@@ -1196,6 +1474,19 @@ extern "C" int spops_set_tuning(int key, int64_t value, int64_t *old) {
       SPOPS_REQUIRE(value >= 64, SPOPS_ERROR_INVALID_VALUE, "slice bytes must be >= 64");
       if (old) *old = g_slice_bytes;
       g_slice_bytes = value;
+    } else if (key == SPOPS_TUNE_SPMV_LAYOUT) {
+      SPOPS_REQUIRE(value >= 0 && value <= 2, SPOPS_ERROR_INVALID_VALUE, "SpMV layout must be 0, 1 or 2");
+      if (old) *old = g_sweep_mode;
+      g_sweep_mode = value;
+    } else if (key == SPOPS_TUNE_SWEEP_WORKGROUPS) {
+      SPOPS_REQUIRE(value >= 1, SPOPS_ERROR_INVALID_VALUE, "workgroup target must be >= 1");
+      if (old) *old = g_sweep_workgroups;
+      g_sweep_workgroups = value;
+    } else if (key == SPOPS_TUNE_SWEEP_MAX_ROWS) {
+      SPOPS_REQUIRE(value >= 1 && value <= kSweepMaxRows, SPOPS_ERROR_INVALID_VALUE,
+                    "swept row block size must be in [1, %d]", kSweepMaxRows);
+      if (old) *old = g_sweep_max_rows;
+      g_sweep_max_rows = value;
     } else if (key == SPOPS_TUNE_DENSE_PRODUCTS) {
       SPOPS_REQUIRE(value >= 1, SPOPS_ERROR_INVALID_VALUE, "dense product threshold must be >= 1");
       if (old) *old = g_dense_products;
@@ -1210,9 +1501,14 @@ extern "C" int spops_set_tuning(int key, int64_t value, int64_t *old) {
 extern "C" int spops_matrix_layout(spops_matrix A, int which, void *host_out) {
   return guarded([&]() {
     SPOPS_REQUIRE(A != nullptr && host_out != nullptr, SPOPS_ERROR_INVALID_VALUE, "null argument");
-    SPOPS_REQUIRE(A->format == SPOPS_FORMAT_SELL, SPOPS_ERROR_NOT_SUPPORTED, "layout arrays exist for SELL only");
+    SPOPS_REQUIRE(A->format == SPOPS_FORMAT_SELL || (which == 2 && A->format == SPOPS_FORMAT_ELL),
+                  SPOPS_ERROR_NOT_SUPPORTED, "layout arrays exist for SELL only");
     if (which == 0) std::memcpy(host_out, A->sell.ref_reorder.data(), A->sell.ref_reorder.size() * 4);
     else if (which == 1) std::memcpy(host_out, A->sell.ref_chunk_len.data(), A->sell.ref_chunk_len.size() * 4);
+    else if (which == 2) {
+      const int64_t d[4] = {A->sweep.rb, A->sweep.H, A->sweep.n_rb, A->sweep.ngroups};
+      std::memcpy(host_out, d, sizeof(d));
+    }
     else throw_error(SPOPS_ERROR_INVALID_VALUE, "unknown layout array %d", which);
     return SPOPS_SUCCESS;
   });

@@ -62,6 +62,38 @@ struct SliceDev {
   std::vector<int32_t> ref_reorder, ref_chunk_len;
 };
 
+// Swept layout (SELL-C-sigma, SELL-C-R and ELL of matrices whose rows gather from all over x):
+// rows are cut into blocks of consecutive rows with about equal entry counts, at most
+// kSweepMaxRows rows (whose FP64 sums fill the 160 KiB of LDS of a CU); a workgroup adds the
+// products of its block into LDS, and the block's entries are stored sorted by column, so the
+// 64 lanes of a gather instruction read 64 consecutive entries of one sweep over x and fall on
+// few cache lines.  An entry is one uint32 word, (column - cb * 2^(32 - rb)) << rb | row in
+// the block (rb: bits of the largest block), plus its value; cb, the column block, comes from
+// the entry's group.  Entries are packed in groups of kSweepGroup (one column block per
+// group, the last group of a column block padded), lane-interleaved so that each 128-bit load
+// of a wave reads one contiguous block and gather u of lane l takes the group's sorted entry
+// 64 u + l.  H workgroups share a row block (contiguous group ranges, H from the CU count);
+// with H > 1 each writes a partial plane (in the y type) and a second pass adds the H planes
+// in order h = 0..H-1 and writes y.  (An in-kernel "last workgroup adds" handoff needs
+// agent-scope release/acquire, which on this chip writes back and invalidates the XCD's L2
+// and with it the cached x: measured slower.)
+constexpr int kSweepMaxRows = 20480;          // 20480 x 8 bytes = 160 KiB of LDS
+constexpr int64_t kSweepRowsTypical = 16384;  // rows of a block for the layout choice
+constexpr int kSweepBlock = 1024;             // threads per workgroup (one workgroup per CU)
+constexpr int kSweepLaneEntries = 8;
+constexpr int kSweepGroup = kWave * kSweepLaneEntries;
+constexpr int kSweepTargetWorkgroups = 256;   // one per CU (default of g_sweep_workgroups)
+constexpr int kSweepMaxSplit = 16;            // H limit
+
+struct SweepDev {
+  DevBuf word, val;           // per group: kSweepGroup words / values, lane-interleaved
+  DevBuf gdesc;               // int per group: column block | valid entries << 16
+  DevBuf wgrp;                // int per workgroup + 1: first group
+  DevBuf rstart;              // int per row block + 1: first row
+  DevBuf part;                // y type [H][n_rows] (H > 1)
+  int rb = 0, H = 0, n_rb = 0, ngroups = 0;
+};
+
 struct BsrDev {
   DevBuf brp, bcol, data;
   int64_t mb = 0, bR = 0, bC = 0, nnzb = 0;
@@ -75,15 +107,22 @@ struct spops_matrix_s {
   int64_t dev_bytes = 0, algo_bytes = 0, slots = 0, ref_nchunks = 0, ell_width = 0;
   spops::CsrDev csr;
   spops::SliceDev sell;
+  spops::SweepDev sweep;      // used instead of `sell` when sweep.H > 0
   spops::BsrDev bsr;
 };
 
 namespace spops {
-extern int64_t g_slice_bytes;
+extern int64_t g_slice_bytes;        // x bytes per slice for matrices created from now on
+// SELL/ELL device layout of matrices created from now on: 0 = chosen from the x footprint,
+// 1 = column-sliced stream, 2 = swept
+extern int64_t g_sweep_mode;
+// swept layout: workgroups aimed for per launch (H = this / fewest row blocks, 1..kSweepMaxSplit)
+extern int64_t g_sweep_workgroups;
+extern int64_t g_sweep_max_rows;     // swept layout: rows of the largest row block
 // The reference's ave_nnz_each_row, Gini_coeff_row and nnz_closed2diag_ratio of a host CSR
 // (src/sparse_features.cpp:609-760): the features the batched SpMM autotuner classifies by.
 void row_features_host(const int64_t *rowptr, const int64_t *col, int64_t n_rows, int64_t n_cols, double out[3]);
-extern int64_t g_dense_products;   // LeSpGEMM: products of a cluster from which it accumulates densely   // x bytes per slice for matrices created from now on
+extern int64_t g_dense_products;   // LeSpGEMM: products of a cluster from which it accumulates densely
 // Builds and uploads the CSR plan of a matrix: SpMV row tasks and long rows (> kCsrLongRow
 // entries); for SpMM the short-row order and the segments of the rows > kSpmmLongRow.
 void upload_csr_plan(CsrDev &P, const std::vector<int64_t> &rp, int64_t n_rows);

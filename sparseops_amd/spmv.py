@@ -130,8 +130,18 @@ class SpMatrix:
 
     @property
     def slices(self) -> int:
-        """Column slices of the SELL/ELL device layout."""
+        """Column slices of the SELL/ELL device layout (0 for the swept layout)."""
         return self.info[12] if self.format in ("sell", "ell") else 1
+
+    @property
+    def sweep(self):
+        """Swept layout parameters (row bits, workgroups per row block, row blocks, entry
+        groups), or None when the matrix uses another layout."""
+        if self.format not in ("sell", "ell"):
+            return None
+        out = np.zeros(4, dtype=np.int64)
+        L.call("spops_matrix_layout", self._h, 2, out.ctypes.data)
+        return tuple(int(v) for v in out) if out[1] > 0 else None
 
     def layout(self, which: int) -> np.ndarray:
         n = self.n_rows if which == 0 else self.info[14]
@@ -201,21 +211,47 @@ def sell_from_chunks(n_rows, n_cols, nnz, C, reorder, chunk_len, chunk_cols, chu
     return SpMatrix(h)
 
 
-class slice_bytes:
-    """Context manager (or plain call) setting the bytes of x per column slice of SELL/ELL
-    matrices created while it is active (library default 2 MiB, one XCD's share of L2)."""
+class tuning:
+    """Context manager (or plain call) setting one process-wide tuning value of the library
+    (spops_set_tuning) for matrices created while it is active; the old value comes back on
+    exit."""
 
-    def __init__(self, nbytes: int):
+    def __init__(self, key: int, value: int):
         old = ctypes.c_int64(0)
-        L.call("spops_set_tuning", L.TUNE_SLICE_BYTES, int(nbytes), ctypes.byref(old))
-        self.old = old.value
+        L.call("spops_set_tuning", key, int(value), ctypes.byref(old))
+        self.key, self.old = key, old.value
 
     def __enter__(self):
         return self
 
     def __exit__(self, *exc):
-        L.call("spops_set_tuning", L.TUNE_SLICE_BYTES, self.old, None)
+        L.call("spops_set_tuning", self.key, self.old, None)
         return False
+
+
+def slice_bytes(nbytes: int) -> tuning:
+    """Bytes of x per column slice of the sliced SELL/ELL layout (default 2 MiB, one XCD's
+    share of L2)."""
+    return tuning(L.TUNE_SLICE_BYTES, nbytes)
+
+
+LAYOUTS = {"auto": 0, "sliced": 1, "swept": 2}
+
+
+def spmv_layout(kind: str) -> tuning:
+    """Device layout of SELL/ELL matrices: "auto" (per matrix), "sliced" (column-sliced stream)
+    or "swept" (row blocks summed in LDS, entries of a block sorted by column)."""
+    return tuning(L.TUNE_SPMV_LAYOUT, LAYOUTS[kind])
+
+
+def sweep_max_rows(n: int) -> tuning:
+    """Rows of the largest row block of the swept layout (1..20480, default 20480)."""
+    return tuning(L.TUNE_SWEEP_MAX_ROWS, n)
+
+
+def sweep_workgroups(n: int) -> tuning:
+    """Workgroups a swept-layout SpMV aims for (default 256); row blocks are shared by up to 16."""
+    return tuning(L.TUNE_SWEEP_WORKGROUPS, n)
 
 
 def SpOps(alpha: float, A: SpMatrix, x: torch.Tensor, beta: float, y: torch.Tensor):

@@ -173,7 +173,7 @@ def test_local_columns_stay_unsliced(dev):
     cols = np.sort((np.arange(n)[:, None] + np.arange(-3, 4)[None, :]) % n, axis=1)
     A = Csr(n, n, (np.arange(n + 1) * 7).astype(np.int32), cols.reshape(-1).astype(np.int32),
             np.random.default_rng(3).standard_normal(n * 7))
-    with S.slice_bytes(64 << 10):                  # x is 256 KiB in FP32
+    with S.slice_bytes(64 << 10), S.spmv_layout("sliced"):     # x is 256 KiB in FP32
         M = S.csr_to_sell_c_sigma(A, 1024, 32, "f32")
         R = S.csr_to_sell_c_sigma(powerlaw(n, lmax=500, seed=2), 1024, 32, "f32")
     assert M.slices == 1 and R.slices == 4
@@ -217,7 +217,7 @@ def test_column_slices(mats, name, vt, P, fmt, dev):
     # the slice count follows the x bytes (in 128-byte lines) one eighth of the rows touches;
     # these matrices have random columns, so an eighth of the rows touches all of x
     xb = -(-A.n_cols * (8 if vt == "f64" else 4) // 128) * 128
-    with S.slice_bytes(-(-xb // P)):
+    with S.slice_bytes(-(-xb // P)), S.spmv_layout("sliced"):
         M = S.csr_to_sell_c_sigma(A, 256, 32, vt) if fmt == "sell" else S.csr_to_ell(A, vt)
     assert M.slices == P
     x = uniform_x(A.n_cols, 8)
@@ -228,3 +228,89 @@ def test_column_slices(mats, name, vt, P, fmt, dev):
     # repeated calls reuse the partial buffer: same result bit for bit
     y2 = _run(M, x, np.full(A.n_rows, np.nan), dev, 1.0, 0.0)
     np.testing.assert_array_equal(y, y2)
+
+
+def _wide_matrix(seed=13):
+    """20000 x 600000 with random columns: two row blocks of the swept layout, three column
+    blocks (2^18 columns each at 2^14 rows per block), rows of 0 to 3000 entries."""
+    rng = np.random.default_rng(seed)
+    n_rows, n_cols = 20000, 600000
+    lens = rng.integers(0, 16, n_rows)
+    lens[::101] = 0
+    lens[5], lens[16384], lens[19999] = 3000, 700, 2049
+    cols = np.concatenate([np.sort(rng.choice(n_cols, size=L, replace=False)) for L in lens])
+    rp = np.zeros(n_rows + 1, np.int64)
+    np.cumsum(lens, out=rp[1:])
+    return Csr(n_rows, n_cols, rp.astype(np.int32), cols.astype(np.int32), rng.standard_normal(cols.size))
+
+
+def _sweep_matrix(mats, name):
+    return _wide_matrix() if name == "wide" else _skewed_matrix() if name == "skew" else mats[name]
+
+
+@pytest.mark.parametrize("wgs,maxrows", [(1, 20480), (256, 20480), (40, 777)])
+@pytest.mark.parametrize("vt", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("name", ["pl", "edge", "skew", "wide", "lap"])
+@pytest.mark.parametrize("fmt", ["sell", "ell"])
+def test_swept(mats, name, vt, fmt, wgs, maxrows, dev):
+    """The swept layout (row blocks summed in LDS, entries sorted by column): one workgroup
+    per row block (wgs=1), blocks shared by up to 16 workgroups whose partial planes a second
+    pass adds (wgs=256), and many short blocks (at most 777 rows, about 40 workgroups)."""
+    A = _sweep_matrix(mats, name)
+    with S.spmv_layout("swept"), S.sweep_workgroups(wgs), S.sweep_max_rows(maxrows):
+        M = S.csr_to_sell_c_sigma(A, 256, 32, vt) if fmt == "sell" else S.csr_to_ell(A, vt)
+    rbits, H, n_rb, groups = M.sweep
+    bmin = -(-A.n_rows // maxrows)
+    assert M.slices == 0 and groups >= -(-A.nnz // 512) and n_rb >= bmin and (1 << rbits) >= -(-A.n_rows // n_rb)
+    assert H == min(16, max(1, wgs // bmin))
+    x = uniform_x(A.n_cols, 8)
+    _check(A, M, x, _y0(A.n_rows), vt, dev=dev)
+    # beta == 0 never reads y; the per-block counters reset, so repeated calls agree
+    y = _run(M, x, np.full(A.n_rows, np.nan), dev, 1.0, 0.0)
+    assert np.all(np.isfinite(y))
+    for _ in range(2):
+        _check(A, M, x, _y0(A.n_rows), vt, dev=dev)
+    if fmt == "sell":   # the reference layout arrays do not depend on the device layout
+        align = 4 if vt == "f64" else 8
+        reorder, cl, _, _, _ = orc.sell_layout(A, 256, 32, align)
+        np.testing.assert_array_equal(M.layout(0), reorder)
+        np.testing.assert_array_equal(M.layout(1), cl)
+
+
+def test_swept_empty_tiny_and_nonfinite(dev):
+    """Empty and one-row matrices; x holding inf/nan outside the gathered columns must not
+    reach y (padding slots never gather)."""
+    with S.spmv_layout("swept"):
+        for n in (1, 2, 63, 65):
+            A = Csr(n, n, np.zeros(n + 1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+            M = S.csr_to_sell_c_sigma(A, 32, 32)
+            y = _run(M, np.ones(n), np.ones(n), dev, 1.0, 0.5)
+            np.testing.assert_array_equal(y, 0.5)
+        n = 3000
+        rng = np.random.default_rng(4)
+        cols = np.stack([np.sort(1 + rng.choice(n - 1, size=5, replace=False)) for _ in range(n)])
+        rp = np.arange(n + 1, dtype=np.int32) * 5
+        A = Csr(n, n, rp, cols.reshape(-1).astype(np.int32), rng.standard_normal(n * 5))
+        for vt in ("f64", "f32"):
+            M = S.csr_to_sell_c_sigma(A, 64, 32, vt)
+            x = uniform_x(n, 3)
+            x[0] = np.inf                        # column 0 holds no entry
+            ref, _ = orc.spmv_exact(A, np.where(np.isfinite(x), x, 0.0))
+            y = _run(M, x, np.zeros(n), dev, 1.0, 0.0)
+            assert np.all(np.isfinite(y))
+            np.testing.assert_allclose(y, ref, rtol=0, atol=(1e-12 if vt == "f64" else 1e-5) * np.abs(ref).max())
+
+
+def test_swept_auto_selection(dev):
+    """The default picks the swept layout for random columns over an x larger than the slice
+    budget and keeps the sliced layout for a band."""
+    n = 1 << 16
+    cols = np.sort((np.arange(n)[:, None] + np.arange(-3, 4)[None, :]) % n, axis=1)
+    band = Csr(n, n, (np.arange(n + 1) * 7).astype(np.int32), cols.reshape(-1).astype(np.int32),
+               np.random.default_rng(3).standard_normal(n * 7))
+    with S.slice_bytes(64 << 10):
+        B = S.csr_to_sell_c_sigma(band, 1024, 32, "f32")
+        R = S.csr_to_sell_c_sigma(powerlaw(n, lmax=500, seed=2), 1024, 32, "f32")
+    assert B.sweep is None and B.slices == 1
+    assert R.sweep is not None and R.slices == 0
+    _check(band, B, uniform_x(n, 9), _y0(n), "f32", dev=dev)
