@@ -26,6 +26,11 @@
 //          the tasks write y directly).  The reference's chunk_len / reorder arrays are
 //          kept on the host for inspection.
 //   ELL  : the same layout.
+//   SELL/ELL swept (matrices whose rows gather from all over x): blocks of consecutive rows
+//          (<= 20480, equal entry counts) whose FP64 sums live in LDS; a block's entries
+//          sorted by column, so the lanes of one gather read consecutive entries of one
+//          sweep over x; H workgroups per block, each writing a partial plane that a second
+//          pass adds in order (spmv_internal.hpp, SweepDev).
 //   BSR  : the reference layout (row_ptr, block_colindex, row-major bR x bC blocks).
 // The streams of A (indices, values) use non-temporal loads so that L2 keeps x.
 #include <algorithm>
