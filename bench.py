@@ -351,6 +351,43 @@ def autograd_measure(scale, k, steps, cpu):
     return out
 
 
+def bsr_measure(n, b, blocks_per_row, steps, warmup):
+    """BSR SpMV on dense b x b blocks: the matrix-core kernel (v_mfma_f64_16x16x4f64) and the
+    VALU kernel on the same matrix, FP64 and FP32 values (algorithmic bytes: block row
+    pointers, block columns, block values, x, y)."""
+    import numpy as np
+    import torch
+    from sparseops_amd import spmv as S
+    from sparseops_amd.gen import block_sparse, uniform_x
+    A = block_sparse(n, b, blocks_per_row, seed=1)
+    x64 = uniform_x(A.n_cols, 0)
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {"workload": f"BSR {b}x{b} dense blocks, {blocks_per_row} per block row, n={n} (nnz={A.nnz}); "
+                       f"and {2 * b}x{2 * b} blocks, {blocks_per_row // 4} per block row (MFMA kernel only)"}
+
+    def run(M, key):
+        x = torch.tensor(x64, dtype=M.xy_dtype, device="cuda:0")
+        y = torch.zeros(A.n_rows, dtype=M.xy_dtype, device="cuda:0")
+        _, kern = time_spmv(M, x, y, steps, warmup, stream)
+        out[key + "_us"] = round(kern * 1e3, 2)
+        out[key + "_gbs"] = round(M.algo_bytes / (kern * 1e-3) / 1e9, 1)
+        M.close()
+
+    for vt in ("f64", "f32"):
+        for mode in ("always", "never"):
+            with S.bsr_mfma(mode):
+                M = S.csr_to_bsr(A, b, b, vt)
+            assert M.info[9] == (1 if mode == "always" else 0)
+            run(M, f"{vt}_{'mfma' if mode == 'always' else 'valu'}")
+    A = block_sparse(n, 2 * b, max(1, blocks_per_row // 4), seed=2)
+    x64 = uniform_x(A.n_cols, 0)
+    for vt in ("f64", "f32"):
+        M = S.csr_to_bsr(A, 2 * b, 2 * b, vt)
+        assert M.info[9] == 1
+        run(M, f"{vt}_{2 * b}x{2 * b}_mfma")
+    return out
+
+
 def batch_measure(count, steps, warmup, cpu, cpu_seconds):
     """Batched SpMM: `count` band matrices (n log-uniform in [1k, 64k], half-width U{1..64},
     0-8 off-band entries per row, k U{1..32}, FP32 N(0,1) values and X), formats chosen by the
@@ -454,6 +491,7 @@ def main():
     ap.add_argument("--lespgemm-iters", type=int, default=5)
     ap.add_argument("--autograd-scale", type=int, default=20, help="R-MAT scale of the autograd SpMM line (0: skip)")
     ap.add_argument("--batch-count", type=int, default=512, help="matrices of the batched SpMM line (0: skip)")
+    ap.add_argument("--bsr-n", type=int, default=1 << 18, help="rows of the dense-block BSR line (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -550,6 +588,12 @@ def main():
                            args.cpu_seconds)
         log(f"batched spmm done in {time.time() - t:.1f}s: {bt}")
 
+    bs = None
+    if args.bsr_n > 0:
+        t = time.time()
+        bs = bsr_measure(args.bsr_n, 16, 8, max(10, args.steps // 4), max(3, args.warmup // 4))
+        log(f"bsr done in {time.time() - t:.1f}s: {bs}")
+
     out = {
         "metric": METRIC,
         "value": round(head["gbs"], 1),
@@ -585,6 +629,7 @@ def main():
         "lespgemm_panels": lp,
         "autograd_spmm": ag,
         "batch_spmm": bt,
+        "bsr_dense_blocks": bs,
     }
     print(json.dumps(out), flush=True)
 

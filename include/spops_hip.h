@@ -199,7 +199,8 @@ int spops_csr_set_values(spops_matrix A, const void *d_values, int values_type, 
 
 /* info[0]=format info[1]=n_rows info[2]=n_cols info[3]=nnz info[4]=value_type
  * info[5]=device bytes owned info[6]=algorithmic bytes of one SpMV (no padding)
- * info[7]=stored slots (incl. padding) info[8]=long rows info[9]=wave tasks
+ * info[7]=stored slots (incl. padding) info[8]=long rows info[9]=wave tasks (BSR: 1 when
+ * the MFMA kernel runs it)
  * info[10]=C info[11]=sigma info[12]=bR (BSR) or column slices (SELL/ELL; 0 for the swept
  * layout, whose info[9] counts entry groups) info[13]=bC
  * (BSR) or max row width (ELL) info[14]=nchunks (reference layout) info[15]=index_type */
@@ -223,12 +224,18 @@ int spops_matrix_info(spops_matrix A, int64_t info[16]);
  *   H = this / B workgroups (1..16), and rows are cut into about this / H blocks of equal
  *   entry counts; the H partial sums of a row are added in a fixed order.
  *   SPOPS_TUNE_SWEEP_MAX_ROWS: rows of the largest row block of the swept layout, 1..20480
- *   (default 20480: 160 KiB of FP64 sums in LDS).  Both apply to matrices created afterwards. */
+ *   (default 20480: 160 KiB of FP64 sums in LDS).  Both apply to matrices created afterwards.
+ *   SPOPS_TUNE_BSR_MFMA: which BSR matrices run on the matrix cores (v_mfma_f64_16x16x4f64,
+ *   FP64 products and sums; dense blocks of at least 8 x 8 with bC % 4 == 0, bR <= 64,
+ *   bR * bC <= 4096): 0 none, 1 (default) those whose blocks exceed the VALU kernel's 256
+ *   entries, 2 all such blocks.  With one right-hand side the VALU kernel is the faster of
+ *   the two where both apply.  Applies to matrices created afterwards. */
 #define SPOPS_TUNE_SLICE_BYTES 0
 #define SPOPS_TUNE_DENSE_PRODUCTS 1
 #define SPOPS_TUNE_SPMV_LAYOUT 2
 #define SPOPS_TUNE_SWEEP_WORKGROUPS 3
 #define SPOPS_TUNE_SWEEP_MAX_ROWS 4
+#define SPOPS_TUNE_BSR_MFMA 5
 int spops_set_tuning(int key, int64_t value, int64_t *old);
 
 /* Device copies of the layout arrays a format conversion produced, for parity checks:

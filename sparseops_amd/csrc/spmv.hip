@@ -575,6 +575,96 @@ __global__ __launch_bounds__(kBlock) void k_bsr_spmv(const int *__restrict__ brp
   }
 }
 
+// BSR with dense blocks of at least 8 x 8 (bC % 4 == 0), on the matrix cores: one wave per
+// block row; each block is cut into tiles of 16 rows x 16 columns, each tile four
+// v_mfma_f64_16x16x4f64 steps.  The reduction index is permuted alike in both operands: at
+// step s, lane i + 16k gives A[i][4k + s] (so lane i + 16k loads the four consecutive
+// entries 4k .. 4k+3 of row i of the tile in one vector load) and x[4k + s] (the same in
+// all 16 columns of the B operand, so every column of D holds the tile's y).  Products are
+// exact in FP64 for FP32 and BF16 values, sums FP64.  Rows past bR, columns past bC and x
+// past n_cols read 0 (out-of-range buffer offsets).  D of this FP64 instruction: lane l,
+// register r holds row l / 16 + 4 r of column l % 16 (measured on gfx950).
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+// MT: 16-row tiles per block (bR <= 16 MT); two accumulator chains per tile (steps s = 0, 2
+// and s = 1, 3) so that consecutive MFMAs do not wait on each other.
+template <int VT, int MT>
+__global__ __launch_bounds__(kBlock) void k_bsr_mfma(const int *__restrict__ brp, const int *__restrict__ bcol,
+                                                     const typename VT_<VT>::V *__restrict__ bdata, int mb, int bR,
+                                                     int bC, int n_rows, int n_cols,
+                                                     const typename VT_<VT>::X *__restrict__ x,
+                                                     typename VT_<VT>::X *__restrict__ y, double alpha, double beta) {
+  using T = VT_<VT>;
+  using V = typename T::V;
+  using X = typename T::X;
+  constexpr uint32_t sV = sizeof(V), sX = sizeof(X);
+  const int i = __builtin_amdgcn_readfirstlane((int)blockIdx.x * (kBlock / kWave) + wave_id());
+  if (i >= mb) return;
+  const int lane = lane_id(), li = lane & 15, lk = lane >> 4;
+  const int tiles = (bR + 15) >> 4, groups = (bC + 15) >> 4;
+  const rsrc_t rx = make_rsrc(x, (uint64_t)n_cols * sX);
+  const int b0 = brp[i], b1 = brp[i + 1];
+  const int bn = bR * bC;
+  const rsrc_t rd = make_rsrc(bdata + (long)b0 * bn, (uint64_t)(b1 - b0) * bn * sV);
+  d4_t acc[MT][2];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) acc[t][0] = acc[t][1] = d4_t{0.0, 0.0, 0.0, 0.0};
+  for (int b = b0; b < b1; ++b) {
+    const long xb = (long)bcol[b] * bC;
+    for (int cg = 0; cg < groups; ++cg) {
+      const int c = cg * 16 + 4 * lk;              // this lane's first column of the group
+      double xv[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const long xi = xb + c + s;
+        xv[s] = (double)bload<X>(rx, (c < bC && xi < n_cols) ? (uint32_t)xi * sX : kOOB);
+      }
+#pragma unroll
+      for (int t = 0; t < MT; ++t) {
+        if (t >= tiles) break;
+        const int row = t * 16 + li;
+        const uint32_t off = (row < bR && c < bC) ? (uint32_t)((b - b0) * bn + row * bC + c) * sV : kOOB;
+        double av[4];
+        if constexpr (sV == 8) {
+          const bu4 lo = bload<bu4, kAuxNT>(rd, off), hi = bload<bu4, kAuxNT>(rd, off == kOOB ? kOOB : off + 16);
+          __builtin_memcpy(&av[0], &lo, 16);
+          __builtin_memcpy(&av[2], &hi, 16);
+        } else if constexpr (sV == 4) {
+          const bu4 w = bload<bu4, kAuxNT>(rd, off);
+          float f[4];
+          __builtin_memcpy(f, &w, 16);
+#pragma unroll
+          for (int s = 0; s < 4; ++s) av[s] = (double)f[s];
+        } else {
+          const bu2 w = bload<bu2, kAuxNT>(rd, off);
+          uint16_t h[4];
+          __builtin_memcpy(h, &w, 8);
+#pragma unroll
+          for (int s = 0; s < 4; ++s) av[s] = (double)T::cvt(h[s]);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+          acc[t][s & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], xv[s], acc[t][s & 1], 0, 0, 0);
+      }
+    }
+  }
+  if (li != 0) return;
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    if (t >= tiles) break;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = t * 16 + lk + 4 * r;
+      const long gr = (long)i * bR + row;
+      if (row < bR && gr < n_rows) {
+        double o = alpha * (acc[t][0][r] + acc[t][1][r]);
+        if (beta != 0.0) o += beta * (double)y[gr];
+        y[gr] = (X)o;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // host: plans and layouts
 // ---------------------------------------------------------------------------------------------
@@ -691,6 +781,21 @@ int64_t g_dense_products = (int64_t)1 << 20;
 int64_t g_sweep_mode = 0;
 int64_t g_sweep_workgroups = kSweepTargetWorkgroups;
 int64_t g_sweep_max_rows = kSweepMaxRows;
+int64_t g_bsr_mfma = 1;   // 0 never, 1 blocks past the VALU kernel's limit, 2 every dense block >= 8 x 8
+
+bool bsr_use_mfma(int64_t bR, int64_t bC) {
+  const bool dense = bR >= kBsrMfmaMinDim && bC >= kBsrMfmaMinDim && bC % 4 == 0 && bR <= kBsrMfmaMaxRows &&
+                     bR * bC <= kBsrMfmaMaxBlock;
+  const bool valu = bR * bC <= kBsrMaxBlock && bR <= kWave;
+  // one right-hand side leaves the matrix cores 1/16 of each 16x16x4 step: on 16x16 blocks
+  // the VALU kernel is faster (FP64 56.9 vs 62.4 us, FP32 26.0 vs 38.5 us on 33.5 M entries),
+  // so by default the MFMA kernel takes the dense blocks the VALU kernel cannot hold
+  if (dense && (g_bsr_mfma == 2 || (g_bsr_mfma == 1 && !valu))) return true;
+  SPOPS_REQUIRE(bR * bC <= kBsrMaxBlock && bR <= kWave, SPOPS_ERROR_LIMIT,
+                "block of %lld x %lld: at most %d entries (or 8..%d x 8.. with bC %% 4 == 0 and at most %d entries "
+                "for the MFMA kernel)", (long long)bR, (long long)bC, kBsrMaxBlock, kBsrMfmaMaxRows, kBsrMfmaMaxBlock);
+  return false;
+}
 
 // Stable LSD radix sort of idx[0..n) by key[idx] (keys < 2^bits), 11-bit digits.
 static void radix_sort_by(std::vector<int32_t> &idx, const std::vector<uint32_t> &key, int bits) {
@@ -1145,9 +1250,18 @@ static void launch_spmv_vt(const spops_matrix_s &M, double alpha, const void *x,
     const auto &B = M.bsr;
     int grid = (int)((B.mb + 3) / 4);
     if (grid == 0) return;
-    hipLaunchKernelGGL((k_bsr_spmv<VT>), dim3(grid), dim3(kBlock), 0, s, B.brp.as<int>(), B.bcol.as<int>(),
-                       B.data.as<typename T::V>(), (int)B.mb, (int)B.bR, (int)B.bC, (int)M.n_rows, (int)M.n_cols,
-                       xx, yy, (Acc)alpha, (Acc)beta);
+    auto mfma = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s, B.brp.as<int>(), B.bcol.as<int>(),
+                         B.data.as<typename T::V>(), (int)B.mb, (int)B.bR, (int)B.bC, (int)M.n_rows, (int)M.n_cols,
+                         xx, yy, alpha, beta);
+    };
+    if (B.mfma && B.bR <= 16) mfma(k_bsr_mfma<VT, 1>);
+    else if (B.mfma && B.bR <= 32) mfma(k_bsr_mfma<VT, 2>);
+    else if (B.mfma) mfma(k_bsr_mfma<VT, kBsrMfmaMaxRows / 16>);
+    else
+      hipLaunchKernelGGL((k_bsr_spmv<VT>), dim3(grid), dim3(kBlock), 0, s, B.brp.as<int>(), B.bcol.as<int>(),
+                         B.data.as<typename T::V>(), (int)B.mb, (int)B.bR, (int)B.bC, (int)M.n_rows, (int)M.n_cols,
+                         xx, yy, (Acc)alpha, (Acc)beta);
   }
   SPOPS_HIP_CHECK(hipGetLastError());
 }
@@ -1390,8 +1504,7 @@ extern "C" int spops_bsr_create(spops_matrix *A, int64_t n_rows, int64_t n_cols,
     SPOPS_REQUIRE(A != nullptr, SPOPS_ERROR_INVALID_VALUE, "null output handle");
     check_vt(in_vt, vt);
     SPOPS_REQUIRE(bR > 0 && bC > 0, SPOPS_ERROR_INVALID_VALUE, "block dims must be positive");
-    SPOPS_REQUIRE(bR * bC <= kBsrMaxBlock && bR <= kWave, SPOPS_ERROR_LIMIT, "block of %lld x %lld exceeds %d entries",
-                  (long long)bR, (long long)bC, kBsrMaxBlock);
+    const bool mfma = bsr_use_mfma(bR, bC);
     std::vector<int64_t> rp = host_index(rowptr, n_rows + 1, index_type);
     std::vector<int64_t> ci = host_index(col, nnz, index_type);
     check_csr(n_rows, n_cols, nnz, rp, ci);
@@ -1434,7 +1547,7 @@ extern "C" int spops_bsr_create(spops_matrix *A, int64_t n_rows, int64_t n_cols,
     M->bsr.brp.upload(brp);
     M->bsr.bcol.upload(bcol);
     M->bsr.data.upload(v);
-    M->bsr.mb = mb; M->bsr.bR = bR; M->bsr.bC = bC; M->bsr.nnzb = nnzb;
+    M->bsr.mb = mb; M->bsr.bR = bR; M->bsr.bC = bC; M->bsr.nnzb = nnzb; M->bsr.mfma = mfma;
     M->slots = nnzb * bR * bC;
     M->dev_bytes = M->bsr.brp.bytes + M->bsr.bcol.bytes + M->bsr.data.bytes;
     int64_t sV = (int64_t)value_bytes(vt), sX = (int64_t)xy_bytes(vt), sI = index_type == SPOPS_INDEX_I32 ? 4 : 8;
@@ -1468,7 +1581,9 @@ extern "C" int spops_matrix_info(spops_matrix A, int64_t info[16]) {
       if (A->sweep.H > 0) { info[8] = 0; info[9] = A->sweep.ngroups; info[12] = 0; }
       if (A->format == SPOPS_FORMAT_ELL) info[13] = A->ell_width;
     }
-    if (A->format == SPOPS_FORMAT_BSR) { info[12] = A->bsr.bR; info[13] = A->bsr.bC; info[14] = A->bsr.nnzb; }
+    if (A->format == SPOPS_FORMAT_BSR) {
+      info[12] = A->bsr.bR; info[13] = A->bsr.bC; info[14] = A->bsr.nnzb; info[9] = A->bsr.mfma ? 1 : 0;
+    }
     return SPOPS_SUCCESS;
   });
 }
@@ -1492,6 +1607,10 @@ extern "C" int spops_set_tuning(int key, int64_t value, int64_t *old) {
                     "swept row block size must be in [1, %d]", kSweepMaxRows);
       if (old) *old = g_sweep_max_rows;
       g_sweep_max_rows = value;
+    } else if (key == SPOPS_TUNE_BSR_MFMA) {
+      SPOPS_REQUIRE(value >= 0 && value <= 2, SPOPS_ERROR_INVALID_VALUE, "BSR MFMA mode must be 0, 1 or 2");
+      if (old) *old = g_bsr_mfma;
+      g_bsr_mfma = value;
     } else if (key == SPOPS_TUNE_DENSE_PRODUCTS) {
       SPOPS_REQUIRE(value >= 1, SPOPS_ERROR_INVALID_VALUE, "dense product threshold must be >= 1");
       if (old) *old = g_dense_products;
@@ -1532,8 +1651,7 @@ extern "C" int spops_bsr_create_from_blocks(spops_matrix *A, int64_t n_rows, int
     check_vt(in_vt, vt);
     SPOPS_REQUIRE(bR > 0 && bC > 0 && mb == (n_rows + bR - 1) / bR, SPOPS_ERROR_INVALID_VALUE,
                   "mb must be ceil(n_rows / bR)");
-    SPOPS_REQUIRE(bR * bC <= kBsrMaxBlock && bR <= kWave, SPOPS_ERROR_LIMIT, "block of %lld x %lld exceeds %d entries",
-                  (long long)bR, (long long)bC, kBsrMaxBlock);
+    const bool mfma = bsr_use_mfma(bR, bC);
     SPOPS_REQUIRE((uint64_t)n_cols * 8 <= kMaxBufBytes, SPOPS_ERROR_LIMIT, "x must fit one 2 GiB buffer resource");
     std::vector<int64_t> rp = host_index(row_ptr, mb + 1, index_type);
     std::vector<int64_t> bc = host_index(block_colindex, nnzb, index_type);
@@ -1556,7 +1674,7 @@ extern "C" int spops_bsr_create_from_blocks(spops_matrix *A, int64_t n_rows, int
     M->bsr.brp.upload(std::vector<int32_t>(rp.begin(), rp.end()));
     M->bsr.bcol.upload(std::vector<int32_t>(bc.begin(), bc.end()));
     M->bsr.data.upload(v);
-    M->bsr.mb = mb; M->bsr.bR = bR; M->bsr.bC = bC; M->bsr.nnzb = nnzb;
+    M->bsr.mb = mb; M->bsr.bR = bR; M->bsr.bC = bC; M->bsr.nnzb = nnzb; M->bsr.mfma = mfma;
     M->slots = nnzb * bR * bC;
     M->dev_bytes = M->bsr.brp.bytes + M->bsr.bcol.bytes + M->bsr.data.bytes;
     int64_t sV = (int64_t)value_bytes(vt), sX = (int64_t)xy_bytes(vt), sI = index_type == SPOPS_INDEX_I32 ? 4 : 8;

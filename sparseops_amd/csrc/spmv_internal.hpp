@@ -12,7 +12,9 @@ constexpr int64_t kCsrLongRow = 2048;  // longer rows: one workgroup each
 constexpr int64_t kSpmmLongRow = 256;  // SpMM: longer rows are summed in segments
 constexpr int64_t kSpmmSegment = 512;  // SpMM: entries per segment of a long row
 constexpr int64_t kSpmmWindow = 4096;  // SpMM: short rows are ordered by length inside windows of this many rows
-constexpr int kBsrMaxBlock = 256;      // bR * bC limit of the wave-per-block-row kernel
+constexpr int kBsrMaxBlock = 256;      // bR * bC limit of the wave-per-block-row VALU kernel
+// MFMA BSR kernel: blocks with bR >= 8, bC >= 8, bC % 4 == 0, at most these sizes
+constexpr int kBsrMfmaMinDim = 8, kBsrMfmaMaxRows = 64, kBsrMfmaMaxBlock = 4096;
 constexpr int64_t kRowSegment = (int64_t)1 << 24;   // entries per buffer resource of a long row
 
 // Column-sliced stream layout (SELL-C-sigma, SELL-C-R, ELL)
@@ -97,6 +99,7 @@ struct SweepDev {
 struct BsrDev {
   DevBuf brp, bcol, data;
   int64_t mb = 0, bR = 0, bC = 0, nnzb = 0;
+  bool mfma = false;          // the MFMA kernel (dense blocks of at least 8 x 8)
 };
 
 }  // namespace spops
@@ -119,6 +122,12 @@ extern int64_t g_sweep_mode;
 // swept layout: workgroups aimed for per launch (H = this / fewest row blocks, 1..kSweepMaxSplit)
 extern int64_t g_sweep_workgroups;
 extern int64_t g_sweep_max_rows;     // swept layout: rows of the largest row block
+// BSR matrices created from now on with dense blocks of at least 8 x 8 (bC % 4 == 0) on the
+// MFMA kernel: 0 never, 1 (default) when the VALU kernel cannot hold the block (more than
+// kBsrMaxBlock entries), 2 always
+extern int64_t g_bsr_mfma;
+// the kernel a BSR matrix of bR x bC blocks gets (throws SPOPS_ERROR_LIMIT if none fits)
+bool bsr_use_mfma(int64_t bR, int64_t bC);
 // The reference's ave_nnz_each_row, Gini_coeff_row and nnz_closed2diag_ratio of a host CSR
 // (src/sparse_features.cpp:609-760): the features the batched SpMM autotuner classifies by.
 void row_features_host(const int64_t *rowptr, const int64_t *col, int64_t n_rows, int64_t n_cols, double out[3]);

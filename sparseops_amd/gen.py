@@ -110,6 +110,23 @@ def powerlaw(n: int = 1 << 21, alpha: float = 2.1, lmin: int = 3, lmax: int = 81
     return Csr(n, n, rowptr.astype(index_dtype), cols.astype(index_dtype), vals)
 
 
+def block_sparse(n: int = 1 << 18, b: int = 16, blocks_per_row: int = 8, seed: int = 0) -> Csr:
+    """Square matrix of dense b x b blocks: every block row holds blocks_per_row distinct block
+    columns, uniform without replacement; N(0,1) values; rows sorted by column."""
+    rng = np.random.default_rng(seed)
+    nb = -(-n // b)
+    bcols = np.stack([np.sort(rng.choice(nb, size=min(blocks_per_row, nb), replace=False)) for _ in range(nb)])
+    k = bcols.shape[1]
+    cols = (bcols[:, None, :, None] * b + np.arange(b)[None, None, None, :]).repeat(b, axis=1)
+    cols = cols.reshape(nb * b, k * b)[:n]
+    keep = cols < n
+    lens = keep.sum(axis=1)
+    rowptr = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(lens, out=rowptr[1:])
+    col = cols[keep].astype(np.int32)
+    return Csr(n, n, rowptr.astype(np.int32), col, rng.standard_normal(col.size))
+
+
 def rmat(scale: int = 20, edge_factor: int = 16, a=0.57, b=0.19, c=0.19, d=0.05,
          seed: int = 0, index_dtype=np.int64) -> Csr:
     """R-MAT graph, duplicate edges merged, self loops kept, values U(0,1]."""

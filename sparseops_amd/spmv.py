@@ -249,6 +249,15 @@ def sweep_max_rows(n: int) -> tuning:
     return tuning(L.TUNE_SWEEP_MAX_ROWS, n)
 
 
+BSR_MFMA = {"never": 0, "large": 1, "always": 2}
+
+
+def bsr_mfma(mode: str) -> tuning:
+    """Which BSR matrices with dense blocks of at least 8 x 8 run on the matrix cores: "never",
+    "large" (default: blocks past the VALU kernel's 256 entries) or "always"."""
+    return tuning(L.TUNE_BSR_MFMA, BSR_MFMA[mode])
+
+
 def sweep_workgroups(n: int) -> tuning:
     """Workgroups a swept-layout SpMV aims for (default 256); row blocks are shared by up to 16."""
     return tuning(L.TUNE_SWEEP_WORKGROUPS, n)

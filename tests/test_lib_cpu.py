@@ -55,8 +55,8 @@ def test_invalid_arguments_map_to_status_codes():
                                 L.VALUE_F64, L.VALUE_F64)
     assert st == 1
     st = L.lib.spops_bsr_create(ctypes.byref(h), 2, 2, 2, L.INDEX_I64, rp.ctypes.data, rp.ctypes.data,
-                                v.ctypes.data, L.VALUE_F64, L.VALUE_F64, 32, 32)
-    assert st == 5
+                                v.ctypes.data, L.VALUE_F64, L.VALUE_F64, 80, 80)
+    assert st == 5                       # neither the VALU (<= 256 entries) nor the MFMA kernel (<= 64 rows)
     assert L.lib.spops_spmv(None, 1.0, None, 0.0, None, None) == 1
     assert L.lib.spops_hierarchical_clustering(3, 0, None, None, 0, None, None, None, 0, 8, None, None, None) == 1
 

@@ -314,3 +314,40 @@ def test_swept_auto_selection(dev):
     assert B.sweep is None and B.slices == 1
     assert R.sweep is not None and R.slices == 0
     _check(band, B, uniform_x(n, 9), _y0(n), "f32", dev=dev)
+
+
+@pytest.mark.parametrize("vt", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("bR,bC", [(8, 8), (16, 16), (16, 8), (8, 12), (24, 16), (40, 8), (32, 32), (64, 64)])
+@pytest.mark.parametrize("name", ["edge", "pl"])
+def test_bsr_mfma(mats, name, vt, bR, bC, dev):
+    """Dense blocks of at least 8 x 8 run on the matrix cores (v_mfma_f64_16x16x4f64); partial
+    16-row tiles, several column groups, blocks past the VALU kernel's 256 entries, and the last
+    block column past n_cols (x there must read 0)."""
+    A = mats[name]
+    with S.bsr_mfma("always"):
+        M = S.csr_to_bsr(A, bR, bC, vt)
+    assert M.info[9] == 1                       # the MFMA kernel runs it
+    _check(A, M, uniform_x(A.n_cols, 6), _y0(A.n_rows), vt, dev=dev)
+    y = _run(M, uniform_x(A.n_cols, 6), np.full(A.n_rows, np.nan), dev, 1.0, 0.0)
+    assert np.all(np.isfinite(y))
+
+
+@pytest.mark.parametrize("vt", ["f64", "f32"])
+@pytest.mark.parametrize("bR,bC", [(8, 8), (16, 16)])
+def test_bsr_mfma_matches_valu(mats, vt, bR, bC, dev):
+    """The same blocks on the VALU kernel and on the matrix cores agree within the sum
+    tolerance, and the switch keeps shapes the VALU kernel cannot hold on the matrix cores."""
+    A = mats["pl"]
+    x = uniform_x(A.n_cols, 12)
+    V = S.csr_to_bsr(A, bR, bC, vt)                 # default: blocks the VALU kernel holds stay there
+    with S.bsr_mfma("always"):
+        M = S.csr_to_bsr(A, bR, bC, vt)
+    assert V.info[9] == 0 and M.info[9] == 1
+    assert S.csr_to_bsr(A, 32, 16, vt).info[9] == 1  # 512 entries: past the VALU kernel, on the matrix cores
+    yv = _check(A, V, x, _y0(A.n_rows), vt, dev=dev)
+    ym = _check(A, M, x, _y0(A.n_rows), vt, dev=dev)
+    ref, s = orc.spmv_exact(A, x, ALPHA, BETA, _y0(A.n_rows))
+    tol = 2e-13 if vt == "f64" else 8e-7
+    assert np.all(np.abs(yv - ym) <= tol * (ALPHA * s + BETA * np.abs(_y0(A.n_rows)) + 1e-300))
+    with S.bsr_mfma("never"), pytest.raises(RuntimeError):
+        S.csr_to_bsr(A, 32, 32, vt)                 # 1024 entries: too large for the VALU kernel
