@@ -426,7 +426,14 @@ __global__ __launch_bounds__(kBlock) void k_emit_large(const int32_t *__restrict
   }
 }
 
-template <typename V>
+// CR: A values of a cluster column held in registers per pass over its B row entries (4 for
+// clusters of at most 4 rows, kMaxClusterRows otherwise); the products of an entry stop at
+// the cluster's size.  With B row cursors, lane group g of a wave keeps, for the first row
+// it handles (all of its rows when one round of the workgroup covers the cluster), that
+// row's next GL entries and, when CR <= 4, its A values in registers from one chunk to the
+// next: the entries are loaded when the row leaves a chunk, so the next chunk starts without
+// a round trip to memory.
+template <typename V, int CR>
 __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__restrict__ list, const int64_t *__restrict__ arp,
                                                            const int64_t *__restrict__ arpv, const int32_t *__restrict__ csz_a,
                                                            const int32_t *__restrict__ acol, const V *__restrict__ aval,
@@ -434,9 +441,12 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
                                                            const V *__restrict__ bval, const int64_t *__restrict__ crp,
                                                            const int64_t *__restrict__ crpv, const int32_t *__restrict__ ccol,
                                                            V *__restrict__ cval, bool b32) {
+  // clusters of at most 4 rows (mostly single rows) take twice the hash slots, so a chunk holds
+  // up to 1024 output columns: half the chunks, each visiting every row of the cluster once
+  constexpr int HS = CR <= 4 ? 2 * kLargeHash : kLargeHash;
   __shared__ V acc[kLargeAccEntries];
-  __shared__ int32_t hkey[kLargeHash];
-  __shared__ int16_t hval[kLargeHash];
+  __shared__ int32_t hkey[HS];
+  __shared__ int16_t hval[HS];
   // per A column (when the cluster has at most kLargeCursors and B's offsets fit 32 bits):
   // the next unread entry of its B row, the row's end, and the column at that entry
   // (INT32_MAX: the row is done), so a chunk reads only B entries and skips rows at once
@@ -448,7 +458,7 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
   const int csz = csz_a[c];
   const int64_t a0 = arp[c], a1 = arp[c + 1], av0 = arpv[c];
   const int64_t c0 = crp[c], nzc = crp[c + 1] - c0, cv0 = crpv[c];
-  const int M = min(kLargeAccEntries / csz, kLargeHash / 2);
+  const int M = min(kLargeAccEntries / csz, HS / 2);
   const bool cursors = a1 - a0 <= kLargeCursors && b32;
   if (cursors)
     for (int64_t t = a0 + threadIdx.x; t < a1; t += kLargeBlock) {
@@ -459,22 +469,39 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
       nextcol[t - a0] = kb < ke ? bcol[kb] : INT32_MAX;
     }
   const int lane = lane_id();
+  // the prefetch slot (see above): entries from position pf_k (-1: none), A values once loaded
+  constexpr bool kCacheA = CR <= 4;           // A values kept from chunk to chunk (registers)
+  constexpr int CA = kCacheA ? CR : 1;
+  int32_t pf_key = INT32_MAX, pf_k = -1;
+  V pf_bv = V(0);
+  V pf_a[CA];
+  bool pf_has_a = false;
+#pragma unroll
+  for (int l = 0; l < CA; ++l) pf_a[l] = V(0);
+  // lanes per B row: about the entries a row brings to one chunk (each output column takes
+  // ~1.2 B entries, shared by the cluster's rows: 1.2 M / rows), a power of two in
+  // [kLargeGroupLanes, 64].  Each step over a row is a dependent round trip to memory, so a
+  // group as wide as the row's part of the chunk takes it in one step, and narrow groups let
+  // one round of the workgroup take more rows when there are many.
+  int GL = kLargeGroupLanes;
+  while (GL < kWave && (int64_t)GL * 5 * (a1 - a0) < (int64_t)6 * M) GL <<= 1;
+  GL = __builtin_amdgcn_readfirstlane(GL);
+  const int G = kWave / GL;
+  const int64_t first_base = a0 + (int64_t)wave_id() * G;
   for (int64_t j0 = 0; j0 < nzc; j0 += M) {
     const int m = (int)min((int64_t)M, nzc - j0);
     const int32_t lo = ccol[c0 + j0], hi = ccol[c0 + j0 + m - 1];
-    for (int i = threadIdx.x; i < kLargeHash; i += kLargeBlock) hkey[i] = -1;
+    for (int i = threadIdx.x; i < HS; i += kLargeBlock) hkey[i] = -1;
     for (int i = threadIdx.x; i < m * csz; i += kLargeBlock) acc[i] = V(0);
     __syncthreads();
     for (int i = threadIdx.x; i < m; i += kLargeBlock) {
       const int32_t key = ccol[c0 + j0 + i];
-      uint32_t h = hslot(key, kLargeHash - 1);
-      while (atomicCAS(&hkey[h], -1, key) != -1) h = (h + 1) & (kLargeHash - 1);
+      uint32_t h = hslot(key, HS - 1);
+      while (atomicCAS(&hkey[h], -1, key) != -1) h = (h + 1) & (HS - 1);
       hval[h] = (int16_t)i;
     }
     __syncthreads();
-    // lane groups of kLargeGroupLanes: each group scans one B row (most rows contribute a few
-    // entries to a chunk, so a whole wave per row would idle)
-    constexpr int GL = kLargeGroupLanes, G = kWave / GL;
+    // lane groups of GL lanes, each scanning one B row
     const int grp = lane / GL, sub = lane % GL;
     const uint64_t gmask = ((GL == 64) ? ~0ull : ((1ull << GL) - 1)) << (grp * GL);
     for (int64_t base = a0 + (int64_t)wave_id() * G; base < a1; base += (int64_t)W * G) {
@@ -499,30 +526,56 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
       }
       const int64_t kstart = k;
       int32_t nxt = INT32_MAX;
-      for (int l0 = 0; l0 < csz; l0 += kMaxClusterRows) {      // one pass unless csz > 16
-        V ar[kMaxClusterRows];
+      const bool slot = cursors && base == first_base;        // wave-uniform
+      for (int l0 = 0; l0 < csz; l0 += CR) {                  // one pass unless csz > CR
+        const int lmax = min(csz - l0, CR);
+        V ar[CR];
+        if (kCacheA && slot && pf_has_a && csz <= CR) {
 #pragma unroll
-        for (int l = 0; l < kMaxClusterRows; ++l) ar[l] = (live && l0 + l < csz) ? a[l0 + l] : V(0);
+          for (int l = 0; l < CA; ++l) ar[l] = live ? pf_a[l] : V(0);
+        } else {
+#pragma unroll
+          for (int l = 0; l < CR; ++l) ar[l] = (live && l < lmax) ? a[l0 + l] : V(0);
+          if (kCacheA && slot && live && csz <= CR) {
+#pragma unroll
+            for (int l = 0; l < CA; ++l) pf_a[l] = ar[l];
+            pf_has_a = true;
+          }
+        }
         k = kstart;
         bool going = live;
+        bool first = true;
         while (__ballot(going)) {
           const int64_t kk = k + sub;
           const bool have = going && kk < ke;
-          const int32_t key = have ? bcol[kk] : INT32_MAX;
-          const V bv = have ? bval[kk] : V(0);
+          const bool use_pf = slot && first && pf_k == (int32_t)k;
+          int32_t key = INT32_MAX;
+          V bv = V(0);
+          if (have) {
+            if (use_pf) {
+              key = pf_key;
+              bv = pf_bv;
+            } else {
+              key = bcol[kk];
+              bv = bval[kk];
+            }
+          }
+          first = false;
           const bool in = going && key <= hi;
           const int n = __builtin_popcountll(__ballot(in) & gmask);
           int h = -1;
           if (in) {                            // every such column is one of the chunk's
-            uint32_t p = hslot(key, kLargeHash - 1);
-            for (int probe = 0; probe < kLargeHash; ++probe, p = (p + 1) & (kLargeHash - 1))
+            uint32_t p = hslot(key, HS - 1);
+            for (int probe = 0; probe < HS; ++probe, p = (p + 1) & (HS - 1))
               if (hkey[p] == key) { h = (int)p; break; }
           }
           if (h >= 0) {
             V *dst = acc + (int)hval[h] * csz + l0;
 #pragma unroll
-            for (int l = 0; l < kMaxClusterRows; ++l)
+            for (int l = 0; l < CR; ++l) {
+              if (l >= lmax) break;
               if (ar[l] != V(0)) atomicAdd(&dst[l], ar[l] * bv);
+            }
           }
           const int32_t first_out = __shfl(key, grp * GL + (n < GL ? n : 0), kWave);
           if (going) {
@@ -537,6 +590,17 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
       if (cursors && live && sub == 0) {
         rpos[t - a0] = (int32_t)k;
         nextcol[t - a0] = nxt;
+      }
+      if (slot && live) {                    // the row's entries for the next chunk it takes part in
+        pf_k = nxt == INT32_MAX ? -1 : (int32_t)k;
+        const int64_t kk = k + sub;
+        if (pf_k >= 0 && kk < ke) {
+          pf_key = bcol[kk];
+          pf_bv = bval[kk];
+        } else {
+          pf_key = INT32_MAX;
+          pf_bv = V(0);
+        }
       }
     }
     __syncthreads();
@@ -973,7 +1037,7 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
   Timer tm(s);
   // the panel's clusters with work, in the plan's bin order
   std::vector<int32_t> psmall, plarge, pregular, pdense;
-  DevBuf dsmall, dlarge, dregular;
+  DevBuf dsmall, dlarge;
   if (!whole) {
     for (int32_t c : P.small) if (c >= c0 && c < c1) psmall.push_back(c);
     for (int32_t c : P.large) if (c >= c0 && c < c1) plarge.push_back(c);
@@ -986,9 +1050,12 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
   const int32_t *llarge = whole ? P.dlarge.as<int32_t>() : dlarge.as<int32_t>();
   for (int32_t c : large) (P.dense[c] ? pdense : pregular).push_back(c);
   const bool split = !pdense.empty();
-  if (split) dregular.upload(pregular);
   const std::vector<int32_t> &regular = split ? pregular : large;
-  const int32_t *lregular = split ? dregular.as<int32_t>() : llarge;
+  std::vector<int32_t> regular4, regular16;
+  for (int32_t c : regular) (P.hcsz[c] <= 4 ? regular4 : regular16).push_back(c);
+  DevBuf dregular4, dregular16;
+  dregular4.upload(regular4);
+  dregular16.upload(regular16);
   int64_t nnzc = 0, nnzv = 0, csr_rows = 0;
   for (int64_t c = c0; c < c1; ++c) {
     nnzc += P.hnz[c];
@@ -1051,12 +1118,17 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
                          A.cluster_sz.as<int32_t>(), A.colids.as<int32_t>(), A.values.as<V>(), P.brp64,
                          P.bcol32.as<int32_t>(), static_cast<const V *>(P.bval), crp, crpv, C.colids.as<int32_t>(),
                          C.values.as<V>());
-    if (!regular.empty())
-      hipLaunchKernelGGL(k_num_large<V>, dim3((unsigned)regular.size()), dim3(kLargeBlock), 0, s, lregular,
-                         A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(), A.cluster_sz.as<int32_t>(),
-                         A.colids.as<int32_t>(), A.values.as<V>(), P.brp64, P.bcol32.as<int32_t>(),
-                         static_cast<const V *>(P.bval), crp, crpv, C.colids.as<int32_t>(), C.values.as<V>(),
-                         P.b_nnz < (int64_t)INT32_MAX);
+    // clusters of at most 4 rows and the others: kernels holding 4 / kMaxClusterRows A values
+    auto large_kernel = [&](auto kern, const std::vector<int32_t> &ids, const int32_t *dids) {
+      if (!ids.empty())
+        hipLaunchKernelGGL(kern, dim3((unsigned)ids.size()), dim3(kLargeBlock), 0, s, dids,
+                           A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(), A.cluster_sz.as<int32_t>(),
+                           A.colids.as<int32_t>(), A.values.as<V>(), P.brp64, P.bcol32.as<int32_t>(),
+                           static_cast<const V *>(P.bval), crp, crpv, C.colids.as<int32_t>(), C.values.as<V>(),
+                           P.b_nnz < (int64_t)INT32_MAX);
+    };
+    large_kernel(k_num_large<V, 4>, regular4, dregular4.as<int32_t>());
+    large_kernel(k_num_large<V, kMaxClusterRows>, regular16, dregular16.as<int32_t>());
     // dense clusters in batches whose accumulators fit kDenseScratch
     size_t i = 0;
     while (i < pdense.size()) {

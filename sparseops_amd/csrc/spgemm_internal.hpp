@@ -13,7 +13,7 @@ constexpr int kLargeBlock = 512;                 // threads of a large-cluster n
 constexpr int kLargeAccEntries = 2304;           // LDS accumulator entries of a numeric chunk
 constexpr int kLargeHash = 1024;                 // LDS hash slots of a chunk's columns
 constexpr int kLargeCursors = 1024;              // A columns whose B-row position is kept in LDS
-constexpr int kLargeGroupLanes = 8;              // lanes scanning one B row in the numeric pass
+constexpr int kLargeGroupLanes = 4;              // fewest lanes scanning one B row in the numeric pass
 constexpr int kMaxClusterRows = 16;              // A values of a column held in registers per pass
 constexpr int kSpgemmStats = 16;
 constexpr int64_t kDenseScratch = (int64_t)1 << 30;    // bytes of dense accumulators live at once
