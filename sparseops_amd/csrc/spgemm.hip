@@ -312,20 +312,31 @@ __device__ __forceinline__ void row_range(const int32_t *__restrict__ bcol, int6
 __device__ __forceinline__ void tile_bits(uint32_t *bm, int64_t t0, int64_t t1, const int32_t *__restrict__ acol,
                                           const int64_t *__restrict__ brp, const int32_t *__restrict__ bcol, int32_t lo,
                                           int32_t hi, bool whole_rows) {
-  // groups of 16 lanes, one A column (B row) each; a row of at most 64 entries is scanned
-  // whole with a range check, a longer one is cut to [lo, hi] by binary search first
-  constexpr int G = 16, NG = kBlock / G;
+  // groups of G threads, one A column (B row) each: G = 16, or more when the cluster has
+  // fewer than 16 columns (up to the whole workgroup for one column), so that long B rows of
+  // a narrow cluster are read by many lanes.  A row of at most 64 entries is scanned whole
+  // with a range check, a longer one is cut to [lo, hi] by binary search first.  Each thread
+  // issues four independent loads per step.
+  const int64_t na = t1 - t0;
+  int G = 16;
+  while (G < kBlock && (int64_t)(kBlock / G) > 2 * na) G <<= 1;
+  const int NG = kBlock / G;
   const int grp = (int)threadIdx.x / G, sub = (int)threadIdx.x % G;
+  constexpr int U = 4;
   for (int64_t t = t0 + grp; t < t1; t += NG) {
     const int32_t b = acol[t];
     int64_t k0 = brp[b], k1 = brp[b + 1];
-    if (!whole_rows && k1 - k0 > 4 * G) row_range(bcol, k0, k1, lo, hi);
-    for (int64_t k = k0 + sub; k < k1; k += G) {
-      const int32_t col = bcol[k];
-      if (whole_rows || (col >= lo && col <= hi)) {
-        const int32_t off = col - lo;
-        atomicOr(&bm[off >> 5], 1u << (off & 31));
-      }
+    if (!whole_rows && k1 - k0 > 64) row_range(bcol, k0, k1, lo, hi);
+    for (int64_t k = k0 + sub; k < k1; k += (int64_t)U * G) {
+      int32_t col[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) col[u] = k + u * G < k1 ? bcol[k + u * G] : -1;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (col[u] >= 0 && (whole_rows || (col[u] >= lo && col[u] <= hi))) {
+          const int32_t off = col[u] - lo;
+          atomicOr(&bm[off >> 5], 1u << (off & 31));
+        }
     }
   }
 }
