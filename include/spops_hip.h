@@ -229,13 +229,17 @@ int spops_matrix_info(spops_matrix A, int64_t info[16]);
  *   FP64 products and sums; dense blocks of at least 8 x 8 with bC % 4 == 0, bR <= 64,
  *   bR * bC <= 4096): 0 none, 1 (default) those whose blocks exceed the VALU kernel's 256
  *   entries, 2 all such blocks.  With one right-hand side the VALU kernel is the faster of
- *   the two where both apply.  Applies to matrices created afterwards. */
+ *   the two where both apply.  Applies to matrices created afterwards.
+ *   SPOPS_TUNE_TASK_PRODUCTS: a large LeSpGEMM cluster's numeric work is split over
+ *   workgroups by ranges of its output-column chunks, about this many products per workgroup
+ *   (default 2^17); applies to numeric calls made afterwards. */
 #define SPOPS_TUNE_SLICE_BYTES 0
 #define SPOPS_TUNE_DENSE_PRODUCTS 1
 #define SPOPS_TUNE_SPMV_LAYOUT 2
 #define SPOPS_TUNE_SWEEP_WORKGROUPS 3
 #define SPOPS_TUNE_SWEEP_MAX_ROWS 4
 #define SPOPS_TUNE_BSR_MFMA 5
+#define SPOPS_TUNE_TASK_PRODUCTS 6
 int spops_set_tuning(int key, int64_t value, int64_t *old);
 
 /* Device copies of the layout arrays a format conversion produced, for parity checks:

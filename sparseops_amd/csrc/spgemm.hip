@@ -38,6 +38,8 @@
 
 namespace spops {
 
+int64_t g_task_products = kLargeTaskProducts;
+
 namespace {
 
 inline unsigned nblk(int64_t n, int per) { return grid_blocks(n, per); }
@@ -444,17 +446,18 @@ __global__ __launch_bounds__(kBlock) void k_emit_large(const int32_t *__restrict
 // row's next GL entries and, when CR <= 4, its A values in registers from one chunk to the
 // next: the entries are loaded when the row leaves a chunk, so the next chunk starts without
 // a round trip to memory.
+// A task is {cluster, first chunk, end chunk}: a cluster with much work is split over several
+// workgroups by ranges of its chunks (each task's row cursors start at its first chunk's
+// column, one binary search per row), so that one cluster does not hold the kernel's tail.
 template <typename V, int CR>
-__global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__restrict__ list, const int64_t *__restrict__ arp,
+__global__ __launch_bounds__(kLargeBlock) void k_num_large(const int4 *__restrict__ tasks, const int64_t *__restrict__ arp,
                                                            const int64_t *__restrict__ arpv, const int32_t *__restrict__ csz_a,
                                                            const int32_t *__restrict__ acol, const V *__restrict__ aval,
                                                            const int64_t *__restrict__ brp, const int32_t *__restrict__ bcol,
                                                            const V *__restrict__ bval, const int64_t *__restrict__ crp,
                                                            const int64_t *__restrict__ crpv, const int32_t *__restrict__ ccol,
                                                            V *__restrict__ cval, bool b32) {
-  // clusters of at most 4 rows (mostly single rows) take twice the hash slots, so a chunk holds
-  // up to 1024 output columns: half the chunks, each visiting every row of the cluster once
-  constexpr int HS = CR <= 4 ? 2 * kLargeHash : kLargeHash;
+  constexpr int HS = large_hash_slots(CR);
   __shared__ V acc[kLargeAccEntries];
   __shared__ int32_t hkey[HS];
   __shared__ int16_t hval[HS];
@@ -465,20 +468,32 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
   __shared__ int32_t rend[kLargeCursors];
   __shared__ int32_t nextcol[kLargeCursors];
   constexpr int W = kLargeBlock / kWave;
-  const int64_t c = list[blockIdx.x];
+  const int4 tk = tasks[blockIdx.x];
+  const int64_t c = tk.x;
   const int csz = csz_a[c];
   const int64_t a0 = arp[c], a1 = arp[c + 1], av0 = arpv[c];
-  const int64_t c0 = crp[c], nzc = crp[c + 1] - c0, cv0 = crpv[c];
-  const int M = min(kLargeAccEntries / csz, HS / 2);
+  const int64_t c0 = crp[c], cv0 = crpv[c];
+  const int M = large_chunk_cols(csz, CR);
+  const int64_t jbeg = (int64_t)tk.y * M, jend = min((int64_t)tk.z * M, crp[c + 1] - c0);
   const bool cursors = a1 - a0 <= kLargeCursors && b32;
-  if (cursors)
+  if (cursors) {
+    const int32_t lo0 = jbeg > 0 && jbeg < jend ? ccol[c0 + jbeg] : INT32_MIN;
     for (int64_t t = a0 + threadIdx.x; t < a1; t += kLargeBlock) {
       const int32_t b = acol[t];
-      const int32_t kb = (int32_t)brp[b], ke = (int32_t)brp[b + 1];
+      int32_t kb = (int32_t)brp[b];
+      const int32_t ke = (int32_t)brp[b + 1];
+      if (lo0 != INT32_MIN) {                 // first entry of the row at or past the task's first column
+        int32_t hi_ = ke;
+        while (kb < hi_) {
+          const int32_t mid = kb + ((hi_ - kb) >> 1);
+          if (bcol[mid] < lo0) kb = mid + 1; else hi_ = mid;
+        }
+      }
       rpos[t - a0] = kb;
       rend[t - a0] = ke;
       nextcol[t - a0] = kb < ke ? bcol[kb] : INT32_MAX;
     }
+  }
   const int lane = lane_id();
   // the prefetch slot (see above): entries from position pf_k (-1: none), A values once loaded
   constexpr bool kCacheA = CR <= 4;           // A values kept from chunk to chunk (registers)
@@ -499,8 +514,8 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int32_t *__rest
   GL = __builtin_amdgcn_readfirstlane(GL);
   const int G = kWave / GL;
   const int64_t first_base = a0 + (int64_t)wave_id() * G;
-  for (int64_t j0 = 0; j0 < nzc; j0 += M) {
-    const int m = (int)min((int64_t)M, nzc - j0);
+  for (int64_t j0 = jbeg; j0 < jend; j0 += M) {
+    const int m = (int)min((int64_t)M, jend - j0);
     const int32_t lo = ccol[c0 + j0], hi = ccol[c0 + j0 + m - 1];
     for (int i = threadIdx.x; i < HS; i += kLargeBlock) hkey[i] = -1;
     for (int i = threadIdx.x; i < m * csz; i += kLargeBlock) acc[i] = V(0);
@@ -1062,8 +1077,17 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
   for (int32_t c : large) (P.dense[c] ? pdense : pregular).push_back(c);
   const bool split = !pdense.empty();
   const std::vector<int32_t> &regular = split ? pregular : large;
-  std::vector<int32_t> regular4, regular16;
-  for (int32_t c : regular) (P.hcsz[c] <= 4 ? regular4 : regular16).push_back(c);
+  // tasks of the large-cluster numeric kernels: {cluster, first chunk, end chunk, 0}, about
+  // g_task_products products each (clusters in decreasing work order)
+  std::vector<int4> regular4, regular16;
+  for (int32_t c : regular) {
+    const int cr = P.hcsz[c] <= 4 ? 4 : kMaxClusterRows;
+    const int64_t chunks = (P.hnz[c] + large_chunk_cols(P.hcsz[c], cr) - 1) / large_chunk_cols(P.hcsz[c], cr);
+    const int64_t pieces = std::max<int64_t>(1, std::min<int64_t>(chunks, P.hflop[c] / g_task_products));
+    for (int64_t i = 0; i < pieces; ++i)
+      (cr == 4 ? regular4 : regular16)
+          .push_back(make_int4(c, (int)(chunks * i / pieces), (int)(chunks * (i + 1) / pieces), 0));
+  }
   DevBuf dregular4, dregular16;
   dregular4.upload(regular4);
   dregular16.upload(regular16);
@@ -1130,7 +1154,7 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
                          P.bcol32.as<int32_t>(), static_cast<const V *>(P.bval), crp, crpv, C.colids.as<int32_t>(),
                          C.values.as<V>());
     // clusters of at most 4 rows and the others: kernels holding 4 / kMaxClusterRows A values
-    auto large_kernel = [&](auto kern, const std::vector<int32_t> &ids, const int32_t *dids) {
+    auto large_kernel = [&](auto kern, const std::vector<int4> &ids, const int4 *dids) {
       if (!ids.empty())
         hipLaunchKernelGGL(kern, dim3((unsigned)ids.size()), dim3(kLargeBlock), 0, s, dids,
                            A.rowptr.as<int64_t>(), A.rowptr_val.as<int64_t>(), A.cluster_sz.as<int32_t>(),
@@ -1138,8 +1162,8 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
                            static_cast<const V *>(P.bval), crp, crpv, C.colids.as<int32_t>(), C.values.as<V>(),
                            P.b_nnz < (int64_t)INT32_MAX);
     };
-    large_kernel(k_num_large<V, 4>, regular4, dregular4.as<int32_t>());
-    large_kernel(k_num_large<V, kMaxClusterRows>, regular16, dregular16.as<int32_t>());
+    large_kernel(k_num_large<V, 4>, regular4, dregular4.as<int4>());
+    large_kernel(k_num_large<V, kMaxClusterRows>, regular16, dregular16.as<int4>());
     // dense clusters in batches whose accumulators fit kDenseScratch
     size_t i = 0;
     while (i < pdense.size()) {

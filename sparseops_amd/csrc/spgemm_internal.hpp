@@ -14,6 +14,15 @@ constexpr int kLargeAccEntries = 2304;           // LDS accumulator entries of a
 constexpr int kLargeHash = 1024;                 // LDS hash slots of a chunk's columns
 constexpr int kLargeCursors = 1024;              // A columns whose B-row position is kept in LDS
 constexpr int kLargeGroupLanes = 4;              // fewest lanes scanning one B row in the numeric pass
+constexpr int64_t kLargeTaskProducts = (int64_t)1 << 17;   // default products per numeric task of a split cluster
+// hash slots of a numeric chunk: clusters of at most 4 rows (CR = 4; mostly single rows) take
+// twice as many, so a chunk holds up to 1024 output columns: half the chunks, each of which
+// visits every row of the cluster once
+constexpr int large_hash_slots(int cr) { return cr <= 4 ? 2 * kLargeHash : kLargeHash; }
+// output columns of a numeric chunk of a cluster of csz rows
+constexpr int large_chunk_cols(int csz, int cr) {
+  return kLargeAccEntries / csz < large_hash_slots(cr) / 2 ? kLargeAccEntries / csz : large_hash_slots(cr) / 2;
+}
 constexpr int kMaxClusterRows = 16;              // A values of a column held in registers per pass
 constexpr int kSpgemmStats = 16;
 constexpr int64_t kDenseScratch = (int64_t)1 << 30;    // bytes of dense accumulators live at once
@@ -60,6 +69,7 @@ struct spops_spgemm_plan_s {
 
 namespace spops {
 extern int64_t g_dense_products;   // products from which a cluster accumulates densely (tuning)
+extern int64_t g_task_products;    // products per numeric task of a large cluster (tuning)
 void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *brp64,
                         const int64_t *bcol64, const void *bval, spops_spgemm_plan_s &P, hipStream_t s);
 // C = rows [c0, c1) of the product as a cluster matrix of c1 - c0 clusters; C's buffers are

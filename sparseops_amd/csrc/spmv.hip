@@ -1611,6 +1611,10 @@ extern "C" int spops_set_tuning(int key, int64_t value, int64_t *old) {
       SPOPS_REQUIRE(value >= 0 && value <= 2, SPOPS_ERROR_INVALID_VALUE, "BSR MFMA mode must be 0, 1 or 2");
       if (old) *old = g_bsr_mfma;
       g_bsr_mfma = value;
+    } else if (key == SPOPS_TUNE_TASK_PRODUCTS) {
+      SPOPS_REQUIRE(value >= 1, SPOPS_ERROR_INVALID_VALUE, "task products must be >= 1");
+      if (old) *old = g_task_products;
+      g_task_products = value;
     } else if (key == SPOPS_TUNE_DENSE_PRODUCTS) {
       SPOPS_REQUIRE(value >= 1, SPOPS_ERROR_INVALID_VALUE, "dense product threshold must be >= 1");
       if (old) *old = g_dense_products;

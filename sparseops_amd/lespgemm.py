@@ -186,6 +186,23 @@ class dense_products:
         L.call("spops_set_tuning", L.TUNE_DENSE_PRODUCTS, self.old, None)
 
 
+class task_products:
+    """Context manager: numeric calls made inside split a large cluster's work over workgroups
+    of about `products` products each (library default 2^17)."""
+
+    def __init__(self, products: int):
+        self.products = int(products)
+
+    def __enter__(self):
+        old = c_i64()
+        L.call("spops_set_tuning", L.TUNE_TASK_PRODUCTS, self.products, ctypes.byref(old))
+        self.old = old.value
+        return self
+
+    def __exit__(self, *exc):
+        L.call("spops_set_tuning", L.TUNE_TASK_PRODUCTS, self.old, None)
+
+
 def _stats(stats):
     s = list(stats)
     return {"b_rows_sorted_copy": s[0], "nnzc": s[1], "nnzv": s[2], "small_clusters": s[3], "large_clusters": s[4],

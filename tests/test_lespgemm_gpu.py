@@ -281,3 +281,28 @@ def test_dense_accumulator_clusters(threshold, budget, dev):
         np.testing.assert_allclose(pval, val[rpv[c0]:rpv[c1]], rtol=1e-12, atol=0)
     assert dense > 10
     plan.close()
+
+
+@pytest.mark.parametrize("products,budget", [(64, 1 << 40), (1000, 1 << 18)])
+def test_split_cluster_tasks(products, budget, dev):
+    """Large clusters split over workgroups by ranges of their output-column chunks (each task
+    starts its B row cursors at its first chunk's column): the product equals the unsplit one,
+    structure exactly and values to 1e-12, whole or in panels."""
+    A = rmat(12, 16, seed=5)
+    dA = LG.DeviceCsr(A, dev)
+    sig = orc.minhash(A, 64, 7)
+    perm, off = orc.hc(A, *orc.lsh_pairs(sig, 64, 16), cluster_size=8)
+    Acl = LG.ClusterMatrix.build(dA, perm, off)
+    with LG.task_products(1 << 40):
+        ref = Acl.multiply(dA)
+    csz, rp, rpv, col, val = ref.download()
+    plan = LG.SpgemmPlan(Acl, dA)
+    P = None
+    with LG.task_products(products):
+        for c0, c1 in plan.panel_ranges(budget):
+            P = plan.numeric(c0, c1, P)
+            pcsz, prp, prpv, pcol, pval = P.download()
+            np.testing.assert_array_equal(prp, rp[c0:c1 + 1] - rp[c0])
+            np.testing.assert_array_equal(pcol, col[rp[c0]:rp[c1]])
+            np.testing.assert_allclose(pval, val[rpv[c0]:rpv[c1]], rtol=1e-12, atol=0)
+    plan.close()
