@@ -212,8 +212,8 @@ int spops_matrix_info(spops_matrix A, int64_t info[16]);
  *   created afterwards (default 2 MiB, one XCD's share of L2); the slice count is the
  *   smallest power of two <= 8 with n_cols * sizeof(x) / slices <= this.
  *   SPOPS_TUNE_DENSE_PRODUCTS: LeSpGEMM clusters with at least this many products (default
- *   2^20) accumulate in a dense block in device memory (many workgroups per cluster) instead
- *   of one workgroup's LDS chunks; applies to plans created afterwards.
+ *   2^22) accumulate in a dense block in device memory (many workgroups per cluster, atomic
+ *   adds) instead of LDS chunks; applies to plans created afterwards.
  *   SPOPS_TUNE_SPMV_LAYOUT: device layout of SELL-C-sigma / SELL-C-R / ELL matrices created
  *   afterwards: 0 (default) chosen per matrix, 1 the column-sliced stream, 2 the swept layout
  *   (blocks of consecutive rows summed in FP64 in LDS, each block's entries sorted by column).

@@ -396,46 +396,41 @@ __global__ __launch_bounds__(kBlock) void k_emit_large(const int32_t *__restrict
   const int32_t cmn = (int32_t)cmin[c], cmx = (int32_t)cmax[c];
   if (crp[c + 1] == crp[c]) return;
   int64_t out = crp[c];
-  // each thread owns kLargeTileBits / 32 / kBlock consecutive words of the tile
-  constexpr int WPT = kLargeTileBits / 32 / kBlock;
   for (int32_t lo = cmn; lo <= cmx; lo += kLargeTileBits) {
     const int32_t hi = (int32_t)min((int64_t)lo + kLargeTileBits - 1, (int64_t)cmx);
-    for (int w = threadIdx.x; w < kLargeTileBits / 32; w += kBlock) bm[w] = 0u;
+    const int nwords = ((hi - lo) >> 5) + 1;          // words that can hold bits of [lo, hi]
+    for (int w = threadIdx.x; w < nwords; w += kBlock) bm[w] = 0u;
     __syncthreads();
     tile_bits(bm, t0, t1, acol, brp, bcol, lo, hi, cmx - cmn < kLargeTileBits);
     __syncthreads();
-    uint32_t word[WPT];
-    int32_t mine = 0;
+    // rounds of kBlock consecutive words, one per thread: the columns of neighbouring threads
+    // are neighbours in C, so each store instruction of the wave covers a short range
+    for (int w0 = 0; w0 < nwords; w0 += kBlock) {
+      const int w = w0 + (int)threadIdx.x;
+      uint32_t bits = w < nwords ? bm[w] : 0u;
+      const int32_t mine = __popc(bits);
+      int32_t inc = mine;                     // block exclusive scan of the per-thread counts
 #pragma unroll
-    for (int i = 0; i < WPT; ++i) {
-      word[i] = bm[threadIdx.x * WPT + i];
-      mine += __popc(word[i]);
-    }
-    int32_t inc = mine;                       // block exclusive scan of per-thread counts
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const int32_t o = __shfl_up(inc, d, kWave);
-      if (lane_id() >= d) inc += o;
-    }
-    if (lane_id() == kWave - 1) wsum[wave_id()] = inc;
-    __syncthreads();
-    int32_t before = 0, total = 0;
-    for (int q = 0; q < kBlock / kWave; ++q) {
-      if (q < wave_id()) before += wsum[q];
-      total += wsum[q];
-    }
-    int64_t r = out + before + inc - mine;
-#pragma unroll
-    for (int i = 0; i < WPT; ++i) {
-      uint32_t bits = word[i];
+      for (int d = 1; d < kWave; d <<= 1) {
+        const int32_t o = __shfl_up(inc, d, kWave);
+        if (lane_id() >= d) inc += o;
+      }
+      if (lane_id() == kWave - 1) wsum[wave_id()] = inc;
+      __syncthreads();
+      int32_t before = 0, total = 0;
+      for (int q = 0; q < kBlock / kWave; ++q) {
+        if (q < wave_id()) before += wsum[q];
+        total += wsum[q];
+      }
+      int64_t r = out + before + inc - mine;
       while (bits) {
         const int b = __ffs(bits) - 1;
-        ccol[r++] = lo + (int32_t)(threadIdx.x * WPT + i) * 32 + b;
+        ccol[r++] = lo + w * 32 + b;
         bits &= bits - 1;
       }
+      out += total;
+      __syncthreads();
     }
-    out += total;
-    __syncthreads();
   }
 }
 

@@ -777,7 +777,7 @@ struct RowSource {
 };
 
 int64_t g_slice_bytes = kSliceBytesDefault;
-int64_t g_dense_products = (int64_t)1 << 20;
+int64_t g_dense_products = (int64_t)1 << 22;
 int64_t g_sweep_mode = 0;
 int64_t g_sweep_workgroups = kSweepTargetWorkgroups;
 int64_t g_sweep_max_rows = kSweepMaxRows;
