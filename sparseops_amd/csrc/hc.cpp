@@ -13,8 +13,9 @@
 // discovered pairs under the comparator (score, (i, j)).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <numeric>
 #include <queue>
-#include <unordered_map>
 
 #include "common.hpp"
 
@@ -61,13 +62,6 @@ struct Csr64 {
   const int64_t *rp, *col;
 };
 
-bool rows_sorted_unique(const Csr64 &A, int64_t n) {
-  for (int64_t r = 0; r < n; ++r)
-    for (int64_t j = A.rp[r] + 1; j < A.rp[r + 1]; ++j)
-      if (A.col[j - 1] >= A.col[j]) return false;
-  return true;
-}
-
 // include/spgemm_utility.h:402-429 (sorted rows) / :362-387 (general, set semantics for row a)
 double jaccard(const Csr64 &A, int64_t a, int64_t b, bool sorted_unique) {
   const int64_t la = A.rp[a + 1] - A.rp[a], lb = A.rp[b + 1] - A.rp[b];
@@ -105,6 +99,56 @@ struct ItemLess {
   bool operator()(const Item<V> &a, const Item<V> &b) const { return item_less(a, b); }
 };
 
+// Small open-addressing map from a score's bit pattern to a dense group id (grows by
+// doubling; the scores of one clustering take few distinct values).
+template <typename V>
+class ScoreTable {
+ public:
+  ScoreTable() { keys_.assign(cap_, kEmpty); vals_.assign(cap_, 0); }
+  int32_t find_or_add(V s, int32_t next_id) {
+    const uint64_t k = bits(s);
+    size_t h = slot(k);
+    for (;;) {
+      if (keys_[h] == k) return vals_[h];
+      if (keys_[h] == kEmpty) {
+        keys_[h] = k;
+        vals_[h] = next_id;
+        if (2 * ++n_ > cap_) grow();
+        return next_id;
+      }
+      h = (h + 1) & (cap_ - 1);
+    }
+  }
+
+ private:
+  static constexpr uint64_t kEmpty = ~0ull;             // no score has this bit pattern here (a NaN payload)
+  static uint64_t bits(V s) {
+    uint64_t b = 0;
+    std::memcpy(&b, &s, sizeof(V));
+    return b;
+  }
+  size_t slot(uint64_t k) const { return (size_t)((k * 0x9e3779b97f4a7c15ull) >> 20) & (cap_ - 1); }
+  void grow() {
+    std::vector<uint64_t> ok;
+    std::vector<int32_t> ov;
+    ok.swap(keys_);
+    ov.swap(vals_);
+    cap_ *= 2;
+    keys_.assign(cap_, kEmpty);
+    vals_.assign(cap_, 0);
+    for (size_t q = 0; q < ok.size(); ++q)
+      if (ok[q] != kEmpty) {
+        size_t h = slot(ok[q]);
+        while (keys_[h] != kEmpty) h = (h + 1) & (cap_ - 1);
+        keys_[h] = ok[q];
+        vals_[h] = ov[q];
+      }
+  }
+  std::vector<uint64_t> keys_;
+  std::vector<int32_t> vals_;
+  size_t cap_ = 1024, n_ = 0;
+};
+
 int64_t find_root(std::vector<int64_t> &cl, int64_t x) {
   int64_t root = x;
   while (cl[root] != root) root = cl[root];
@@ -126,16 +170,64 @@ int64_t emit(std::vector<int64_t> &cl, int64_t n, int64_t *perm, int64_t *offset
 }
 
 // std::lower_bound over the input pairs with the comparator of spgemm_utility.h:789-796.
-bool initial_has(int64_t np, const int64_t *pi, const int64_t *pj, int64_t i, int64_t j) {
-  int64_t first = 0, len = np;
-  while (len > 0) {
-    int64_t half = len >> 1, mid = first + half;
-    bool less = pi[mid] != i ? pi[mid] < i : pj[mid] < j;
-    if (less) { first = mid + 1; len = len - half - 1; }
-    else len = half;
+// When the input is sorted by (i, j) (the order that search assumes; LSH emits it so) and
+// every i is a row, the search runs inside row i's pairs only, found through a row index:
+// same answer, a few cache lines instead of ~24 misses over all pairs.
+class InitialPairs {
+ public:
+  InitialPairs(int64_t n, int64_t np, const int64_t *pi, const int64_t *pj) : np_(np), pi_(pi), pj_(pj) {
+    bool sorted = true;
+    for (int64_t t = 0; t < np && sorted; ++t) {
+      if (pi[t] < 0 || pi[t] >= n) sorted = false;
+      else if (t > 0 && (pi[t - 1] > pi[t] || (pi[t - 1] == pi[t] && pj[t - 1] > pj[t]))) sorted = false;
+    }
+    if (!sorted) return;
+    first_.assign((size_t)n + 1, 0);
+    for (int64_t t = 0; t < np; ++t) ++first_[(size_t)pi[t] + 1];
+    for (int64_t r = 0; r < n; ++r) first_[(size_t)r + 1] += first_[(size_t)r];
   }
-  return first < np && pi[first] == i && pj[first] == j;
-}
+  bool has(int64_t i, int64_t j) const {
+    int64_t first = 0, len = np_;
+    if (!first_.empty()) {
+      if (i < 0 || i + 1 >= (int64_t)first_.size()) return false;
+      first = first_[(size_t)i];
+      len = first_[(size_t)i + 1] - first;
+    }
+    while (len > 0) {
+      const int64_t half = len >> 1, mid = first + half;
+      const bool less = pi_[mid] != i ? pi_[mid] < i : pj_[mid] < j;
+      if (less) { first = mid + 1; len = len - half - 1; }
+      else len = half;
+    }
+    return first < np_ && pi_[first] == i && pj_[first] == j;
+  }
+
+ private:
+  int64_t np_;
+  const int64_t *pi_, *pj_;
+  std::vector<int64_t> first_;     // per row: its first pair (empty: unsorted input)
+};
+
+// Whether the rows are strictly sorted (spgemm_utility.h:390), asked per row when a Jaccard
+// is computed: the sorted-merge and set-semantics counts agree on rows that are sorted and
+// duplicate free, so deciding per pair of rows gives the global flag's results.
+class RowOrder {
+ public:
+  RowOrder(const Csr64 &A, int64_t n) : A_(A), state_((size_t)n, 0) {}
+  bool sorted(int64_t r) {
+    int8_t &st = state_[(size_t)r];
+    if (st == 0) {
+      st = 1;
+      for (int64_t j = A_.rp[r] + 1; j < A_.rp[r + 1]; ++j)
+        if (A_.col[j - 1] >= A_.col[j]) { st = 2; break; }
+    }
+    return st == 1;
+  }
+
+ private:
+  const Csr64 &A_;
+  std::vector<int8_t> state_;      // 0 unknown, 1 strictly ascending, 2 not
+};
 
 template <typename V>
 int64_t hc_v0_fast(int64_t n, const Csr64 &A, int64_t np, const int64_t *pi, const int64_t *pj, const V *score,
@@ -144,38 +236,36 @@ int64_t hc_v0_fast(int64_t n, const Csr64 &A, int64_t np, const int64_t *pi, con
   std::vector<char> valid(n, 1);
   for (int64_t i = 0; i < n; ++i) cl[i] = i;
   // Initial stream: group equal scores (counting sort over the distinct values), scores
-  // descending, input position descending inside a score.
-  std::vector<int64_t> keep;
-  keep.reserve((size_t)np);
-  for (int64_t t = 0; t < np; ++t) {
-    int64_t a = std::min(pi[t], pj[t]), b = std::max(pi[t], pj[t]);
-    if (a == b || a >= n || b >= n) continue;
-    keep.push_back(t);
-  }
-  std::unordered_map<V, int64_t> bucket_of;
+  // descending, input position descending inside a score.  The distinct scores are few
+  // (multiples of 1/k for MinHash estimates): a small open-addressing table keyed by the
+  // score's bits (+0.0 for -0.0) gives each pair its group in one pass.
+  ScoreTable<V> tab;
+  std::vector<int32_t> gid((size_t)np, -1);                 // per pair: group in first-seen order, -1: dropped
   std::vector<V> distinct;
-  for (int64_t t : keep) {
-    V s = score[t] == V(0) ? V(0) : score[t];
-    if (bucket_of.emplace(s, 0).second) distinct.push_back(s);
+  std::vector<int64_t> gcount;
+  for (int64_t t = 0; t < np; ++t) {
+    const int64_t a = std::min(pi[t], pj[t]), b = std::max(pi[t], pj[t]);
+    if (a == b || a >= n || b >= n) continue;
+    const V sc = score[t] == V(0) ? V(0) : score[t];
+    const int32_t g = tab.find_or_add(sc, (int32_t)distinct.size());
+    if (g == (int32_t)distinct.size()) { distinct.push_back(sc); gcount.push_back(0); }
+    gid[(size_t)t] = g;
+    ++gcount[(size_t)g];
   }
-  std::sort(distinct.begin(), distinct.end(), [](V a, V b) { return a > b; });
-  for (size_t q = 0; q < distinct.size(); ++q) bucket_of[distinct[q]] = (int64_t)q;
-  std::vector<int64_t> bstart(distinct.size() + 1, 0);
-  std::vector<int64_t> bid(keep.size());
-  for (size_t q = 0; q < keep.size(); ++q) {
-    V s = score[keep[q]] == V(0) ? V(0) : score[keep[q]];
-    bid[q] = bucket_of[s];
-    bstart[bid[q] + 1]++;
-  }
-  for (size_t q = 0; q < distinct.size(); ++q) bstart[q + 1] += bstart[q];
-  std::vector<int64_t> order(keep.size());
-  {
-    std::vector<int64_t> fill(bstart.begin(), bstart.end() - 1);
-    for (int64_t q = (int64_t)keep.size() - 1; q >= 0; --q) order[fill[bid[q]]++] = keep[q];   // descending position
-  }
+  // groups in descending score; bstart over that order
+  std::vector<int32_t> rank(distinct.size());
+  std::iota(rank.begin(), rank.end(), 0);
+  std::sort(rank.begin(), rank.end(), [&](int32_t x, int32_t y) { return distinct[(size_t)x] > distinct[(size_t)y]; });
+  std::vector<int64_t> fill(distinct.size());                 // per group (first-seen id): next slot
+  int64_t total = 0;
+  for (int32_t r : rank) { fill[(size_t)r] = total; total += gcount[(size_t)r]; }
+  std::vector<int64_t> order((size_t)total);
+  for (int64_t t = np - 1; t >= 0; --t)                       // descending position
+    if (gid[(size_t)t] >= 0) order[(size_t)fill[(size_t)gid[(size_t)t]]++] = t;
   std::priority_queue<Item<V>, std::vector<Item<V>>, ItemLess<V>> dyn;
   PairSet discovered((size_t)np / 8 + 1024);
-  const bool su = rows_sorted_unique(A, n);
+  const InitialPairs initial(n, np, pi, pj);
+  RowOrder order_of(A, n);
   size_t next = 0;
   auto initial_item = [&](size_t q) {
     int64_t t = order[q];
@@ -211,8 +301,8 @@ int64_t hc_v0_fast(int64_t n, const Csr64 &A, int64_t np, const int64_t *pi, con
       if (!valid[ri] || !valid[rj]) continue;
       if (ri != rj) {
         const int64_t a = std::min(ri, rj), b = std::max(ri, rj);
-        if (!initial_has(np, pi, pj, a, b) && discovered.insert(a, b))
-          dyn.push(Item<V>{(V)jaccard(A, a, b, su), a, b});
+        if (!initial.has(a, b) && discovered.insert(a, b))
+          dyn.push(Item<V>{(V)jaccard(A, a, b, order_of.sorted(a) && order_of.sorted(b)), a, b});
       }
     }
   }
