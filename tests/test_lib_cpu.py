@@ -10,7 +10,7 @@ import pytest
 
 import oracle_bind as orc
 from sparseops_amd import _lib as L
-from sparseops_amd.gen import rmat
+from sparseops_amd.gen import Csr, rmat
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLD = os.path.join(ROOT, "tests", "golden")
@@ -86,6 +86,33 @@ def test_library_hc_matches_oracle_on_other_inputs():
         for version in (0, 1):
             want = orc.hc(A, pi, pj, ps, cs, version=version)
             got = hierarchical_clustering(A, pi, pj, ps, cs, version)
+            np.testing.assert_array_equal(got[0], want[0])
+            np.testing.assert_array_equal(got[1], want[1])
+
+
+def test_library_hc_fallbacks_match_oracle():
+    """Pairs in shuffled order (the search for a discovered pair among the input runs over
+    all of them, as the reference's lower_bound does) and rows out of order or with repeated
+    columns (the Jaccard of such rows takes set semantics): same clustering as the oracle."""
+    from sparseops_amd.lespgemm import hierarchical_clustering
+    A = rmat(11, 8, seed=7)
+    sig = orc.minhash(A, 32, 3)
+    pi, pj, ps = orc.lsh_pairs(sig, 32, 8)
+    shuffled = np.random.default_rng(0).permutation(len(pi))
+    rows = [A.col[A.rowptr[r]:A.rowptr[r + 1]].astype(np.int64) for r in range(A.n_rows)]
+    for r in range(A.n_rows):
+        if r % 7 == 0:
+            rows[r] = rows[r][::-1].copy()
+        if r % 11 == 0 and len(rows[r]):
+            rows[r] = np.concatenate([rows[r], rows[r][:1]])
+    rp = np.zeros(A.n_rows + 1, np.int64)
+    np.cumsum([len(c) for c in rows], out=rp[1:])
+    col = np.concatenate(rows)
+    B = Csr(A.n_rows, A.n_cols, rp, col, np.ones(len(col)))
+    for version in (0, 1):
+        for M, qi, qj, qs in ((A, pi[shuffled], pj[shuffled], ps[shuffled]), (B, pi, pj, ps)):
+            want = orc.hc(M, qi, qj, qs, 4, version=version)
+            got = hierarchical_clustering(M, qi, qj, qs, 4, version)
             np.testing.assert_array_equal(got[0], want[0])
             np.testing.assert_array_equal(got[1], want[1])
 
