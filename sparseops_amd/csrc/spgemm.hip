@@ -343,22 +343,80 @@ __device__ __forceinline__ void tile_bits(uint32_t *bm, int64_t t0, int64_t t1, 
   }
 }
 
+// tile_bits with a cursor per A column (cur[t - t0], in LDS): the entries of a B row below the
+// tile were consumed by the earlier tiles (tiles ascend, and the first tile starts at or below
+// every column of the cluster's rows), so a row's part of the tile starts at its cursor and
+// ends at its first column past hi: found by the scan itself, with no binary search.  Lane
+// groups are at most a wave (the end is a group-wide minimum); four loads per lane in flight.
+constexpr int kTileCursors = 1024;
+__device__ __forceinline__ void tile_bits_cur(uint32_t *bm, int64_t t0, int64_t t1, const int32_t *__restrict__ acol,
+                                              const int64_t *__restrict__ brp, const int32_t *__restrict__ bcol,
+                                              int32_t lo, int32_t hi, int32_t *cur) {
+  const int64_t na = t1 - t0;
+  int G = 16;
+  while (G < kWave && (int64_t)(kBlock / G) > 2 * na) G <<= 1;
+  const int NG = kBlock / G;
+  const int grp = (int)threadIdx.x / G, sub = (int)threadIdx.x % G;
+  constexpr int U = 16;
+  for (int64_t t = t0 + grp; t < t1; t += NG) {
+    const int32_t k1 = (int32_t)brp[acol[t] + 1];
+    int32_t k = cur[t - t0];
+    bool going = k < k1;                      // uniform inside the group
+    while (going) {
+      int32_t col[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t pos = k + sub + u * G;
+        col[u] = pos < k1 ? bcol[pos] : INT32_MAX;
+      }
+      int32_t past = INT32_MAX;               // this lane's first position past the tile
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t pos = k + sub + u * G;
+        if (col[u] <= hi) {
+          if (col[u] >= lo) {
+            const int32_t off = col[u] - lo;
+            atomicOr(&bm[off >> 5], 1u << (off & 31));
+          }
+        } else {
+          past = min(past, pos < k1 ? pos : k1);
+        }
+      }
+      for (int d = 1; d < G; d <<= 1) past = min(past, __shfl_xor(past, d, kWave));
+      if (past != INT32_MAX) {
+        k = past;
+        going = false;
+      } else {
+        k += U * G;
+        going = k < k1;
+        if (!going) k = k1;
+      }
+    }
+    if (sub == 0) cur[t - t0] = k;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_sym_large(const int32_t *__restrict__ list, const int64_t *__restrict__ arp,
                                                       const int32_t *__restrict__ acol, const int64_t *__restrict__ brp,
                                                       const int32_t *__restrict__ bcol, int32_t b_cols,
                                                       int64_t *__restrict__ nz, int64_t *__restrict__ cmin,
-                                                      int64_t *__restrict__ cmax) {
+                                                      int64_t *__restrict__ cmax, int b32) {
   __shared__ uint32_t bm[kLargeTileBits / 32];
+  __shared__ int32_t cur[kTileCursors];
   __shared__ int cnt, mn, mx;
   const int64_t c = list[blockIdx.x];
   const int64_t t0 = arp[c], t1 = arp[c + 1];
   if (threadIdx.x == 0) { cnt = 0; mn = INT32_MAX; mx = -1; }
   const bool whole = b_cols <= kLargeTileBits;
+  const bool cursors = !whole && b32 && t1 - t0 <= kTileCursors;
+  if (cursors)
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBlock) cur[t - t0] = (int32_t)brp[acol[t]];
   for (int32_t lo = 0; lo < b_cols; lo += kLargeTileBits) {
     const int32_t hi = (int32_t)min((int64_t)lo + kLargeTileBits, (int64_t)b_cols) - 1;
     for (int w = threadIdx.x; w < kLargeTileBits / 32; w += kBlock) bm[w] = 0u;
     __syncthreads();
-    tile_bits(bm, t0, t1, acol, brp, bcol, lo, hi, whole);
+    if (cursors) tile_bits_cur(bm, t0, t1, acol, brp, bcol, lo, hi, cur);
+    else tile_bits(bm, t0, t1, acol, brp, bcol, lo, hi, whole);
     __syncthreads();
     int n = 0, first = INT32_MAX, last = -1;
     for (int w = threadIdx.x; w < kLargeTileBits / 32; w += kBlock) {
@@ -388,20 +446,25 @@ __global__ __launch_bounds__(kBlock) void k_emit_large(const int32_t *__restrict
                                                        const int32_t *__restrict__ acol, const int64_t *__restrict__ brp,
                                                        const int32_t *__restrict__ bcol, const int64_t *__restrict__ crp,
                                                        const int64_t *__restrict__ cmin, const int64_t *__restrict__ cmax,
-                                                       int32_t *__restrict__ ccol) {
+                                                       int32_t *__restrict__ ccol, int b32) {
   __shared__ uint32_t bm[kLargeTileBits / 32];
+  __shared__ int32_t cur[kTileCursors];
   __shared__ int32_t wsum[kBlock / kWave];
   const int64_t c = list[blockIdx.x];
   const int64_t t0 = arp[c], t1 = arp[c + 1];
   const int32_t cmn = (int32_t)cmin[c], cmx = (int32_t)cmax[c];
   if (crp[c + 1] == crp[c]) return;
   int64_t out = crp[c];
+  const bool cursors = cmx - cmn >= kLargeTileBits && b32 && t1 - t0 <= kTileCursors;
+  if (cursors)
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBlock) cur[t - t0] = (int32_t)brp[acol[t]];
   for (int32_t lo = cmn; lo <= cmx; lo += kLargeTileBits) {
     const int32_t hi = (int32_t)min((int64_t)lo + kLargeTileBits - 1, (int64_t)cmx);
     const int nwords = ((hi - lo) >> 5) + 1;          // words that can hold bits of [lo, hi]
     for (int w = threadIdx.x; w < nwords; w += kBlock) bm[w] = 0u;
     __syncthreads();
-    tile_bits(bm, t0, t1, acol, brp, bcol, lo, hi, cmx - cmn < kLargeTileBits);
+    if (cursors) tile_bits_cur(bm, t0, t1, acol, brp, bcol, lo, hi, cur);
+    else tile_bits(bm, t0, t1, acol, brp, bcol, lo, hi, cmx - cmn < kLargeTileBits);
     __syncthreads();
     // rounds of kBlock consecutive words, one per thread: the columns of neighbouring threads
     // are neighbours in C, so each store instruction of the wave covers a short range
@@ -1011,7 +1074,7 @@ void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols
   if (!P.large.empty())
     hipLaunchKernelGGL(k_sym_large, dim3((unsigned)P.large.size()), dim3(kBlock), 0, s, P.dlarge.as<int32_t>(),
                        A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64, P.bcol32.as<int32_t>(), (int32_t)b_cols,
-                       P.nz.as<int64_t>(), P.cmin.as<int64_t>(), P.cmax.as<int64_t>());
+                       P.nz.as<int64_t>(), P.cmin.as<int64_t>(), P.cmax.as<int64_t>(), (int)(b_nnz < (int64_t)INT32_MAX));
   SPOPS_HIP_CHECK(hipGetLastError());
   P.hnz.assign(nc, 0);
   if (nc > 0) SPOPS_HIP_CHECK(hipMemcpyAsync(P.hnz.data(), P.nz.p, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
@@ -1138,7 +1201,7 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
   if (!large.empty())
     hipLaunchKernelGGL(k_emit_large, dim3((unsigned)large.size()), dim3(kBlock), 0, s, llarge, A.rowptr.as<int64_t>(),
                        A.colids.as<int32_t>(), P.brp64, P.bcol32.as<int32_t>(), crp, P.cmin.as<int64_t>(),
-                       P.cmax.as<int64_t>(), C.colids.as<int32_t>());
+                       P.cmax.as<int64_t>(), C.colids.as<int32_t>(), (int)(P.b_nnz < (int64_t)INT32_MAX));
   SPOPS_HIP_CHECK(hipGetLastError());
   auto numeric = [&](auto tag) {
     using V = decltype(tag);
