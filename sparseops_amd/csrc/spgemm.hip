@@ -507,7 +507,10 @@ __global__ __launch_bounds__(kBlock) void k_emit_large(const int32_t *__restrict
 // A task is {cluster, first chunk, end chunk}: a cluster with much work is split over several
 // workgroups by ranges of its chunks (each task's row cursors start at its first chunk's
 // column, one binary search per row), so that one cluster does not hold the kernel's tail.
-template <typename V, int CR>
+// CURS: A columns whose cursors fit LDS (kLargeCursors, or kLargeCursorsWide for a variant
+// with 78 KB of LDS, two workgroups per CU, that takes the tasks of clusters with more
+// columns: cursors instead of two binary searches per row and chunk).
+template <typename V, int CR, int CURS>
 __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int4 *__restrict__ tasks, const int64_t *__restrict__ arp,
                                                            const int64_t *__restrict__ arpv, const int32_t *__restrict__ csz_a,
                                                            const int32_t *__restrict__ acol, const V *__restrict__ aval,
@@ -522,9 +525,9 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int4 *__restric
   // per A column (when the cluster has at most kLargeCursors and B's offsets fit 32 bits):
   // the next unread entry of its B row, the row's end, and the column at that entry
   // (INT32_MAX: the row is done), so a chunk reads only B entries and skips rows at once
-  __shared__ int32_t rpos[kLargeCursors];
-  __shared__ int32_t rend[kLargeCursors];
-  __shared__ int32_t nextcol[kLargeCursors];
+  __shared__ int32_t rpos[CURS];
+  __shared__ int32_t rend[CURS];
+  __shared__ int32_t nextcol[CURS];
   constexpr int W = kLargeBlock / kWave;
   const int4 tk = tasks[blockIdx.x];
   const int64_t c = tk.x;
@@ -533,7 +536,7 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int4 *__restric
   const int64_t c0 = crp[c], cv0 = crpv[c];
   const int M = large_chunk_cols(csz, CR);
   const int64_t jbeg = (int64_t)tk.y * M, jend = min((int64_t)tk.z * M, crp[c + 1] - c0);
-  const bool cursors = a1 - a0 <= kLargeCursors && b32;
+  const bool cursors = a1 - a0 <= CURS && b32;
   if (cursors) {
     const int32_t lo0 = jbeg > 0 && jbeg < jend ? ccol[c0 + jbeg] : INT32_MIN;
     for (int64_t t = a0 + threadIdx.x; t < a1; t += kLargeBlock) {
@@ -1137,18 +1140,25 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
   const std::vector<int32_t> &regular = split ? pregular : large;
   // tasks of the large-cluster numeric kernels: {cluster, first chunk, end chunk, 0}, about
   // g_task_products products each (clusters in decreasing work order)
-  std::vector<int4> regular4, regular16;
+  // (clusters with more than kLargeCursors columns and B offsets of 32 bits: the variant with
+  // kLargeCursorsWide cursors)
+  std::vector<int4> regular4, regular16, wide4, wide16;
+  const bool b32 = P.b_nnz < (int64_t)INT32_MAX;
   for (int32_t c : regular) {
     const int cr = P.hcsz[c] <= 4 ? 4 : kMaxClusterRows;
     const int64_t chunks = (P.hnz[c] + large_chunk_cols(P.hcsz[c], cr) - 1) / large_chunk_cols(P.hcsz[c], cr);
     const int64_t pieces = std::max<int64_t>(1, std::min<int64_t>(chunks, P.hflop[c] / g_task_products));
+    const int64_t na = P.harp[c + 1] - P.harp[c];
+    const bool wide = b32 && na > kLargeCursors && na <= kLargeCursorsWide;
+    std::vector<int4> &to = wide ? (cr == 4 ? wide4 : wide16) : (cr == 4 ? regular4 : regular16);
     for (int64_t i = 0; i < pieces; ++i)
-      (cr == 4 ? regular4 : regular16)
-          .push_back(make_int4(c, (int)(chunks * i / pieces), (int)(chunks * (i + 1) / pieces), 0));
+      to.push_back(make_int4(c, (int)(chunks * i / pieces), (int)(chunks * (i + 1) / pieces), 0));
   }
-  DevBuf dregular4, dregular16;
+  DevBuf dregular4, dregular16, dwide4, dwide16;
   dregular4.upload(regular4);
   dregular16.upload(regular16);
+  dwide4.upload(wide4);
+  dwide16.upload(wide16);
   int64_t nnzc = 0, nnzv = 0, csr_rows = 0;
   for (int64_t c = c0; c < c1; ++c) {
     nnzc += P.hnz[c];
@@ -1220,8 +1230,10 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
                            static_cast<const V *>(P.bval), crp, crpv, C.colids.as<int32_t>(), C.values.as<V>(),
                            P.b_nnz < (int64_t)INT32_MAX);
     };
-    large_kernel(k_num_large<V, 4>, regular4, dregular4.as<int4>());
-    large_kernel(k_num_large<V, kMaxClusterRows>, regular16, dregular16.as<int4>());
+    large_kernel(k_num_large<V, 4, kLargeCursorsWide>, wide4, dwide4.as<int4>());
+    large_kernel(k_num_large<V, kMaxClusterRows, kLargeCursorsWide>, wide16, dwide16.as<int4>());
+    large_kernel(k_num_large<V, 4, kLargeCursors>, regular4, dregular4.as<int4>());
+    large_kernel(k_num_large<V, kMaxClusterRows, kLargeCursors>, regular16, dregular16.as<int4>());
     // dense clusters in batches whose accumulators fit kDenseScratch
     size_t i = 0;
     while (i < pdense.size()) {

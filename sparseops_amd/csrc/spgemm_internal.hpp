@@ -13,6 +13,7 @@ constexpr int kLargeBlock = 512;                 // threads of a large-cluster n
 constexpr int kLargeAccEntries = 2304;           // LDS accumulator entries of a numeric chunk
 constexpr int kLargeHash = 1024;                 // LDS hash slots of a chunk's columns
 constexpr int kLargeCursors = 1024;              // A columns whose B-row position is kept in LDS
+constexpr int kLargeCursorsWide = 4096;          // the same in the variant for wider clusters (two workgroups per CU)
 constexpr int kLargeGroupLanes = 4;              // fewest lanes scanning one B row in the numeric pass
 constexpr int64_t kLargeTaskProducts = (int64_t)1 << 17;   // default products per numeric task of a split cluster
 // hash slots of a numeric chunk: clusters of at most 4 rows (CR = 4; mostly single rows) take

@@ -306,3 +306,27 @@ def test_split_cluster_tasks(products, budget, dev):
             np.testing.assert_array_equal(pcol, col[rp[c0]:rp[c1]])
             np.testing.assert_allclose(pval, val[rpv[c0]:rpv[c1]], rtol=1e-12, atol=0)
     plan.close()
+
+
+def test_wide_clusters_keep_cursors(dev):
+    """Rows of 1500-3000 columns make clusters with more than 1024 A columns, whose tasks run
+    in the numeric variant with 4096 LDS cursors (also split over workgroups): same product
+    as the oracle, structure exactly and values to 1e-12."""
+    rng = np.random.default_rng(21)
+    n = 6000
+    lens = rng.integers(1, 12, n)
+    lens[[5, 700, 3001, 5999]] = [3000, 1500, 2200, 2600]
+    cols = [np.sort(rng.choice(n, size=L, replace=False)) for L in lens]
+    rp = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=rp[1:])
+    col = np.concatenate(cols).astype(np.int64)
+    A = Csr(n, n, rp, col, rng.random(col.size) + 0.5)
+    want = orc.lespgemm(A, A, k=64, bands=16, seed=7, cluster_size=8, use_long_double=True)
+    for products in (1 << 40, 4096):
+        with LG.task_products(products):
+            got = LG.lespgemm(A, A, k=64, bands=16, seed=7, cluster_size=8, device=dev)
+        np.testing.assert_array_equal(got["perm"], want["perm"])
+        np.testing.assert_array_equal(got["C_rowptr"], want["C_rowptr"])
+        np.testing.assert_array_equal(got["C_col"], want["C_col"])
+        rel = np.abs(got["C_val"] - want["C_val"]) / np.abs(want["C_val"])
+        assert rel.max() <= 1e-12, rel.max()
