@@ -14,7 +14,9 @@ the launch stream; its HBM traffic comes from the committed rocprofv3 counter su
 the same command.  A secondary LeSpGEMM measurement (C = A*A,
 R-MAT, MinHash-LSH clustering, GFLOP/s) is reported with the reference's OpenMP pipeline
 timed on the same host; so are the differentiable SpMM layer (R-MAT scale 20, BF16 X / G,
-forward and backward) and the batched SpMM (512 band matrices, one HIP-graph replay per step).
+forward and backward), the batched SpMM (512 band matrices, one HIP-graph replay per step)
+and the plain FP64 CSR SpMV on the 5-point Laplacian of a 256 x 256 grid (device-resident,
+through the drop-in driver, and the reference's OpenMP CSR kernel on the same host).
 
 Prints ONE JSON line (rank 0, N = 1).  --gpus N > 1 exits: the benchmark runs on one GPU.
 """
@@ -473,6 +475,67 @@ def batch_measure(count, steps, warmup, cpu, cpu_seconds):
     return out
 
 
+def laplacian_measure(nx, steps, warmup, cpu_threads, cpu, cpu_seconds):
+    """Plain FP64 CSR SpMV on the 5-point Laplacian of an nx-by-nx grid (x ~ U[-1,1), seed 0):
+    the device-resident kernel (HIP events, many steps: one SpMV moves 5.2 MB, so it is
+    launch-bound), the drop-in driver benchmark_spmv_csr on the same matrix as a .mtx (its
+    host-array timing includes the PCIe copies of x and y), and the reference's OpenMP CSR
+    kernel (kernel_flag 2) on this host's cores.  Error against an FP64 scipy product."""
+    import numpy as np
+    import scipy.sparse as sp
+    import torch
+    from sparseops_amd import spmv as S
+    from sparseops_amd.gen import laplacian_2d, uniform_x
+    from sparseops_amd.io import write_csr_bin, write_mtx
+
+    A = laplacian_2d(nx)
+    x64 = uniform_x(A.n_cols, 0)
+    As = sp.csr_matrix((A.val, A.col, A.rowptr), shape=(A.n_rows, A.n_cols))
+    y64 = As @ x64
+    absrow = abs(As) @ np.abs(x64)
+    M = S.csr(A, "f64")
+    dev = torch.device("cuda:0")
+    x = torch.tensor(x64, device=dev)
+    y = torch.zeros(A.n_rows, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    wall_ms, kern_ms = time_spmv(M, x, y, steps, warmup, stream)
+    err = errors(y.cpu().numpy(), y64, absrow)
+    out = {"n": A.n_rows, "nnz": A.nnz, "algo_bytes": M.algo_bytes, "kernel_us": round(kern_ms * 1e3, 3),
+           "wall_us_per_step": round(wall_ms * 1e3, 3),
+           "kernel_gbs": round(M.algo_bytes / (kern_ms * 1e-3) / 1e9, 1),
+           "pct_hbm_peak": round(100.0 * M.algo_bytes / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+           "gflops": round(2.0 * A.nnz / (kern_ms * 1e-3) / 1e9, 2), "max_rel_err": err["max_rel_err"],
+           "dropin": None, "cpu_reference": None}
+    M.close()
+    exe = os.path.join(ROOT, "sparseops_amd", "cli", "benchmark_spmv_csr")
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        if os.path.exists(exe):
+            path = os.path.join(d, f"laplacian_{nx}.mtx")
+            write_mtx(path, A)
+            r = subprocess.run([exe, path, "--precision=64", f"--perf_file={os.path.join(d, 'perf.txt')}"],
+                               capture_output=True, text=True, timeout=300)
+            dropin = {"returncode": r.returncode, "passed": "PASSED" in r.stdout}
+            for line in r.stdout.splitlines():
+                if "benchmarking" in line and "[" in line:
+                    tag = line.split("[")[1].split("]")[0]
+                    ms = float(line.split("]:")[1].split("ms")[0])
+                    dropin["host_arrays_us" if tag == "host arrays" else "device_us"] = round(ms * 1e3, 3)
+            out["dropin"] = dropin
+        if cpu and os.path.exists(harness):
+            path = os.path.join(d, "A.csr")
+            write_csr_bin(path, A)
+            env = dict(os.environ, OMP_NUM_THREADS=str(cpu_threads), OMP_PLACES="cores", OMP_PROC_BIND="spread")
+            r = subprocess.run([harness, "bench_spmv", path, "csr", "64", str(cpu_threads), str(cpu_seconds)],
+                               capture_output=True, text=True, env=env, timeout=300)
+            if r.returncode == 0:
+                c = json.loads(r.stdout.strip().splitlines()[-1])
+                out["cpu_reference"] = {"us": round(c["seconds_per_iter"] * 1e6, 3), "cores": c["threads"],
+                                        "iterations": c["iterations"],
+                                        "gbs": round(out["algo_bytes"] / c["seconds_per_iter"] / 1e9, 1)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -492,6 +555,8 @@ def main():
     ap.add_argument("--autograd-scale", type=int, default=20, help="R-MAT scale of the autograd SpMM line (0: skip)")
     ap.add_argument("--batch-count", type=int, default=512, help="matrices of the batched SpMM line (0: skip)")
     ap.add_argument("--bsr-n", type=int, default=1 << 18, help="rows of the dense-block BSR line (0: skip)")
+    ap.add_argument("--laplacian-nx", type=int, default=256,
+                    help="grid side of the FP64 CSR Laplacian line (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -594,6 +659,13 @@ def main():
         bs = bsr_measure(args.bsr_n, 16, 8, max(10, args.steps // 4), max(3, args.warmup // 4))
         log(f"bsr done in {time.time() - t:.1f}s: {bs}")
 
+    lap = None
+    if args.laplacian_nx > 0:
+        t = time.time()
+        lap = laplacian_measure(args.laplacian_nx, max(200, args.steps * 5), max(20, args.warmup), args.cpu_threads,
+                                not args.no_cpu, min(2.0, args.cpu_seconds))
+        log(f"laplacian done in {time.time() - t:.1f}s: {lap}")
+
     out = {
         "metric": METRIC,
         "value": round(head["gbs"], 1),
@@ -630,6 +702,7 @@ def main():
         "autograd_spmm": ag,
         "batch_spmm": bt,
         "bsr_dense_blocks": bs,
+        "laplacian_csr_f64": lap,
     }
     print(json.dumps(out), flush=True)
 
