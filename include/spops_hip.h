@@ -231,8 +231,13 @@ int spops_matrix_info(spops_matrix A, int64_t info[16]);
  *   entries, 2 all such blocks.  With one right-hand side the VALU kernel is the faster of
  *   the two where both apply.  Applies to matrices created afterwards.
  *   SPOPS_TUNE_TASK_PRODUCTS: a large LeSpGEMM cluster's numeric work is split over
- *   workgroups by ranges of its output-column chunks, about this many products per workgroup
- *   (default 2^17); applies to numeric calls made afterwards. */
+ *   workgroups by ranges of its output-column chunks, about this many products per workgroup;
+ *   0 (default) picks it per numeric call: 2^15, halved down to 2^13 while the call has fewer
+ *   than 32768 such tasks.  Applies to numeric calls made afterwards.
+ *   SPOPS_TUNE_WIDE_CURSORS: 1 runs LeSpGEMM clusters of 1025..4096 A columns (B with < 2^31
+ *   entries) in a variant that keeps one B-row cursor per column in LDS, at two workgroups
+ *   per CU, instead of searching each B row once per chunk; 0 (default) does not.  Applies
+ *   to numeric calls made afterwards. */
 #define SPOPS_TUNE_SLICE_BYTES 0
 #define SPOPS_TUNE_DENSE_PRODUCTS 1
 #define SPOPS_TUNE_SPMV_LAYOUT 2
@@ -240,6 +245,7 @@ int spops_matrix_info(spops_matrix A, int64_t info[16]);
 #define SPOPS_TUNE_SWEEP_MAX_ROWS 4
 #define SPOPS_TUNE_BSR_MFMA 5
 #define SPOPS_TUNE_TASK_PRODUCTS 6
+#define SPOPS_TUNE_WIDE_CURSORS 7
 int spops_set_tuning(int key, int64_t value, int64_t *old);
 
 /* Device copies of the layout arrays a format conversion produced, for parity checks:
@@ -319,7 +325,9 @@ int spops_cluster_destroy(spops_cluster A);
  * C's columns ascend inside each cluster.  B's rows need not be sorted (an unsorted B is
  * sorted into a temporary copy).  stats (may be NULL): [0]=1 if B was sorted into a copy
  * [1]=nnzc [2]=nnzv [3]=small clusters (one LDS block each) [4]=large clusters (chunked LDS
- * accumulator) [5..8]=setup/symbolic/malloc/numeric in us [9]=cluster-wise products. */
+ * accumulator) [5..8]=setup/symbolic/malloc/numeric in us [9]=cluster-wise products
+ * [10]=dense clusters [11]=numeric tasks of the large clusters [12]=those of them in the
+ * wide-cursor variant. */
 int spops_lespgemm(spops_cluster A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *d_browptr,
                    const int64_t *d_bcol, const void *d_bval, spops_cluster *C, int64_t stats[16], void *stream);
 

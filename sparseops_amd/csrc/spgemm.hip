@@ -38,7 +38,8 @@
 
 namespace spops {
 
-int64_t g_task_products = kLargeTaskProducts;
+int64_t g_task_products = 0;
+int g_wide_cursors = 0;
 
 namespace {
 
@@ -1139,17 +1140,42 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
   const bool split = !pdense.empty();
   const std::vector<int32_t> &regular = split ? pregular : large;
   // tasks of the large-cluster numeric kernels: {cluster, first chunk, end chunk, 0}, about
-  // g_task_products products each (clusters in decreasing work order)
+  // task_products products each (clusters in decreasing work order)
   // (clusters with more than kLargeCursors columns and B offsets of 32 bits: the variant with
   // kLargeCursorsWide cursors)
   std::vector<int4> regular4, regular16, wide4, wide16;
   const bool b32 = P.b_nnz < (int64_t)INT32_MAX;
-  for (int32_t c : regular) {
+  auto pieces_of = [&](int32_t c, int64_t per_task, int64_t *chunks_out) {
     const int cr = P.hcsz[c] <= 4 ? 4 : kMaxClusterRows;
     const int64_t chunks = (P.hnz[c] + large_chunk_cols(P.hcsz[c], cr) - 1) / large_chunk_cols(P.hcsz[c], cr);
-    const int64_t pieces = std::max<int64_t>(1, std::min<int64_t>(chunks, P.hflop[c] / g_task_products));
+    if (chunks_out) *chunks_out = chunks;
+    return std::max<int64_t>(1, std::min<int64_t>(chunks, P.hflop[c] / per_task));
+  };
+  // products per task: the tuned value, else kLargeTaskProducts halved (down to
+  // kLargeTaskProductsMin) while the call has fewer than kLargeTasksMin tasks -- the smaller
+  // the product, the more the tail of the launch matters
+  int64_t task_products = g_task_products;
+  if (task_products <= 0) {
+    task_products = kLargeTaskProducts;
+    while (task_products > kLargeTaskProductsMin) {
+      int64_t tasks = 0;
+      for (int32_t c : regular) tasks += pieces_of(c, task_products, nullptr);
+      if (tasks >= kLargeTasksMin) break;
+      task_products >>= 1;
+    }
+  }
+  // clusters of 1025..kLargeCursorsWide A columns in the variant with kLargeCursorsWide
+  // cursors, when asked for (it runs at half the occupancy of the regular kernel)
+  const bool use_wide = g_wide_cursors != 0;
+  auto is_wide = [&](int32_t c) {
     const int64_t na = P.harp[c + 1] - P.harp[c];
-    const bool wide = b32 && na > kLargeCursors && na <= kLargeCursorsWide;
+    return b32 && na > kLargeCursors && na <= kLargeCursorsWide;
+  };
+  for (int32_t c : regular) {
+    const int cr = P.hcsz[c] <= 4 ? 4 : kMaxClusterRows;
+    int64_t chunks = 0;
+    const int64_t pieces = pieces_of(c, task_products, &chunks);
+    const bool wide = use_wide && is_wide(c);
     std::vector<int4> &to = wide ? (cr == 4 ? wide4 : wide16) : (cr == 4 ? regular4 : regular16);
     for (int64_t i = 0; i < pieces; ++i)
       to.push_back(make_int4(c, (int)(chunks * i / pieces), (int)(chunks * (i + 1) / pieces), 0));
@@ -1286,6 +1312,8 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
   stats[3] = (int64_t)small.size();
   stats[4] = (int64_t)large.size();
   stats[10] = (int64_t)pdense.size();
+  stats[11] = (int64_t)(regular4.size() + regular16.size());
+  stats[12] = (int64_t)(wide4.size() + wide16.size());
   stats[5] = (int64_t)((whole ? P.ms_setup : 0.0) * 1e3);
   stats[6] = (int64_t)((whole ? P.ms_symbolic : 0.0) * 1e3 + ms_scan * 1e3);
   stats[7] = (int64_t)(ms_malloc * 1e3);

@@ -15,7 +15,9 @@ constexpr int kLargeHash = 1024;                 // LDS hash slots of a chunk's 
 constexpr int kLargeCursors = 1024;              // A columns whose B-row position is kept in LDS
 constexpr int kLargeCursorsWide = 4096;          // the same in the variant for wider clusters (two workgroups per CU)
 constexpr int kLargeGroupLanes = 4;              // fewest lanes scanning one B row in the numeric pass
-constexpr int64_t kLargeTaskProducts = (int64_t)1 << 17;   // default products per numeric task of a split cluster
+constexpr int64_t kLargeTaskProducts = (int64_t)1 << 15;    // products per numeric task of a split cluster ...
+constexpr int64_t kLargeTaskProductsMin = (int64_t)1 << 13; // ... halved down to this while a call has
+constexpr int64_t kLargeTasksMin = 32768;                    // fewer tasks than this
 // hash slots of a numeric chunk: clusters of at most 4 rows (CR = 4; mostly single rows) take
 // twice as many, so a chunk holds up to 1024 output columns: half the chunks, each of which
 // visits every row of the cluster once
@@ -70,7 +72,8 @@ struct spops_spgemm_plan_s {
 
 namespace spops {
 extern int64_t g_dense_products;   // products from which a cluster accumulates densely (tuning)
-extern int64_t g_task_products;    // products per numeric task of a large cluster (tuning)
+extern int64_t g_task_products;    // products per numeric task of a large cluster, 0: chosen per call (tuning)
+extern int g_wide_cursors;         // wide-cluster cursor variant: 0 off, 1 on (tuning)
 void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *brp64,
                         const int64_t *bcol64, const void *bval, spops_spgemm_plan_s &P, hipStream_t s);
 // C = rows [c0, c1) of the product as a cluster matrix of c1 - c0 clusters; C's buffers are

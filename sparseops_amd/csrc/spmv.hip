@@ -1612,9 +1612,13 @@ extern "C" int spops_set_tuning(int key, int64_t value, int64_t *old) {
       if (old) *old = g_bsr_mfma;
       g_bsr_mfma = value;
     } else if (key == SPOPS_TUNE_TASK_PRODUCTS) {
-      SPOPS_REQUIRE(value >= 1, SPOPS_ERROR_INVALID_VALUE, "task products must be >= 1");
+      SPOPS_REQUIRE(value >= 0, SPOPS_ERROR_INVALID_VALUE, "task products must be >= 0");
       if (old) *old = g_task_products;
       g_task_products = value;
+    } else if (key == SPOPS_TUNE_WIDE_CURSORS) {
+      SPOPS_REQUIRE(value >= 0 && value <= 1, SPOPS_ERROR_INVALID_VALUE, "wide cursor mode must be 0 or 1");
+      if (old) *old = g_wide_cursors;
+      g_wide_cursors = (int)value;
     } else if (key == SPOPS_TUNE_DENSE_PRODUCTS) {
       SPOPS_REQUIRE(value >= 1, SPOPS_ERROR_INVALID_VALUE, "dense product threshold must be >= 1");
       if (old) *old = g_dense_products;

@@ -188,7 +188,8 @@ class dense_products:
 
 class task_products:
     """Context manager: numeric calls made inside split a large cluster's work over workgroups
-    of about `products` products each (library default 2^17)."""
+    of about `products` products each (0, the library default: chosen per call, 2^15 halved
+    down to 2^13 while the call has fewer than 32768 tasks)."""
 
     def __init__(self, products: int):
         self.products = int(products)
@@ -203,11 +204,29 @@ class task_products:
         L.call("spops_set_tuning", L.TUNE_TASK_PRODUCTS, self.old, None)
 
 
+class wide_cursors:
+    """Context manager: numeric calls made inside run clusters of 1025..4096 A columns in the
+    variant with 4096 LDS cursors (two workgroups per CU) when mode is 1; 0 is the library
+    default (per-chunk searches of each B row in the regular kernel)."""
+
+    def __init__(self, mode: int):
+        self.mode = int(mode)
+
+    def __enter__(self):
+        old = c_i64()
+        L.call("spops_set_tuning", L.TUNE_WIDE_CURSORS, self.mode, ctypes.byref(old))
+        self.old = old.value
+        return self
+
+    def __exit__(self, *exc):
+        L.call("spops_set_tuning", L.TUNE_WIDE_CURSORS, self.old, None)
+
+
 def _stats(stats):
     s = list(stats)
     return {"b_rows_sorted_copy": s[0], "nnzc": s[1], "nnzv": s[2], "small_clusters": s[3], "large_clusters": s[4],
             "setup_ms": s[5] / 1e3, "symbolic_ms": s[6] / 1e3, "malloc_ms": s[7] / 1e3, "numeric_ms": s[8] / 1e3,
-            "cluster_products": s[9], "dense_clusters": s[10]}
+            "cluster_products": s[9], "dense_clusters": s[10], "large_tasks": s[11], "wide_tasks": s[12]}
 
 
 class SpgemmPlan:

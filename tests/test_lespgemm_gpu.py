@@ -310,8 +310,10 @@ def test_split_cluster_tasks(products, budget, dev):
 
 def test_wide_clusters_keep_cursors(dev):
     """Rows of 1500-3000 columns make clusters with more than 1024 A columns, whose tasks run
-    in the numeric variant with 4096 LDS cursors (also split over workgroups): same product
-    as the oracle, structure exactly and values to 1e-12."""
+    in the numeric variant with 4096 LDS cursors when it is on (mode 1; also split over
+    workgroups) and in the regular variant with per-chunk searches otherwise (also with the
+    per-call task size, products 0): same product as the oracle, structure exactly and values
+    to 1e-12."""
     rng = np.random.default_rng(21)
     n = 6000
     lens = rng.integers(1, 12, n)
@@ -322,8 +324,8 @@ def test_wide_clusters_keep_cursors(dev):
     col = np.concatenate(cols).astype(np.int64)
     A = Csr(n, n, rp, col, rng.random(col.size) + 0.5)
     want = orc.lespgemm(A, A, k=64, bands=16, seed=7, cluster_size=8, use_long_double=True)
-    for products in (1 << 40, 4096):
-        with LG.task_products(products):
+    for products, mode in ((1 << 40, 1), (4096, 1), (0, 1), (1 << 40, 0), (0, 0)):
+        with LG.task_products(products), LG.wide_cursors(mode):
             got = LG.lespgemm(A, A, k=64, bands=16, seed=7, cluster_size=8, device=dev)
         np.testing.assert_array_equal(got["perm"], want["perm"])
         np.testing.assert_array_equal(got["C_rowptr"], want["C_rowptr"])
