@@ -237,7 +237,11 @@ int spops_matrix_info(spops_matrix A, int64_t info[16]);
  *   SPOPS_TUNE_WIDE_CURSORS: 1 runs LeSpGEMM clusters of 1025..4096 A columns (B with < 2^31
  *   entries) in a variant that keeps one B-row cursor per column in LDS, at two workgroups
  *   per CU, instead of searching each B row once per chunk; 0 (default) does not.  Applies
- *   to numeric calls made afterwards. */
+ *   to numeric calls made afterwards.
+ *   SPOPS_TUNE_HEAVY_PRODUCTS: LeSpGEMM clusters with at least this many products (default
+ *   2^20) get their symbolic bitmap built by several workgroups (parts of about 2^18
+ *   products) and kept by the plan for the emit pass, while the part bitmaps (1 GiB) and the
+ *   kept ones (256 MiB) fit; applies to plans created afterwards. */
 #define SPOPS_TUNE_SLICE_BYTES 0
 #define SPOPS_TUNE_DENSE_PRODUCTS 1
 #define SPOPS_TUNE_SPMV_LAYOUT 2
@@ -246,6 +250,7 @@ int spops_matrix_info(spops_matrix A, int64_t info[16]);
 #define SPOPS_TUNE_BSR_MFMA 5
 #define SPOPS_TUNE_TASK_PRODUCTS 6
 #define SPOPS_TUNE_WIDE_CURSORS 7
+#define SPOPS_TUNE_HEAVY_PRODUCTS 8
 int spops_set_tuning(int key, int64_t value, int64_t *old);
 
 /* Device copies of the layout arrays a format conversion produced, for parity checks:

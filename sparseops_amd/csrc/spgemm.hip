@@ -40,6 +40,7 @@ namespace spops {
 
 int64_t g_task_products = 0;
 int g_wide_cursors = 0;
+int64_t g_heavy_products = kHeavyProducts;
 
 namespace {
 
@@ -397,6 +398,41 @@ __device__ __forceinline__ void tile_bits_cur(uint32_t *bm, int64_t t0, int64_t 
   }
 }
 
+// writes the columns base + 32 w + b of the set bits b of words[0, nwords) to ccol[out...] in
+// ascending order (out advances): rounds of kBlock consecutive words, one per thread, with a
+// block scan of the per-thread counts; the columns of neighbouring threads are neighbours in
+// C, so each store instruction of the wave covers a short range
+constexpr int kTileWords = kLargeTileBits / 32;
+__device__ __forceinline__ void emit_words(const uint32_t *words, int nwords, int32_t base, int64_t &out,
+                                           int32_t *__restrict__ ccol, int32_t *wsum) {
+  for (int w0 = 0; w0 < nwords; w0 += kBlock) {
+    const int w = w0 + (int)threadIdx.x;
+    uint32_t bits = w < nwords ? words[w] : 0u;
+    const int32_t mine = __popc(bits);
+    int32_t inc = mine;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int32_t o = __shfl_up(inc, d, kWave);
+      if (lane_id() >= d) inc += o;
+    }
+    if (lane_id() == kWave - 1) wsum[wave_id()] = inc;
+    __syncthreads();
+    int32_t before = 0, total = 0;
+    for (int q = 0; q < kBlock / kWave; ++q) {
+      if (q < wave_id()) before += wsum[q];
+      total += wsum[q];
+    }
+    int64_t r = out + before + inc - mine;
+    while (bits) {
+      const int b = __ffs(bits) - 1;
+      ccol[r++] = base + w * 32 + b;
+      bits &= bits - 1;
+    }
+    out += total;
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_sym_large(const int32_t *__restrict__ list, const int64_t *__restrict__ arp,
                                                       const int32_t *__restrict__ acol, const int64_t *__restrict__ brp,
                                                       const int32_t *__restrict__ bcol, int32_t b_cols,
@@ -467,34 +503,105 @@ __global__ __launch_bounds__(kBlock) void k_emit_large(const int32_t *__restrict
     if (cursors) tile_bits_cur(bm, t0, t1, acol, brp, bcol, lo, hi, cur);
     else tile_bits(bm, t0, t1, acol, brp, bcol, lo, hi, cmx - cmn < kLargeTileBits);
     __syncthreads();
-    // rounds of kBlock consecutive words, one per thread: the columns of neighbouring threads
-    // are neighbours in C, so each store instruction of the wave covers a short range
-    for (int w0 = 0; w0 < nwords; w0 += kBlock) {
-      const int w = w0 + (int)threadIdx.x;
-      uint32_t bits = w < nwords ? bm[w] : 0u;
-      const int32_t mine = __popc(bits);
-      int32_t inc = mine;                     // block exclusive scan of the per-thread counts
-#pragma unroll
-      for (int d = 1; d < kWave; d <<= 1) {
-        const int32_t o = __shfl_up(inc, d, kWave);
-        if (lane_id() >= d) inc += o;
-      }
-      if (lane_id() == kWave - 1) wsum[wave_id()] = inc;
-      __syncthreads();
-      int32_t before = 0, total = 0;
-      for (int q = 0; q < kBlock / kWave; ++q) {
-        if (q < wave_id()) before += wsum[q];
-        total += wsum[q];
-      }
-      int64_t r = out + before + inc - mine;
-      while (bits) {
-        const int b = __ffs(bits) - 1;
-        ccol[r++] = lo + w * 32 + b;
-        bits &= bits - 1;
-      }
-      out += total;
-      __syncthreads();
+    emit_words(bm, nwords, lo, out, ccol, wsum);
+  }
+}
+
+// clusters of many products ("heavy"): the symbolic bitmap over all of B's columns is built
+// by several workgroups, each taking a part of the cluster's A columns (at most kTileCursors,
+// about kHeavyPartProducts products) and writing its bitmap to a slice of its own; a merge pass
+// ORs a cluster's slices into the bitmap kept by the plan and counts it (nz, min, max column),
+// and the emit pass writes C's columns straight from that bitmap.
+struct HeavyPart {
+  int64_t t0, t1;          // A columns of the part (cluster column pointers)
+  int32_t slice, pad;      // bitmap slice written
+};
+
+__global__ __launch_bounds__(kBlock) void k_heavy_part(const HeavyPart *__restrict__ parts,
+                                                       const int32_t *__restrict__ acol, const int64_t *__restrict__ brp,
+                                                       const int32_t *__restrict__ bcol, int32_t b_cols, int64_t words,
+                                                       uint32_t *__restrict__ slices, int b32) {
+  __shared__ uint32_t bm[kLargeTileBits / 32];
+  __shared__ int32_t cur[kTileCursors];
+  const HeavyPart p = parts[blockIdx.x];
+  const bool whole = b_cols <= kLargeTileBits;
+  const bool cursors = !whole && b32 && p.t1 - p.t0 <= kTileCursors;
+  if (cursors)
+    for (int64_t t = p.t0 + threadIdx.x; t < p.t1; t += kBlock) cur[t - p.t0] = (int32_t)brp[acol[t]];
+  uint32_t *out = slices + (int64_t)p.slice * words;
+  for (int64_t lo64 = 0; lo64 < b_cols; lo64 += kLargeTileBits) {
+    const int32_t lo = (int32_t)lo64;
+    const int32_t hi = (int32_t)(min(lo64 + kLargeTileBits, (int64_t)b_cols) - 1);
+    const int nwords = ((hi - lo) >> 5) + 1;
+    for (int w = threadIdx.x; w < nwords; w += kBlock) bm[w] = 0u;
+    __syncthreads();
+    if (cursors) tile_bits_cur(bm, p.t0, p.t1, acol, brp, bcol, lo, hi, cur);
+    else tile_bits(bm, p.t0, p.t1, acol, brp, bcol, lo, hi, whole);
+    __syncthreads();
+    for (int w = threadIdx.x; w < nwords; w += kBlock) out[(lo >> 5) + w] = bm[w];
+    __syncthreads();
+  }
+}
+
+// per heavy cluster {cluster, first slice, end slice, index h}: OR of its slices -> bitmap h,
+// and the cluster's nz / min / max column
+__global__ __launch_bounds__(kBlock) void k_heavy_merge(const int4 *__restrict__ heavy, const uint32_t *__restrict__ slices,
+                                                        int64_t words, uint32_t *__restrict__ gbm,
+                                                        int64_t *__restrict__ nz, int64_t *__restrict__ cmin,
+                                                        int64_t *__restrict__ cmax) {
+  __shared__ int64_t scnt[kBlock / kWave];
+  __shared__ int64_t smn[kBlock / kWave], smx[kBlock / kWave];
+  const int4 hv = heavy[blockIdx.x];
+  uint32_t *dst = gbm + (int64_t)hv.w * words;
+  int64_t n = 0, first = INT64_MAX, last = -1;
+  for (int64_t w = threadIdx.x; w < words; w += kBlock) {
+    uint32_t x = 0u;
+    for (int q = hv.y; q < hv.z; ++q) x |= slices[(int64_t)q * words + w];
+    dst[w] = x;
+    if (x) {
+      n += __popc(x);
+      first = min(first, w * 32 + __ffs(x) - 1);
+      last = max(last, w * 32 + 31 - __clz(x));
     }
+  }
+  for (int d = 1; d < kWave; d <<= 1) {
+    n += __shfl_xor(n, d, kWave);
+    first = min(first, __shfl_xor(first, d, kWave));
+    last = max(last, __shfl_xor(last, d, kWave));
+  }
+  if (lane_id() == 0) {
+    scnt[wave_id()] = n;
+    smn[wave_id()] = first;
+    smx[wave_id()] = last;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < kBlock / kWave; ++q) {
+      n += scnt[q];
+      first = min(first, smn[q]);
+      last = max(last, smx[q]);
+    }
+    nz[hv.x] = n;
+    cmin[hv.x] = n ? first : 0;
+    cmax[hv.x] = n ? last : 0;
+  }
+}
+
+// C's sorted columns of heavy clusters {cluster, bitmap h} from the kept bitmaps
+__global__ __launch_bounds__(kBlock) void k_heavy_emit(const int2 *__restrict__ list, const uint32_t *__restrict__ gbm,
+                                                       int64_t words, const int64_t *__restrict__ crp,
+                                                       const int64_t *__restrict__ cmin, const int64_t *__restrict__ cmax,
+                                                       int32_t *__restrict__ ccol) {
+  __shared__ int32_t wsum[kBlock / kWave];
+  const int2 e = list[blockIdx.x];
+  const int64_t c = e.x;
+  if (crp[c + 1] == crp[c]) return;
+  int64_t out = crp[c];
+  const int64_t w0 = cmin[c] >> 5, w1 = cmax[c] >> 5;
+  const uint32_t *src = gbm + (int64_t)e.y * words + w0;
+  for (int64_t w = 0; w <= w1 - w0; w += kTileWords) {
+    const int nwords = (int)min((int64_t)kTileWords, w1 - w0 + 1 - w);
+    emit_words(src + w, nwords, (int32_t)((w0 + w) * 32), out, ccol, wsum);
   }
 }
 
@@ -1061,7 +1168,43 @@ void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols
   // large clusters in descending work, so that the longest start first
   std::stable_sort(P.large.begin(), P.large.end(), [&](int32_t a, int32_t b) { return P.hflop[a] > P.hflop[b]; });
   P.dsmall.upload(P.small);
-  P.dlarge.upload(P.large);
+  P.harp.assign(nc + 1, 0);
+  if (nc > 0)
+    SPOPS_HIP_CHECK(hipMemcpyAsync(P.harp.data(), A.rowptr.p, (size_t)(nc + 1) * 8, hipMemcpyDeviceToHost, s));
+  SPOPS_HIP_CHECK(hipStreamSynchronize(s));
+  // heavy clusters (by decreasing work, while the part slices and kept bitmaps fit their
+  // budgets): parts of at most kTileCursors A columns and about kHeavyPartProducts products
+  const int64_t words = (b_cols + 31) / 32;
+  std::vector<int32_t> light;
+  std::vector<int4> heavy;                 // {cluster, first slice, end slice, bitmap}
+  std::vector<HeavyPart> parts;
+  int64_t nslices = 0;
+  P.heavy_id.assign(nc, -1);
+  for (int32_t c : P.large) {
+    const int64_t na = P.harp[c + 1] - P.harp[c];
+    int64_t np = std::max<int64_t>((P.hflop[c] + kHeavyPartProducts - 1) / kHeavyPartProducts,
+                                   (na + kTileCursors - 1) / kTileCursors);
+    np = std::min<int64_t>(np, na);
+    const bool is_heavy = P.hflop[c] >= g_heavy_products && na > 0 && words > 0 &&
+                          (nslices + np) * words * 4 <= kHeavySliceBytes &&
+                          ((int64_t)heavy.size() + 1) * words * 4 <= kHeavyBitmapBytes;
+    if (!is_heavy) {
+      light.push_back(c);
+      continue;
+    }
+    P.heavy_id[c] = (int32_t)heavy.size();
+    heavy.push_back(make_int4(c, (int)nslices, (int)(nslices + np), (int)heavy.size()));
+    for (int64_t i = 0; i < np; ++i)
+      parts.push_back(HeavyPart{P.harp[c] + na * i / np, P.harp[c] + na * (i + 1) / np, (int32_t)(nslices + i), 0});
+    nslices += np;
+  }
+  std::vector<int2> heavy_list;
+  for (const int4 &h : heavy) heavy_list.push_back(make_int2(h.x, h.w));
+  P.dlight.upload(light);
+  P.dheavy.upload(heavy_list);
+  P.nlight = (int64_t)light.size();
+  P.nheavy = (int64_t)heavy.size();
+  P.heavy_words = words;
   const size_t nb = (size_t)std::max<int64_t>(nc, 1) * 8;
   P.nz.alloc(nb);
   P.cmin.alloc(nb);
@@ -1071,14 +1214,28 @@ void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols
   SPOPS_HIP_CHECK(hipMemsetAsync(P.cmax.p, 0, nb, s));
   P.ms_setup = tm.stop();
   // ---- symbolic, every cluster
+  const int b32 = (int)(b_nnz < (int64_t)INT32_MAX);
   if (!P.small.empty())
     hipLaunchKernelGGL(k_sym_small, dim3((unsigned)P.small.size()), dim3(kBlock), 0, s, P.dsmall.as<int32_t>(),
                        (int64_t)P.small.size(), A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64,
                        P.bcol32.as<int32_t>(), P.nz.as<int64_t>(), P.cmin.as<int64_t>(), P.cmax.as<int64_t>());
-  if (!P.large.empty())
-    hipLaunchKernelGGL(k_sym_large, dim3((unsigned)P.large.size()), dim3(kBlock), 0, s, P.dlarge.as<int32_t>(),
+  DevBuf dparts, dheavy4, slices;
+  if (!heavy.empty()) {
+    dparts.upload(parts);
+    dheavy4.upload(heavy);
+    slices.alloc((size_t)nslices * words * 4);
+    P.heavy_bm.alloc((size_t)heavy.size() * words * 4);
+    hipLaunchKernelGGL(k_heavy_part, dim3((unsigned)parts.size()), dim3(kBlock), 0, s, dparts.as<HeavyPart>(),
+                       A.colids.as<int32_t>(), brp64, P.bcol32.as<int32_t>(), (int32_t)b_cols, words,
+                       slices.as<uint32_t>(), b32);
+    hipLaunchKernelGGL(k_heavy_merge, dim3((unsigned)heavy.size()), dim3(kBlock), 0, s, dheavy4.as<int4>(),
+                       slices.as<uint32_t>(), words, P.heavy_bm.as<uint32_t>(), P.nz.as<int64_t>(),
+                       P.cmin.as<int64_t>(), P.cmax.as<int64_t>());
+  }
+  if (!light.empty())
+    hipLaunchKernelGGL(k_sym_large, dim3((unsigned)light.size()), dim3(kBlock), 0, s, P.dlight.as<int32_t>(),
                        A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64, P.bcol32.as<int32_t>(), (int32_t)b_cols,
-                       P.nz.as<int64_t>(), P.cmin.as<int64_t>(), P.cmax.as<int64_t>(), (int)(b_nnz < (int64_t)INT32_MAX));
+                       P.nz.as<int64_t>(), P.cmin.as<int64_t>(), P.cmax.as<int64_t>(), b32);
   SPOPS_HIP_CHECK(hipGetLastError());
   P.hnz.assign(nc, 0);
   if (nc > 0) SPOPS_HIP_CHECK(hipMemcpyAsync(P.hnz.data(), P.nz.p, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
@@ -1091,11 +1248,9 @@ void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols
   // clusters with very many products accumulate densely (see k_dense_products)
   P.hcmin.assign(nc, 0);
   P.hcmax.assign(nc, 0);
-  P.harp.assign(nc + 1, 0);
   if (nc > 0) {
     SPOPS_HIP_CHECK(hipMemcpyAsync(P.hcmin.data(), P.cmin.p, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
     SPOPS_HIP_CHECK(hipMemcpyAsync(P.hcmax.data(), P.cmax.p, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
-    SPOPS_HIP_CHECK(hipMemcpyAsync(P.harp.data(), A.rowptr.p, (size_t)(nc + 1) * 8, hipMemcpyDeviceToHost, s));
   }
   SPOPS_HIP_CHECK(hipStreamSynchronize(s));
   P.dense.assign(nc, 0);
@@ -1125,17 +1280,31 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
   Timer tm(s);
   // the panel's clusters with work, in the plan's bin order
   std::vector<int32_t> psmall, plarge, pregular, pdense;
-  DevBuf dsmall, dlarge;
+  DevBuf dsmall;
   if (!whole) {
     for (int32_t c : P.small) if (c >= c0 && c < c1) psmall.push_back(c);
     for (int32_t c : P.large) if (c >= c0 && c < c1) plarge.push_back(c);
     dsmall.upload(psmall);
-    dlarge.upload(plarge);
   }
   const std::vector<int32_t> &small = whole ? P.small : psmall;
   const std::vector<int32_t> &large = whole ? P.large : plarge;
   const int32_t *lsmall = whole ? P.dsmall.as<int32_t>() : dsmall.as<int32_t>();
-  const int32_t *llarge = whole ? P.dlarge.as<int32_t>() : dlarge.as<int32_t>();
+  // the emit pass: large clusters without a kept bitmap, and the heavy ones
+  std::vector<int32_t> plight;
+  std::vector<int2> pheavy;
+  DevBuf dlight, dheavy;
+  if (!whole) {
+    for (int32_t c : plarge) {
+      if (P.heavy_id[c] >= 0) pheavy.push_back(make_int2(c, P.heavy_id[c]));
+      else plight.push_back(c);
+    }
+    dlight.upload(plight);
+    dheavy.upload(pheavy);
+  }
+  const int64_t nlight = whole ? P.nlight : (int64_t)plight.size();
+  const int64_t nheavy = whole ? P.nheavy : (int64_t)pheavy.size();
+  const int32_t *llight = whole ? P.dlight.as<int32_t>() : dlight.as<int32_t>();
+  const int2 *lheavy = whole ? P.dheavy.as<int2>() : dheavy.as<int2>();
   for (int32_t c : large) (P.dense[c] ? pdense : pregular).push_back(c);
   const bool split = !pdense.empty();
   const std::vector<int32_t> &regular = split ? pregular : large;
@@ -1234,10 +1403,13 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
   // so they get the panel's arrays shifted by c0 (only ids in [c0, c1) are touched).
   int64_t *crp = C.rowptr.as<int64_t>() - c0;
   int64_t *crpv = C.rowptr_val.as<int64_t>() - c0;
-  if (!large.empty())
-    hipLaunchKernelGGL(k_emit_large, dim3((unsigned)large.size()), dim3(kBlock), 0, s, llarge, A.rowptr.as<int64_t>(),
+  if (nlight > 0)
+    hipLaunchKernelGGL(k_emit_large, dim3((unsigned)nlight), dim3(kBlock), 0, s, llight, A.rowptr.as<int64_t>(),
                        A.colids.as<int32_t>(), P.brp64, P.bcol32.as<int32_t>(), crp, P.cmin.as<int64_t>(),
                        P.cmax.as<int64_t>(), C.colids.as<int32_t>(), (int)(P.b_nnz < (int64_t)INT32_MAX));
+  if (nheavy > 0)
+    hipLaunchKernelGGL(k_heavy_emit, dim3((unsigned)nheavy), dim3(kBlock), 0, s, lheavy, P.heavy_bm.as<uint32_t>(),
+                       P.heavy_words, crp, P.cmin.as<int64_t>(), P.cmax.as<int64_t>(), C.colids.as<int32_t>());
   SPOPS_HIP_CHECK(hipGetLastError());
   auto numeric = [&](auto tag) {
     using V = decltype(tag);

@@ -27,6 +27,12 @@ constexpr int large_chunk_cols(int csz, int cr) {
   return kLargeAccEntries / csz < large_hash_slots(cr) / 2 ? kLargeAccEntries / csz : large_hash_slots(cr) / 2;
 }
 constexpr int kMaxClusterRows = 16;              // A values of a column held in registers per pass
+// heavy clusters (at least g_heavy_products products): symbolic bitmaps built in parts of
+// about kHeavyPartProducts products by several workgroups and kept for the emit pass
+constexpr int64_t kHeavyProducts = (int64_t)1 << 20;        // default of g_heavy_products
+constexpr int64_t kHeavyPartProducts = (int64_t)1 << 18;
+constexpr int64_t kHeavySliceBytes = (int64_t)1 << 30;      // part bitmaps live during the symbolic pass
+constexpr int64_t kHeavyBitmapBytes = (int64_t)1 << 28;     // bitmaps a plan keeps
 constexpr int kSpgemmStats = 16;
 constexpr int64_t kDenseScratch = (int64_t)1 << 30;    // bytes of dense accumulators live at once
 constexpr int kDenseTaskCols = 32;                      // A columns per dense-product workgroup
@@ -64,8 +70,13 @@ struct spops_spgemm_plan_s {
   std::vector<int64_t> harp;               // A's cluster column pointers (host copy)
   std::vector<int64_t> hcmin, hcmax;       // per cluster: column range of C
   spops::DevBuf scratch;                   // dense accumulators (kDenseScratch at most)
-  spops::DevBuf dsmall, dlarge;
+  spops::DevBuf dsmall;
   spops::DevBuf nz, cmin, cmax;            // int64 [rows]
+  std::vector<int32_t> heavy_id;           // per cluster: index of its kept bitmap, or -1
+  spops::DevBuf heavy_bm;                  // uint32 [heavy clusters][heavy_words]
+  spops::DevBuf dlight;                    // int32: large clusters without a kept bitmap
+  spops::DevBuf dheavy;                    // int2 {cluster, bitmap}: the heavy clusters
+  int64_t heavy_words = 0, nheavy = 0, nlight = 0;
   int64_t nnzc = 0, nnzv = 0, products = 0;
   double ms_setup = 0, ms_symbolic = 0;
 };
@@ -74,6 +85,7 @@ namespace spops {
 extern int64_t g_dense_products;   // products from which a cluster accumulates densely (tuning)
 extern int64_t g_task_products;    // products per numeric task of a large cluster, 0: chosen per call (tuning)
 extern int g_wide_cursors;         // wide-cluster cursor variant: 0 off, 1 on (tuning)
+extern int64_t g_heavy_products;   // products from which a large cluster's bitmap is built in parts (tuning)
 void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *brp64,
                         const int64_t *bcol64, const void *bval, spops_spgemm_plan_s &P, hipStream_t s);
 // C = rows [c0, c1) of the product as a cluster matrix of c1 - c0 clusters; C's buffers are

@@ -1619,6 +1619,10 @@ extern "C" int spops_set_tuning(int key, int64_t value, int64_t *old) {
       SPOPS_REQUIRE(value >= 0 && value <= 1, SPOPS_ERROR_INVALID_VALUE, "wide cursor mode must be 0 or 1");
       if (old) *old = g_wide_cursors;
       g_wide_cursors = (int)value;
+    } else if (key == SPOPS_TUNE_HEAVY_PRODUCTS) {
+      SPOPS_REQUIRE(value >= 1, SPOPS_ERROR_INVALID_VALUE, "heavy product threshold must be >= 1");
+      if (old) *old = g_heavy_products;
+      g_heavy_products = value;
     } else if (key == SPOPS_TUNE_DENSE_PRODUCTS) {
       SPOPS_REQUIRE(value >= 1, SPOPS_ERROR_INVALID_VALUE, "dense product threshold must be >= 1");
       if (old) *old = g_dense_products;

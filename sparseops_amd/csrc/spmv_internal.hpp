@@ -134,6 +134,7 @@ void row_features_host(const int64_t *rowptr, const int64_t *col, int64_t n_rows
 extern int64_t g_dense_products;   // LeSpGEMM: products of a cluster from which it accumulates densely
 extern int64_t g_task_products;    // LeSpGEMM: products per numeric task of a large cluster (0: per call)
 extern int g_wide_cursors;         // LeSpGEMM: wide-cluster cursor variant (0 off, 1 on)
+extern int64_t g_heavy_products;   // LeSpGEMM: products from which a cluster's bitmap is built in parts
 // Builds and uploads the CSR plan of a matrix: SpMV row tasks and long rows (> kCsrLongRow
 // entries); for SpMM the short-row order and the segments of the rows > kSpmmLongRow.
 void upload_csr_plan(CsrDev &P, const std::vector<int64_t> &rp, int64_t n_rows);

@@ -204,6 +204,24 @@ class task_products:
         L.call("spops_set_tuning", L.TUNE_TASK_PRODUCTS, self.old, None)
 
 
+class heavy_products:
+    """Context manager: plans created inside build the symbolic bitmap of clusters with at
+    least `products` products with several workgroups and keep it for the emit pass (library
+    default 2^20)."""
+
+    def __init__(self, products: int):
+        self.products = int(products)
+
+    def __enter__(self):
+        old = c_i64()
+        L.call("spops_set_tuning", L.TUNE_HEAVY_PRODUCTS, self.products, ctypes.byref(old))
+        self.old = old.value
+        return self
+
+    def __exit__(self, *exc):
+        L.call("spops_set_tuning", L.TUNE_HEAVY_PRODUCTS, self.old, None)
+
+
 class wide_cursors:
     """Context manager: numeric calls made inside run clusters of 1025..4096 A columns in the
     variant with 4096 LDS cursors (two workgroups per CU) when mode is 1; 0 is the library

@@ -167,6 +167,55 @@ def test_lespgemm_wide_b_several_bitmap_tiles(dev):
     _check_against_oracle(A, B, out)
 
 
+@pytest.mark.parametrize("case", ["rmat", "wide_b", "narrow_b", "wide_rows"])
+def test_heavy_clusters_split_bitmaps(case, dev):
+    """Every large cluster heavy (threshold 1 product): its symbolic bitmap is built in parts
+    of the A columns by several workgroups, merged, kept by the plan, and C's columns are
+    emitted from it -- whole and in panels.  B with 600,000 columns (several 2^18-column
+    tiles with B-row cursors), with 700 columns (one tile, whole rows), and A rows of up to
+    3000 columns (parts of at most 1024 A columns): same C as the oracle."""
+    if case == "rmat":
+        A = rmat(11, 16, seed=5)
+        B = _random_b(A.n_cols, A.n_cols, 40, seed=6)
+    elif case == "wide_b":
+        A = rmat(10, 16, seed=3)
+        B = _random_b(A.n_cols, 600_000, 48, seed=4)
+    elif case == "narrow_b":
+        A = rmat(10, 16, seed=8)
+        B = _random_b(A.n_cols, 700, 30, seed=9)
+    else:
+        rng = np.random.default_rng(31)
+        n = 5000
+        lens = rng.integers(1, 10, n)
+        lens[[3, 900, 4999]] = [3000, 2100, 1500]
+        cols = [np.sort(rng.choice(n, size=L, replace=False)) for L in lens]
+        rp = np.zeros(n + 1, np.int64)
+        np.cumsum(lens, out=rp[1:])
+        col = np.concatenate(cols).astype(np.int64)
+        A = Csr(n, n, rp, col, rng.random(col.size) + 0.5)
+        B = A
+    with LG.heavy_products(1):
+        out = LG.lespgemm(A, B, device=dev)
+    assert out["stats"]["large_clusters"] > 0
+    _check_against_oracle(A, B, out)
+    # panels of a plan made with the heavy path, against the whole product
+    dA, dB = LG.DeviceCsr(A, dev), LG.DeviceCsr(B, dev)
+    perm, off = out["perm"], out["offsets"]
+    Acl = LG.ClusterMatrix.build(dA, perm, off)
+    whole = Acl.multiply(dB)
+    csz, rp, rpv, col, val = whole.download()
+    with LG.heavy_products(1):
+        plan = LG.SpgemmPlan(Acl, dB)
+    P = None
+    for c0, c1 in plan.panel_ranges(1 << 16):
+        P = plan.numeric(c0, c1, P)
+        pcsz, prp, prpv, pcol, pval = P.download()
+        np.testing.assert_array_equal(prp, rp[c0:c1 + 1] - rp[c0])
+        np.testing.assert_array_equal(pcol, col[rp[c0]:rp[c1]])
+        np.testing.assert_allclose(pval, val[rpv[c0]:rpv[c1]], rtol=1e-12, atol=0)
+    plan.close()
+
+
 def test_lespgemm_chunked_accumulator(dev):
     """Clusters whose C block has more than 1024 columns: the numeric pass runs several LDS
     accumulator chunks per cluster."""
