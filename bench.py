@@ -594,7 +594,11 @@ def main():
     res = {}
     for name, fmt, vt in variants:
         t = time.time()
-        M = S.csr(A, vt) if fmt == "csr" else S.csr_to_sell_c_sigma(A, 1024, 32, vt)
+        if fmt == "csrrows":                   # CSR with its row kernels only (no swept layout)
+            with S.spmv_layout("sliced"):
+                M = S.csr(A, vt)
+        else:
+            M = S.csr(A, vt) if fmt == "csr" else S.csr_to_sell_c_sigma(A, 1024, 32, vt)
         dt = M.xy_dtype
         x = torch.tensor(x64, dtype=dt, device=dev)
         y = torch.zeros(A.n_rows, dtype=dt, device=dev)
@@ -603,9 +607,9 @@ def main():
         r = {"wall_ms": wall_ms, "kernel_ms": kern_ms, "algo_bytes": M.algo_bytes,
              "gbs": M.algo_bytes / (wall_ms * 1e-3) / 1e9, "kernel_gbs": M.algo_bytes / (kern_ms * 1e-3) / 1e9,
              "device_bytes": M.device_bytes, "build_s": time.time() - t, "sweep": M.sweep,
-             "kernels": (["k_csr_spmv"] if M.format not in ("sell", "ell")
-                         else (["k_sweep_spmv", "k_sweep_reduce"] if M.sweep[1] > 1 else ["k_sweep_spmv"])
-                         if M.sweep else (["k_slice_spmv", "k_slice_reduce"] if M.slices > 1 else ["k_slice_spmv"]))}
+             "kernels": ((["k_sweep_spmv", "k_sweep_reduce"] if M.sweep[1] > 1 else ["k_sweep_spmv"]) if M.sweep
+                         else ["k_csr_spmv"] if M.format not in ("sell", "ell")
+                         else (["k_slice_spmv", "k_slice_reduce"] if M.slices > 1 else ["k_slice_spmv"]))}
         r.update(errors(yh, y64, absrow))
         res[name] = r
         log(f"{name}: {r['kernel_ms'] * 1e3:.1f} us/kernel, {r['kernel_gbs']:.0f} GB/s, "

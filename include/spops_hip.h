@@ -193,8 +193,10 @@ int spops_sddmm(spops_matrix A, int64_t k, const void *d_G, int g_type, const vo
 /* AT = A^T as a new CSR handle; AT's entry e is A's entry perm[e] (kept inside AT), rows of
  * AT hold A's entries in A's row order */
 int spops_csr_transpose(spops_matrix A, spops_matrix *AT);
-/* overwrite the stored values from a device array (cast to the stored type).  For a handle
- * made by spops_csr_transpose the array is in the ORIGINAL matrix's entry order. */
+/* overwrite the stored values from a device array (cast to the stored type; BF16 storage
+ * rounds each value once, to nearest even, from its FP64 / FP32 value).  For a handle made
+ * by spops_csr_transpose the array is in the ORIGINAL matrix's entry order.  A handle with a
+ * swept SpMV layout gets the new values there too. */
 int spops_csr_set_values(spops_matrix A, const void *d_values, int values_type, void *stream);
 
 /* info[0]=format info[1]=n_rows info[2]=n_cols info[3]=nnz info[4]=value_type
@@ -203,7 +205,8 @@ int spops_csr_set_values(spops_matrix A, const void *d_values, int values_type, 
  * the MFMA kernel runs it)
  * info[10]=C info[11]=sigma info[12]=bR (BSR) or column slices (SELL/ELL; 0 for the swept
  * layout, whose info[9] counts entry groups) info[13]=bC
- * (BSR) or max row width (ELL) info[14]=nchunks (reference layout) info[15]=index_type */
+ * (BSR) or max row width (ELL) or the entry groups of a CSR handle's swept SpMV layout (0
+ * when its SpMV runs the row kernels) info[14]=nchunks (reference layout) info[15]=index_type */
 int spops_matrix_info(spops_matrix A, int64_t info[16]);
 
 /* Tuning (process-wide; no reference counterpart).  Returns the previous value in *old (may
@@ -218,7 +221,10 @@ int spops_matrix_info(spops_matrix A, int64_t info[16]);
  *   afterwards: 0 (default) chosen per matrix, 1 the column-sliced stream, 2 the swept layout
  *   (blocks of consecutive rows summed in FP64 in LDS, each block's entries sorted by column).
  *   The default picks the swept layout when one XCD's rows gather from more than the slice
- *   bytes of x and 16384 rows hold at least one entry per 128-byte line of x.
+ *   bytes of x and 16384 rows hold at least one entry per 128-byte line of x.  CSR matrices
+ *   created afterwards follow the same choice for their SpMV: 1 keeps the row kernels, 2 and
+ *   the default's swept choice add the swept layout beside the CSR arrays (which SpMM, SDDMM
+ *   and the transpose keep using; spops_csr_set_values updates both copies).
  *   SPOPS_TUNE_SWEEP_WORKGROUPS: workgroups one swept-layout SpMV aims for (default 256, one
  *   per CU).  With B the fewest row blocks the row limit allows, a block is shared by
  *   H = this / B workgroups (1..16), and rows are cut into about this / H blocks of equal

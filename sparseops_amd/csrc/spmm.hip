@@ -42,9 +42,15 @@ template <> struct Ty<SPOPS_VALUE_F32> {
 template <> struct Ty<SPOPS_VALUE_BF16> {
   using T = uint16_t;
   static __device__ __forceinline__ double ld(uint16_t v) { return (double)bf16_to_float(v); }
-  // float -> bf16, round to nearest even (the device twin of float_to_bf16_rne)
+  // double -> float with round-to-odd, then float -> bf16 round to nearest even: a direct
+  // double -> bf16 rounding (the device twin of double_to_bf16_rne)
   static __device__ __forceinline__ uint16_t st(double d) {
-    const float f = (float)d;
+    float f = (float)d;
+    if ((double)f != d && f != 0.0f && f == f && f - f == 0.0f) {
+      uint32_t o = __float_as_uint(f);
+      if ((o & 1u) == 0u) o = (((double)f < d) == (f > 0.0f)) ? o + 1u : o - 1u;
+      f = __uint_as_float(o);
+    }
     uint32_t u = __float_as_uint(f);
     if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x0040u);
     u += 0x7fffu + ((u >> 16) & 1u);
@@ -285,6 +291,16 @@ __global__ __launch_bounds__(kBlock) void k_convert_values(const typename Ty<SC>
 }
 
 inline unsigned nblk(int64_t n, int per) { return grid_blocks(n, per); }
+
+// swept-layout values from the CSR values (bit copies; padding slots stay zero)
+template <typename B>
+__global__ __launch_bounds__(kBlock) void k_sweep_refresh(const int32_t *__restrict__ src, int64_t slots,
+                                                          const B *__restrict__ csr_val, B *__restrict__ sweep_val) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= slots) return;
+  const int32_t e = src[i];
+  sweep_val[i] = e >= 0 ? csr_val[e] : B(0);
+}
 
 void require_csr(const spops_matrix_s *A) {
   SPOPS_REQUIRE(A != nullptr, SPOPS_ERROR_INVALID_VALUE, "null matrix");
@@ -530,8 +546,27 @@ extern "C" int spops_csr_set_values(spops_matrix A, const void *d_values, int va
     require_type(values_type, "values");
     SPOPS_REQUIRE(A->nnz == 0 || d_values != nullptr, SPOPS_ERROR_INVALID_VALUE, "null values");
     const bool perm = A->csr.tperm.p != nullptr;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
     convert_values(d_values, values_type, perm ? A->csr.tperm.p : nullptr, A->csr.tperm64, A->nnz, A->csr.val.p,
-                   A->value_type, static_cast<hipStream_t>(stream));
+                   A->value_type, s);
+    if (A->sweep.H > 0 && A->slots > 0) {   // the swept SpMV layout's copy of the values
+      const int64_t slots = (int64_t)A->sweep.ngroups * kSweepGroup;
+      const int32_t *src = A->sweep.src.as<int32_t>();
+      switch (value_bytes(A->value_type)) {
+        case 8:
+          hipLaunchKernelGGL(k_sweep_refresh<uint64_t>, dim3(nblk(slots, kBlock)), dim3(kBlock), 0, s, src, slots,
+                             A->csr.val.as<uint64_t>(), A->sweep.val.as<uint64_t>());
+          break;
+        case 4:
+          hipLaunchKernelGGL(k_sweep_refresh<uint32_t>, dim3(nblk(slots, kBlock)), dim3(kBlock), 0, s, src, slots,
+                             A->csr.val.as<uint32_t>(), A->sweep.val.as<uint32_t>());
+          break;
+        default:
+          hipLaunchKernelGGL(k_sweep_refresh<uint16_t>, dim3(nblk(slots, kBlock)), dim3(kBlock), 0, s, src, slots,
+                             A->csr.val.as<uint16_t>(), A->sweep.val.as<uint16_t>());
+      }
+      SPOPS_HIP_CHECK(hipGetLastError());
+    }
     return SPOPS_SUCCESS;
   });
 }

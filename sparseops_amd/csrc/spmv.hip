@@ -774,6 +774,8 @@ struct RowSource {
   virtual int64_t length(int64_t g) const = 0;
   virtual int64_t orig_row(int64_t g) const = 0;
   virtual void entry(int64_t g, int64_t k, int64_t &col, double &val) const = 0;
+  // index of the entry in the source arrays (CSR sources; -1 when there is none)
+  virtual int64_t entry_index(int64_t g, int64_t k) const { (void)g; (void)k; return -1; }
 };
 
 int64_t g_slice_bytes = kSliceBytesDefault;
@@ -812,8 +814,10 @@ static void radix_sort_by(std::vector<int32_t> &idx, const std::vector<uint32_t>
 }
 
 // The swept layout (see SweepDev).  pos[r] is the SELL position of original row r, len[r] its length.
+// With slot_src, the source entry of every slot (-1 for padding) is kept on the device, so
+// that new values of the source arrays can be copied into the layout (CSR handles).
 static void build_sweep_layout(spops_matrix_s &M, const RowSource &src, const std::vector<int64_t> &pos,
-                               const std::vector<int64_t> &len, int vt) {
+                               const std::vector<int64_t> &len, int vt, bool slot_src = false) {
   auto &W = M.sweep;
   const int64_t n = M.n_rows, m = M.n_cols;
   // row blocks: H workgroups per block and about g_sweep_workgroups / H blocks of equal entry
@@ -864,8 +868,9 @@ static void build_sweep_layout(spops_matrix_s &M, const RowSource &src, const st
   std::vector<int32_t> idx;
   std::vector<uint32_t> bcol, bword;
   std::vector<double> bval;
+  std::vector<int32_t> bsrc, ssrc;
   for (int64_t b = 0; b < n_rb; ++b) {
-    bcol.clear(); bword.clear(); bval.clear();
+    bcol.clear(); bword.clear(); bval.clear(); bsrc.clear();
     for (int64_t r = rstart[b]; r < rstart[b + 1]; ++r)
       for (int64_t k = 0; k < len[r]; ++k) {
         int64_t c; double v;
@@ -873,6 +878,7 @@ static void build_sweep_layout(spops_matrix_s &M, const RowSource &src, const st
         bcol.push_back((uint32_t)c);
         bword.push_back((uint32_t)(((uint64_t)c & (((uint64_t)1 << cbits) - 1)) << rb) | (uint32_t)(r - rstart[b]));
         bval.push_back(v);
+        if (slot_src) bsrc.push_back((int32_t)src.entry_index(pos[r], k));
       }
     idx.resize(bcol.size());
     std::iota(idx.begin(), idx.end(), 0);
@@ -888,11 +894,13 @@ static void build_sweep_layout(spops_matrix_s &M, const RowSource &src, const st
         const size_t at = word.size();
         word.resize(at + kSweepGroup, 0u);
         dval.resize(at + kSweepGroup, 0.0);
+        if (slot_src) ssrc.resize(at + kSweepGroup, -1);
         for (int j = 0; j < valid; ++j) {
           const int l = j % kWave, u = j / kWave;
           const int32_t s = idx[g0 + j];
           word[at + (size_t)((u / gc) * kWave + l) * gc + u % gc] = bword[s];
           dval[at + (size_t)((u / gv) * kWave + l) * gv + u % gv] = bval[s];
+          if (slot_src) ssrc[at + (size_t)((u / gv) * kWave + l) * gv + u % gv] = bsrc[s];
         }
       }
       i = e;
@@ -912,6 +920,7 @@ static void build_sweep_layout(spops_matrix_s &M, const RowSource &src, const st
   for (double v : dval) pack_values(val, vt, v);
   W.word.upload(word);
   W.val.upload(val);
+  if (slot_src) W.src.upload(ssrc);
   W.gdesc.upload(gdesc);
   W.wgrp.upload(wgrp);
   W.rstart.upload(rstart);
@@ -921,18 +930,15 @@ static void build_sweep_layout(spops_matrix_s &M, const RowSource &src, const st
   W.n_rb = (int)n_rb;
   W.ngroups = (int)ngroups;
   M.slots = ngroups * kSweepGroup;
-  M.dev_bytes = W.word.bytes + W.val.bytes + W.gdesc.bytes + W.wgrp.bytes + W.rstart.bytes + W.part.bytes;
+  M.dev_bytes = W.word.bytes + W.val.bytes + W.src.bytes + W.gdesc.bytes + W.wgrp.bytes + W.rstart.bytes +
+                W.part.bytes;
 }
 
-void build_slice_layout(spops_matrix_s &M, const RowSource &src, int vt) {
-  auto &S = M.sell;
+// Column slices of the sliced layout: P from the x footprint of one XCD's rows (128-byte lines
+// of x touched by an eighth of the rows, the largest eighth).
+static int slice_count(const spops_matrix_s &M, const RowSource &src, const std::vector<int64_t> &pos,
+                       const std::vector<int64_t> &len, int vt) {
   const int64_t n = M.n_rows, m = M.n_cols;
-  std::vector<int64_t> pos(n);                          // original row -> SELL position
-  for (int64_t g = 0; g < n; ++g) pos[src.orig_row(g)] = g;
-  std::vector<int64_t> len(n);
-  for (int64_t r = 0; r < n; ++r) len[r] = src.length(pos[r]);
-  // slices: P from the x footprint of one XCD's rows (128-byte lines of x touched by an
-  // eighth of the rows, the largest eighth), column cuts at equal entry counts
   const int64_t sx = (int64_t)xy_bytes(vt);
   int64_t foot = 0;
   if (m * sx > g_slice_bytes) {
@@ -960,12 +966,29 @@ void build_slice_layout(spops_matrix_s &M, const RowSource &src, int vt) {
   }
   int P = 1;
   while (P < kXcds && foot > (int64_t)P * g_slice_bytes) P *= 2;
-  // Rows that gather from more x than one XCD's L2 holds: the swept layout, when a row block
-  // of kSweepRowsTypical rows holds at least one entry per 128-byte line of x (so the sorted
-  // sweep reads every line it fetches more than once on average).
-  if (g_sweep_mode == 2 || (g_sweep_mode == 0 && P > 1 && n > 0 &&
-                            (double)M.nnz * (double)std::min<int64_t>(n, kSweepRowsTypical) /
-                                    (double)n >= (double)((m * sx + 127) / 128))) {
+  return P;
+}
+
+// Rows that gather from more x than one XCD's L2 holds (P > 1) take the swept layout when a
+// row block of kSweepRowsTypical rows holds at least one entry per 128-byte line of x (so the
+// sorted sweep reads every line it fetches more than once on average); g_sweep_mode 1 / 2
+// force the other layout / the swept one.
+static bool sweep_wanted(const spops_matrix_s &M, int P, int vt) {
+  const int64_t n = M.n_rows, m = M.n_cols, sx = (int64_t)xy_bytes(vt);
+  return g_sweep_mode == 2 || (g_sweep_mode == 0 && P > 1 && n > 0 &&
+                               (double)M.nnz * (double)std::min<int64_t>(n, kSweepRowsTypical) / (double)n >=
+                                   (double)((m * sx + 127) / 128));
+}
+
+void build_slice_layout(spops_matrix_s &M, const RowSource &src, int vt) {
+  auto &S = M.sell;
+  const int64_t n = M.n_rows, m = M.n_cols;
+  std::vector<int64_t> pos(n);                          // original row -> SELL position
+  for (int64_t g = 0; g < n; ++g) pos[src.orig_row(g)] = g;
+  std::vector<int64_t> len(n);
+  for (int64_t r = 0; r < n; ++r) len[r] = src.length(pos[r]);
+  const int P = slice_count(M, src, pos, len, vt);
+  if (sweep_wanted(M, P, vt)) {
     build_sweep_layout(M, src, pos, len, vt);
     return;
   }
@@ -1171,7 +1194,7 @@ static void launch_spmv_vt(const spops_matrix_s &M, double alpha, const void *x,
   using Acc = typename T::Acc;
   const auto *xx = static_cast<const typename T::X *>(x);
   auto *yy = static_cast<typename T::X *>(y);
-  if (M.format == SPOPS_FORMAT_CSR) {
+  if (M.format == SPOPS_FORMAT_CSR && M.sweep.H == 0) {
     const auto &P = M.csr;
     int grid = P.nlong + (P.ntasks + 3) / 4;
     if (grid == 0) return;
@@ -1183,7 +1206,7 @@ static void launch_spmv_vt(const spops_matrix_s &M, double alpha, const void *x,
       hipLaunchKernelGGL((k_csr_spmv<VT, int>), dim3(grid), dim3(kBlock), 0, s, P.rowptr.as<int>(), P.col.as<int>(),
                          P.val.as<typename T::V>(), xx, (int)M.n_cols, yy, (Acc)alpha, (Acc)beta, P.tasks.as<int2>(), P.ntasks,
                          P.longs.as<int>(), P.nlong);
-  } else if ((M.format == SPOPS_FORMAT_SELL || M.format == SPOPS_FORMAT_ELL) && M.sweep.H > 0) {
+  } else if (M.sweep.H > 0) {
     const auto &W = M.sweep;
     if (M.n_rows == 0) return;
     SweepArgs wa;
@@ -1313,6 +1336,7 @@ struct CsrRows : RowSource {
     col = ci[j];
     val = host_value(vals, in_vt, j);
   }
+  int64_t entry_index(int64_t g, int64_t k) const override { return rp[perm[g]] + k; }
 };
 
 // the reference's per-slice order: sort by (nnz, row) descending inside each sigma-slice
@@ -1338,6 +1362,22 @@ std::vector<int32_t> sell_ref_chunk_len(const std::vector<int64_t> &rp, const st
   for (int64_t g = 0; g < n; ++g) cl[g / C] = std::max<int32_t>(cl[g / C], (int32_t)(rp[perm[g] + 1] - rp[perm[g]]));
   for (auto &v : cl) v = (int32_t)((v + align - 1) / align * align);
   return cl;
+}
+
+// CSR handles: the swept layout beside the CSR arrays (SpMV only), chosen as for SELL.
+static void add_csr_sweep(spops_matrix_s &M, const std::vector<int64_t> &rp, const std::vector<int64_t> &ci,
+                          const void *values, int in_vt, int vt) {
+  const int64_t n = M.n_rows;
+  if (g_sweep_mode == 1 || n == 0 || M.nnz == 0) return;
+  std::vector<int64_t> ident(n);
+  std::iota(ident.begin(), ident.end(), 0);
+  CsrRows src(rp, ci, ident, values, in_vt);
+  std::vector<int64_t> len(n);
+  for (int64_t r = 0; r < n; ++r) len[r] = rp[r + 1] - rp[r];
+  if (g_sweep_mode != 2 && !sweep_wanted(M, slice_count(M, src, ident, len, vt), vt)) return;
+  const int64_t slots = M.slots;
+  build_sweep_layout(M, src, ident, len, vt, true);
+  M.slots = slots;
 }
 
 }  // namespace
@@ -1366,7 +1406,9 @@ extern "C" int spops_csr_create(spops_matrix *A, int64_t n_rows, int64_t n_cols,
     P.val.upload(v);
     upload_csr_plan(P, rp, n_rows);
     M->slots = nnz;
-    M->dev_bytes = P.rowptr.bytes + P.col.bytes + P.val.bytes + P.tasks.bytes + P.longs.bytes;
+    add_csr_sweep(*M, rp, ci, values, in_vt, vt);
+    M->dev_bytes = P.rowptr.bytes + P.col.bytes + P.val.bytes + P.tasks.bytes + P.longs.bytes +
+                   (M->sweep.H > 0 ? M->dev_bytes : 0);
     M->algo_bytes = spmv_algo_bytes(*M, index_type == SPOPS_INDEX_I32 ? 4 : 8);
     *A = M.release();
     return SPOPS_SUCCESS;
@@ -1574,7 +1616,10 @@ extern "C" int spops_matrix_info(spops_matrix A, int64_t info[16]) {
     info[0] = A->format; info[1] = A->n_rows; info[2] = A->n_cols; info[3] = A->nnz; info[4] = A->value_type;
     info[5] = A->dev_bytes; info[6] = A->algo_bytes; info[7] = A->slots;
     info[15] = A->index_type;
-    if (A->format == SPOPS_FORMAT_CSR) { info[8] = A->csr.nlong; info[9] = A->csr.ntasks; }
+    if (A->format == SPOPS_FORMAT_CSR) {
+      info[8] = A->csr.nlong; info[9] = A->csr.ntasks;
+      if (A->sweep.H > 0) info[13] = A->sweep.ngroups;     // SpMV runs on the swept layout
+    }
     if (A->format == SPOPS_FORMAT_SELL || A->format == SPOPS_FORMAT_ELL) {
       info[8] = A->sell.nlong; info[9] = A->sell.ntasks;
       info[10] = A->sell.C; info[11] = A->sell.sigma; info[12] = A->sell.P; info[14] = A->ref_nchunks;
@@ -1637,7 +1682,8 @@ extern "C" int spops_set_tuning(int key, int64_t value, int64_t *old) {
 extern "C" int spops_matrix_layout(spops_matrix A, int which, void *host_out) {
   return guarded([&]() {
     SPOPS_REQUIRE(A != nullptr && host_out != nullptr, SPOPS_ERROR_INVALID_VALUE, "null argument");
-    SPOPS_REQUIRE(A->format == SPOPS_FORMAT_SELL || (which == 2 && A->format == SPOPS_FORMAT_ELL),
+    SPOPS_REQUIRE(A->format == SPOPS_FORMAT_SELL ||
+                      (which == 2 && (A->format == SPOPS_FORMAT_ELL || A->format == SPOPS_FORMAT_CSR)),
                   SPOPS_ERROR_NOT_SUPPORTED, "layout arrays exist for SELL only");
     if (which == 0) std::memcpy(host_out, A->sell.ref_reorder.data(), A->sell.ref_reorder.size() * 4);
     else if (which == 1) std::memcpy(host_out, A->sell.ref_chunk_len.data(), A->sell.ref_chunk_len.size() * 4);

@@ -89,6 +89,7 @@ constexpr int kSweepMaxSplit = 16;            // H limit
 
 struct SweepDev {
   DevBuf word, val;           // per group: kSweepGroup words / values, lane-interleaved
+  DevBuf src;                 // int32 per slot (CSR handles): its CSR entry, -1 for padding
   DevBuf gdesc;               // int per group: column block | valid entries << 16
   DevBuf wgrp;                // int per workgroup + 1: first group
   DevBuf rstart;              // int per row block + 1: first row
@@ -110,7 +111,7 @@ struct spops_matrix_s {
   int64_t dev_bytes = 0, algo_bytes = 0, slots = 0, ref_nchunks = 0, ell_width = 0;
   spops::CsrDev csr;
   spops::SliceDev sell;
-  spops::SweepDev sweep;      // used instead of `sell` when sweep.H > 0
+  spops::SweepDev sweep;      // SpMV layout used instead of `sell` (or beside `csr`) when sweep.H > 0
   spops::BsrDev bsr;
 };
 

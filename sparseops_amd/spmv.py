@@ -136,8 +136,9 @@ class SpMatrix:
     @property
     def sweep(self):
         """Swept layout parameters (row bits, workgroups per row block, row blocks, entry
-        groups), or None when the matrix uses another layout."""
-        if self.format not in ("sell", "ell"):
+        groups), or None when the matrix's SpMV uses another layout (CSR handles keep it beside
+        the CSR arrays, which SpMM, SDDMM and the transpose use)."""
+        if self.format not in ("sell", "ell", "csr"):
             return None
         out = np.zeros(4, dtype=np.int64)
         L.call("spops_matrix_layout", self._h, 2, out.ctypes.data)
@@ -239,8 +240,10 @@ LAYOUTS = {"auto": 0, "sliced": 1, "swept": 2}
 
 
 def spmv_layout(kind: str) -> tuning:
-    """Device layout of SELL/ELL matrices: "auto" (per matrix), "sliced" (column-sliced stream)
-    or "swept" (row blocks summed in LDS, entries of a block sorted by column)."""
+    """Device layout of SELL/ELL matrices created inside: "auto" (per matrix), "sliced"
+    (column-sliced stream) or "swept" (row blocks summed in LDS, entries of a block sorted by
+    column).  CSR matrices keep their row kernels under "sliced", add the swept layout for
+    SpMV under "swept", and choose as SELL does under "auto"."""
     return tuning(L.TUNE_SPMV_LAYOUT, LAYOUTS[kind])
 
 

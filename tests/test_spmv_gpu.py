@@ -99,9 +99,58 @@ def test_laplacian_closed_form(dev):
 @pytest.mark.parametrize("vt", ["f64", "f32", "bf16"])
 @pytest.mark.parametrize("name", ["lap", "pl", "edge"])
 def test_csr(mats, name, vt, dev):
+    """CSR row kernels (the swept layout held off)."""
     A = mats[name]
     x = uniform_x(A.n_cols, 1)
-    _check(A, S.csr(A, vt), x, _y0(A.n_rows), vt, dev=dev)
+    with S.spmv_layout("sliced"):
+        M = S.csr(A, vt)
+    assert M.sweep is None
+    _check(A, M, x, _y0(A.n_rows), vt, dev=dev)
+
+
+@pytest.mark.parametrize("wgs,maxrows", [(1, 20480), (256, 20480), (40, 777)])
+@pytest.mark.parametrize("vt", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("name", ["pl", "edge", "wide", "lap"])
+def test_csr_swept(mats, name, vt, wgs, maxrows, dev):
+    """CSR handles with the swept SpMV layout beside the CSR arrays: SpMV on the swept layout,
+    new values through set_values reach both copies (padding slots stay zero), and SpMM keeps
+    using the CSR arrays."""
+    A = _sweep_matrix(mats, name)
+    with S.spmv_layout("swept"), S.sweep_workgroups(wgs), S.sweep_max_rows(maxrows):
+        M = S.csr(A, vt)
+    assert M.sweep is not None and M.info[13] == M.sweep[3]
+    x = uniform_x(A.n_cols, 8)
+    _check(A, M, x, _y0(A.n_rows), vt, dev=dev)
+    y = _run(M, x, np.full(A.n_rows, np.nan), dev, 1.0, 0.0)
+    assert np.all(np.isfinite(y))
+    vals2 = np.random.default_rng(5).standard_normal(A.nnz)
+    M.set_values(torch.tensor(vals2, device=dev))
+    A2 = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, vals2)
+    _check(A2, M, x, _y0(A.n_rows), vt, dev=dev)
+    if vt == "f64":
+        X = np.random.default_rng(6).standard_normal((A.n_cols, 3))
+        Y = torch.zeros((A.n_rows, 3), dtype=torch.float64, device=dev)
+        M.spmm(torch.tensor(X, device=dev), Y)
+        torch.cuda.synchronize()
+        for j in range(3):
+            ref, sc = orc.spmv_exact(A2, X[:, j])
+            assert np.all(np.abs(Y[:, j].cpu().numpy() - ref) <= 1e-13 * sc + 1e-300)
+
+
+def test_csr_swept_auto_selection(dev):
+    """CSR handles add the swept layout under the same rule as SELL (random columns over an x
+    larger than the slice budget) and keep the row kernels for a band."""
+    n = 1 << 16
+    cols = np.sort((np.arange(n)[:, None] + np.arange(-3, 4)[None, :]) % n, axis=1)
+    band = Csr(n, n, (np.arange(n + 1) * 7).astype(np.int32), cols.reshape(-1).astype(np.int32),
+               np.random.default_rng(3).standard_normal(n * 7))
+    P = powerlaw(n, lmax=500, seed=2)
+    with S.slice_bytes(64 << 10):
+        B = S.csr(band, "f32")
+        R = S.csr(P, "f32")
+    assert B.sweep is None and R.sweep is not None
+    _check(band, B, uniform_x(n, 9), _y0(n), "f32", dev=dev)
+    _check(P, R, uniform_x(n, 9), _y0(n), "f32", dev=dev)
 
 
 @pytest.mark.parametrize("vt", ["f64", "f32", "bf16"])
