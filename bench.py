@@ -5,10 +5,10 @@
 A step is one y = A*x of that matrix with A, x, y resident in HBM.  The headline line is the
 FP32-value SELL-C-sigma (C = 32, sigma = 1024) SpMV, in achieved GB/s of algorithmic bytes
 ((n+1)*4 + nnz*(4+4) + n*4 (x) + n*4 (y) + n*4 (reorder) + chunks*4); the FP64 CSR drop-in,
-FP64 SELL, BF16 SELL and FP32 CSR run on the same matrix and are reported beside it with
-their error against an FP64 product (scipy, float64).  The SELL handles of this matrix use
-the swept device layout (row blocks summed in FP64 in LDS, each block's entries sorted by
-column).  The roofline object covers one SpMV of the headline variant (its two launches:
+FP64 SELL, BF16 SELL, FP32 CSR and the FP64 CSR row kernels run on the same matrix and are
+reported beside it with their error against an FP64 product (scipy, float64).  The SELL and
+CSR handles of this matrix use the swept device layout for SpMV (row blocks summed in FP64 in
+LDS, each block's entries sorted by column).  The roofline object covers one SpMV of the headline variant (its two launches:
 the swept kernel and the pass that adds the two partial planes), timed with HIP events on
 the launch stream; its HBM traffic comes from the committed rocprofv3 counter summary of
 the same command.  A secondary LeSpGEMM measurement (C = A*A,
@@ -545,8 +545,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variants", default="sell_f32,csr_f64,sell_f64,sell_bf16,csr_f32",
-                    help="comma list of <fmt>_<value type>; the first one is the headline")
+    ap.add_argument("--variants", default="sell_f32,csr_f64,sell_f64,sell_bf16,csr_f32,csrrows_f64",
+                    help="comma list of <fmt>_<value type> (fmt: sell, csr, or csrrows = CSR held to its row "
+                         "kernels); the first one is the headline")
     ap.add_argument("--lespgemm-scale", type=int, default=16, help="R-MAT scale of the LeSpGEMM line (0: skip)")
     ap.add_argument("--lespgemm-panels-scale", type=int, default=20,
                     help="R-MAT scale of the panelled LeSpGEMM line (0: skip)")
@@ -701,6 +702,8 @@ def main():
                                     if "csr_f64" in res and "sell_f32" in res else None),
         "fp32_sell_over_fp64_sell": (round(res["sell_f64"]["kernel_ms"] / res["sell_f32"]["kernel_ms"], 3)
                                      if "sell_f64" in res and "sell_f32" in res else None),
+        "fp32_sell_over_fp64_csr_row_kernels": (round(res["csrrows_f64"]["kernel_ms"] / res["sell_f32"]["kernel_ms"], 3)
+                                                if "csrrows_f64" in res and "sell_f32" in res else None),
         "lespgemm": lg,
         "lespgemm_panels": lp,
         "autograd_spmm": ag,
