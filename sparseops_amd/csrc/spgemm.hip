@@ -636,6 +636,7 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int4 *__restric
   __shared__ int32_t rpos[CURS];
   __shared__ int32_t rend[CURS];
   __shared__ int32_t nextcol[CURS];
+  __shared__ int next_row;                     // next row (relative to the cluster's first) to claim
   constexpr int W = kLargeBlock / kWave;
   const int4 tk = tasks[blockIdx.x];
   const int64_t c = tk.x;
@@ -688,6 +689,7 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int4 *__restric
     const int32_t lo = ccol[c0 + j0], hi = ccol[c0 + j0 + m - 1];
     for (int i = threadIdx.x; i < HS; i += kLargeBlock) hkey[i] = -1;
     for (int i = threadIdx.x; i < m * csz; i += kLargeBlock) acc[i] = V(0);
+    if (threadIdx.x == 0) next_row = W * G;   // rows past the first round, handed out on demand
     __syncthreads();
     for (int i = threadIdx.x; i < m; i += kLargeBlock) {
       const int32_t key = ccol[c0 + j0 + i];
@@ -699,6 +701,131 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int4 *__restric
     // lane groups of GL lanes, each scanning one B row
     const int grp = lane / GL, sub = lane % GL;
     const uint64_t gmask = ((GL == 64) ? ~0ull : ((1ull << GL) - 1)) << (grp * GL);
+    if (csz <= CR) {
+      // One pass per row.  Each lane group starts with its row of the first round (its
+      // prefetch-slot row) and then claims the next row from the LDS counter as soon as its
+      // row is done, so a few long B rows do not hold the other groups: with rows dealt out
+      // round-robin the slowest wave of a chunk took twice the mean number of steps.
+      int64_t t = first_base + grp;
+      bool slot = cursors, first = true, going = false, fetch = false, done = false;
+      int64_t k = 0, ke = 0;
+      V ar[CR];
+#pragma unroll
+      for (int l = 0; l < CR; ++l) ar[l] = V(0);
+      // row t's entries in this chunk start at k (false: none)
+      auto start_row = [&]() {
+        bool live = t < a1 && !(cursors && nextcol[t - a0] > hi);
+        k = 0;
+        ke = 0;
+        if (live) {
+          if (cursors) {
+            k = rpos[t - a0];
+            ke = rend[t - a0];
+          } else {
+            const int32_t b = acol[t];
+            k = brp[b];
+            ke = brp[b + 1];
+            int64_t k1 = ke;
+            row_range(bcol, k, k1, lo, hi);
+          }
+          live = k < ke;
+        }
+        if (live) {
+          if (kCacheA && slot && pf_has_a) {
+#pragma unroll
+            for (int l = 0; l < CA; ++l) ar[l] = pf_a[l];
+          } else {
+            const V *a = aval + av0 + (t - a0) * csz;
+#pragma unroll
+            for (int l = 0; l < CR; ++l) ar[l] = l < csz ? a[l] : V(0);
+            if (kCacheA && slot) {
+#pragma unroll
+              for (int l = 0; l < CA; ++l) pf_a[l] = ar[l];
+              pf_has_a = true;
+            }
+          }
+        }
+        first = true;
+        return live;
+      };
+      going = start_row();
+      fetch = !going;
+      for (;;) {
+        // groups without a row claim one (one claim per group and iteration)
+        int claim = 0;
+        if (fetch && !done && sub == 0) claim = atomicAdd(&next_row, 1);
+        claim = __shfl(claim, grp * GL, kWave);
+        if (fetch && !done) {
+          if (a0 + claim >= a1) {
+            done = true;
+          } else {
+            t = a0 + claim;
+            slot = false;
+            going = start_row();
+          }
+          fetch = !going && !done;
+        }
+        if (!__ballot(going)) {
+          if (__ballot(fetch)) continue;
+          break;
+        }
+        const int64_t kk = k + sub;
+        const bool have = going && kk < ke;
+        const bool use_pf = slot && first && pf_k == (int32_t)k;
+        int32_t key = INT32_MAX;
+        V bv = V(0);
+        if (have) {
+          if (use_pf) {
+            key = pf_key;
+            bv = pf_bv;
+          } else {
+            key = bcol[kk];
+            bv = bval[kk];
+          }
+        }
+        first = false;
+        const bool in = going && key <= hi;
+        const int n = __builtin_popcountll(__ballot(in) & gmask);
+        int h = -1;
+        if (in) {                              // every such column is one of the chunk's
+          uint32_t p = hslot(key, HS - 1);
+          for (int probe = 0; probe < HS; ++probe, p = (p + 1) & (HS - 1))
+            if (hkey[p] == key) { h = (int)p; break; }
+        }
+        if (h >= 0) {
+          V *dst = acc + (int)hval[h] * csz;
+#pragma unroll
+          for (int l = 0; l < CR; ++l) {
+            if (l >= csz) break;
+            if (ar[l] != V(0)) atomicAdd(&dst[l], ar[l] * bv);
+          }
+        }
+        const int32_t first_out = __shfl(key, grp * GL + (n < GL ? n : 0), kWave);
+        if (going) {
+          k += n;
+          if (n < GL) {                        // the row's part of the chunk is done
+            const int32_t nxt = first_out;     // its first column past hi (INT32_MAX: row end)
+            going = false;
+            fetch = true;
+            if (cursors && sub == 0) {
+              rpos[t - a0] = (int32_t)k;
+              nextcol[t - a0] = nxt;
+            }
+            if (slot) {                        // the entries it brings to its next chunk
+              pf_k = nxt == INT32_MAX ? -1 : (int32_t)k;
+              const int64_t k2 = k + sub;
+              if (pf_k >= 0 && k2 < ke) {
+                pf_key = bcol[k2];
+                pf_bv = bval[k2];
+              } else {
+                pf_key = INT32_MAX;
+                pf_bv = V(0);
+              }
+            }
+          }
+        }
+      }
+    } else {
     for (int64_t base = a0 + (int64_t)wave_id() * G; base < a1; base += (int64_t)W * G) {
       // every lane of the wave takes part in the ballots below, groups without a row idle
       const int64_t t = base + grp;
@@ -797,6 +924,7 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int4 *__restric
           pf_bv = V(0);
         }
       }
+    }
     }
     __syncthreads();
     V *out = cval + cv0 + j0 * csz;
@@ -975,6 +1103,32 @@ void cluster_build(spops_cluster_s &A, int64_t n, int64_t n_cols, int64_t nnz, c
 }
 
 namespace {
+
+// ids by decreasing flop[id], ties in their current order: a stable LSD radix sort of
+// (max - flop) with 16-bit digits (a comparison sort of the ~10^5..10^6 large clusters took
+// tens of milliseconds on the host)
+void sort_by_work_desc(std::vector<int32_t> &ids, const std::vector<int64_t> &flop) {
+  const size_t n = ids.size();
+  if (n < 2) return;
+  int64_t mx = 0;
+  for (int32_t c : ids) mx = std::max(mx, flop[c]);
+  std::vector<uint64_t> key(n), ktmp(n);
+  std::vector<int32_t> tmp(n);
+  for (size_t i = 0; i < n; ++i) key[i] = (uint64_t)(mx - flop[ids[i]]);
+  std::vector<size_t> cnt(65536 + 1);
+  for (int sh = 0; sh < 64 && ((uint64_t)mx >> sh) != 0; sh += 16) {
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (size_t i = 0; i < n; ++i) ++cnt[((key[i] >> sh) & 0xffffu) + 1];
+    for (size_t d = 0; d < 65536; ++d) cnt[d + 1] += cnt[d];
+    for (size_t i = 0; i < n; ++i) {
+      const size_t at = cnt[(key[i] >> sh) & 0xffffu]++;
+      tmp[at] = ids[i];
+      ktmp[at] = key[i];
+    }
+    ids.swap(tmp);
+    key.swap(ktmp);
+  }
+}
 
 struct Timer {
   hipEvent_t a, b;
@@ -1165,8 +1319,8 @@ void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols
     if (P.hflop[c] <= kSmallFlop && P.hflop[c] * P.hcsz[c] <= kSmallAcc) P.small.push_back((int32_t)c);
     else P.large.push_back((int32_t)c);
   }
-  // large clusters in descending work, so that the longest start first
-  std::stable_sort(P.large.begin(), P.large.end(), [&](int32_t a, int32_t b) { return P.hflop[a] > P.hflop[b]; });
+  // large clusters in descending work, so that the longest start first (ties in ascending id)
+  sort_by_work_desc(P.large, P.hflop);
   P.dsmall.upload(P.small);
   P.harp.assign(nc + 1, 0);
   if (nc > 0)
@@ -1325,13 +1479,15 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
   // the product, the more the tail of the launch matters
   int64_t task_products = g_task_products;
   if (task_products <= 0) {
-    task_products = kLargeTaskProducts;
-    while (task_products > kLargeTaskProductsMin) {
-      int64_t tasks = 0;
-      for (int32_t c : regular) tasks += pieces_of(c, task_products, nullptr);
-      if (tasks >= kLargeTasksMin) break;
-      task_products >>= 1;
-    }
+    constexpr int kSizes = 8;                  // candidate sizes kLargeTaskProducts >> i
+    int64_t tasks[kSizes] = {0};
+    int nsz = 1;
+    while (nsz < kSizes && (kLargeTaskProducts >> nsz) >= kLargeTaskProductsMin) ++nsz;
+    for (int32_t c : regular)
+      for (int i = 0; i < nsz; ++i) tasks[i] += pieces_of(c, kLargeTaskProducts >> i, nullptr);
+    int i = 0;
+    while (i + 1 < nsz && tasks[i] < kLargeTasksMin) ++i;
+    task_products = kLargeTaskProducts >> i;
   }
   // clusters of 1025..kLargeCursorsWide A columns in the variant with kLargeCursorsWide
   // cursors, when asked for (it runs at half the occupancy of the regular kernel)
