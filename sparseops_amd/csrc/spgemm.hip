@@ -288,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_num_small(const int32_t *__restrict_
 //             gives the count and the min / max column; a second kernel (emit) builds the
 //             same bitmaps over [min, max] and writes C's sorted columns (word popcount
 //             prefix = rank).
-//   numeric:  C's columns of the cluster are taken in chunks of m <= kLargeAccEntries / csz;
+//   numeric:  C's columns of the cluster are taken in chunks of m <= large_acc_entries / csz;
 //             a chunk's columns go into an LDS hash (column -> rank inside the chunk) and
 //             its csz-wide values accumulate in LDS; each B row contributes only its entries
 //             inside the chunk's column range (binary search, B rows are sorted).
@@ -619,7 +619,8 @@ __global__ __launch_bounds__(kBlock) void k_heavy_emit(const int2 *__restrict__ 
 // with 78 KB of LDS, two workgroups per CU, that takes the tasks of clusters with more
 // columns: cursors instead of two binary searches per row and chunk).
 template <typename V, int CR, int CURS>
-__global__ __launch_bounds__(kLargeBlock) void k_num_large(const int4 *__restrict__ tasks, const int64_t *__restrict__ arp,
+__global__ __launch_bounds__(kLargeBlock) __attribute__((amdgpu_waves_per_eu(CR <= 4 && CURS <= kLargeCursors ? 8 : 1)))
+void k_num_large(const int4 *__restrict__ tasks, const int64_t *__restrict__ arp,
                                                            const int64_t *__restrict__ arpv, const int32_t *__restrict__ csz_a,
                                                            const int32_t *__restrict__ acol, const V *__restrict__ aval,
                                                            const int64_t *__restrict__ brp, const int32_t *__restrict__ bcol,
@@ -627,7 +628,7 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int4 *__restric
                                                            const int64_t *__restrict__ crpv, const int32_t *__restrict__ ccol,
                                                            V *__restrict__ cval, bool b32) {
   constexpr int HS = large_hash_slots(CR);
-  __shared__ V acc[kLargeAccEntries];
+  __shared__ V acc[large_acc_entries(CR)];
   __shared__ int32_t hkey[HS];
   __shared__ int16_t hval[HS];
   // per A column (when the cluster has at most kLargeCursors and B's offsets fit 32 bits):
@@ -701,7 +702,7 @@ __global__ __launch_bounds__(kLargeBlock) void k_num_large(const int4 *__restric
     // lane groups of GL lanes, each scanning one B row
     const int grp = lane / GL, sub = lane % GL;
     const uint64_t gmask = ((GL == 64) ? ~0ull : ((1ull << GL) - 1)) << (grp * GL);
-    if (csz <= CR) {
+    if (CR <= 4 || csz <= CR) {              // (the CR = 4 kernel only gets clusters of <= 4 rows)
       // One pass per row.  Each lane group starts with its row of the first round (its
       // prefetch-slot row) and then claims the next row from the LDS counter as soon as its
       // row is done, so a few long B rows do not hold the other groups: with rows dealt out

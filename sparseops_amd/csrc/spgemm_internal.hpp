@@ -10,9 +10,9 @@ constexpr int kSmallTable = 2048;                // LDS hash slots (load factor 
 constexpr int kSmallAcc = 4096;                  // LDS accumulator entries (nz * csz)
 constexpr int kLargeTileBits = 1 << 18;         // columns per LDS bitmap pass (32 KiB)
 constexpr int kLargeBlock = 512;                 // threads of a large-cluster numeric workgroup
-constexpr int kLargeAccEntries = 2304;           // LDS accumulator entries of a numeric chunk
+constexpr int kLargeAccEntries = 2304;           // LDS accumulator entries of a numeric chunk (CR = 16)
 constexpr int kLargeHash = 1024;                 // LDS hash slots of a chunk's columns
-constexpr int kLargeCursors = 1024;              // A columns whose B-row position is kept in LDS
+constexpr int kLargeCursors = 1016;              // A columns whose B-row position is kept in LDS
 constexpr int kLargeCursorsWide = 4096;          // the same in the variant for wider clusters (two workgroups per CU)
 constexpr int kLargeGroupLanes = 4;              // fewest lanes scanning one B row in the numeric pass
 constexpr int64_t kLargeTaskProducts = (int64_t)1 << 15;    // products per numeric task of a split cluster ...
@@ -23,8 +23,12 @@ constexpr int64_t kLargeTasksMin = 32768;                    // fewer tasks than
 // visits every row of the cluster once
 constexpr int large_hash_slots(int cr) { return cr <= 4 ? 2 * kLargeHash : kLargeHash; }
 // output columns of a numeric chunk of a cluster of csz rows
+// LDS accumulator entries: 2048 for clusters of at most 4 rows, so that with 1016 cursors the
+// CR = 4 kernel takes 40 KiB of LDS and four workgroups (eight waves per SIMD) fit one CU
+constexpr int large_acc_entries(int cr) { return cr <= 4 ? 2048 : kLargeAccEntries; }
 constexpr int large_chunk_cols(int csz, int cr) {
-  return kLargeAccEntries / csz < large_hash_slots(cr) / 2 ? kLargeAccEntries / csz : large_hash_slots(cr) / 2;
+  return large_acc_entries(cr) / csz < large_hash_slots(cr) / 2 ? large_acc_entries(cr) / csz
+                                                                 : large_hash_slots(cr) / 2;
 }
 constexpr int kMaxClusterRows = 16;              // A values of a column held in registers per pass
 // heavy clusters (at least g_heavy_products products): symbolic bitmaps built in parts of
