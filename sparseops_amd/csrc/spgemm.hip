@@ -294,6 +294,10 @@ __global__ __launch_bounds__(kBlock) void k_num_small(const int32_t *__restrict_
 //             inside the chunk's column range (binary search, B rows are sorted).
 // ---------------------------------------------------------------------------------------------
 
+// threads of the bitmap kernels (symbolic, emit, heavy parts): 36 KiB of LDS each, so four
+// workgroups per CU; eight waves each keep eight per SIMD in flight
+constexpr int kBitmapBlock = 512;
+
 // [k0, k1) narrowed to the entries of a sorted B row with columns in [lo, hi]
 __device__ __forceinline__ void row_range(const int32_t *__restrict__ bcol, int64_t &k0, int64_t &k1, int32_t lo,
                                           int32_t hi) {
@@ -323,8 +327,8 @@ __device__ __forceinline__ void tile_bits(uint32_t *bm, int64_t t0, int64_t t1, 
   // issues four independent loads per step.
   const int64_t na = t1 - t0;
   int G = 16;
-  while (G < kBlock && (int64_t)(kBlock / G) > 2 * na) G <<= 1;
-  const int NG = kBlock / G;
+  while (G < kBitmapBlock && (int64_t)(kBitmapBlock / G) > 2 * na) G <<= 1;
+  const int NG = kBitmapBlock / G;
   const int grp = (int)threadIdx.x / G, sub = (int)threadIdx.x % G;
   constexpr int U = 4;
   for (int64_t t = t0 + grp; t < t1; t += NG) {
@@ -356,8 +360,8 @@ __device__ __forceinline__ void tile_bits_cur(uint32_t *bm, int64_t t0, int64_t 
                                               int32_t lo, int32_t hi, int32_t *cur) {
   const int64_t na = t1 - t0;
   int G = 16;
-  while (G < kWave && (int64_t)(kBlock / G) > 2 * na) G <<= 1;
-  const int NG = kBlock / G;
+  while (G < kWave && (int64_t)(kBitmapBlock / G) > 2 * na) G <<= 1;
+  const int NG = kBitmapBlock / G;
   const int grp = (int)threadIdx.x / G, sub = (int)threadIdx.x % G;
   constexpr int U = 16;
   for (int64_t t = t0 + grp; t < t1; t += NG) {
@@ -399,13 +403,14 @@ __device__ __forceinline__ void tile_bits_cur(uint32_t *bm, int64_t t0, int64_t 
 }
 
 // writes the columns base + 32 w + b of the set bits b of words[0, nwords) to ccol[out...] in
-// ascending order (out advances): rounds of kBlock consecutive words, one per thread, with a
+// ascending order (out advances): rounds of BLK consecutive words, one per thread, with a
 // block scan of the per-thread counts; the columns of neighbouring threads are neighbours in
 // C, so each store instruction of the wave covers a short range
 constexpr int kTileWords = kLargeTileBits / 32;
+template <int BLK>
 __device__ __forceinline__ void emit_words(const uint32_t *words, int nwords, int32_t base, int64_t &out,
                                            int32_t *__restrict__ ccol, int32_t *wsum) {
-  for (int w0 = 0; w0 < nwords; w0 += kBlock) {
+  for (int w0 = 0; w0 < nwords; w0 += BLK) {
     const int w = w0 + (int)threadIdx.x;
     uint32_t bits = w < nwords ? words[w] : 0u;
     const int32_t mine = __popc(bits);
@@ -418,7 +423,7 @@ __device__ __forceinline__ void emit_words(const uint32_t *words, int nwords, in
     if (lane_id() == kWave - 1) wsum[wave_id()] = inc;
     __syncthreads();
     int32_t before = 0, total = 0;
-    for (int q = 0; q < kBlock / kWave; ++q) {
+    for (int q = 0; q < BLK / kWave; ++q) {
       if (q < wave_id()) before += wsum[q];
       total += wsum[q];
     }
@@ -433,7 +438,7 @@ __device__ __forceinline__ void emit_words(const uint32_t *words, int nwords, in
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_sym_large(const int32_t *__restrict__ list, const int64_t *__restrict__ arp,
+__global__ __launch_bounds__(kBitmapBlock) void k_sym_large(const int32_t *__restrict__ list, const int64_t *__restrict__ arp,
                                                       const int32_t *__restrict__ acol, const int64_t *__restrict__ brp,
                                                       const int32_t *__restrict__ bcol, int32_t b_cols,
                                                       int64_t *__restrict__ nz, int64_t *__restrict__ cmin,
@@ -447,16 +452,16 @@ __global__ __launch_bounds__(kBlock) void k_sym_large(const int32_t *__restrict_
   const bool whole = b_cols <= kLargeTileBits;
   const bool cursors = !whole && b32 && t1 - t0 <= kTileCursors;
   if (cursors)
-    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBlock) cur[t - t0] = (int32_t)brp[acol[t]];
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBitmapBlock) cur[t - t0] = (int32_t)brp[acol[t]];
   for (int32_t lo = 0; lo < b_cols; lo += kLargeTileBits) {
     const int32_t hi = (int32_t)min((int64_t)lo + kLargeTileBits, (int64_t)b_cols) - 1;
-    for (int w = threadIdx.x; w < kLargeTileBits / 32; w += kBlock) bm[w] = 0u;
+    for (int w = threadIdx.x; w < kLargeTileBits / 32; w += kBitmapBlock) bm[w] = 0u;
     __syncthreads();
     if (cursors) tile_bits_cur(bm, t0, t1, acol, brp, bcol, lo, hi, cur);
     else tile_bits(bm, t0, t1, acol, brp, bcol, lo, hi, whole);
     __syncthreads();
     int n = 0, first = INT32_MAX, last = -1;
-    for (int w = threadIdx.x; w < kLargeTileBits / 32; w += kBlock) {
+    for (int w = threadIdx.x; w < kLargeTileBits / 32; w += kBitmapBlock) {
       const uint32_t x = bm[w];
       if (x) {
         n += __popc(x);
@@ -479,14 +484,14 @@ __global__ __launch_bounds__(kBlock) void k_sym_large(const int32_t *__restrict_
 }
 
 // C's sorted columns of each large cluster
-__global__ __launch_bounds__(kBlock) void k_emit_large(const int32_t *__restrict__ list, const int64_t *__restrict__ arp,
+__global__ __launch_bounds__(kBitmapBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_emit_large(const int32_t *__restrict__ list, const int64_t *__restrict__ arp,
                                                        const int32_t *__restrict__ acol, const int64_t *__restrict__ brp,
                                                        const int32_t *__restrict__ bcol, const int64_t *__restrict__ crp,
                                                        const int64_t *__restrict__ cmin, const int64_t *__restrict__ cmax,
                                                        int32_t *__restrict__ ccol, int b32) {
   __shared__ uint32_t bm[kLargeTileBits / 32];
   __shared__ int32_t cur[kTileCursors];
-  __shared__ int32_t wsum[kBlock / kWave];
+  __shared__ int32_t wsum[kBitmapBlock / kWave];
   const int64_t c = list[blockIdx.x];
   const int64_t t0 = arp[c], t1 = arp[c + 1];
   const int32_t cmn = (int32_t)cmin[c], cmx = (int32_t)cmax[c];
@@ -494,16 +499,16 @@ __global__ __launch_bounds__(kBlock) void k_emit_large(const int32_t *__restrict
   int64_t out = crp[c];
   const bool cursors = cmx - cmn >= kLargeTileBits && b32 && t1 - t0 <= kTileCursors;
   if (cursors)
-    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBlock) cur[t - t0] = (int32_t)brp[acol[t]];
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBitmapBlock) cur[t - t0] = (int32_t)brp[acol[t]];
   for (int32_t lo = cmn; lo <= cmx; lo += kLargeTileBits) {
     const int32_t hi = (int32_t)min((int64_t)lo + kLargeTileBits - 1, (int64_t)cmx);
     const int nwords = ((hi - lo) >> 5) + 1;          // words that can hold bits of [lo, hi]
-    for (int w = threadIdx.x; w < nwords; w += kBlock) bm[w] = 0u;
+    for (int w = threadIdx.x; w < nwords; w += kBitmapBlock) bm[w] = 0u;
     __syncthreads();
     if (cursors) tile_bits_cur(bm, t0, t1, acol, brp, bcol, lo, hi, cur);
     else tile_bits(bm, t0, t1, acol, brp, bcol, lo, hi, cmx - cmn < kLargeTileBits);
     __syncthreads();
-    emit_words(bm, nwords, lo, out, ccol, wsum);
+    emit_words<kBitmapBlock>(bm, nwords, lo, out, ccol, wsum);
   }
 }
 
@@ -517,7 +522,7 @@ struct HeavyPart {
   int32_t slice, pad;      // bitmap slice written
 };
 
-__global__ __launch_bounds__(kBlock) void k_heavy_part(const HeavyPart *__restrict__ parts,
+__global__ __launch_bounds__(kBitmapBlock) void k_heavy_part(const HeavyPart *__restrict__ parts,
                                                        const int32_t *__restrict__ acol, const int64_t *__restrict__ brp,
                                                        const int32_t *__restrict__ bcol, int32_t b_cols, int64_t words,
                                                        uint32_t *__restrict__ slices, int b32) {
@@ -527,18 +532,18 @@ __global__ __launch_bounds__(kBlock) void k_heavy_part(const HeavyPart *__restri
   const bool whole = b_cols <= kLargeTileBits;
   const bool cursors = !whole && b32 && p.t1 - p.t0 <= kTileCursors;
   if (cursors)
-    for (int64_t t = p.t0 + threadIdx.x; t < p.t1; t += kBlock) cur[t - p.t0] = (int32_t)brp[acol[t]];
+    for (int64_t t = p.t0 + threadIdx.x; t < p.t1; t += kBitmapBlock) cur[t - p.t0] = (int32_t)brp[acol[t]];
   uint32_t *out = slices + (int64_t)p.slice * words;
   for (int64_t lo64 = 0; lo64 < b_cols; lo64 += kLargeTileBits) {
     const int32_t lo = (int32_t)lo64;
     const int32_t hi = (int32_t)(min(lo64 + kLargeTileBits, (int64_t)b_cols) - 1);
     const int nwords = ((hi - lo) >> 5) + 1;
-    for (int w = threadIdx.x; w < nwords; w += kBlock) bm[w] = 0u;
+    for (int w = threadIdx.x; w < nwords; w += kBitmapBlock) bm[w] = 0u;
     __syncthreads();
     if (cursors) tile_bits_cur(bm, p.t0, p.t1, acol, brp, bcol, lo, hi, cur);
     else tile_bits(bm, p.t0, p.t1, acol, brp, bcol, lo, hi, whole);
     __syncthreads();
-    for (int w = threadIdx.x; w < nwords; w += kBlock) out[(lo >> 5) + w] = bm[w];
+    for (int w = threadIdx.x; w < nwords; w += kBitmapBlock) out[(lo >> 5) + w] = bm[w];
     __syncthreads();
   }
 }
@@ -601,7 +606,7 @@ __global__ __launch_bounds__(kBlock) void k_heavy_emit(const int2 *__restrict__ 
   const uint32_t *src = gbm + (int64_t)e.y * words + w0;
   for (int64_t w = 0; w <= w1 - w0; w += kTileWords) {
     const int nwords = (int)min((int64_t)kTileWords, w1 - w0 + 1 - w);
-    emit_words(src + w, nwords, (int32_t)((w0 + w) * 32), out, ccol, wsum);
+    emit_words<kBlock>(src + w, nwords, (int32_t)((w0 + w) * 32), out, ccol, wsum);
   }
 }
 
@@ -1380,7 +1385,7 @@ void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols
     dheavy4.upload(heavy);
     slices.alloc((size_t)nslices * words * 4);
     P.heavy_bm.alloc((size_t)heavy.size() * words * 4);
-    hipLaunchKernelGGL(k_heavy_part, dim3((unsigned)parts.size()), dim3(kBlock), 0, s, dparts.as<HeavyPart>(),
+    hipLaunchKernelGGL(k_heavy_part, dim3((unsigned)parts.size()), dim3(kBitmapBlock), 0, s, dparts.as<HeavyPart>(),
                        A.colids.as<int32_t>(), brp64, P.bcol32.as<int32_t>(), (int32_t)b_cols, words,
                        slices.as<uint32_t>(), b32);
     hipLaunchKernelGGL(k_heavy_merge, dim3((unsigned)heavy.size()), dim3(kBlock), 0, s, dheavy4.as<int4>(),
@@ -1388,7 +1393,7 @@ void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols
                        P.cmin.as<int64_t>(), P.cmax.as<int64_t>());
   }
   if (!light.empty())
-    hipLaunchKernelGGL(k_sym_large, dim3((unsigned)light.size()), dim3(kBlock), 0, s, P.dlight.as<int32_t>(),
+    hipLaunchKernelGGL(k_sym_large, dim3((unsigned)light.size()), dim3(kBitmapBlock), 0, s, P.dlight.as<int32_t>(),
                        A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64, P.bcol32.as<int32_t>(), (int32_t)b_cols,
                        P.nz.as<int64_t>(), P.cmin.as<int64_t>(), P.cmax.as<int64_t>(), b32);
   SPOPS_HIP_CHECK(hipGetLastError());
@@ -1561,7 +1566,7 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
   int64_t *crp = C.rowptr.as<int64_t>() - c0;
   int64_t *crpv = C.rowptr_val.as<int64_t>() - c0;
   if (nlight > 0)
-    hipLaunchKernelGGL(k_emit_large, dim3((unsigned)nlight), dim3(kBlock), 0, s, llight, A.rowptr.as<int64_t>(),
+    hipLaunchKernelGGL(k_emit_large, dim3((unsigned)nlight), dim3(kBitmapBlock), 0, s, llight, A.rowptr.as<int64_t>(),
                        A.colids.as<int32_t>(), P.brp64, P.bcol32.as<int32_t>(), crp, P.cmin.as<int64_t>(),
                        P.cmax.as<int64_t>(), C.colids.as<int32_t>(), (int)(P.b_nnz < (int64_t)INT32_MAX));
   if (nheavy > 0)
