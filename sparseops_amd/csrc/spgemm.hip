@@ -354,10 +354,13 @@ __device__ __forceinline__ void tile_bits(uint32_t *bm, int64_t t0, int64_t t1, 
 // every column of the cluster's rows), so a row's part of the tile starts at its cursor and
 // ends at its first column past hi: found by the scan itself, with no binary search.  Lane
 // groups are at most a wave (the end is a group-wide minimum); four loads per lane in flight.
-constexpr int kTileCursors = 1024;
-__device__ __forceinline__ void tile_bits_cur(uint32_t *bm, int64_t t0, int64_t t1, const int32_t *__restrict__ acol,
-                                              const int64_t *__restrict__ brp, const int32_t *__restrict__ bcol,
-                                              int32_t lo, int32_t hi, int32_t *cur) {
+// The end of each row (cend, LDS) is read once with the first cursor, so a row's visit of a
+// tile costs one round trip (its entries) instead of three (A column, row end, entries).
+// 1020 cursors: 32 KiB of bitmap + 2 x 4080 bytes of cursors and ends leave four workgroups
+// per CU.
+constexpr int kTileCursors = 1020;
+__device__ __forceinline__ void tile_bits_cur(uint32_t *bm, int64_t t0, int64_t t1, const int32_t *__restrict__ bcol,
+                                              int32_t lo, int32_t hi, int32_t *cur, const int32_t *cend) {
   const int64_t na = t1 - t0;
   int G = 16;
   while (G < kWave && (int64_t)(kBitmapBlock / G) > 2 * na) G <<= 1;
@@ -365,7 +368,7 @@ __device__ __forceinline__ void tile_bits_cur(uint32_t *bm, int64_t t0, int64_t 
   const int grp = (int)threadIdx.x / G, sub = (int)threadIdx.x % G;
   constexpr int U = 16;
   for (int64_t t = t0 + grp; t < t1; t += NG) {
-    const int32_t k1 = (int32_t)brp[acol[t] + 1];
+    const int32_t k1 = cend[t - t0];
     int32_t k = cur[t - t0];
     bool going = k < k1;                      // uniform inside the group
     while (going) {
@@ -442,22 +445,31 @@ __global__ __launch_bounds__(kBitmapBlock) void k_sym_large(const int32_t *__res
                                                       const int32_t *__restrict__ acol, const int64_t *__restrict__ brp,
                                                       const int32_t *__restrict__ bcol, int32_t b_cols,
                                                       int64_t *__restrict__ nz, int64_t *__restrict__ cmin,
-                                                      int64_t *__restrict__ cmax, int b32) {
+                                                      int64_t *__restrict__ cmax, int b32, int32_t *gcur,
+                                                      int32_t *gend) {
   __shared__ uint32_t bm[kLargeTileBits / 32];
-  __shared__ int32_t cur[kTileCursors];
+  __shared__ int32_t cur[kTileCursors], cend[kTileCursors];
   __shared__ int cnt, mn, mx;
   const int64_t c = list[blockIdx.x];
   const int64_t t0 = arp[c], t1 = arp[c + 1];
   if (threadIdx.x == 0) { cnt = 0; mn = INT32_MAX; mx = -1; }
   const bool whole = b_cols <= kLargeTileBits;
-  const bool cursors = !whole && b32 && t1 - t0 <= kTileCursors;
+  // cursors in LDS, or (clusters of more columns) in device memory at the cluster's columns
+  const bool lds_cur = t1 - t0 <= kTileCursors;
+  const bool cursors = !whole && b32 && (lds_cur || gcur != nullptr);
+  int32_t *cu = lds_cur ? cur : gcur + t0;
+  int32_t *ce = lds_cur ? cend : gend + t0;
   if (cursors)
-    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBitmapBlock) cur[t - t0] = (int32_t)brp[acol[t]];
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBitmapBlock) {
+      const int32_t b = acol[t];
+      cu[t - t0] = (int32_t)brp[b];
+      ce[t - t0] = (int32_t)brp[b + 1];
+    }
   for (int32_t lo = 0; lo < b_cols; lo += kLargeTileBits) {
     const int32_t hi = (int32_t)min((int64_t)lo + kLargeTileBits, (int64_t)b_cols) - 1;
     for (int w = threadIdx.x; w < kLargeTileBits / 32; w += kBitmapBlock) bm[w] = 0u;
     __syncthreads();
-    if (cursors) tile_bits_cur(bm, t0, t1, acol, brp, bcol, lo, hi, cur);
+    if (cursors) tile_bits_cur(bm, t0, t1, bcol, lo, hi, cu, ce);
     else tile_bits(bm, t0, t1, acol, brp, bcol, lo, hi, whole);
     __syncthreads();
     int n = 0, first = INT32_MAX, last = -1;
@@ -488,24 +500,32 @@ __global__ __launch_bounds__(kBitmapBlock) __attribute__((amdgpu_waves_per_eu(8)
                                                        const int32_t *__restrict__ acol, const int64_t *__restrict__ brp,
                                                        const int32_t *__restrict__ bcol, const int64_t *__restrict__ crp,
                                                        const int64_t *__restrict__ cmin, const int64_t *__restrict__ cmax,
-                                                       int32_t *__restrict__ ccol, int b32) {
+                                                       int32_t *__restrict__ ccol, int b32, int32_t *gcur,
+                                                       int32_t *gend) {
   __shared__ uint32_t bm[kLargeTileBits / 32];
-  __shared__ int32_t cur[kTileCursors];
+  __shared__ int32_t cur[kTileCursors], cend[kTileCursors];
   __shared__ int32_t wsum[kBitmapBlock / kWave];
   const int64_t c = list[blockIdx.x];
   const int64_t t0 = arp[c], t1 = arp[c + 1];
   const int32_t cmn = (int32_t)cmin[c], cmx = (int32_t)cmax[c];
   if (crp[c + 1] == crp[c]) return;
   int64_t out = crp[c];
-  const bool cursors = cmx - cmn >= kLargeTileBits && b32 && t1 - t0 <= kTileCursors;
+  const bool lds_cur = t1 - t0 <= kTileCursors;
+  const bool cursors = cmx - cmn >= kLargeTileBits && b32 && (lds_cur || gcur != nullptr);
+  int32_t *cu = lds_cur ? cur : gcur + t0;
+  int32_t *ce = lds_cur ? cend : gend + t0;
   if (cursors)
-    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBitmapBlock) cur[t - t0] = (int32_t)brp[acol[t]];
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += kBitmapBlock) {
+      const int32_t b = acol[t];
+      cu[t - t0] = (int32_t)brp[b];
+      ce[t - t0] = (int32_t)brp[b + 1];
+    }
   for (int32_t lo = cmn; lo <= cmx; lo += kLargeTileBits) {
     const int32_t hi = (int32_t)min((int64_t)lo + kLargeTileBits - 1, (int64_t)cmx);
     const int nwords = ((hi - lo) >> 5) + 1;          // words that can hold bits of [lo, hi]
     for (int w = threadIdx.x; w < nwords; w += kBitmapBlock) bm[w] = 0u;
     __syncthreads();
-    if (cursors) tile_bits_cur(bm, t0, t1, acol, brp, bcol, lo, hi, cur);
+    if (cursors) tile_bits_cur(bm, t0, t1, bcol, lo, hi, cu, ce);
     else tile_bits(bm, t0, t1, acol, brp, bcol, lo, hi, cmx - cmn < kLargeTileBits);
     __syncthreads();
     emit_words<kBitmapBlock>(bm, nwords, lo, out, ccol, wsum);
@@ -527,12 +547,16 @@ __global__ __launch_bounds__(kBitmapBlock) void k_heavy_part(const HeavyPart *__
                                                        const int32_t *__restrict__ bcol, int32_t b_cols, int64_t words,
                                                        uint32_t *__restrict__ slices, int b32) {
   __shared__ uint32_t bm[kLargeTileBits / 32];
-  __shared__ int32_t cur[kTileCursors];
+  __shared__ int32_t cur[kTileCursors], cend[kTileCursors];
   const HeavyPart p = parts[blockIdx.x];
   const bool whole = b_cols <= kLargeTileBits;
   const bool cursors = !whole && b32 && p.t1 - p.t0 <= kTileCursors;
   if (cursors)
-    for (int64_t t = p.t0 + threadIdx.x; t < p.t1; t += kBitmapBlock) cur[t - p.t0] = (int32_t)brp[acol[t]];
+    for (int64_t t = p.t0 + threadIdx.x; t < p.t1; t += kBitmapBlock) {
+      const int32_t b = acol[t];
+      cur[t - p.t0] = (int32_t)brp[b];
+      cend[t - p.t0] = (int32_t)brp[b + 1];
+    }
   uint32_t *out = slices + (int64_t)p.slice * words;
   for (int64_t lo64 = 0; lo64 < b_cols; lo64 += kLargeTileBits) {
     const int32_t lo = (int32_t)lo64;
@@ -540,7 +564,7 @@ __global__ __launch_bounds__(kBitmapBlock) void k_heavy_part(const HeavyPart *__
     const int nwords = ((hi - lo) >> 5) + 1;
     for (int w = threadIdx.x; w < nwords; w += kBitmapBlock) bm[w] = 0u;
     __syncthreads();
-    if (cursors) tile_bits_cur(bm, p.t0, p.t1, acol, brp, bcol, lo, hi, cur);
+    if (cursors) tile_bits_cur(bm, p.t0, p.t1, bcol, lo, hi, cur, cend);
     else tile_bits(bm, p.t0, p.t1, acol, brp, bcol, lo, hi, whole);
     __syncthreads();
     for (int w = threadIdx.x; w < nwords; w += kBitmapBlock) out[(lo >> 5) + w] = bm[w];
@@ -1375,6 +1399,11 @@ void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols
   P.ms_setup = tm.stop();
   // ---- symbolic, every cluster
   const int b32 = (int)(b_nnz < (int64_t)INT32_MAX);
+  // B-row cursors (position, end) per cluster column, for clusters of more than kTileCursors
+  // columns (the bitmap passes keep the others in LDS)
+  bool wide_cluster = false;
+  for (int32_t c : light) wide_cluster |= P.harp[c + 1] - P.harp[c] > kTileCursors;
+  if (b32 && b_cols > kLargeTileBits && wide_cluster) P.gcur.alloc((size_t)A.nnzc * 8);
   if (!P.small.empty())
     hipLaunchKernelGGL(k_sym_small, dim3((unsigned)P.small.size()), dim3(kBlock), 0, s, P.dsmall.as<int32_t>(),
                        (int64_t)P.small.size(), A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64,
@@ -1395,7 +1424,8 @@ void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols
   if (!light.empty())
     hipLaunchKernelGGL(k_sym_large, dim3((unsigned)light.size()), dim3(kBitmapBlock), 0, s, P.dlight.as<int32_t>(),
                        A.rowptr.as<int64_t>(), A.colids.as<int32_t>(), brp64, P.bcol32.as<int32_t>(), (int32_t)b_cols,
-                       P.nz.as<int64_t>(), P.cmin.as<int64_t>(), P.cmax.as<int64_t>(), b32);
+                       P.nz.as<int64_t>(), P.cmin.as<int64_t>(), P.cmax.as<int64_t>(), b32, P.gcur.as<int32_t>(),
+                       P.gcur.p ? P.gcur.as<int32_t>() + A.nnzc : nullptr);
   SPOPS_HIP_CHECK(hipGetLastError());
   P.hnz.assign(nc, 0);
   if (nc > 0) SPOPS_HIP_CHECK(hipMemcpyAsync(P.hnz.data(), P.nz.p, (size_t)nc * 8, hipMemcpyDeviceToHost, s));
@@ -1568,7 +1598,8 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
   if (nlight > 0)
     hipLaunchKernelGGL(k_emit_large, dim3((unsigned)nlight), dim3(kBitmapBlock), 0, s, llight, A.rowptr.as<int64_t>(),
                        A.colids.as<int32_t>(), P.brp64, P.bcol32.as<int32_t>(), crp, P.cmin.as<int64_t>(),
-                       P.cmax.as<int64_t>(), C.colids.as<int32_t>(), (int)(P.b_nnz < (int64_t)INT32_MAX));
+                       P.cmax.as<int64_t>(), C.colids.as<int32_t>(), (int)(P.b_nnz < (int64_t)INT32_MAX),
+                       P.gcur.as<int32_t>(), P.gcur.p ? P.gcur.as<int32_t>() + A.nnzc : nullptr);
   if (nheavy > 0)
     hipLaunchKernelGGL(k_heavy_emit, dim3((unsigned)nheavy), dim3(kBlock), 0, s, lheavy, P.heavy_bm.as<uint32_t>(),
                        P.heavy_words, crp, P.cmin.as<int64_t>(), P.cmax.as<int64_t>(), C.colids.as<int32_t>());

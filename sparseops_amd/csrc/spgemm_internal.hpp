@@ -75,6 +75,7 @@ struct spops_spgemm_plan_s {
   std::vector<int64_t> hcmin, hcmax;       // per cluster: column range of C
   spops::DevBuf scratch;                   // dense accumulators (kDenseScratch at most)
   spops::DevBuf dsmall;
+  spops::DevBuf gcur;                      // int32 [2][A's nnzc]: B-row cursors of wide clusters
   spops::DevBuf nz, cmin, cmax;            // int64 [rows]
   std::vector<int32_t> heavy_id;           // per cluster: index of its kept bitmap, or -1
   spops::DevBuf heavy_bm;                  // uint32 [heavy clusters][heavy_words]
