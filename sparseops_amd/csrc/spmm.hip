@@ -540,6 +540,28 @@ extern "C" int spops_sddmm(spops_matrix A, int64_t k, const void *d_G, int g_typ
   });
 }
 
+namespace spops {
+void refresh_sweep_values(spops_matrix_s &A, hipStream_t s) {
+  const int64_t slots = (int64_t)A.sweep.ngroups * kSweepGroup;
+  if (A.sweep.H == 0 || slots == 0 || A.sweep.src.p == nullptr) return;
+  const int32_t *src = A.sweep.src.as<int32_t>();
+  switch (value_bytes(A.value_type)) {
+    case 8:
+      hipLaunchKernelGGL(k_sweep_refresh<uint64_t>, dim3(nblk(slots, kBlock)), dim3(kBlock), 0, s, src, slots,
+                         A.csr.val.as<uint64_t>(), A.sweep.val.as<uint64_t>());
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_sweep_refresh<uint32_t>, dim3(nblk(slots, kBlock)), dim3(kBlock), 0, s, src, slots,
+                         A.csr.val.as<uint32_t>(), A.sweep.val.as<uint32_t>());
+      break;
+    default:
+      hipLaunchKernelGGL(k_sweep_refresh<uint16_t>, dim3(nblk(slots, kBlock)), dim3(kBlock), 0, s, src, slots,
+                         A.csr.val.as<uint16_t>(), A.sweep.val.as<uint16_t>());
+  }
+  SPOPS_HIP_CHECK(hipGetLastError());
+}
+}  // namespace spops
+
 extern "C" int spops_csr_set_values(spops_matrix A, const void *d_values, int values_type, void *stream) {
   return guarded([&]() {
     require_csr(A);
@@ -549,24 +571,8 @@ extern "C" int spops_csr_set_values(spops_matrix A, const void *d_values, int va
     const hipStream_t s = static_cast<hipStream_t>(stream);
     convert_values(d_values, values_type, perm ? A->csr.tperm.p : nullptr, A->csr.tperm64, A->nnz, A->csr.val.p,
                    A->value_type, s);
-    if (A->sweep.H > 0 && A->slots > 0) {   // the swept SpMV layout's copy of the values
-      const int64_t slots = (int64_t)A->sweep.ngroups * kSweepGroup;
-      const int32_t *src = A->sweep.src.as<int32_t>();
-      switch (value_bytes(A->value_type)) {
-        case 8:
-          hipLaunchKernelGGL(k_sweep_refresh<uint64_t>, dim3(nblk(slots, kBlock)), dim3(kBlock), 0, s, src, slots,
-                             A->csr.val.as<uint64_t>(), A->sweep.val.as<uint64_t>());
-          break;
-        case 4:
-          hipLaunchKernelGGL(k_sweep_refresh<uint32_t>, dim3(nblk(slots, kBlock)), dim3(kBlock), 0, s, src, slots,
-                             A->csr.val.as<uint32_t>(), A->sweep.val.as<uint32_t>());
-          break;
-        default:
-          hipLaunchKernelGGL(k_sweep_refresh<uint16_t>, dim3(nblk(slots, kBlock)), dim3(kBlock), 0, s, src, slots,
-                             A->csr.val.as<uint16_t>(), A->sweep.val.as<uint16_t>());
-      }
-      SPOPS_HIP_CHECK(hipGetLastError());
-    }
+    // the swept SpMV layout's copy follows at the next SpMV (SpMM / SDDMM callers never pay it)
+    if (A->sweep.H > 0 && A->slots > 0) A->sweep_stale = true;
     return SPOPS_SUCCESS;
   });
 }

@@ -1604,6 +1604,10 @@ extern "C" int spops_spmv(spops_matrix A, double alpha, const void *d_x, double 
     SPOPS_REQUIRE(A != nullptr, SPOPS_ERROR_INVALID_VALUE, "null matrix");
     SPOPS_REQUIRE((d_x != nullptr || A->n_cols == 0) && (d_y != nullptr || A->n_rows == 0), SPOPS_ERROR_INVALID_VALUE,
                   "null x or y");
+    if (A->sweep_stale) {                  // values changed since the swept copy was made
+      refresh_sweep_values(*A, static_cast<hipStream_t>(stream));
+      A->sweep_stale = false;
+    }
     launch_spmv(*A, alpha, d_x, beta, d_y, static_cast<hipStream_t>(stream));
     return SPOPS_SUCCESS;
   });

@@ -112,6 +112,7 @@ struct spops_matrix_s {
   spops::CsrDev csr;
   spops::SliceDev sell;
   spops::SweepDev sweep;      // SpMV layout used instead of `sell` (or beside `csr`) when sweep.H > 0
+  bool sweep_stale = false;   // CSR values set since: the next SpMV copies them into `sweep` first
   spops::BsrDev bsr;
 };
 
@@ -136,6 +137,8 @@ extern int64_t g_dense_products;   // LeSpGEMM: products of a cluster from which
 extern int64_t g_task_products;    // LeSpGEMM: products per numeric task of a large cluster (0: per call)
 extern int g_wide_cursors;         // LeSpGEMM: wide-cluster cursor variant (0 off, 1 on)
 extern int64_t g_heavy_products;   // LeSpGEMM: products from which a cluster's bitmap is built in parts
+// a CSR handle's swept SpMV copy of the values from its CSR values (stream-ordered)
+void refresh_sweep_values(spops_matrix_s &A, hipStream_t s);
 // Builds and uploads the CSR plan of a matrix: SpMV row tasks and long rows (> kCsrLongRow
 // entries); for SpMM the short-row order and the segments of the rows > kSpmmLongRow.
 void upload_csr_plan(CsrDev &P, const std::vector<int64_t> &rp, int64_t n_rows);
