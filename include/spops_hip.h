@@ -240,10 +240,11 @@ int spops_matrix_info(spops_matrix A, int64_t info[16]);
  *   workgroups by ranges of its output-column chunks, about this many products per workgroup;
  *   0 (default) picks it per numeric call: 2^15, halved down to 2^13 while the call has fewer
  *   than 32768 such tasks.  Applies to numeric calls made afterwards.
- *   SPOPS_TUNE_WIDE_CURSORS: 1 runs LeSpGEMM clusters of 1025..4096 A columns (B with < 2^31
+ *   SPOPS_TUNE_WIDE_CURSORS: LeSpGEMM clusters of 1017..4096 A columns (B with < 2^31
  *   entries) in a variant that keeps one B-row cursor per column in LDS, at two workgroups
- *   per CU, instead of searching each B row once per chunk; 0 (default) does not.  Applies
- *   to numeric calls made afterwards.
+ *   per CU, instead of searching each B row once per chunk: 0 never, 1 always, 2 (default)
+ *   when a numeric call has at least 8192 such tasks.
+ *   Applies to numeric calls made afterwards.
  *   SPOPS_TUNE_HEAVY_PRODUCTS: LeSpGEMM clusters with at least this many products (default
  *   2^20) get their symbolic bitmap built by several workgroups (parts of about 2^18
  *   products) and kept by the plan for the emit pass, while the part bitmaps (1 GiB) and the

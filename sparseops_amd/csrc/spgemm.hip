@@ -39,7 +39,7 @@
 namespace spops {
 
 int64_t g_task_products = 0;
-int g_wide_cursors = 0;
+int g_wide_cursors = 2;
 int64_t g_heavy_products = kHeavyProducts;
 
 namespace {
@@ -1525,13 +1525,23 @@ void spgemm_plan_numeric(spops_spgemm_plan_s &P, int64_t c0, int64_t c1, spops_c
     while (i + 1 < nsz && tasks[i] < kLargeTasksMin) ++i;
     task_products = kLargeTaskProducts >> i;
   }
-  // clusters of 1025..kLargeCursorsWide A columns in the variant with kLargeCursorsWide
-  // cursors, when asked for (it runs at half the occupancy of the regular kernel)
-  const bool use_wide = g_wide_cursors != 0;
+  // clusters of kLargeCursors+1..kLargeCursorsWide A columns in the variant with
+  // kLargeCursorsWide cursors (instead of two binary searches per row and chunk).  It runs at
+  // a quarter of the regular kernel's waves, so per call (mode 2) it takes them when they are
+  // at least kWideMinTasks tasks, 16 per workgroup slot of the variant (measured: R-MAT 20,
+  // 1.8-39 K such tasks per panel, +4.6 % with the variant on all panels; R-MAT 16 and 18,
+  // 3.5 K and 5.3 K tasks: -4 % and 0 %)
   auto is_wide = [&](int32_t c) {
     const int64_t na = P.harp[c + 1] - P.harp[c];
     return b32 && na > kLargeCursors && na <= kLargeCursorsWide;
   };
+  bool use_wide = g_wide_cursors == 1;
+  if (g_wide_cursors == 2) {
+    int64_t wide_tasks = 0;
+    for (int32_t c : regular)
+      if (is_wide(c)) wide_tasks += pieces_of(c, task_products, nullptr);
+    use_wide = wide_tasks >= kWideMinTasks;
+  }
   for (int32_t c : regular) {
     const int cr = P.hcsz[c] <= 4 ? 4 : kMaxClusterRows;
     int64_t chunks = 0;

@@ -14,6 +14,7 @@ constexpr int kLargeAccEntries = 2304;           // LDS accumulator entries of a
 constexpr int kLargeHash = 1024;                 // LDS hash slots of a chunk's columns
 constexpr int kLargeCursors = 1016;              // A columns whose B-row position is kept in LDS
 constexpr int kLargeCursorsWide = 4096;          // the same in the variant for wider clusters (two workgroups per CU)
+constexpr int64_t kWideMinTasks = 8192;          // per-call choice: fewest tasks for that variant (16 per slot)
 constexpr int kLargeGroupLanes = 4;              // fewest lanes scanning one B row in the numeric pass
 constexpr int64_t kLargeTaskProducts = (int64_t)1 << 15;    // products per numeric task of a split cluster ...
 constexpr int64_t kLargeTaskProductsMin = (int64_t)1 << 13; // ... halved down to this while a call has
@@ -89,7 +90,7 @@ struct spops_spgemm_plan_s {
 namespace spops {
 extern int64_t g_dense_products;   // products from which a cluster accumulates densely (tuning)
 extern int64_t g_task_products;    // products per numeric task of a large cluster, 0: chosen per call (tuning)
-extern int g_wide_cursors;         // wide-cluster cursor variant: 0 off, 1 on (tuning)
+extern int g_wide_cursors;         // wide-cluster cursor variant: 0 off, 1 on, 2 per call (tuning)
 extern int64_t g_heavy_products;   // products from which a large cluster's bitmap is built in parts (tuning)
 void spgemm_plan_create(const spops_cluster_s &A, int64_t b_rows, int64_t b_cols, int64_t b_nnz, const int64_t *brp64,
                         const int64_t *bcol64, const void *bval, spops_spgemm_plan_s &P, hipStream_t s);

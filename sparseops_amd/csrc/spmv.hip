@@ -1665,7 +1665,7 @@ extern "C" int spops_set_tuning(int key, int64_t value, int64_t *old) {
       if (old) *old = g_task_products;
       g_task_products = value;
     } else if (key == SPOPS_TUNE_WIDE_CURSORS) {
-      SPOPS_REQUIRE(value >= 0 && value <= 1, SPOPS_ERROR_INVALID_VALUE, "wide cursor mode must be 0 or 1");
+      SPOPS_REQUIRE(value >= 0 && value <= 2, SPOPS_ERROR_INVALID_VALUE, "wide cursor mode must be 0, 1 or 2");
       if (old) *old = g_wide_cursors;
       g_wide_cursors = (int)value;
     } else if (key == SPOPS_TUNE_HEAVY_PRODUCTS) {

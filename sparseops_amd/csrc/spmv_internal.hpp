@@ -135,7 +135,7 @@ bool bsr_use_mfma(int64_t bR, int64_t bC);
 void row_features_host(const int64_t *rowptr, const int64_t *col, int64_t n_rows, int64_t n_cols, double out[3]);
 extern int64_t g_dense_products;   // LeSpGEMM: products of a cluster from which it accumulates densely
 extern int64_t g_task_products;    // LeSpGEMM: products per numeric task of a large cluster (0: per call)
-extern int g_wide_cursors;         // LeSpGEMM: wide-cluster cursor variant (0 off, 1 on)
+extern int g_wide_cursors;         // LeSpGEMM: wide-cluster cursor variant (0 off, 1 on, 2 per call)
 extern int64_t g_heavy_products;   // LeSpGEMM: products from which a cluster's bitmap is built in parts
 // a CSR handle's swept SpMV copy of the values from its CSR values (stream-ordered)
 void refresh_sweep_values(spops_matrix_s &A, hipStream_t s);

@@ -223,9 +223,10 @@ class heavy_products:
 
 
 class wide_cursors:
-    """Context manager: numeric calls made inside run clusters of 1025..4096 A columns in the
-    variant with 4096 LDS cursors (two workgroups per CU) when mode is 1; 0 is the library
-    default (per-chunk searches of each B row in the regular kernel)."""
+    """Context manager: numeric calls made inside run clusters of 1017..4096 A columns in the
+    variant with 4096 LDS cursors (two workgroups per CU): mode 0 never (per-chunk searches of
+    each B row in the regular kernel), 1 always, 2 (library default) when the call has at
+    least 8192 such tasks."""
 
     def __init__(self, mode: int):
         self.mode = int(mode)

@@ -373,7 +373,7 @@ def test_wide_clusters_keep_cursors(dev):
     col = np.concatenate(cols).astype(np.int64)
     A = Csr(n, n, rp, col, rng.random(col.size) + 0.5)
     want = orc.lespgemm(A, A, k=64, bands=16, seed=7, cluster_size=8, use_long_double=True)
-    for products, mode in ((1 << 40, 1), (4096, 1), (0, 1), (1 << 40, 0), (0, 0)):
+    for products, mode in ((1 << 40, 1), (4096, 1), (0, 1), (1 << 40, 0), (0, 0), (0, 2)):
         with LG.task_products(products), LG.wide_cursors(mode):
             got = LG.lespgemm(A, A, k=64, bands=16, seed=7, cluster_size=8, device=dev)
         np.testing.assert_array_equal(got["perm"], want["perm"])
