@@ -160,7 +160,8 @@ def lespgemm_panels_measure(scale, budget_gb, cpu_threads, cpu, cpu_share=0.02):
     whose output stays under budget_gb, output buffers reused from panel to panel.  Timed
     twice; the second (warm) pass is reported.  The CPU leg runs the reference's OpenMP
     pipeline and its LeSpGEMM_VLength on two cluster ranges of about cpu_share of the work
-    each; the GPU computes the same ranges, and their checksums are compared."""
+    each; the GPU computes the same ranges, and their checksums are compared.  C = A*A^T on
+    the same clustering is timed the same way (GPU only) under "aat"."""
     import numpy as np
     import torch
     from sparseops_amd import lespgemm as LG
@@ -278,6 +279,29 @@ def lespgemm_panels_measure(scale, budget_gb, cpu_threads, cpu, cpu_share=0.02):
     if C is not None:
         C.close()
     plan.close()
+    # C = A * A^T on the same clustering (the second product of the R-MAT configuration), GPU
+    # only, timed like C = A * A (plan + panels, second of two passes)
+    from sparseops_amd.gen import transpose
+    At = transpose(A)
+    dAt = LG.DeviceCsr(At, dev)
+    flop_t = LG.spgemm_flop(dA, dAt)
+    Ct, walls = None, []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        pt = LG.SpgemmPlan(Acl, dAt)
+        ranges_t = pt.panel_ranges(budget)
+        for c0, c1 in ranges_t:
+            Ct = pt.numeric(c0, c1, Ct)
+        torch.cuda.synchronize()
+        walls.append((time.perf_counter() - t) * 1e3)
+        nnzc_t = pt.nnzc
+        pt.close()
+    Ct.close()
+    out["aat"] = {"workload": "C = A * A^T, same matrix and clustering", "flop": int(flop_t),
+                  "gpu_ms": round(walls[-1], 2), "gpu_gflops": round(flop_t / walls[-1] / 1e6, 2),
+                  "nnzc": int(nnzc_t), "panels": len(ranges_t)}
+    del dAt
     Acl.close()
     torch.cuda.synchronize()
     return out
