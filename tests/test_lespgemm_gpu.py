@@ -216,6 +216,33 @@ def test_heavy_clusters_split_bitmaps(case, dev):
     plan.close()
 
 
+def test_wide_clusters_over_wide_b_keep_cursors_in_memory(dev):
+    """Rows of 1100-2500 A columns times a B of 600,000 columns with 1-3 entries per row: the
+    clusters are not heavy (few products) but have more columns than the LDS cursors hold, so
+    the symbolic and emit bitmap passes keep their B-row cursors in device memory across the
+    2^18-column tiles; in every wide-cursor mode the product equals the oracle's."""
+    rng = np.random.default_rng(41)
+    n, nb_cols = 3000, 600_000
+    lens = rng.integers(1, 8, n)
+    lens[[7, 1200, 2999]] = [2500, 1100, 1800]
+    cols = [np.sort(rng.choice(n, size=L, replace=False)) for L in lens]
+    rp = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=rp[1:])
+    col = np.concatenate(cols).astype(np.int64)
+    A = Csr(n, n, rp, col, rng.random(col.size) + 0.5)
+    blens = rng.integers(1, 4, n)
+    bcols = [np.sort(rng.choice(nb_cols, size=L, replace=False)) for L in blens]
+    brp = np.zeros(n + 1, np.int64)
+    np.cumsum(blens, out=brp[1:])
+    bcol = np.concatenate(bcols).astype(np.int64)
+    B = Csr(n, nb_cols, brp, bcol, rng.random(bcol.size) + 0.5)
+    for mode in (0, 1):
+        with LG.wide_cursors(mode):
+            out = LG.lespgemm(A, B, device=dev)
+        assert int(out["cmax"].max()) - int(out["cmin"].min()) >= (1 << 18)
+        _check_against_oracle(A, B, out)
+
+
 def test_lespgemm_chunked_accumulator(dev):
     """Clusters whose C block has more than 1024 columns: the numeric pass runs several LDS
     accumulator chunks per cluster."""
