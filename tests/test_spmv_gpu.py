@@ -326,6 +326,21 @@ def test_swept(mats, name, vt, fmt, wgs, maxrows, dev):
         np.testing.assert_array_equal(M.layout(1), cl)
 
 
+@pytest.mark.parametrize("n", [20001, 20002, 20003, 20004])
+@pytest.mark.parametrize("vt", ["f32", "bf16", "f64"])
+def test_swept_reduce_row_counts(n, vt, dev):
+    """The pass adding the partial planes: four rows per thread for FP32 planes when n % 4 == 0,
+    one row per thread otherwise (and for FP64); alpha, beta and beta == 0 (y not read)."""
+    A = powerlaw(n, lmax=400, seed=n)
+    with S.spmv_layout("swept"), S.sweep_workgroups(256):
+        M = S.csr_to_sell_c_sigma(A, 256, 32, vt)
+    assert M.sweep[1] > 1
+    x = uniform_x(A.n_cols, 12)
+    _check(A, M, x, _y0(A.n_rows), vt, dev=dev)
+    y = _run(M, x, np.full(A.n_rows, np.nan), dev, 1.0, 0.0)
+    assert np.all(np.isfinite(y))
+
+
 def test_swept_empty_tiny_and_nonfinite(dev):
     """Empty and one-row matrices; x holding inf/nan outside the gathered columns must not
     reach y (padding slots never gather)."""
