@@ -632,7 +632,8 @@ def main():
         r = {"wall_ms": wall_ms, "kernel_ms": kern_ms, "algo_bytes": M.algo_bytes,
              "gbs": M.algo_bytes / (wall_ms * 1e-3) / 1e9, "kernel_gbs": M.algo_bytes / (kern_ms * 1e-3) / 1e9,
              "device_bytes": M.device_bytes, "build_s": time.time() - t, "sweep": M.sweep,
-             "kernels": ((["k_sweep_spmv", "k_sweep_reduce4" if vt == "f32" and A.n_rows % 4 == 0 else "k_sweep_reduce"]
+             "kernels": ((["k_sweep_spmv", "k_sweep_reduce4" if vt in ("f32", "bf16") and A.n_rows % 4 == 0
+                                           else "k_sweep_reduce"]
                           if M.sweep[1] > 1 else ["k_sweep_spmv"]) if M.sweep
                          else ["k_csr_spmv"] if M.format not in ("sell", "ell")
                          else (["k_slice_spmv", "k_slice_reduce"] if M.slices > 1 else ["k_slice_spmv"]))}
