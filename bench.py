@@ -212,21 +212,10 @@ def lespgemm_panels_measure(scale, budget_gb, cpu_threads, cpu, cpu_share=0.02):
            "gpu_plan_ms": round(w["plan_ms"], 2), "gpu_panels_ms": round(w["panels_ms"], 2),
            "gpu_genpairs_ms": round(genpairs_ms, 1), "gpu_hc_ms": round(hc_ms, 1),
            "gpu_conversion_ms": round(conv_ms, 1), "generate_s": round(gen_s, 1), "pairs": int(len(pi))}
-    # per-cluster flop in get_spgemm_flop's sense (2 * sum over the cluster's rows of nnz(B_k))
-    blen = np.diff(A.rowptr.astype(np.int64))
-    rowf = np.zeros(A.n_rows, np.int64)
-    nzr = blen > 0
-    if A.nnz:
-        rowf[nzr] = np.add.reduceat(blen[A.col.astype(np.int64)], A.rowptr[:-1].astype(np.int64)[nzr])
-    csz = np.diff(offsets)
-    cf = 2 * np.add.reduceat(rowf[perm], offsets[:-1]) * (csz > 0)
-    cum = np.cumsum(cf)
-    tot = int(cum[-1])
-    picks = []
-    for start in (0.0, 0.5):
-        c0 = int(np.searchsorted(cum, start * tot, side="right"))
-        c1 = int(np.searchsorted(cum, cum[c0 - 1] + cpu_share * tot if c0 else cpu_share * tot)) + 1
-        picks.append((c0, min(c1, plan.clusters)))
+    # per-cluster flop in get_spgemm_flop's sense, and two ranges of about cpu_share of it
+    cf = LG.cluster_flops(A, A.rowptr, perm, offsets)
+    tot = int(cf.sum())
+    picks = LG.sample_ranges(cf, cpu_share)
     gpu_pick = []
     for c0, c1 in picks:
         P = plan.numeric(c0, c1)

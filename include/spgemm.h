@@ -51,22 +51,38 @@ void LeSpGEMM_VLength(const CSR_VlengthCluster<IndexType, ValueType> &A, const C
   if (A.cols != B.num_rows) throw std::invalid_argument("LeSpGEMM_VLength: A.cols != B.num_rows");
   std::lock_guard<std::mutex> g(state().mu);
   const int64_t nc = A.rows;
-  std::vector<int64_t> csz(A.cluster_sz, A.cluster_sz + nc), rp(A.rowptr, A.rowptr + nc + 1),
-      rpv(A.rowptr_val, A.rowptr_val + nc + 1), col(A.colids, A.colids + A.nnzc);
-  spops_cluster hA = nullptr, hC = nullptr;
-  check(spops_cluster_create_from_host(&hA, A.csr_rows, A.cols, nc, csz.data(), rp.data(), rpv.data(), col.data(),
-                                       A.values, value_code<V>()),
-        "spops_cluster_create_from_host");
+  std::vector<int64_t> csz(A.cluster_sz, A.cluster_sz + nc), rp(A.rowptr, A.rowptr + nc + 1);
+  // A stays on the device between calls (cached and checked like the SpMV matrices)
+  Source src;
+  src.kind = 200 + (int)sizeof(I) * 10 + (int)sizeof(V);
+  src.dims[0] = A.csr_rows; src.dims[1] = A.cols; src.dims[2] = nc; src.dims[3] = A.nnzc; src.dims[4] = A.nnzv;
+  src.arrays[0] = A.cluster_sz; src.arrays[1] = A.rowptr; src.arrays[2] = A.rowptr_val; src.arrays[3] = A.colids;
+  src.arrays[4] = A.values;
+  spops_cluster hA = cached_entry(
+                         A.rowptr, src,
+                         [&](Digest &d, int mode) {
+                           d.array(A.cluster_sz, sizeof(I), nc, mode);
+                           d.array(A.rowptr, sizeof(I), nc + 1, mode);
+                           d.array(A.rowptr_val, sizeof(I), nc + 1, mode);
+                           d.array(A.colids, sizeof(I), A.nnzc, mode);
+                           d.array(A.values, sizeof(V), A.nnzv, mode);
+                         },
+                         [&](Entry &e) {
+                           std::vector<int64_t> rpv(A.rowptr_val, A.rowptr_val + nc + 1),
+                               col(A.colids, A.colids + A.nnzc);
+                           check(spops_cluster_create_from_host(&e.cl, A.csr_rows, A.cols, nc, csz.data(), rp.data(),
+                                                                rpv.data(), col.data(), A.values, value_code<V>()),
+                                 "spops_cluster_create_from_host");
+                         })
+                         .cl;
+  spops_cluster hC = nullptr;
   int64_t stats[16] = {0};
   int st;
   {
-    const int64_t *drp = upload_index64(0, B.row_offset, (int64_t)B.num_rows + 1);
-    const int64_t *dcol = upload_index64(1, B.col_index, (int64_t)B.num_nnzs);
-    void *dval = state().scratch[2].get((size_t)(B.num_nnzs ? B.num_nnzs : 1) * sizeof(V));
-    check(spops_memcpy_htod(dval, B.values, (size_t)B.num_nnzs * sizeof(V), nullptr), "upload B values");
-    st = spops_lespgemm(hA, B.num_rows, B.num_cols, B.num_nnzs, drp, dcol, dval, &hC, stats, nullptr);
+    // B stays on the device between calls (cached and checked like the SpMV matrices)
+    DeviceCsr<V> dB = cached_csr64<I, V>(B.row_offset, B.col_index, B.values, B.num_rows, B.num_cols, B.num_nnzs);
+    st = spops_lespgemm(hA, B.num_rows, B.num_cols, B.num_nnzs, dB.rowptr, dB.col, dB.values, &hC, stats, nullptr);
   }
-  spops_cluster_destroy(hA);
   check(st, "spops_lespgemm");
   int64_t info[8];
   spops_cluster_info(hC, info);

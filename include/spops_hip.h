@@ -86,6 +86,15 @@ int spops_csr_create(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t nn
                      int index_type, const void *rowptr, const void *col,
                      const void *values, int in_value_type, int value_type);
 
+/* The same CSR handle with its SpMV layout chosen explicitly instead of by the process-wide
+ * SPOPS_TUNE_SPMV_LAYOUT: 0 chosen from the matrix (as spops_csr_create does by default),
+ * 1 the row kernels only (no swept copy of the entries: for handles used by SpMM, SDDMM and
+ * the transpose, which never run SpMV, so that spops_csr_set_values updates one copy),
+ * 2 the swept layout beside the CSR arrays.  Backs SpOps_csr like spops_csr_create. */
+int spops_csr_create_layout(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t nnz,
+                            int index_type, const void *rowptr, const void *col,
+                            const void *values, int in_value_type, int value_type, int spmv_layout);
+
 /* SELL-C-sigma from CSR with the reference's ordering: each sigma-slice sorted by
  * (nnz desc, row desc), chunks of C sorted rows (include/sparse_conversion.h:345-487).
  * sigma = n_rows gives SELL-C-R (:491-606).  sigma % C != 0 -> SPOPS_ERROR_INVALID_VALUE.
@@ -196,7 +205,8 @@ int spops_csr_transpose(spops_matrix A, spops_matrix *AT);
 /* overwrite the stored values from a device array (cast to the stored type; BF16 storage
  * rounds each value once, to nearest even, from its FP64 / FP32 value).  For a handle made
  * by spops_csr_transpose the array is in the ORIGINAL matrix's entry order.  A handle with a
- * swept SpMV layout gets the new values there too. */
+ * swept SpMV layout gets the new values there too, on the same stream, so every SpMV
+ * ordered after this call (stream order, events, or a graph captured after it) sees them. */
 int spops_csr_set_values(spops_matrix A, const void *d_values, int values_type, void *stream);
 
 /* info[0]=format info[1]=n_rows info[2]=n_cols info[3]=nnz info[4]=value_type

@@ -186,7 +186,7 @@ int run_spmv(const HostCsr &h, const std::vector<double> &xd, const std::string 
 
 template <typename V>
 int run_lespgemm(const HostCsr &ha, const HostCsr &hb, const std::string &out, int k, int bands,
-                 uint64_t seed, int hc_v, int64_t cluster_size, int kernel) {
+                 uint64_t seed, int hc_v, int64_t cluster_size, int kernel, double l2_fraction) {
   typedef int64_t I;
   CSR_Matrix<I, V> A = to_ref_csr<I, V>(ha);
   CSR_Matrix<I, V> B = to_ref_csr<I, V>(hb);
@@ -224,9 +224,13 @@ int run_lespgemm(const HostCsr &ha, const HostCsr &hb, const std::string &out, i
   Ccl.acc_flag = new_array<char>(Acl.rows);
   Ccl.min_ccol = new_array<I>(Acl.rows);
   Ccl.max_ccol = new_array<I>(Acl.rows);
-  LeSpGEMM_VLength<false, I, V>(Acl, B, Ccl, kernel, -1.0);
+  LeSpGEMM_VLength<false, I, V>(Acl, B, Ccl, kernel, l2_fraction);
   int64_t ccl_meta[3] = {Ccl.rows, Ccl.nnzc, Ccl.nnzv};
   dump(out, "ccl_meta", ccl_meta, 3);
+  if (kernel == 3) {
+    int64_t dense = Ccl.dense_cluster_count;
+    dump(out, "dense_cluster_count", &dense, 1);
+  }
   dump(out, "ccl_rowptr", Ccl.rowptr, (size_t)Ccl.rows + 1);
   if (kernel == 3) {
     std::vector<int64_t> mn(Ccl.min_ccol, Ccl.min_ccol + Ccl.rows), mx(Ccl.max_ccol, Ccl.max_ccol + Ccl.rows);
@@ -364,7 +368,7 @@ CSR_VlengthCluster<I, V> cluster_slice(const CSR_VlengthCluster<I, V> &A, I c0, 
 }
 
 int bench_lespgemm_panels(const HostCsr &ha, const HostCsr &hb, int iterations, int k, int bands, uint64_t seed,
-                          int64_t cluster_size, int kernel, const char *ranges) {
+                          int64_t cluster_size, int kernel, const char *ranges, const char *dump_dir) {
   typedef int64_t I; typedef double V;
   CSR_Matrix<I, V> A = to_ref_csr<I, V>(ha);
   CSR_Matrix<I, V> B = to_ref_csr<I, V>(hb);
@@ -376,6 +380,10 @@ int bench_lespgemm_panels(const HostCsr &ha, const HostCsr &hb, int iterations, 
   t0 = omp_get_wtime();
   hierarchical_clustering_v0_fast<I, V>(A.row_offset, A.col_index, A.num_rows, pairs, (I)cluster_size, perm, offsets);
   double t_hc = omp_get_wtime() - t0;
+  if (dump_dir != nullptr) {
+    dump(dump_dir, "perm", perm.data(), perm.size());
+    dump(dump_dir, "offsets", offsets.data(), offsets.size());
+  }
   CSR_Matrix<I, V> Ar = csr_reorder_rows<I, V>(A, perm);
   t0 = omp_get_wtime();
   CSR_VlengthCluster<I, V> Acl = csr_to_vlength_cluster<I, V>(Ar, offsets);
@@ -461,8 +469,9 @@ int main(int argc, char **argv) {
     int64_t cs = std::atoll(argv[9]);
     int kernel = std::atoi(argv[10]);
     int prec = std::atoi(argv[11]);
-    return prec == 32 ? run_lespgemm<float>(A, B, argv[4], k, bands, seed, hc_v, cs, kernel)
-                      : run_lespgemm<double>(A, B, argv[4], k, bands, seed, hc_v, cs, kernel);
+    double l2f = argc > 12 ? std::atof(argv[12]) : -1.0;   // kernel 3's L2 fraction (-1: its default)
+    return prec == 32 ? run_lespgemm<float>(A, B, argv[4], k, bands, seed, hc_v, cs, kernel, l2f)
+                      : run_lespgemm<double>(A, B, argv[4], k, bands, seed, hc_v, cs, kernel, l2f);
   }
   if (mode == "bench_spmv" && argc >= 7) {
     HostCsr A = read_csr_file(argv[2]);
@@ -494,7 +503,7 @@ int main(int argc, char **argv) {
     Le_set_thread_num(std::atoi(argv[4]));
     return bench_lespgemm_panels(A, B, std::atoi(argv[5]), std::atoi(argv[6]), std::atoi(argv[7]),
                                  std::strtoull(argv[8], nullptr, 10), std::atoll(argv[9]), std::atoi(argv[10]),
-                                 argv[11]);
+                                 argv[11], argc > 12 ? argv[12] : nullptr);
   }
   std::fprintf(stderr, "bad arguments for mode %s\n", mode.c_str());
   return 2;

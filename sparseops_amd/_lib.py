@@ -47,6 +47,7 @@ _SIGNATURES = [
     ("spops_event_record", c_int, [c_vp, c_vp]),
     ("spops_event_elapsed_ms", c_int, [c_vp, c_vp, c_vp]),
     ("spops_csr_create", c_int, [c_vp, c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_int, c_int]),
+    ("spops_csr_create_layout", c_int, [c_vp, c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_int, c_int, c_int]),
     ("spops_sell_create", c_int, [c_vp, c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_int, c_int, c_i64, c_i64]),
     ("spops_sell_create_from_chunks", c_int,
      [c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_int, c_int]),

@@ -23,7 +23,8 @@ class SparsePattern:
     and its transpose, built once."""
 
     def __init__(self, A: Csr, value_type="f32"):
-        self.A = S.csr(A, value_type)
+        # SpMM, SDDMM and the transpose read the CSR arrays only: no swept SpMV copy to refresh
+        self.A = S.csr(A, value_type, spmv_layout=1)
         self.AT = self.A.transpose()
         self.n_rows, self.n_cols, self.nnz = A.n_rows, A.n_cols, A.nnz
 

@@ -157,7 +157,7 @@ static int run(const char *path, const Opts &o) {
   {
     using namespace spops_dropin;
     std::lock_guard<std::mutex> g(state().mu);
-    spops_matrix A = state().matrices.at(F.key());
+    spops_matrix A = state().entries.at(F.key())->m;
     void *dx = nullptr, *dy = nullptr;
     check(spops_malloc(&dx, (size_t)(csr.num_cols ? csr.num_cols : 1) * sizeof(V)), "malloc x");
     check(spops_malloc(&dy, (size_t)(csr.num_rows ? csr.num_rows : 1) * sizeof(V)), "malloc y");

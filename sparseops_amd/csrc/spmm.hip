@@ -571,8 +571,10 @@ extern "C" int spops_csr_set_values(spops_matrix A, const void *d_values, int va
     const hipStream_t s = static_cast<hipStream_t>(stream);
     convert_values(d_values, values_type, perm ? A->csr.tperm.p : nullptr, A->csr.tperm64, A->nnz, A->csr.val.p,
                    A->value_type, s);
-    // the swept SpMV layout's copy follows at the next SpMV (SpMM / SDDMM callers never pay it)
-    if (A->sweep.H > 0 && A->slots > 0) A->sweep_stale = true;
+    // the swept SpMV layout's copy follows on the same stream, so any later SpMV ordered after
+    // this call (or captured after it) reads the new values; handles made for SpMM / SDDMM only
+    // (spops_csr_create_layout with spmv_layout 1) have no such copy and pay nothing here
+    if (A->sweep.H > 0 && A->slots > 0) refresh_sweep_values(*A, s);
     return SPOPS_SUCCESS;
   });
 }

@@ -1405,15 +1405,15 @@ std::vector<int32_t> sell_ref_chunk_len(const std::vector<int64_t> &rp, const st
 
 // CSR handles: the swept layout beside the CSR arrays (SpMV only), chosen as for SELL.
 static void add_csr_sweep(spops_matrix_s &M, const std::vector<int64_t> &rp, const std::vector<int64_t> &ci,
-                          const void *values, int in_vt, int vt) {
+                          const void *values, int in_vt, int vt, int64_t mode) {
   const int64_t n = M.n_rows;
-  if (g_sweep_mode == 1 || n == 0 || M.nnz == 0) return;
+  if (mode == 1 || n == 0 || M.nnz == 0) return;
   std::vector<int64_t> ident(n);
   std::iota(ident.begin(), ident.end(), 0);
   CsrRows src(rp, ci, ident, values, in_vt);
   std::vector<int64_t> len(n);
   for (int64_t r = 0; r < n; ++r) len[r] = rp[r + 1] - rp[r];
-  if (g_sweep_mode != 2 && !sweep_wanted(M, slice_count(M, src, ident, len, vt), vt)) return;
+  if (mode != 2 && !sweep_wanted(M, slice_count(M, src, ident, len, vt), vt)) return;
   const int64_t slots = M.slots;
   build_sweep_layout(M, src, ident, len, vt, true);
   M.slots = slots;
@@ -1421,10 +1421,12 @@ static void add_csr_sweep(spops_matrix_s &M, const std::vector<int64_t> &rp, con
 
 }  // namespace
 
-extern "C" int spops_csr_create(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t nnz, int index_type,
-                                const void *rowptr, const void *col, const void *values, int in_vt, int vt) {
+static int csr_create_common(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t nnz, int index_type,
+                             const void *rowptr, const void *col, const void *values, int in_vt, int vt,
+                             int64_t spmv_layout) {
   return guarded([&]() {
     SPOPS_REQUIRE(A != nullptr, SPOPS_ERROR_INVALID_VALUE, "null output handle");
+    SPOPS_REQUIRE(spmv_layout >= 0 && spmv_layout <= 2, SPOPS_ERROR_INVALID_VALUE, "spmv_layout must be 0, 1 or 2");
     check_vt(in_vt, vt);
     std::vector<int64_t> rp = host_index(rowptr, n_rows + 1, index_type);
     std::vector<int64_t> ci = host_index(col, nnz, index_type);
@@ -1445,13 +1447,24 @@ extern "C" int spops_csr_create(spops_matrix *A, int64_t n_rows, int64_t n_cols,
     P.val.upload(v);
     upload_csr_plan(P, rp, n_rows);
     M->slots = nnz;
-    add_csr_sweep(*M, rp, ci, values, in_vt, vt);
+    add_csr_sweep(*M, rp, ci, values, in_vt, vt, spmv_layout);
     M->dev_bytes = P.rowptr.bytes + P.col.bytes + P.val.bytes + P.tasks.bytes + P.longs.bytes +
                    (M->sweep.H > 0 ? M->dev_bytes : 0);
     M->algo_bytes = spmv_algo_bytes(*M, index_type == SPOPS_INDEX_I32 ? 4 : 8);
     *A = M.release();
     return SPOPS_SUCCESS;
   });
+}
+
+extern "C" int spops_csr_create(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t nnz, int index_type,
+                                const void *rowptr, const void *col, const void *values, int in_vt, int vt) {
+  return csr_create_common(A, n_rows, n_cols, nnz, index_type, rowptr, col, values, in_vt, vt, g_sweep_mode);
+}
+
+extern "C" int spops_csr_create_layout(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t nnz,
+                                       int index_type, const void *rowptr, const void *col, const void *values,
+                                       int in_vt, int vt, int spmv_layout) {
+  return csr_create_common(A, n_rows, n_cols, nnz, index_type, rowptr, col, values, in_vt, vt, spmv_layout);
 }
 
 static int sell_create_common(spops_matrix *A, int64_t n_rows, int64_t n_cols, int64_t nnz, int index_type,
@@ -1643,10 +1656,6 @@ extern "C" int spops_spmv(spops_matrix A, double alpha, const void *d_x, double 
     SPOPS_REQUIRE(A != nullptr, SPOPS_ERROR_INVALID_VALUE, "null matrix");
     SPOPS_REQUIRE((d_x != nullptr || A->n_cols == 0) && (d_y != nullptr || A->n_rows == 0), SPOPS_ERROR_INVALID_VALUE,
                   "null x or y");
-    if (A->sweep_stale) {                  // values changed since the swept copy was made
-      refresh_sweep_values(*A, static_cast<hipStream_t>(stream));
-      A->sweep_stale = false;
-    }
     launch_spmv(*A, alpha, d_x, beta, d_y, static_cast<hipStream_t>(stream));
     return SPOPS_SUCCESS;
   });

@@ -112,7 +112,6 @@ struct spops_matrix_s {
   spops::CsrDev csr;
   spops::SliceDev sell;
   spops::SweepDev sweep;      // SpMV layout used instead of `sell` (or beside `csr`) when sweep.H > 0
-  bool sweep_stale = false;   // CSR values set since: the next SpMV copies them into `sweep` first
   spops::BsrDev bsr;
 };
 

@@ -297,6 +297,41 @@ class SpgemmPlan:
             pass
 
 
+def cluster_flops(A: Csr, B_rowptr, perm, offsets) -> np.ndarray:
+    """Flop of each cluster of C = A * B in get_spgemm_flop's sense (src/spgemm.cpp:24-42:
+    2 * the B-row lengths of the cluster rows' entries), from host arrays."""
+    blen = np.diff(np.asarray(B_rowptr, dtype=np.int64))
+    rp = np.asarray(A.rowptr, dtype=np.int64)
+    col = np.asarray(A.col, dtype=np.int64)
+    rowf = np.zeros(A.n_rows, np.int64)
+    nzr = np.diff(rp) > 0
+    if A.nnz:
+        inb = col < len(blen)
+        w = np.where(inb, blen[np.minimum(col, max(len(blen) - 1, 0))], 0)
+        rowf[nzr] = np.add.reduceat(w, rp[:-1][nzr])
+    perm = np.asarray(perm, dtype=np.int64)
+    offsets = np.asarray(offsets, dtype=np.int64)
+    csz = np.diff(offsets)
+    out = np.zeros(len(csz), np.int64)
+    ne = csz > 0
+    out[ne] = 2 * np.add.reduceat(rowf[perm], offsets[:-1][ne])
+    return out
+
+
+def sample_ranges(cf, share: float, starts=(0.0, 0.5)):
+    """Cluster ranges [c0, c1) of about `share` of the total flop each, starting at the given
+    fractions of the flop prefix (the sampled panels timed against the reference)."""
+    cum = np.cumsum(np.asarray(cf, dtype=np.int64))
+    tot = int(cum[-1]) if len(cum) else 0
+    out = []
+    for start in starts:
+        c0 = int(np.searchsorted(cum, start * tot, side="right"))
+        base = cum[c0 - 1] if c0 else 0
+        c1 = int(np.searchsorted(cum, base + share * tot)) + 1
+        out.append((c0, min(c1, len(cum))))
+    return out
+
+
 def spgemm_flop(dA: DeviceCsr, dB: DeviceCsr) -> int:
     f = c_i64()
     L.call("spops_spgemm_flop", dA.nnz, dA.col.data_ptr(), dB.n_rows, dB.rowptr.data_ptr(), ctypes.byref(f),

@@ -171,8 +171,13 @@ def _create(fn: str, A: Csr, value_type, *extra) -> SpMatrix:
     return SpMatrix(h)
 
 
-def csr(A: Csr, value_type="f64") -> SpMatrix:
-    return _create("spops_csr_create", A, value_type)
+def csr(A: Csr, value_type="f64", spmv_layout=None) -> SpMatrix:
+    """CSR handle.  spmv_layout None follows the process-wide tuning; 0 chooses from the matrix,
+    1 keeps the row kernels only (handles used for SpMM / SDDMM / the transpose), 2 adds the
+    swept SpMV layout."""
+    if spmv_layout is None:
+        return _create("spops_csr_create", A, value_type)
+    return _create("spops_csr_create_layout", A, value_type, int(spmv_layout))
 
 
 def csr_to_sell_c_sigma(A: Csr, sigma: int, C: int, value_type="f64") -> SpMatrix:

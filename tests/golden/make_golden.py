@@ -143,9 +143,95 @@ def features_case(name, A):
     np.savez(os.path.join(OUT, f"{name}.npz"), features=vec, input_digest=digest(A.rowptr, A.col, A.val))
 
 
+def classify_case(name, A, fractions, cs=8):
+    """Kernel 3's cluster classification (src/spgemm_Vlength_mixed.cpp:268-319) as the
+    reference computes it, for several L2 fractions of its budget (-1: its default 0.75)."""
+    out = {"input_digest": digest(A.rowptr, A.col, A.val), "fractions": np.array(fractions, np.float64),
+           "cluster_size": cs}
+    with tempfile.TemporaryDirectory() as d:
+        write_csr_bin(os.path.join(d, "A.csr"), A)
+        for t, f in enumerate(fractions):
+            run(["lespgemm", os.path.join(d, "A.csr"), os.path.join(d, "A.csr"), d, "64", "16", "7", "0", str(cs),
+                 "3", "64", repr(f)], d)
+            out[f"acc_flag_{t}"] = rd(d, "ccl_acc_flag", np.int8)
+            out[f"dense_count_{t}"] = rd(d, "dense_cluster_count", np.int64)
+            if t == 0:
+                out["perm"], out["offsets"] = rd(d, "perm", np.int64), rd(d, "offsets", np.int64)
+                out["ccl_min"], out["ccl_max"] = rd(d, "ccl_min", np.int64), rd(d, "ccl_max", np.int64)
+                out["ccl_rowptr"] = rd(d, "ccl_rowptr", np.int64)
+    np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **out)
+    print("wrote", name, "dense counts", [int(out[f"dense_count_{t}"][0]) for t in range(len(fractions))])
+
+
+def fingerprint_case(name, A, block=1):
+    """C = A * A at a scale whose C is too large to keep: the reference's clustering and C
+    as digests (perm, offsets, C row_ptr, C columns sorted within rows) plus C's row sums
+    (summed over blocks of `block` rows)."""
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        write_csr_bin(os.path.join(d, "A.csr"), A)
+        run(["lespgemm", os.path.join(d, "A.csr"), os.path.join(d, "A.csr"), d, "64", "16", "7", "0", "8", "3",
+             "64"], d)
+        perm, off = rd(d, "perm", np.int64), rd(d, "offsets", np.int64)
+        crp = rd(d, "C_rowptr", np.int64)
+        ccol, cval = rd(d, "C_col", np.int64), rd(d, "C_val", np.float64)
+        rows = np.repeat(np.arange(A.n_rows), np.diff(crp))
+        o = np.lexsort((ccol, rows))
+        rs = np.bincount(rows, weights=cval, minlength=A.n_rows)
+        out = {"input_digest": digest(A.rowptr, A.col, A.val), "perm_digest": digest(perm),
+               "offsets_digest": digest(off), "nclusters": len(off) - 1, "C_rowptr_digest": digest(crp),
+               "C_col_digest": digest(ccol[o]), "nnz": int(crp[-1]), "block": block,
+               "C_rowsum": rs.reshape(-1, block).sum(axis=1), "flops": rd(d, "flops", np.int64),
+               "ccl_meta": rd(d, "ccl_meta", np.int64)}
+    np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **out)
+    print("wrote", name, "nnz(C)", out["nnz"], "clusters", out["nclusters"])
+
+
+def panels_case(name, scale, share=0.02):
+    """C = A * A on R-MAT(scale, 16, seed 0) without forming C: the reference's clustering
+    fingerprints and its LeSpGEMM_VLength checksums on two cluster ranges of about `share` of
+    the flop each (the ranges the sampled-panel benchmark uses)."""
+    from sparseops_amd.lespgemm import cluster_flops, sample_ranges
+    A = rmat(scale, 16, seed=0)
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        path = os.path.join(d, "A.csr")
+        write_csr_bin(path, A)
+        env = dict(os.environ, OMP_NUM_THREADS=str(os.cpu_count() or 8))
+        subprocess.check_call([HARNESS, "bench_lespgemm_panels", path, path, env["OMP_NUM_THREADS"], "0", "64", "16",
+                               "7", "8", "3", "", d], stdout=subprocess.DEVNULL, env=env)
+        perm, off = rd(d, "perm", np.int64), rd(d, "offsets", np.int64)
+        picks = sample_ranges(cluster_flops(A, A.rowptr, perm, off), share)
+        rl = ",".join(f"{a}:{b}" for a, b in picks)
+        txt = subprocess.check_output([HARNESS, "bench_lespgemm_panels", path, path, env["OMP_NUM_THREADS"], "0",
+                                       "64", "16", "7", "8", "3", rl], env=env, text=True)
+    import json
+    lines = [json.loads(t) for t in txt.splitlines() if t.strip()]
+    head, ps = lines[0], lines[1:]
+    out = {"input_digest": digest(A.rowptr, A.col, A.val), "scale": scale, "share": share,
+           "perm_digest": digest(perm), "offsets_digest": digest(off), "nclusters": len(off) - 1,
+           "perm_fingerprint": np.uint64(head["perm_fingerprint"]),
+           "offsets_fingerprint": np.uint64(head["offsets_fingerprint"]),
+           "ranges": np.array(picks, np.int64),
+           "nnzc": np.array([p["nnzc"] for p in ps], np.int64), "nnzv": np.array([p["nnzv"] for p in ps], np.int64),
+           "colid_sum": np.array([int(p["colid_sum"]) for p in ps], np.int64),
+           "nonzeros": np.array([int(p["nonzeros"]) for p in ps], np.int64),
+           "sum": np.array([p["sum"] for p in ps]), "sum_abs": np.array([p["sum_abs"] for p in ps])}
+    np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **out)
+    print("wrote", name, "ranges", picks, "nnzc", out["nnzc"])
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: run make -C oracle ref (needs the reference sources)")
+    only = set(sys.argv[1:])
+    if only:   # regenerate the named groups only: spmv, lsh, features, classify, fingerprints, panels
+        groups = {"classify": lambda: (classify_case("classify_rmat10", rmat(10, 16, seed=10), [-1.0, 2e-4, 5e-5, 0.0]),
+                                       classify_case("classify_rmat12", rmat(12, 16, seed=12), [-1.0, 2e-3, 1e-3, 2e-4])),
+                  "fingerprints": lambda: (fingerprint_case("lsh_rmat14_fp", rmat(14, 16, seed=14)),
+                                           fingerprint_case("lsh_rmat16_fp", rmat(16, 16, seed=16), block=16)),
+                  "panels": lambda: panels_case("lsh_rmat20_panels", 20)}
+        for g in sorted(only):
+            groups[g]()
+        return
     spmv_case("spmv_lap16_f64", laplacian_2d(16), 64, 8, 4, 4, 64, True)
     spmv_case("spmv_lap16_f32", laplacian_2d(16), 64, 8, 4, 8, 32, True)
     spmv_case("spmv_pl3000_f64", powerlaw(3000, lmax=300, seed=2), 256, 32, 16, 4, 64, False)

@@ -196,6 +196,15 @@ def cluster_symbolic(clr, colids, B):
     return nz, mn, mx
 
 
+def cluster_classify(clr, csz, nz, mn, mx, b_rowdense, l2_budget, thr=0.15, vbytes=8):
+    """Kernel 3's dense (1) / hash (0) flag per cluster and the dense count."""
+    nc = len(csz)
+    flag = np.zeros(nc, np.int8)
+    dense = lib.orc_cluster_classify(i64(nc), P(_i64(clr)), P(_i64(csz)), P(_i64(nz)), P(_i64(mn)), P(_i64(mx)),
+                                     c_dbl(b_rowdense), i64(int(l2_budget)), c_dbl(thr), i64(vbytes), P(flag))
+    return flag, int(dense)
+
+
 def cluster_numeric(clr, clrv, csz, colids, values, B, nz, use_long_double=False):
     nc = len(clr) - 1
     crp = np.zeros(nc + 1, np.int64)

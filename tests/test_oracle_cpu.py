@@ -373,3 +373,207 @@ def test_features_closed_forms():
     total = c[-1] * m / 2
     np.testing.assert_allclose(f[16], (total - area) / total, rtol=1e-14)
     assert f[10] == 1 and f[11] == m and f[4] == m
+
+
+# ------------------------------------------------ extended-precision SpMV (the SpMV checker)
+def _exact_spmv(A, x, alpha, beta, y0):
+    """y = alpha * A x + beta * y0 and sum_j |a_rj x_j| in exact rational arithmetic."""
+    from fractions import Fraction as F
+    fa, fb = F(alpha), F(beta)
+    y, s = [], []
+    for r in range(A.n_rows):
+        acc, aa = F(0), F(0)
+        for j in range(int(A.rowptr[r]), int(A.rowptr[r + 1])):
+            p = F(float(A.val[j])) * F(float(x[int(A.col[j])]))
+            acc += p
+            aa += abs(p)
+        y.append(fa * acc + fb * F(float(y0[r])))
+        s.append(aa)
+    return y, s
+
+
+def _spmv_exact_checks(fn):
+    """Every pin of the extended-precision SpMV, run on a candidate `fn(A, x, alpha, beta, y0)
+    -> (y, absrow)`: the reference's own FP64 outputs (serial and nnz-balanced CSR kernels,
+    alpha/beta and alpha=1/beta=0), scipy, and exact rational sums on a small matrix (y
+    correctly rounded to within one unit in the last place plus the extended sum's error; the
+    condition scale the same)."""
+    for name in SPMV_CASES:
+        g = gold(name)
+        A = SPMV_CASES[name]()
+        x = uniform_x(A.n_cols, int(g["x_seed"]))
+        a, b, y0 = float(g["alpha"]), float(g["beta"]), g["y0"]
+        y, s = fn(A, x, a, b, y0)
+        tol = 1e-13 * (abs(a) * s + abs(b) * np.abs(y0)) + 1e-300
+        assert np.all(np.abs(y - g["y_csr_ab"]) <= tol), f"{name}: y_csr_ab"
+        assert np.all(np.abs(y - g["y_csr_lb_ab"]) <= tol), f"{name}: y_csr_lb_ab"
+        y1, s1 = fn(A, x, 1.0, 0.0, np.zeros(A.n_rows))
+        assert np.all(np.abs(y1 - g["y_csr_a1b0"]) <= 1e-13 * s1 + 1e-300), f"{name}: y_csr_a1b0"
+        D = np.abs(scipy_csr(A)) @ np.abs(x)
+        np.testing.assert_allclose(s, D, rtol=1e-13, atol=0)
+    A = powerlaw(5000, lmax=700, seed=4)
+    x = uniform_x(A.n_cols, 9)
+    y0 = np.linspace(-1, 1, A.n_rows)
+    y, s = fn(A, x, 0.3, -1.25, y0)
+    ref = 0.3 * (scipy_csr(A) @ x) - 1.25 * y0
+    assert np.all(np.abs(y - ref) <= 1e-13 * (0.3 * s + 1.25 * np.abs(y0))), "scipy"
+    A = powerlaw(300, lmax=40, seed=7)
+    x = uniform_x(A.n_cols, 3)
+    y0 = 0.25 * ((np.arange(A.n_rows) * 7919) % 13) - 1.0
+    ey, es = _exact_spmv(A, x, 0.8, 0.7, y0)
+    y, s = fn(A, x, 0.8, 0.7, y0)
+    for r in range(A.n_rows):
+        bound = 2.0 ** -53 * abs(float(ey[r])) + 2.0 ** -60 * float(es[r] * 1 + abs(0.7 * y0[r]))
+        assert abs(y[r] - float(ey[r])) <= bound, f"exact: row {r}"
+        assert abs(s[r] - float(es[r])) <= 2.0 ** -52 * float(es[r]), f"exact scale: row {r}"
+
+
+def test_spmv_exact_pinned():
+    _spmv_exact_checks(orc.spmv_exact)
+
+
+def test_spmv_exact_planted_mistakes_fail():
+    """Mistakes planted in the extended-precision SpMV (on the CPU) must fail its pins: a wrong
+    beta, a dropped row, the last entry of every row dropped, the transposed operand, and a
+    condition scale that sums signed products."""
+    def wrong_beta(A, x, a, b, y0):
+        return orc.spmv_exact(A, x, a, b + 1e-6, y0)
+
+    def dropped_row(A, x, a, b, y0):
+        v = np.array(A.val, dtype=np.float64)
+        v[A.rowptr[5]:A.rowptr[6]] = 0.0
+        return orc.spmv_exact(A, x, a, b, y0, values=v)
+
+    def dropped_last(A, x, a, b, y0):
+        v = np.array(A.val, dtype=np.float64)
+        ends = np.asarray(A.rowptr[1:], np.int64)
+        starts = np.asarray(A.rowptr[:-1], np.int64)
+        v[(ends - 1)[ends > starts]] = 0.0
+        return orc.spmv_exact(A, x, a, b, y0, values=v)
+
+    def transposed(A, x, a, b, y0):
+        return orc.spmv_exact(transpose(A), x, a, b, y0)
+
+    def signed_scale(A, x, a, b, y0):
+        y, _ = orc.spmv_exact(A, x, a, b, y0)
+        return y, scipy_csr(A) @ x
+
+    for mutant in (wrong_beta, dropped_row, dropped_last, transposed, signed_scale):
+        with pytest.raises(AssertionError):
+            _spmv_exact_checks(mutant)
+
+
+@pytest.mark.parametrize("name", ["spmv_lap16_f32", "spmv_pl3000_f32"])
+def test_spmv_csr_f32_matches_reference(name):
+    """The FP32 CSR restatement against the reference's FP32 build on the same inputs.  The
+    two sum in the same order but the reference's compiler may contract products into fused
+    multiply-adds, so the bound is two FP32 recursive-summation errors of the row:
+    2 * (L + 2) * 2^-24 * (|alpha| sum|a x| + |beta y0|)."""
+    g = gold(name)
+    A = SPMV_CASES[name.replace("_f32", "_f64")]()
+    x = uniform_x(A.n_cols, int(g["x_seed"]))
+    xs = x.astype(np.float32).astype(np.float64)
+    vs = A.val.astype(np.float32).astype(np.float64)
+    y0 = g["y0"].astype(np.float64)
+    _, s = orc.spmv_exact(A, xs, values=vs)
+    L = np.diff(np.asarray(A.rowptr, np.int64))
+    for a, b, want in ((0.8, 0.7, g["y_csr_ab"]), (1.0, 0.0, g["y_csr_a1b0"])):
+        y = orc.spmv_csr_f32(A, x, np.float32(a), np.float32(b), g["y0"]).astype(np.float64)
+        tol = 2 * (L + 2) * 2.0 ** -24 * (abs(a) * s + abs(b) * np.abs(y0)) + 1e-30
+        assert np.all(np.abs(y - want.astype(np.float64)) <= tol)
+        exact, _ = orc.spmv_exact(A, xs, a, b, y0, values=vs)
+        assert np.all(np.abs(y - exact) <= tol / 2)
+    # a planted mistake (alpha and beta swapped) fails the same bound
+    y = orc.spmv_csr_f32(A, x, np.float32(0.7), np.float32(0.8), g["y0"]).astype(np.float64)
+    assert not np.all(np.abs(y - g["y_csr_ab"].astype(np.float64)) <= 2 * (L + 2) * 2.0 ** -24 * (s + np.abs(y0)))
+
+
+# ---------------------------------------------------- kernel-3 classification (acc_flag)
+@pytest.mark.parametrize("name,scale", [("classify_rmat10", 10), ("classify_rmat12", 12)])
+def test_cluster_classify_matches_reference(name, scale):
+    """acc_flag and dense_cluster_count against the reference's classify_clusters
+    (src/spgemm_Vlength_mixed.cpp:268-319) at several L2 fractions of its budget
+    (src/spgemm.cpp:609-614: CPU_L2CACHE_SIZE = 64 MiB times the fraction, 0.75 when the
+    fraction is outside [0, 2])."""
+    g = gold(name)
+    A = rmat(scale, 16, seed=scale)
+    assert digest(A.rowptr, A.col, A.val) == str(g["input_digest"])
+    clr, clrv, csz, colids, vals = orc.cluster_build(A, g["perm"], g["offsets"])
+    nz, mn, mx = orc.cluster_symbolic(clr, colids, A)
+    np.testing.assert_array_equal(np.concatenate([[0], np.cumsum(nz)]), g["ccl_rowptr"])
+    np.testing.assert_array_equal(mn, g["ccl_min"])
+    np.testing.assert_array_equal(mx, g["ccl_max"])
+    rowdense = A.nnz / A.n_rows
+    for t, f in enumerate(g["fractions"]):
+        frac = f if 0.0 <= f <= 2.0 else 0.75
+        budget = int(float(64 << 20) * frac)
+        flag, dense = orc.cluster_classify(clr, csz, nz, mn, mx, rowdense, budget)
+        np.testing.assert_array_equal(flag, g[f"acc_flag_{t}"])
+        assert dense == int(g[f"dense_count_{t}"][0])
+
+
+def test_cluster_classify_closed_form():
+    """Hand-computed clusters at each boundary of the rule: dense iff
+    acols * rowdense * 16 + range * csz * 8 <= budget and nz / range >= 0.15."""
+    # cluster:         0     1     2     3     4     5
+    clr = np.array([0, 2, 4, 6, 8, 10, 12])      # 2 A columns each
+    csz = np.array([4, 4, 4, 4, 4, 1])
+    nz = np.array([0, 15, 14, 20, 20, 20])
+    mn = np.array([0, 0, 0, 0, 0, 0])
+    mx = np.array([9, 99, 99, 99, 99, 99])       # range 10 or 100
+    rowdense = 3.0                               # B bytes: 2 * 3 * 16 = 96
+    # dense bytes: range 100 * csz 4 * 8 = 3200 (cluster 5: 800); budget 3296 admits 96 + 3200
+    flag, dense = orc.cluster_classify(clr, csz, nz, mn, mx, rowdense, 3296)
+    # 0: empty; 1: density 0.15 (>=, dense); 2: 0.14; 3, 4: dense; 5: dense (smaller block)
+    np.testing.assert_array_equal(flag, [0, 1, 0, 1, 1, 1])
+    assert dense == 4
+    flag, dense = orc.cluster_classify(clr, csz, nz, mn, mx, rowdense, 3295)   # one byte short
+    np.testing.assert_array_equal(flag, [0, 0, 0, 0, 0, 1])
+    assert dense == 1
+    flag, _ = orc.cluster_classify(clr, csz, nz, mn, mx, rowdense, 3296, thr=0.2)
+    np.testing.assert_array_equal(flag, [0, 0, 0, 1, 1, 1])
+
+
+# ---------------------------------------------------- LeSpGEMM numeric, extended precision
+@pytest.mark.parametrize("name,B_of", [("lsh_rmat8", None), ("lsh_rmat10_AAt", "T"), ("lsh_rmat12", None)])
+def test_lespgemm_long_double_matches_reference_and_scipy(name, B_of):
+    """The extended-precision branch of the cluster numeric (the checker of the GPU values):
+    structure equal to the double branch and to scipy's product, values within 1e-12 of
+    scipy's and of the reference's, row sums within 1e-12 of the reference's."""
+    g = gold(name)
+    scale = {"lsh_rmat8": 8, "lsh_rmat12": 12}.get(name, 10)
+    A = rmat(scale, 16, seed=scale)
+    B = transpose(A) if B_of == "T" else A
+    out = orc.lespgemm(A, B, use_long_double=True)
+    np.testing.assert_array_equal(out["C_rowptr"], g["C_rowptr"])
+    assert digest(out["C_col"]) == str(g["C_col_digest"])
+    P = (scipy_csr(A) @ scipy_csr(B)).tocsr()
+    P.sort_indices()
+    np.testing.assert_array_equal(out["C_col"], P.indices)
+    np.testing.assert_allclose(out["C_val"], P.data, rtol=1e-12, atol=0)
+    rows = np.repeat(np.arange(A.n_rows), np.diff(out["C_rowptr"]))
+    rs = np.bincount(rows, weights=out["C_val"], minlength=A.n_rows)
+    np.testing.assert_allclose(rs, g["C_rowsum"], rtol=1e-12, atol=0)
+    if "C_val" in g:
+        np.testing.assert_allclose(out["C_val"], g["C_val"], rtol=1e-12, atol=0)
+    # a planted mistake (one B value perturbed by 1e-9 relative) must break the 1e-12 bound
+    Bv = np.array(B.val, dtype=np.float64)
+    Bv[len(Bv) // 2] *= 1 + 1e-9
+    bad = orc.lespgemm(A, Csr(B.n_rows, B.n_cols, B.rowptr, B.col, Bv), use_long_double=True)
+    assert not np.allclose(bad["C_val"], P.data, rtol=1e-12, atol=0)
+
+
+def test_lespgemm_scale14_matches_reference_fingerprints():
+    """C = A * A at R-MAT scale 14 against the reference's fingerprints (clustering and C's
+    structure as digests, C's row sums)."""
+    g = gold("lsh_rmat14_fp")
+    A = rmat(14, 16, seed=14)
+    assert digest(A.rowptr, A.col, A.val) == str(g["input_digest"])
+    out = orc.lespgemm(A, A, use_long_double=True)
+    assert digest(out["perm"]) == str(g["perm_digest"])
+    assert digest(out["offsets"]) == str(g["offsets_digest"])
+    assert digest(out["C_rowptr"]) == str(g["C_rowptr_digest"])
+    assert digest(out["C_col"]) == str(g["C_col_digest"])
+    rows = np.repeat(np.arange(A.n_rows), np.diff(out["C_rowptr"]))
+    rs = np.bincount(rows, weights=out["C_val"], minlength=A.n_rows)
+    np.testing.assert_allclose(rs, g["C_rowsum"], rtol=1e-12, atol=0)

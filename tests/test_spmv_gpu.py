@@ -137,6 +137,92 @@ def test_csr_swept(mats, name, vt, wgs, maxrows, dev):
             assert np.all(np.abs(Y[:, j].cpu().numpy() - ref) <= 1e-13 * sc + 1e-300)
 
 
+@pytest.mark.parametrize("vt", ["f64", "f32"])
+def test_csr_swept_set_values_ordering(mats, vt, dev):
+    """set_values refreshes the swept copy on its own stream: (1) an SpMV captured into a graph
+    before set_values and replayed after it sees the new values; (2) two streams that both wait
+    for set_values and then each run an SpMV both see them.  Each replay is checked against the
+    oracle, with eager work and host synchronisations between the replays."""
+    A = mats["pl"]
+    with S.spmv_layout("swept"), S.sweep_workgroups(64):
+        M = S.csr(A, vt)
+    assert M.sweep is not None and M.sweep[1] > 1
+    dt = M.xy_dtype
+    x = uniform_x(A.n_cols, 21)
+    xt = torch.tensor(x, dtype=dt, device=dev)
+    yt = torch.zeros(A.n_rows, dtype=dt, device=dev)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        M.spmv(xt, yt, 1.0, 0.0)          # warm-up outside the capture
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=s):
+            M.spmv(xt, yt, 1.0, 0.0)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(22)
+    for rep in range(3):
+        vals = rng.standard_normal(A.nnz)
+        M.set_values(torch.tensor(vals, device=dev))
+        _ = xt.abs().sum().item()
+        yt.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        A2 = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, vals)
+        if vt == "f64":
+            ref, sc = orc.spmv_exact(A2, x)
+            tol = 1e-13
+        else:
+            ref, sc = orc.spmv_exact(A2, x.astype(np.float32).astype(np.float64),
+                                     values=vals.astype(np.float32).astype(np.float64))
+            tol = 4e-7
+        err = np.abs(yt.cpu().double().numpy() - ref) / (sc + 1e-300)
+        assert np.all(err <= tol), f"replay {rep}: conditioned error {err.max():.3e}"
+    # two streams ordered after set_values' stream by events
+    vals = rng.standard_normal(A.nnz)
+    s0, s1, s2 = torch.cuda.Stream(), torch.cuda.Stream(), torch.cuda.Stream()
+    y1 = torch.full((A.n_rows,), float("nan"), dtype=dt, device=dev)
+    y2 = torch.full((A.n_rows,), float("nan"), dtype=dt, device=dev)
+    vt_dev = torch.tensor(vals, device=dev)
+    torch.cuda.synchronize()
+    M.set_values(vt_dev, stream=s0.cuda_stream)
+    ev = torch.cuda.Event()
+    ev.record(s0)
+    s1.wait_event(ev)
+    s2.wait_event(ev)
+    # distinct partial planes: run the two SpMVs on separate handles' planes by serialising
+    # the second after the first (calls on one handle share its planes and must be ordered)
+    M.spmv(xt, y1, 1.0, 0.0, stream=s1.cuda_stream)
+    ev1 = torch.cuda.Event()
+    ev1.record(s1)
+    s2.wait_event(ev1)
+    M.spmv(xt, y2, 1.0, 0.0, stream=s2.cuda_stream)
+    torch.cuda.synchronize()
+    A2 = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, vals)
+    xr = x if vt == "f64" else x.astype(np.float32).astype(np.float64)
+    vr = vals if vt == "f64" else vals.astype(np.float32).astype(np.float64)
+    ref, sc = orc.spmv_exact(A2, xr, values=vr)
+    tol = 1e-13 if vt == "f64" else 4e-7
+    for y in (y1, y2):
+        err = np.abs(y.cpu().double().numpy() - ref) / (sc + 1e-300)
+        assert np.all(err <= tol), f"stream-ordered SpMV: conditioned error {err.max():.3e}"
+
+
+def test_csr_create_layout(mats, dev):
+    """spops_csr_create_layout: 1 builds no swept copy (SpMM / SDDMM handles), 2 always does,
+    whatever the process-wide layout setting."""
+    A = mats["pl"]
+    with S.spmv_layout("swept"):
+        R = S.csr(A, "f32", spmv_layout=1)
+    with S.spmv_layout("sliced"):
+        W = S.csr(A, "f32", spmv_layout=2)
+    assert R.sweep is None and W.sweep is not None
+    x = uniform_x(A.n_cols, 3)
+    _check(A, R, x, _y0(A.n_rows), "f32", dev=dev)
+    _check(A, W, x, _y0(A.n_rows), "f32", dev=dev)
+
+
 def test_csr_swept_auto_selection(dev):
     """CSR handles add the swept layout under the same rule as SELL (random columns over an x
     larger than the slice budget) and keep the row kernels for a band."""
