@@ -195,6 +195,17 @@ int spops_spmm(spops_matrix A, int64_t k, double alpha, const void *d_X, double 
 /* the same with explicit X and Y element types (SPOPS_VALUE_F64 / F32 / BF16) */
 int spops_spmm_typed(spops_matrix A, int64_t k, double alpha, const void *d_X, int x_type, double beta, void *d_Y,
                      int y_type, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc): spops_spmm_typed
+ * with the same arguments and results, computed by the same kernels, whose only scratch (the
+ * partial sums of rows longer than 256 entries, one row of k sums per 512-entry segment) is
+ * a static device array of 64 MiB instead of a per-handle buffer grown on demand.  It makes
+ * no allocation.  A request needing more (segments x k x 8 bytes for FP64 values or X, x 4
+ * otherwise) returns SPOPS_ERROR_LIMIT.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (different streams, or host threads without ordering) race;
+ * order them. */
+int spops_spmm_typed_noalloc(spops_matrix A, int64_t k, double alpha, const void *d_X, int x_type, double beta,
+                             void *d_Y, int y_type, void *stream);
 /* out[e] = sum_c G[row_e, c] * X[col_e, c] for every entry e of A (A's values unused);
  * G row-major (n_rows x k), X row-major (n_cols x k), out[nnz] */
 int spops_sddmm(spops_matrix A, int64_t k, const void *d_G, int g_type, const void *d_X, int x_type, void *d_out,

@@ -58,6 +58,7 @@ _SIGNATURES = [
     ("spops_spmv", c_int, [c_vp, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
     ("spops_spmm", c_int, [c_vp, c_i64, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
     ("spops_spmm_typed", c_int, [c_vp, c_i64, c_dbl, c_vp, c_int, c_dbl, c_vp, c_int, c_vp]),
+    ("spops_spmm_typed_noalloc", c_int, [c_vp, c_i64, c_dbl, c_vp, c_int, c_dbl, c_vp, c_int, c_vp]),
     ("spops_sddmm", c_int, [c_vp, c_i64, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_vp]),
     ("spops_csr_transpose", c_int, [c_vp, c_vp]),
     ("spops_csr_set_values", c_int, [c_vp, c_vp, c_int, c_vp]),

@@ -19,6 +19,7 @@
 //                    G and X rows, cross-lane sum.
 #include <algorithm>
 #include <memory>
+#include <mutex>
 #include <numeric>
 
 #include "common.hpp"
@@ -326,7 +327,7 @@ inline int lanes_for(int k, int vec) {
 
 template <int VC, int XC, int YC, typename RP, int VEC>
 void spmm_launch_vec(const spops_matrix_s &A, int k, double alpha, const void *X, double beta, void *Y,
-                     hipStream_t s) {
+                     hipStream_t s, void *ext_part) {
   using Acc = AccT<VC, XC>;
   const auto &C = A.csr;
   const RP *rp = C.rowptr.as<RP>();
@@ -341,13 +342,19 @@ void spmm_launch_vec(const spops_matrix_s &A, int k, double alpha, const void *X
     SPOPS_HIP_CHECK(hipGetLastError());
   }
   if (C.nslong > 0) {
-    const size_t need = (size_t)C.nlseg * (size_t)k * sizeof(Acc);
-    if (C.spmm_part.bytes < need) C.spmm_part.alloc(need);
+    // the segments' partial sums: the caller's scratch (no allocation), else the handle's
+    // buffer, grown on demand
+    Acc *part = static_cast<Acc *>(ext_part);
+    if (part == nullptr) {
+      const size_t need = (size_t)C.nlseg * (size_t)k * sizeof(Acc);
+      if (C.spmm_part.bytes < need) C.spmm_part.alloc(need);
+      part = C.spmm_part.as<Acc>();
+    }
     hipLaunchKernelGGL((k_spmm_seg<VC, XC, RP, VEC>), dim3((unsigned)C.nlseg), dim3(kBlock), 0, s, rp, C.col.as<int>(),
-                       v, C.slongs.as<int>(), C.lsegs.as<int2>(), k, L, x, C.spmm_part.as<Acc>());
+                       v, C.slongs.as<int>(), C.lsegs.as<int2>(), k, L, x, part);
     SPOPS_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL((k_spmm_seg_sum<Acc, YC>), dim3((unsigned)C.nslong), dim3(kBlock), 0, s, C.slongs.as<int>(),
-                       C.lsegoff.as<int>(), C.spmm_part.as<Acc>(), k, y, alpha, beta);
+                       C.lsegoff.as<int>(), part, k, y, alpha, beta);
     SPOPS_HIP_CHECK(hipGetLastError());
   }
 }
@@ -355,40 +362,42 @@ void spmm_launch_vec(const spops_matrix_s &A, int k, double alpha, const void *X
 // One vector access covers 16 bytes of X when k is a multiple of that many elements and X
 // and Y are aligned for it; otherwise one element per lane.
 template <int VC, int XC, int YC, typename RP>
-void spmm_launch(const spops_matrix_s &A, int k, double alpha, const void *X, double beta, void *Y, hipStream_t s) {
+void spmm_launch(const spops_matrix_s &A, int k, double alpha, const void *X, double beta, void *Y, hipStream_t s,
+                 void *ext) {
   constexpr int V = 16 / (int)sizeof(typename Ty<XC>::T);
   if (k % V == 0 && aligned_to(X, 16) && aligned_to(Y, V * sizeof(typename Ty<YC>::T)))
-    spmm_launch_vec<VC, XC, YC, RP, V>(A, k, alpha, X, beta, Y, s);
+    spmm_launch_vec<VC, XC, YC, RP, V>(A, k, alpha, X, beta, Y, s, ext);
   else
-    spmm_launch_vec<VC, XC, YC, RP, 1>(A, k, alpha, X, beta, Y, s);
+    spmm_launch_vec<VC, XC, YC, RP, 1>(A, k, alpha, X, beta, Y, s, ext);
 }
 
 template <int VC, int XC, typename RP>
-void spmm_y(const spops_matrix_s &A, int k, double alpha, const void *X, double beta, void *Y, int yt, hipStream_t s) {
+void spmm_y(const spops_matrix_s &A, int k, double alpha, const void *X, double beta, void *Y, int yt, hipStream_t s,
+            void *ext) {
   switch (yt) {
-    case SPOPS_VALUE_F64: spmm_launch<VC, XC, SPOPS_VALUE_F64, RP>(A, k, alpha, X, beta, Y, s); break;
-    case SPOPS_VALUE_F32: spmm_launch<VC, XC, SPOPS_VALUE_F32, RP>(A, k, alpha, X, beta, Y, s); break;
-    default: spmm_launch<VC, XC, SPOPS_VALUE_BF16, RP>(A, k, alpha, X, beta, Y, s); break;
+    case SPOPS_VALUE_F64: spmm_launch<VC, XC, SPOPS_VALUE_F64, RP>(A, k, alpha, X, beta, Y, s, ext); break;
+    case SPOPS_VALUE_F32: spmm_launch<VC, XC, SPOPS_VALUE_F32, RP>(A, k, alpha, X, beta, Y, s, ext); break;
+    default: spmm_launch<VC, XC, SPOPS_VALUE_BF16, RP>(A, k, alpha, X, beta, Y, s, ext); break;
   }
 }
 
 template <int VC, typename RP>
 void spmm_x(const spops_matrix_s &A, int k, double alpha, const void *X, int xt, double beta, void *Y, int yt,
-            hipStream_t s) {
+            hipStream_t s, void *ext) {
   switch (xt) {
-    case SPOPS_VALUE_F64: spmm_y<VC, SPOPS_VALUE_F64, RP>(A, k, alpha, X, beta, Y, yt, s); break;
-    case SPOPS_VALUE_F32: spmm_y<VC, SPOPS_VALUE_F32, RP>(A, k, alpha, X, beta, Y, yt, s); break;
-    default: spmm_y<VC, SPOPS_VALUE_BF16, RP>(A, k, alpha, X, beta, Y, yt, s); break;
+    case SPOPS_VALUE_F64: spmm_y<VC, SPOPS_VALUE_F64, RP>(A, k, alpha, X, beta, Y, yt, s, ext); break;
+    case SPOPS_VALUE_F32: spmm_y<VC, SPOPS_VALUE_F32, RP>(A, k, alpha, X, beta, Y, yt, s, ext); break;
+    default: spmm_y<VC, SPOPS_VALUE_BF16, RP>(A, k, alpha, X, beta, Y, yt, s, ext); break;
   }
 }
 
 template <typename RP>
 void spmm_v(const spops_matrix_s &A, int k, double alpha, const void *X, int xt, double beta, void *Y, int yt,
-            hipStream_t s) {
+            hipStream_t s, void *ext) {
   switch (A.value_type) {
-    case SPOPS_VALUE_F64: spmm_x<SPOPS_VALUE_F64, RP>(A, k, alpha, X, xt, beta, Y, yt, s); break;
-    case SPOPS_VALUE_F32: spmm_x<SPOPS_VALUE_F32, RP>(A, k, alpha, X, xt, beta, Y, yt, s); break;
-    default: spmm_x<SPOPS_VALUE_BF16, RP>(A, k, alpha, X, xt, beta, Y, yt, s); break;
+    case SPOPS_VALUE_F64: spmm_x<SPOPS_VALUE_F64, RP>(A, k, alpha, X, xt, beta, Y, yt, s, ext); break;
+    case SPOPS_VALUE_F32: spmm_x<SPOPS_VALUE_F32, RP>(A, k, alpha, X, xt, beta, Y, yt, s, ext); break;
+    default: spmm_x<SPOPS_VALUE_BF16, RP>(A, k, alpha, X, xt, beta, Y, yt, s, ext); break;
   }
 }
 
@@ -481,7 +490,7 @@ void convert_values(const void *src, int st, const void *perm, bool perm64, int6
 }  // namespace
 
 void launch_spmm(const spops_matrix_s &A, int64_t k, double alpha, const void *X, int xt, double beta, void *Y, int yt,
-                 hipStream_t s) {
+                 hipStream_t s, void *ext_part) {
   require_csr(&A);
   require_type(xt, "X");
   require_type(yt, "Y");
@@ -489,11 +498,39 @@ void launch_spmm(const spops_matrix_s &A, int64_t k, double alpha, const void *X
   SPOPS_REQUIRE((uint64_t)A.n_cols * (uint64_t)k < ((uint64_t)1 << 40) &&
                     (uint64_t)A.n_rows * (uint64_t)k < ((uint64_t)1 << 40),
                 SPOPS_ERROR_LIMIT, "n * k too large");
-  if (A.csr.rowptr64) spmm_v<long>(A, (int)k, alpha, X, xt, beta, Y, yt, s);
-  else spmm_v<int>(A, (int)k, alpha, X, xt, beta, Y, yt, s);
+  if (A.csr.rowptr64) spmm_v<long>(A, (int)k, alpha, X, xt, beta, Y, yt, s, ext_part);
+  else spmm_v<int>(A, (int)k, alpha, X, xt, beta, Y, yt, s, ext_part);
 }
 
 }  // namespace spops
+
+// Scratch of spops_spmm_typed_noalloc: the partial sums of the long rows' segments (one row
+// of k sums per segment of kSpmmSegment entries, FP64 or FP32).  Each GPU holds its own copy
+// for the life of the process; every call on that GPU shares it.
+constexpr size_t kSpmmScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kSpmmScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (the array's name is a host-side
+// stand-in there), looked up on the first call on each device and kept.
+void *spmm_scratch_address() {
+  int dev = 0;
+  SPOPS_HIP_CHECK(hipGetDevice(&dev));
+  SPOPS_REQUIRE(dev >= 0 && dev < kMaxDevices, SPOPS_ERROR_LIMIT, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void *p = nullptr;
+    SPOPS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SPOPS_REQUIRE(p != nullptr, SPOPS_ERROR_HIP, "no device address for the SpMM scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
 
 using namespace spops;
 
@@ -504,7 +541,7 @@ extern "C" int spops_spmm(spops_matrix A, int64_t k, double alpha, const void *d
     SPOPS_REQUIRE((d_X != nullptr || A->n_cols == 0) && (d_Y != nullptr || A->n_rows == 0), SPOPS_ERROR_INVALID_VALUE,
                   "null X or Y");
     const int t = A->value_type == SPOPS_VALUE_F64 ? SPOPS_VALUE_F64 : SPOPS_VALUE_F32;
-    launch_spmm(*A, k, alpha, d_X, t, beta, d_Y, t, static_cast<hipStream_t>(stream));
+    launch_spmm(*A, k, alpha, d_X, t, beta, d_Y, t, static_cast<hipStream_t>(stream), nullptr);
     return SPOPS_SUCCESS;
   });
 }
@@ -515,7 +552,28 @@ extern "C" int spops_spmm_typed(spops_matrix A, int64_t k, double alpha, const v
     require_csr(A);
     SPOPS_REQUIRE((d_X != nullptr || A->n_cols == 0) && (d_Y != nullptr || A->n_rows == 0), SPOPS_ERROR_INVALID_VALUE,
                   "null X or Y");
-    launch_spmm(*A, k, alpha, d_X, x_type, beta, d_Y, y_type, static_cast<hipStream_t>(stream));
+    launch_spmm(*A, k, alpha, d_X, x_type, beta, d_Y, y_type, static_cast<hipStream_t>(stream), nullptr);
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_spmm_typed_noalloc(spops_matrix A, int64_t k, double alpha, const void *d_X, int x_type,
+                                        double beta, void *d_Y, int y_type, void *stream) {
+  return guarded([&]() {
+    require_csr(A);
+    require_type(x_type, "X");
+    SPOPS_REQUIRE((d_X != nullptr || A->n_cols == 0) && (d_Y != nullptr || A->n_rows == 0), SPOPS_ERROR_INVALID_VALUE,
+                  "null X or Y");
+    SPOPS_REQUIRE(k > 0 && k < (1 << 20), SPOPS_ERROR_INVALID_VALUE, "k must be in [1, 2^20)");
+    const size_t acc = (A->value_type == SPOPS_VALUE_F64 || x_type == SPOPS_VALUE_F64) ? 8 : 4;
+    const size_t need = (size_t)A->csr.nlseg * (size_t)k * acc;
+    SPOPS_REQUIRE(need <= kSpmmScratchBytes, SPOPS_ERROR_LIMIT,
+                  "spops_spmm_typed_noalloc: %zu bytes of segment partial sums (%d segments x k=%lld) exceed its "
+                  "static scratch of %zu bytes; use spops_spmm_typed",
+                  need, A->csr.nlseg, (long long)k, kSpmmScratchBytes);
+    void *part = A->csr.nlseg > 0 ? spmm_scratch_address() : nullptr;
+    launch_spmm(*A, k, alpha, d_X, x_type, beta, d_Y, y_type, static_cast<hipStream_t>(stream),
+                part);
     return SPOPS_SUCCESS;
   });
 }
