@@ -34,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+HOST = {}            # the CPU baselines' host (host_cpu()), reported with each of them
 METRIC = "SpMV achieved GB/s (% HBM peak), SpGEMM GFLOP/s; mixed vs FP64 at rel err"
 
 
@@ -78,6 +79,53 @@ def time_spmv(M, x, y, steps, warmup, stream):
     return wall_ms, tot / steps
 
 
+def host_cpu():
+    """The host the CPU baselines run on: logical CPUs (nproc), the CPUs this process may run
+    on, the physical cores among them (distinct (socket, core) pairs of /proc/cpuinfo), the
+    sockets, the CPU model, and the OpenMP thread count the baselines use: one per physical
+    core this process may use, capped by OMP_NUM_THREADS when that is set."""
+    allowed = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count() or 1))
+    model, cores, sockets = "unknown", set(), set()
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as f:
+            for line in list(f) + ["\n"]:
+                if not line.strip():
+                    if "processor" in cur and int(cur["processor"]) in allowed:
+                        cores.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                        sockets.add(cur.get("physical id", "0"))
+                        model = cur.get("model name", model)
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+    except OSError:
+        pass
+    phys = len(cores) or len(allowed)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(phys, int(env)) if env.isdigit() and int(env) > 0 else phys
+    return {"nproc": os.cpu_count(), "allowed_cpus": len(allowed), "physical_cores": phys,
+            "sockets": max(1, len(sockets)), "cpu_model": model, "threads": threads}
+
+
+def ref_bytes_per_spmv(fmt, vt, A, M):
+    """The reference's own GB/s model, bytes_per_spmv (include/spmv_benchmark.h:43-51 for CSR,
+    :107-122 for SELL-C-sigma, which counts each chunk's values with sizeof(IndexType)), for
+    int32 indices; None for BF16 values, which the reference does not have."""
+    if vt == "bf16":
+        return None
+    sI, sV, n, nnz = 4, 8 if vt == "f64" else 4, A.n_rows, A.nnz
+    if fmt in ("csr", "csrrows"):
+        return 2 * sI * n + sI * nnz + 2 * sV * nnz + 2 * sV * n
+    cl = M.layout(1).astype("int64")
+    C = M.info[10]
+    return sI * len(cl) + sI * n + int(cl.sum()) * C * (sI + sI) + sV * nnz + 2 * sV * n
+
+
+def cpu_env(threads):
+    return dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores", OMP_PROC_BIND="spread")
+
+
 def cpu_baseline(A, seconds, threads):
     """The reference's own OpenMP SELL-C-sigma SpMV (kernel_flag 2, FP32, C=32, sigma=1024),
     built from its sources into oracle/_ref/ref_harness, on this host's cores."""
@@ -88,9 +136,8 @@ def cpu_baseline(A, seconds, threads):
     with tempfile.TemporaryDirectory(dir="/tmp") as d:
         path = os.path.join(d, "A.csr")
         write_csr_bin(path, A)
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PLACES="cores", OMP_PROC_BIND="spread")
         out = subprocess.run([exe, "bench_spmv", path, "sell", "32", str(threads), str(seconds), "1024", "32"],
-                             capture_output=True, text=True, env=env, timeout=600)
+                             capture_output=True, text=True, env=cpu_env(threads), timeout=600)
     if out.returncode != 0:
         log(f"cpu baseline failed: {out.stderr[-500:]}")
         return None
@@ -113,6 +160,15 @@ def committed_traffic():
     return vals.get("bytes")
 
 
+def spgemm_roofline(nnz_a, nnz_b, n_a, n_b, nnz_c, n_c, ms):
+    """Roofline of one C = A * B: compulsory bytes = A, B and C once each as int64 + FP64 CSR
+    ((rows + 1) * 8 + nnz * 16 each) over the time of the whole product (plan + panels)."""
+    comp = (n_a + 1) * 8 + nnz_a * 16 + (n_b + 1) * 8 + nnz_b * 16 + (n_c + 1) * 8 + nnz_c * 16
+    gbs = comp / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "compulsory_bytes": int(comp)}
+
+
 def lespgemm_measure(scale, iterations, cpu_threads, cpu):
     """LeSpGEMM C = A*A on R-MAT(scale, edge factor 16), k=64, bands=16, seed=7,
     cluster_size=8, hc_v=0: GPU pipeline and, unless cpu is False, the reference's own
@@ -129,18 +185,19 @@ def lespgemm_measure(scale, iterations, cpu_threads, cpu):
            "flop": int(r["flop"]), "gpu_ms": round(gpu_ms, 3), "gpu_gflops": round(r["flop"] / gpu_ms / 1e6, 2),
            "gpu_genpairs_ms": round(r["times"]["genpairs_ms"], 2), "gpu_hc_ms": round(r["times"]["hc_ms"], 2),
            "gpu_conversion_ms": round(r["times"]["conversion_ms"], 2), "nnz_C": int(len(r["C_col"])),
-           "clusters": int(r["nclusters"])}
+           "clusters": int(r["nclusters"]),
+           "roofline": spgemm_roofline(A.nnz, A.nnz, A.n_rows, A.n_rows, len(r["C_col"]), A.n_rows, gpu_ms)}
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     if cpu and os.path.exists(exe):
         with tempfile.TemporaryDirectory(dir="/tmp") as d:
             path = os.path.join(d, "A.csr")
             write_csr_bin(path, A)
-            env = dict(os.environ, OMP_NUM_THREADS=str(cpu_threads), OMP_PLACES="cores", OMP_PROC_BIND="spread")
+            env = cpu_env(cpu_threads)
             p = subprocess.run([exe, "bench_lespgemm", path, path, str(cpu_threads), "3", "64", "16", "7", "8", "3"],
                                capture_output=True, text=True, env=env, timeout=900)
         if p.returncode == 0:
             c = json.loads(p.stdout.strip().splitlines()[-1])
-            out["cpu_reference"] = {"gflops": round(c["gflops"], 3), "lespgemm_ms": c["lespgemm_ms"],
+            out["cpu_reference"] = {**HOST, "gflops": round(c["gflops"], 3), "lespgemm_ms": c["lespgemm_ms"],
                                     "genpairs_ms": c["genpairs_ms"], "hc_ms": c["hc_ms"],
                                     "conversion_ms": c["conversion_ms"], "threads": c["threads"],
                                     "kind": "reference", "iterations": 3}
@@ -204,6 +261,11 @@ def lespgemm_panels_measure(scale, budget_gb, cpu_threads, cpu, cpu_share=0.02):
             plan.close()
     plan = last
     w = passes[-1]
+    # nonzeros of C (its CSR nnz, exact zeros dropped): an untimed pass over the panels
+    nnz_c = 0
+    for c0, c1 in ranges:
+        C = plan.numeric(c0, c1, C)
+        nnz_c += C.checksum()["nonzeros"]
     out = {"workload": f"R-MAT scale {scale} (n={A.n_rows}, nnz={A.nnz}), C = A*A, kernel 3, cluster_size 8, k=64, "
                        f"bands=16, seed=7, hc_v=0; output in cluster panels of <= {budget_gb} GiB",
            "flop": int(flop), "clusters": int(plan.clusters), "nnzc": int(plan.nnzc), "nnzv": int(plan.nnzv),
@@ -211,7 +273,9 @@ def lespgemm_panels_measure(scale, budget_gb, cpu_threads, cpu, cpu_share=0.02):
            "gpu_ms": round(w["wall_ms"], 2), "gpu_gflops": round(flop / w["wall_ms"] / 1e6, 2),
            "gpu_plan_ms": round(w["plan_ms"], 2), "gpu_panels_ms": round(w["panels_ms"], 2),
            "gpu_genpairs_ms": round(genpairs_ms, 1), "gpu_hc_ms": round(hc_ms, 1),
-           "gpu_conversion_ms": round(conv_ms, 1), "generate_s": round(gen_s, 1), "pairs": int(len(pi))}
+           "gpu_conversion_ms": round(conv_ms, 1), "generate_s": round(gen_s, 1), "pairs": int(len(pi)),
+           "nnz_C": int(nnz_c),
+           "roofline": spgemm_roofline(A.nnz, A.nnz, A.n_rows, A.n_rows, nnz_c, A.n_rows, w["wall_ms"])}
     # per-cluster flop in get_spgemm_flop's sense, and two ranges of about cpu_share of it
     cf = LG.cluster_flops(A, A.rowptr, perm, offsets)
     tot = int(cf.sum())
@@ -235,7 +299,7 @@ def lespgemm_panels_measure(scale, budget_gb, cpu_threads, cpu, cpu_share=0.02):
         with tempfile.TemporaryDirectory(dir="/tmp") as d:
             path = os.path.join(d, "A.csr")
             write_csr_bin(path, A)
-            env = dict(os.environ, OMP_NUM_THREADS=str(cpu_threads), OMP_PLACES="cores", OMP_PROC_BIND="spread")
+            env = cpu_env(cpu_threads)
             rl = ",".join(f"{a}:{b}" for a, b in picks)
             proc = subprocess.Popen([exe, "bench_lespgemm_panels", path, path, str(cpu_threads), "1", "64", "16", "7",
                                      "8", "3", rl], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
@@ -252,7 +316,8 @@ def lespgemm_panels_measure(scale, budget_gb, cpu_threads, cpu, cpu_share=0.02):
             match = all(c["nnzc"] == g["nnzc"] and c["nnzv"] == g["nnzv"] and int(c["colid_sum"]) == g["colid_sum"]
                         and int(c["nonzeros"]) == g["nonzeros"]
                         and abs(c["sum"] - g["sum"]) <= 1e-10 * max(c["sum_abs"], 1e-300) for c, g in zip(cp, gpu_pick))
-            out["cpu_reference"] = {"kind": "reference", "threads": head["threads"], "genpairs_ms": head["genpairs_ms"],
+            out["cpu_reference"] = {**HOST, "kind": "reference", "threads": head["threads"],
+                                    "genpairs_ms": head["genpairs_ms"],
                                     "hc_ms": head["hc_ms"], "conversion_ms": head["conversion_ms"],
                                     "sampled_lespgemm_ms": round(cpu_ms, 2),
                                     "sampled_gflops": round(pflop / cpu_ms / 1e6, 3),
@@ -285,18 +350,24 @@ def lespgemm_panels_measure(scale, budget_gb, cpu_threads, cpu, cpu_share=0.02):
         torch.cuda.synchronize()
         walls.append((time.perf_counter() - t) * 1e3)
         nnzc_t = pt.nnzc
+        if len(walls) == 2:        # untimed: C's nonzeros for the roofline
+            nnz_ct = 0
+            for c0, c1 in ranges_t:
+                Ct = pt.numeric(c0, c1, Ct)
+                nnz_ct += Ct.checksum()["nonzeros"]
         pt.close()
     Ct.close()
     out["aat"] = {"workload": "C = A * A^T, same matrix and clustering", "flop": int(flop_t),
                   "gpu_ms": round(walls[-1], 2), "gpu_gflops": round(flop_t / walls[-1] / 1e6, 2),
-                  "nnzc": int(nnzc_t), "panels": len(ranges_t)}
+                  "nnzc": int(nnzc_t), "panels": len(ranges_t), "nnz_C": int(nnz_ct),
+                  "roofline": spgemm_roofline(A.nnz, At.nnz, A.n_rows, At.n_rows, nnz_ct, A.n_rows, walls[-1])}
     del dAt
     Acl.close()
     torch.cuda.synchronize()
     return out
 
 
-def autograd_measure(scale, k, steps, cpu):
+def autograd_measure(scale, k, steps, cpu, cpu_threads):
     """The differentiable SpMM layer Y = A(vals) X: R-MAT(scale, edge factor 32) with symmetric
     normalised FP32 values, X and G of width k in BF16, FP32 accumulation.  Forward and backward
     (dX = A^T G and dvals = SDDMM(G, X)) timed with HIP events; the CPU leg is the oracle's
@@ -348,20 +419,47 @@ def autograd_measure(scale, k, steps, cpu):
            "forward_gbs": round(b_fwd / fw / 1e6, 1), "backward_gbs": round(b_bwd / bw / 1e6, 1),
            "forward_gflops": round(2 * nnz * k / fw / 1e6, 1),
            "algo_bytes_forward": b_fwd, "algo_bytes_backward": b_bwd}
-    if cpu:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_bind as orc
-        from sparseops_amd.gen import Csr
-        r1 = max(1, n // 64)
-        sub = Csr(r1, A.n_cols, A.rowptr[:r1 + 1].astype(np.int64), A.col[:A.rowptr[r1]].astype(np.int64),
-                  A.val[:A.rowptr[r1]].astype(np.float64))
-        Xh = X.detach().double().cpu().numpy()
-        t = time.perf_counter()
-        orc.spmm(sub, Xh)
-        dt = (time.perf_counter() - t) * n / r1
-        out["cpu_baseline"] = {"forward_ms": round(dt * 1e3, 1), "kind": "port", "cores": 1,
-                               "sample": f"oracle CSR SpMM (FP64, one thread) on rows 0..{r1 - 1}, scaled by n/{r1}"}
-        out["gpu_over_cpu_forward"] = round(dt * 1e3 / fw, 1)
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if cpu and os.path.exists(exe):
+        from sparseops_amd.io import write_csr_bin
+        res = {}
+        with tempfile.TemporaryDirectory(dir="/tmp") as d:
+            path = os.path.join(d, "A.csr")
+            write_csr_bin(path, A)
+            for name, tr in (("forward", 0), ("backward_dx", 1)):
+                r = subprocess.run([exe, "bench_spmm", path, str(k), str(cpu_threads), "2", "32", str(tr)],
+                                   capture_output=True, text=True, env=cpu_env(cpu_threads), timeout=900)
+                if r.returncode == 0:
+                    res[name] = json.loads(r.stdout.strip().splitlines()[-1])
+                else:
+                    log(f"cpu autograd {name} failed: {r.stderr[-300:]}")
+        if len(res) == 2:
+            # dvals = SDDMM(G, X) has no reference counterpart: the oracle's one-thread loop on
+            # the first 1/64 of the rows, scaled to the whole matrix
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_bind as orc
+            from sparseops_amd.gen import Csr
+            r1 = max(1, n // 64)
+            sub = Csr(r1, A.n_cols, A.rowptr[:r1 + 1].astype(np.int64), A.col[:A.rowptr[r1]].astype(np.int64),
+                      A.val[:A.rowptr[r1]].astype(np.float64))
+            Xh = X.detach().double().cpu().numpy()
+            Gh = G.double().cpu().numpy()
+            t = time.perf_counter()
+            orc.sddmm(sub, Gh[:r1], Xh)
+            sddmm_ms = (time.perf_counter() - t) * n / r1 * 1e3
+            fwd_ms = res["forward"]["seconds_per_spmm"] * 1e3
+            dx_ms = res["backward_dx"]["seconds_per_spmm"] * 1e3
+            out["cpu_baseline"] = {"value": round(fwd_ms, 2), "unit": "ms (forward)", "kind": "reference",
+                                   "cores": cpu_threads, "forward_ms": round(fwd_ms, 2),
+                                   "backward_dx_ms": round(dx_ms, 2),
+                                   "backward_dvals_ms_port_1thread": round(sddmm_ms, 1),
+                                   "sample": f"the whole matrix: forward and dX = A^T G as {k} calls of the "
+                                             f"reference's OpenMP SpOps_csr (kernel_flag 2, FP32) per product, "
+                                             f"on A and on csr_transpose(A); dvals (no reference counterpart) "
+                                             f"by the oracle's one-thread SDDMM on rows 0..{r1 - 1}, scaled",
+                                   **HOST}
+            out["gpu_over_cpu_forward"] = round(fwd_ms / fw, 1)
+            out["gpu_over_cpu_backward_dx"] = round(dx_ms / bw, 1)
     pat.close()
     return out
 
@@ -403,7 +501,7 @@ def bsr_measure(n, b, blocks_per_row, steps, warmup):
     return out
 
 
-def batch_measure(count, steps, warmup, cpu, cpu_seconds):
+def batch_measure(count, steps, warmup, cpu, cpu_threads):
     """Batched SpMM: `count` band matrices (n log-uniform in [1k, 64k], half-width U{1..64},
     0-8 off-band entries per row, k U{1..32}, FP32 N(0,1) values and X), formats chosen by the
     autotuner, one persistent launch per step replayed from a captured HIP graph and timed with
@@ -465,25 +563,38 @@ def batch_measure(count, steps, warmup, cpu, cpu_seconds):
         got = B.y_view(Yh, i).numpy()
         worst = max(worst, float(np.max(np.abs(got - want) / (scale + 1e-30))))
     out["max_scaled_err_sampled"] = worst
-    if cpu:
-        t = time.perf_counter()
-        done_flop, used = 0, 0
-        for i in range(count):
-            A, k = mats[i], ks[i]
-            A64 = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, A.val.astype(np.float64))
-            Xi = np.random.default_rng(i).standard_normal((A.n_cols, k))
-            t0 = time.perf_counter()
-            orc.spmm(A64, Xi)
-            used += time.perf_counter() - t0
-            done_flop += 2 * A.nnz * k
-            if time.perf_counter() - t > cpu_seconds:
-                break
-        cpu_ms = B.flop / done_flop * used * 1e3
-        out["cpu_baseline"] = {"ms_per_step": round(cpu_ms, 1), "gflops": round(done_flop / used / 1e9, 3),
-                               "kind": "port", "cores": 1,
-                               "sample": f"oracle CSR SpMM (FP64, one thread) over matrices 0..{i} of the batch "
-                                         f"({done_flop:.3e} flop), scaled by flop to the whole batch"}
-        out["gpu_over_cpu"] = round(cpu_ms / step_ms, 1)
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if cpu and os.path.exists(exe):
+        # the reference's OpenMP SpOps_csr (kernel_flag 2, FP32) once per right-hand side, on the
+        # first matrices of the batch up to share_flop of its flop, scaled by flop
+        from sparseops_amd.io import write_csr_bin
+        share_flop, done = 0.05, 0
+        with tempfile.TemporaryDirectory(dir="/tmp") as d:
+            lines = []
+            for i in range(count):
+                pth = os.path.join(d, f"m{i}.csr")
+                write_csr_bin(pth, mats[i])
+                lines.append(f"{pth} {ks[i]}\n")
+                done += 2 * mats[i].nnz * ks[i]
+                if done >= share_flop * B.flop:
+                    break
+            lst = os.path.join(d, "list.txt")
+            with open(lst, "w") as f:
+                f.writelines(lines)
+            r = subprocess.run([exe, "bench_spmm_batch", lst, str(cpu_threads), "32"], capture_output=True, text=True,
+                               env=cpu_env(cpu_threads), timeout=900)
+        if r.returncode == 0:
+            c = json.loads(r.stdout.strip().splitlines()[-1])
+            cpu_ms = B.flop / c["flop"] * c["seconds"] * 1e3
+            out["cpu_baseline"] = {"value": round(cpu_ms, 1), "unit": "ms per batch", "kind": "reference",
+                                   "cores": c["threads"], "gflops": round(c["flop"] / c["seconds"] / 1e9, 3),
+                                   "sample": f"matrices 0..{c['matrices'] - 1} of the batch ({c['flop']:.3e} of "
+                                             f"{B.flop:.3e} flop): each k-column product as k calls of the "
+                                             f"reference's OpenMP SpOps_csr (kernel_flag 2, FP32) after a warm-up, "
+                                             f"scaled by flop to the whole batch", **HOST}
+            out["gpu_over_cpu"] = round(cpu_ms / step_ms, 1)
+        else:
+            log(f"cpu batch baseline failed: {r.stderr[-300:]}")
     B.close()
     return out
 
@@ -538,12 +649,12 @@ def laplacian_measure(nx, steps, warmup, cpu_threads, cpu, cpu_seconds):
         if cpu and os.path.exists(harness):
             path = os.path.join(d, "A.csr")
             write_csr_bin(path, A)
-            env = dict(os.environ, OMP_NUM_THREADS=str(cpu_threads), OMP_PLACES="cores", OMP_PROC_BIND="spread")
+            env = cpu_env(cpu_threads)
             r = subprocess.run([harness, "bench_spmv", path, "csr", "64", str(cpu_threads), str(cpu_seconds)],
                                capture_output=True, text=True, env=env, timeout=300)
             if r.returncode == 0:
                 c = json.loads(r.stdout.strip().splitlines()[-1])
-                out["cpu_reference"] = {"us": round(c["seconds_per_iter"] * 1e6, 3), "cores": c["threads"],
+                out["cpu_reference"] = {**HOST, "us": round(c["seconds_per_iter"] * 1e6, 3), "cores": c["threads"],
                                         "iterations": c["iterations"],
                                         "gbs": round(out["algo_bytes"] / c["seconds_per_iter"] / 1e9, 1)}
     return out
@@ -556,7 +667,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1 << 21)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="OpenMP threads of the CPU baselines (0: one per physical core this process may use, "
+                         "capped by OMP_NUM_THREADS)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variants", default="sell_f32,csr_f64,sell_f64,sell_bf16,csr_f32,csrrows_f64",
                     help="comma list of <fmt>_<value type> (fmt: sell, csr, or csrrows = CSR held to its row "
@@ -574,6 +687,10 @@ def main():
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    HOST.update(host_cpu())
+    if args.cpu_threads <= 0:
+        args.cpu_threads = HOST["threads"]
+    HOST["threads"] = args.cpu_threads
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "the sharded multi-GPU form is a design, not run here.", file=sys.stderr)
@@ -627,6 +744,9 @@ def main():
                          else ["k_csr_spmv"] if M.format not in ("sell", "ell")
                          else (["k_slice_spmv", "k_slice_reduce"] if M.slices > 1 else ["k_slice_spmv"]))}
         r.update(errors(yh, y64, absrow))
+        rb = ref_bytes_per_spmv(fmt, vt, A, M)
+        r["ref_bytes_per_spmv"] = rb
+        r["ref_model_gbs"] = round(rb / (kern_ms * 1e-3) / 1e9, 1) if rb else None
         res[name] = r
         log(f"{name}: {r['kernel_ms'] * 1e3:.1f} us/kernel, {r['kernel_gbs']:.0f} GB/s, "
             f"max rel err {r['max_rel_err']:.2e}, normwise {r['normwise_rel_err']:.2e}")
@@ -643,7 +763,7 @@ def main():
                    "cores": c["threads"], "kind": "reference",
                    "sample": f"the full matrix, {c['iterations']} SpMV iterations of the reference's OpenMP "
                              f"SELL-C-sigma kernel (kernel_flag 2, FP32, C=32, sigma=1024), "
-                             f"{c['seconds_per_iter'] * 1e3:.2f} ms each"}
+                             f"{c['seconds_per_iter'] * 1e3:.2f} ms each", **HOST}
         log(f"cpu baseline done in {time.time() - t:.1f}s: {cpu}")
 
     traffic = committed_traffic() if head_name == "sell_f32" and args.n == 1 << 21 else None
@@ -663,14 +783,14 @@ def main():
     ag = None
     if args.autograd_scale > 0:
         t = time.time()
-        ag = autograd_measure(args.autograd_scale, 64, 5, not args.no_cpu)
+        ag = autograd_measure(args.autograd_scale, 64, 5, not args.no_cpu, args.cpu_threads)
         log(f"autograd spmm done in {time.time() - t:.1f}s: {ag}")
 
     bt = None
     if args.batch_count > 0:
         t = time.time()
         bt = batch_measure(args.batch_count, max(5, args.steps // 10), max(2, args.warmup // 5), not args.no_cpu,
-                           args.cpu_seconds)
+                           args.cpu_threads)
         log(f"batched spmm done in {time.time() - t:.1f}s: {bt}")
 
     bs = None
@@ -699,8 +819,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "SELL-C-sigma SpMV, FP32 values + FP32 accumulation, power-law n=2,097,152 "
-                               f"(nnz={A.nnz}, L^-2.1 on [3,8192]), C=32, sigma=1024",
+        "config": {"workload": "SELL-C-sigma SpMV, FP32 values and x/y (products and row sums in FP64), "
+                               f"power-law n=2,097,152 (nnz={A.nnz}, mean row length {A.nnz / A.n_rows:.2f}, "
+                               "L^-2.1 on [3,8192]), C=32, sigma=1024",
                    "n": A.n_rows, "nnz": A.nnz, "C": 32, "sigma": 1024},
         "pct_hbm_peak": round(100.0 * head["gbs"] / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(head["kernel_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -712,7 +833,12 @@ def main():
                             for k, v in res.items()},
         "variants": {k: {"gbs": round(v["gbs"], 1), "kernel_us": round(v["kernel_ms"] * 1e3, 2),
                          "pct_hbm_peak": round(100 * v["kernel_gbs"] / HBM_PEAK_GBS, 2), "kernels": v["kernels"],
-                         "sweep": v["sweep"]} for k, v in res.items()},
+                         "sweep": v["sweep"], "algo_bytes": v["algo_bytes"],
+                         "ref_bytes_per_spmv": v["ref_bytes_per_spmv"], "ref_model_gbs": v["ref_model_gbs"]}
+                     for k, v in res.items()},
+        "reference_bytes_model": {"bytes_per_spmv": head["ref_bytes_per_spmv"], "gbs": head["ref_model_gbs"],
+                                  "note": "the reference's bytes_per_spmv (x counted once per entry, padded SELL "
+                                          "slots included); value and roofline use the algorithmic bytes"},
         "fp32_sell_over_fp64_csr": (round(res["csr_f64"]["kernel_ms"] / res["sell_f32"]["kernel_ms"], 3)
                                     if "csr_f64" in res and "sell_f32" in res else None),
         "fp32_sell_over_fp64_sell": (round(res["sell_f64"]["kernel_ms"] / res["sell_f32"]["kernel_ms"], 3)

@@ -19,6 +19,13 @@
 //                         <c0:c1,c0:c1,...>   LeSpGEMM_VLength on cluster ranges of the clustered A
 //   features <A.mtx>      the reference's feature extractor (MTX<int,double>: MtxLoad,
 //                         CalculateFeatures, FeaturesPrint) on a Matrix Market file
+//   bench_spmm <A.csr> <k> <threads> <min_seconds> <prec> <transpose 0|1>
+//                         k right-hand sides as k calls of the reference's OpenMP SpOps_csr
+//                         (kernel_flag 2, nnz-balanced rows) on A (or on csr_transpose(A),
+//                         built before the timing); the reference has no SpMM entry point
+//   bench_spmm_batch <list> <threads> <prec>
+//                         the same over a batch: <list> holds lines "<A.csr> <k>"; every
+//                         matrix is timed on one k-column product after a warm-up
 #include "SpOps.h"
 #include "spgemm.h"
 #include "spgemm_MinHashLSH.h"
@@ -295,6 +302,85 @@ int bench_spmv(const HostCsr &h, const std::string &fmt, double min_seconds, int
   return 0;
 }
 
+// One k-column product: k calls of SpOps_csr (kernel_flag 2) on columns of X (column-major,
+// n_cols x k) into Y (n_rows x k).  X holds a deterministic pattern (timing does not depend on
+// the values).
+template <typename V>
+struct SpmmCase {
+  CSR_Matrix<int, V> A;
+  int k = 0;
+  std::vector<V> X, Y;
+  void prepare(int threads) {
+    X.resize((size_t)A.num_cols * k);
+    Y.assign((size_t)A.num_rows * k, (V)0);
+    for (size_t i = 0; i < X.size(); ++i) X[i] = (V)((double)((i * 2654435761ULL) % 1000) / 1000.0 - 0.5);
+    A.kernel_flag = 2;
+    A.partition = new_array<int>(threads + 1);
+    balanced_partition_row_by_nnz(A.row_offset, A.num_rows, threads, A.partition);
+  }
+  void run() {
+    for (int j = 0; j < k; ++j)
+      SpOps_csr<int, V>((V)1, A, X.data() + (size_t)j * A.num_cols, (V)0, Y.data() + (size_t)j * A.num_rows);
+  }
+  void release() {
+    delete_array(A.partition);
+    A.partition = nullptr;
+    delete_host_matrix(A);
+  }
+};
+
+template <typename V>
+int bench_spmm(const HostCsr &h, int k, double min_seconds, bool transpose) {
+  SpmmCase<V> c;
+  CSR_Matrix<int, V> A = to_ref_csr<int, V>(h);
+  if (transpose) {
+    c.A = csr_transpose<int, V>(A);
+    delete_host_matrix(A);
+  } else {
+    c.A = A;
+  }
+  c.k = k;
+  const int threads = Le_get_thread_num();
+  c.prepare(threads);
+  c.run();                                              // warm-up
+  int iters = 0;
+  double t0 = omp_get_wtime();
+  do { c.run(); ++iters; } while (iters < 3 || omp_get_wtime() - t0 < min_seconds);
+  const double t = (omp_get_wtime() - t0) / iters;
+  std::printf("{\"threads\": %d, \"k\": %d, \"nnz\": %d, \"iterations\": %d, \"seconds_per_spmm\": %.9e}\n",
+              threads, k, c.A.num_nnzs, iters, t);
+  c.release();
+  return 0;
+}
+
+template <typename V>
+int bench_spmm_batch(const char *list) {
+  FILE *f = std::fopen(list, "r");
+  if (!f) { std::fprintf(stderr, "cannot open %s\n", list); return 2; }
+  char path[4096];
+  int k = 0;
+  const int threads = Le_get_thread_num();
+  double total = 0.0, flop = 0.0;
+  int count = 0;
+  while (std::fscanf(f, "%4095s %d", path, &k) == 2) {
+    SpmmCase<V> c;
+    c.A = to_ref_csr<int, V>(read_csr_file(path));
+    c.k = k;
+    c.prepare(threads);
+    c.run();                                            // warm-up
+    const double t0 = omp_get_wtime();
+    c.run();
+    total += omp_get_wtime() - t0;
+    flop += 2.0 * (double)c.A.num_nnzs * k;
+    ++count;
+    c.release();
+  }
+  std::fclose(f);
+  std::printf("{\"threads\": %d, \"matrices\": %d, \"flop\": %.17g, \"seconds\": %.9e}\n", threads, count, flop,
+              total);
+  return 0;
+}
+
 int bench_lespgemm(const HostCsr &ha, const HostCsr &hb, int iterations, int k, int bands, uint64_t seed,
                    int64_t cluster_size, int kernel) {
   typedef int64_t I; typedef double V;
@@ -488,6 +574,18 @@ int main(int argc, char **argv) {
     Le_set_thread_num(std::atoi(argv[4]));
     return bench_lespgemm(A, B, std::atoi(argv[5]), std::atoi(argv[6]), std::atoi(argv[7]),
                           std::strtoull(argv[8], nullptr, 10), std::atoll(argv[9]), std::atoi(argv[10]));
+  }
+  if (mode == "bench_spmm" && argc >= 8) {
+    HostCsr A = read_csr_file(argv[2]);
+    Le_set_thread_num(std::atoi(argv[4]));
+    const int k = std::atoi(argv[3]), prec = std::atoi(argv[6]);
+    const double secs = std::atof(argv[5]);
+    const bool tr = std::atoi(argv[7]) != 0;
+    return prec == 32 ? bench_spmm<float>(A, k, secs, tr) : bench_spmm<double>(A, k, secs, tr);
+  }
+  if (mode == "bench_spmm_batch" && argc >= 5) {
+    Le_set_thread_num(std::atoi(argv[3]));
+    return std::atoi(argv[4]) == 32 ? bench_spmm_batch<float>(argv[2]) : bench_spmm_batch<double>(argv[2]);
   }
   if (mode == "features" && argc >= 3) {
     MTX<int, double> m(0);
