@@ -9,15 +9,28 @@
  * y = alpha*A*x + beta*y with host x and y, computed by the gfx950 kernels of
  * libsparseops_hip.so.  kernel_flag (serial / OMP / balanced) is accepted and ignored: every
  * flag runs the GPU kernel.  beta == 0 does not read y.  The matrix is uploaded on first
- * use and cached (spops_dropin.h).  Link with -lsparseops_hip. */
+ * use and cached (spops_dropin.h).  Link with -lsparseops_hip.
+ *
+ * Like the reference's umbrella header (include/SpOps.h:4-41) it also brings in the
+ * configuration, I/O, conversion, benchmark, test-routine, feature and SpGEMM headers, so a
+ * caller that includes only this file finds every name the reference's umbrella provides for
+ * these formats. */
 #ifndef SPOPS_DROPIN_SPOPS_H
 #define SPOPS_DROPIN_SPOPS_H
 
 #include <vector>
 
+#include "general_config.h"
+#include "plat_config.h"
 #include "memopt.h"
-#include "sparse_conversion.h"
+#include "mmio.h"
+#include "thread.h"
+#include "sparse_io.h"
 #include "sparse_format.h"
+#include "sparse_operation.h"
+#include "sparse_partition.h"
+#include "sparse_conversion.h"
+#include "spmv_benchmark.h"
 #include "spops_dropin.h"
 
 namespace spops_dropin {
@@ -164,5 +177,6 @@ void SpOps_bsr(const V alpha, const BSR_Matrix<I, V> &b, const V *x, const V bet
 }
 
 #include "spgemm.h"
+#include "spmv_testroutine.h"
 
 #endif

@@ -27,24 +27,25 @@
 #include "spgemm_cluster.h"
 #include "spops_dropin.h"
 
-#ifndef SIMD_WIDTH
-#define SIMD_WIDTH 256   /* the reference's plat_config.h value: alignment = 256/8/sizeof(V) */
-#endif
-#ifndef SELL_SIGMA
-#define SELL_SIGMA 16384
-#endif
-#ifndef CHUNK_SIZE
-#define CHUNK_SIZE 4
-#endif
-#ifndef BSR_BlockDimRow
-#define BSR_BlockDimRow 16
-#endif
+#include "general_config.h"
+#include "plat_config.h"
 
 template <typename T>
 inline T *spops_checked(T *p) {
   if (!p) throw std::bad_alloc();
   return p;
 }
+
+/* The reference's allocation check (include/sparse_conversion.h:25-35): a null pointer
+ * prints where it happened and throws std::bad_alloc. */
+inline void checkAlloc(const void *ptr, const char *func, const char *file, int line) {
+  if (ptr != nullptr) return;
+  std::cerr << "Memory allocation failed in " << func << " (" << file << ":" << line << ")" << std::endl;
+  throw std::bad_alloc();
+}
+#ifndef CHECK_ALLOC
+#define CHECK_ALLOC(ptr) checkAlloc((ptr), __FUNCTION__, __FILE__, __LINE__)
+#endif
 
 /* Sum duplicate entries row by row and drop exact-zero sums; the columns of a row come out
  * in reverse order of their first appearance (the order the reference's linked list yields,

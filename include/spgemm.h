@@ -26,9 +26,7 @@
 #ifndef MIXED_L2_FRACTION
 #define MIXED_L2_FRACTION 0.75
 #endif
-#ifndef CPU_L2CACHE_SIZE
-#define CPU_L2CACHE_SIZE 67108864   /* the reference's plat_config.h value */
-#endif
+#include "plat_config.h"   /* CPU_L2CACHE_SIZE: the kernel-3 budget */
 
 template <typename IndexType, typename ValueType>
 long long int get_spgemm_flop(const CSR_Matrix<IndexType, ValueType> &A, const CSR_Matrix<IndexType, ValueType> &B) {
