@@ -471,4 +471,30 @@ CSR_Matrix<I, V> csr_transpose(const CSR_Matrix<I, V> &csr) {
   return t;
 }
 
+/* csr_to_binary_pattern (include/sparse_conversion.h:2121-2158): the same structure with every
+ * value 1; an empty matrix gets a zero row pointer and null column / value arrays. */
+template <class I, class V>
+CSR_Matrix<I, V> csr_to_binary_pattern(const CSR_Matrix<I, V> &csr) {
+  CSR_Matrix<I, V> b;
+  b.num_rows = csr.num_rows;
+  b.num_cols = csr.num_cols;
+  b.num_nnzs = csr.num_nnzs;
+  b.tag = 0;
+  b.kernel_flag = csr.kernel_flag;
+  b.partition = nullptr;
+  b.row_offset = spops_checked(new_array<I>((size_t)b.num_rows + 1));
+  if (csr.num_nnzs == 0) {
+    std::fill_n(b.row_offset, (size_t)b.num_rows + 1, (I)0);
+    b.col_index = nullptr;
+    b.values = nullptr;
+    return b;
+  }
+  b.col_index = spops_checked(new_array<I>((size_t)b.num_nnzs));
+  b.values = spops_checked(new_array<V>((size_t)b.num_nnzs));
+  std::copy(csr.row_offset, csr.row_offset + b.num_rows + 1, b.row_offset);
+  std::copy(csr.col_index, csr.col_index + b.num_nnzs, b.col_index);
+  std::fill_n(b.values, (size_t)b.num_nnzs, (V)1);
+  return b;
+}
+
 #endif

@@ -1,6 +1,11 @@
-/* Cluster-wise SpGEMM of the LeSpGEMM path (drop-in for the reference's include/spgemm.h):
+/* SpGEMM of the LeSpGEMM path (drop-in for the reference's include/spgemm.h):
  *   get_spgemm_flop   src/spgemm.cpp:24-42
  *   LeSpGEMM_VLength  src/spgemm.cpp:662-680 (template <bool sortOutput, I, V>)
+ *   LeSpGEMM          src/spgemm.cpp:684-705, row-wise, and its kernels LeSpGEMM_hash_rowwise
+ *                     :45-104, LeSpGEMM_array_rowwise_new :110-169, LeSpGEMM_spa_rowwise :175-236
+ *                     (spops_spgemm_csr: one C for every flag, columns ascending, zero sums
+ *                     kept; the reference's hash kernel leaves hash order when sortOutput = 0)
+ *   HashSpGEMMTopK    src/spgemm.cpp:707-842 (spops_spgemm_topk_jaccard)
  * C = A_cluster * B on the GPU (spops_lespgemm, the hybrid LDS-hash / HBM-bitmap
  * accumulator).  kernel_flag 1, 2 and 3 all produce the same C here (they differ only in
  * the reference's CPU accumulator); for kernel 3 the caller-allocated acc_flag, min_ccol and
@@ -136,6 +141,92 @@ void LeSpGEMM_VLength(const CSR_VlengthCluster<IndexType, ValueType> &A, const C
     }
   }
   (void)sortOutput;
+}
+
+namespace spops_dropin {
+// Host CSR from three device arrays the library allocated (then freed)
+template <typename I, typename V>
+void take_device_csr(CSR_Matrix<I, V> &C, int64_t rows, int64_t cols, int64_t nnz, int64_t *drp, int64_t *dcol,
+                     void *dval) {
+  std::vector<int64_t> rp((size_t)rows + 1), col((size_t)nnz);
+  std::vector<V> val((size_t)nnz);
+  int st = spops_memcpy_dtoh(rp.data(), drp, rp.size() * 8, nullptr);
+  if (st == SPOPS_SUCCESS && nnz) st = spops_memcpy_dtoh(col.data(), dcol, col.size() * 8, nullptr);
+  if (st == SPOPS_SUCCESS && nnz) st = spops_memcpy_dtoh(val.data(), dval, val.size() * sizeof(V), nullptr);
+  spops_free(drp);
+  spops_free(dcol);
+  spops_free(dval);
+  check(st, "download C");
+  C.num_rows = (I)rows;
+  C.num_cols = (I)cols;
+  C.num_nnzs = (I)nnz;
+  C.tag = 0;
+  C.partition = nullptr;
+  C.row_offset = new_array<I>((size_t)rows + 1);
+  C.col_index = new_array<I>(nnz ? (size_t)nnz : 1);
+  C.values = new_array<V>(nnz ? (size_t)nnz : 1);
+  std::copy(rp.begin(), rp.end(), C.row_offset);
+  std::copy(col.begin(), col.end(), C.col_index);
+  std::copy(val.begin(), val.end(), C.values);
+}
+
+template <typename I, typename V>
+void rowwise_product(const CSR_Matrix<I, V> &A, const CSR_Matrix<I, V> &B, CSR_Matrix<I, V> &C) {
+  if (A.num_cols != B.num_rows) throw std::invalid_argument("LeSpGEMM: A.num_cols != B.num_rows");
+  std::lock_guard<std::mutex> g(state().mu);
+  const DeviceCsr<V> dA = cached_csr64(A.row_offset, A.col_index, A.values, A.num_rows, A.num_cols, A.num_nnzs);
+  const DeviceCsr<V> dB = cached_csr64(B.row_offset, B.col_index, B.values, B.num_rows, B.num_cols, B.num_nnzs);
+  int64_t nnz = 0, *drp = nullptr, *dcol = nullptr;
+  void *dval = nullptr;
+  check(spops_spgemm_csr(A.num_rows, A.num_cols, A.num_nnzs, dA.rowptr, dA.col, dA.values, B.num_cols, B.num_nnzs,
+                         dB.rowptr, dB.col, dB.values, value_code<V>(), &nnz, &drp, &dcol, &dval, nullptr, nullptr),
+        "spops_spgemm_csr");
+  take_device_csr(C, A.num_rows, B.num_cols, nnz, drp, dcol, dval);
+}
+}  // namespace spops_dropin
+
+template <bool sortOutput = true, typename IndexType, typename ValueType>
+void LeSpGEMM_hash_rowwise(const CSR_Matrix<IndexType, ValueType> &A, const CSR_Matrix<IndexType, ValueType> &B,
+                           CSR_Matrix<IndexType, ValueType> &C, int kernel_flag) {
+  spops_dropin::rowwise_product(A, B, C);
+  C.kernel_flag = kernel_flag;
+}
+
+template <bool sortOutput = true, typename IndexType, typename ValueType>
+void LeSpGEMM_array_rowwise_new(const CSR_Matrix<IndexType, ValueType> &A, const CSR_Matrix<IndexType, ValueType> &B,
+                                CSR_Matrix<IndexType, ValueType> &C) {
+  spops_dropin::rowwise_product(A, B, C);
+  C.kernel_flag = 2;
+}
+
+template <bool sortOutput = true, typename IndexType, typename ValueType>
+void LeSpGEMM_spa_rowwise(const CSR_Matrix<IndexType, ValueType> &A, const CSR_Matrix<IndexType, ValueType> &B,
+                          CSR_Matrix<IndexType, ValueType> &C) {
+  spops_dropin::rowwise_product(A, B, C);
+  C.kernel_flag = 3;
+}
+
+template <bool sortOutput = true, typename IndexType, typename ValueType>
+void LeSpGEMM(const CSR_Matrix<IndexType, ValueType> &A, const CSR_Matrix<IndexType, ValueType> &B,
+              CSR_Matrix<IndexType, ValueType> &C, int kernel_flag = 1) {
+  if (kernel_flag == 2) LeSpGEMM_array_rowwise_new<sortOutput>(A, B, C);
+  else if (kernel_flag == 3) LeSpGEMM_spa_rowwise<sortOutput>(A, B, C);
+  else LeSpGEMM_hash_rowwise<sortOutput>(A, B, C, kernel_flag);
+}
+
+template <typename IndexType, typename ValueType>
+void HashSpGEMMTopK(const CSR_Matrix<IndexType, ValueType> &A, CSR_Matrix<IndexType, ValueType> &C,
+                    IndexType top_k = 7) {
+  using namespace spops_dropin;
+  std::lock_guard<std::mutex> g(state().mu);
+  const DeviceCsr<ValueType> dA = cached_csr64(A.row_offset, A.col_index, A.values, A.num_rows, A.num_cols, A.num_nnzs);
+  int64_t nnz = 0, *drp = nullptr, *dcol = nullptr;
+  void *dval = nullptr;
+  check(spops_spgemm_topk_jaccard(A.num_rows, A.num_cols, A.num_nnzs, dA.rowptr, dA.col, value_code<ValueType>(),
+                                  (int64_t)top_k, 0, &nnz, &drp, &dcol, &dval, nullptr),
+        "spops_spgemm_topk_jaccard");
+  take_device_csr(C, A.num_rows, A.num_rows, nnz, drp, dcol, dval);
+  C.kernel_flag = 0;
 }
 
 #endif

@@ -1,13 +1,18 @@
 /* Row clustering of the LeSpGEMM path (drop-in for the reference's
- * include/spgemm_utility.h): hierarchical_clustering_v0_fast :743-915 and
- * hierarchical_clustering_v1 :609-733 with the same signatures, bit-identical permutation
- * and offsets (host code in libsparseops_hip.so, spops_hierarchical_clustering), plus the
- * Jaccard helpers :362-429 and CumulativeSum :334-343. */
+ * include/spgemm_utility.h): hierarchical_clustering_v0_fast :743-915,
+ * hierarchical_clustering_v1 :609-733 and the map-based hierarchical_clustering_v0 :517-606
+ * with reordered_dict_to_permutation_and_offsets :921-939, with the same signatures and
+ * bit-identical permutation and offsets (host code in libsparseops_hip.so,
+ * spops_hierarchical_clustering / spops_hierarchical_clustering_v0), plus the Jaccard helpers
+ * :362-429 and CumulativeSum :334-343.  hierarchical_clustering_v0 does not add the pairs it
+ * scores on the way to the caller's map (the reference does; its callers do not read them). */
 #ifndef SPOPS_DROPIN_SPGEMM_UTILITY_H
 #define SPOPS_DROPIN_SPGEMM_UTILITY_H
 
+#include <map>
 #include <set>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "spops_dropin.h"
@@ -71,6 +76,49 @@ inline void hierarchical_clustering_v1(const IndexType *rowptr, const IndexType 
                                        std::vector<IndexType> &permutation_out, std::vector<IndexType> &offsets_out) {
   spops_dropin::hierarchical_clustering<IndexType, ValueType>(1, rowptr, colids, num_rows, pairs, cluster_size,
                                                               permutation_out, offsets_out);
+}
+
+template <typename IndexType, typename ValueType>
+inline std::map<IndexType, std::vector<IndexType>> hierarchical_clustering_v0(
+    const IndexType *rowptr, const IndexType *colids, IndexType num_rows,
+    std::map<std::pair<IndexType, IndexType>, ValueType> &close_pairs, IndexType cluster_size) {
+  using namespace spops_dropin;
+  const int64_t n = num_rows, nnz = num_rows > 0 ? (int64_t)rowptr[num_rows] : 0;
+  std::vector<int64_t> rp(rowptr, rowptr + n + 1), ci(colids, colids + nnz);
+  std::vector<int64_t> pi, pj;
+  std::vector<ValueType> sc;
+  pi.reserve(close_pairs.size());
+  pj.reserve(close_pairs.size());
+  sc.reserve(close_pairs.size());
+  for (const auto &kv : close_pairs) {
+    pi.push_back((int64_t)kv.first.first);
+    pj.push_back((int64_t)kv.first.second);
+    sc.push_back(kv.second);
+  }
+  std::vector<int64_t> perm((size_t)n), offsets((size_t)n + 1), roots((size_t)(n > 0 ? n : 1));
+  int64_t nc = 0;
+  check(spops_hierarchical_clustering_v0(n, rp.data(), ci.data(), (int64_t)pi.size(), pi.data(), pj.data(), sc.data(),
+                                         value_code<ValueType>(), (int64_t)cluster_size, perm.data(), offsets.data(),
+                                         roots.data(), &nc),
+        "spops_hierarchical_clustering_v0");
+  std::map<IndexType, std::vector<IndexType>> dict;
+  for (int64_t c = 0; c < nc; ++c)
+    dict[(IndexType)roots[(size_t)c]].assign(perm.begin() + offsets[(size_t)c], perm.begin() + offsets[(size_t)c + 1]);
+  return dict;
+}
+
+// permutation[new_row] = original row, clusters in the map's (root) order; offsets [0, len0, ...]
+template <typename IndexType>
+inline void reordered_dict_to_permutation_and_offsets(const std::map<IndexType, std::vector<IndexType>> &reordered_dict,
+                                                      IndexType num_rows, std::vector<IndexType> &permutation_out,
+                                                      std::vector<IndexType> &offsets_out) {
+  (void)num_rows;
+  permutation_out.clear();
+  offsets_out.assign(1, 0);
+  for (const auto &kv : reordered_dict) {
+    permutation_out.insert(permutation_out.end(), kv.second.begin(), kv.second.end());
+    offsets_out.push_back((IndexType)permutation_out.size());
+  }
 }
 
 #endif

@@ -326,6 +326,18 @@ int spops_hierarchical_clustering(int version, int64_t n_rows, const int64_t *ro
                                   int score_type, int64_t cluster_size, int64_t *perm, int64_t *offsets,
                                   int64_t *nclusters);
 
+/* hierarchical_clustering_v0 (include/spgemm_utility.h:517-606), the map-based original that
+ * test/test_spgemm_hc.cpp runs on candidate pairs from a file (e.g. the A * A^T top-k pairs of
+ * spops_spgemm_topk_jaccard).  The pairs (pi, pj, score) in the order of the caller's map,
+ * keys as it holds them (non-negative; i > j allowed).  perm and offsets as above; roots[c]
+ * receives the root row of cluster c (the reference returns map<root, rows>, clusters ordered
+ * by root, rows ascending).  The caller's map is not extended with the pairs scored on the way
+ * (the reference inserts them). */
+int spops_hierarchical_clustering_v0(int64_t n_rows, const int64_t *rowptr, const int64_t *col, int64_t npairs,
+                                     const int64_t *pi, const int64_t *pj, const void *score, int score_type,
+                                     int64_t cluster_size, int64_t *perm, int64_t *offsets, int64_t *roots,
+                                     int64_t *nclusters);
+
 /* ---- LeSpGEMM: cluster format and cluster-wise SpGEMM (device) ----------------------------
  * A spops_cluster is a device CSR_VlengthCluster (include/spgemm_cluster.h:108-143):
  * cluster c = rows offsets[c]..offsets[c+1] of the permuted matrix, its distinct columns
@@ -392,6 +404,30 @@ int spops_cluster_checksum(spops_cluster C, double out[4], void *stream);
  * columns ascending.  The three device arrays are allocated here; free with spops_free. */
 int spops_cluster_to_csr(spops_cluster C, spops_cluster A, int64_t *nnz, int64_t **d_rowptr, int64_t **d_col,
                          void **d_val, void *stream);
+
+/* Row-wise C = A * B in CSR: the reference's LeSpGEMM (src/spgemm.cpp:684-705), whose kernel
+ * flags 1 (hash, :45-104), 2 (array, :110-169) and 3 (SPA, :175-236) all give this C.  A and B
+ * on the device (int64 rowptr/col, values of value_type F64 or F32; B's rows need not be
+ * sorted).  C's columns ascend in every row (what sortOutput = true asks; with false the
+ * reference leaves its hash order) and every column of the symbolic product is kept, zero sums
+ * included.  The three output arrays (rowptr a_rows + 1, col and values c_nnz) are allocated
+ * here; free with spops_free.  stats as spops_lespgemm (may be NULL).  Synchronous. */
+int spops_spgemm_csr(int64_t a_rows, int64_t a_cols, int64_t a_nnz, const int64_t *d_arowptr, const int64_t *d_acol,
+                     const void *d_aval, int64_t b_cols, int64_t b_nnz, const int64_t *d_browptr, const int64_t *d_bcol,
+                     const void *d_bval, int value_type, int64_t *c_nnz, int64_t **d_crowptr, int64_t **d_ccol,
+                     void **d_cval, int64_t stats[16], void *stream);
+
+/* HashSpGEMMTopK (src/spgemm.cpp:707-842): the candidate pairs of the A * A^T front end of the
+ * hierarchical clustering.  Row i of C holds the rows j != i of A that share a column with row
+ * i, scored by s = |A_i & A_j| / (|A_i| + |A_j| - |A_i & A_j|) computed in the value type
+ * (row lengths |A_i| count stored entries, duplicates included), at most top_k of them,
+ * ordered by s descending then j ascending (src/spgemm_hash.cpp:99-108, 169-230).  A's pattern
+ * on the device (int64 rowptr/col; its values are not read).  The intersection counts are
+ * computed in row panels whose output stays under budget_bytes (<= 0: a quarter of the free
+ * device memory).  Output arrays allocated here (free with spops_free).  Synchronous. */
+int spops_spgemm_topk_jaccard(int64_t n_rows, int64_t n_cols, int64_t nnz, const int64_t *d_rowptr,
+                              const int64_t *d_col, int value_type, int64_t top_k, int64_t budget_bytes,
+                              int64_t *c_nnz, int64_t **d_crowptr, int64_t **d_ccol, void **d_cval, void *stream);
 
 /* get_spgemm_flop (src/spgemm.cpp:24-42): 2 * sum over A entries (i,k), k < b_rows, of nnz(B_k). */
 int spops_spgemm_flop(int64_t a_nnz, const int64_t *d_acol, int64_t b_rows, const int64_t *d_browptr, int64_t *flops,

@@ -537,6 +537,9 @@ static int64_t hc_output(int64_t n, int64_t *cl, int64_t *perm, int64_t **offset
 
 /* version 0: hierarchical_clustering_v0_fast, include/spgemm_utility.h:743-915
  * version 1: hierarchical_clustering_v1,      include/spgemm_utility.h:609-733
+ * version 2: hierarchical_clustering_v0,      include/spgemm_utility.h:517-606 (the map-based
+ *            original: pairs in the map's order, keys as the map holds them; a discovered root
+ *            pair is scored unless its (min, max) key is a key already; set-semantics Jaccard)
  * score_f32 != 0 reproduces ValueType=float (scores and Jaccard stored as float).
  * Returns the number of clusters; *offsets is malloc'ed (nclusters+1 entries). */
 int64_t orc_hc(int version, int64_t n, const int64_t *rowptr, const int64_t *col,
@@ -549,10 +552,38 @@ int64_t orc_hc(int version, int64_t n, const int64_t *rowptr, const int64_t *col
   const int su = rows_sorted_unique(n, rowptr, col);
   heap_t dyn = {0, 0, 0};
   pset_t seen;
-  pset_init(&seen, version == 1 ? np : np / 8 + 1024);
+  pset_init(&seen, version >= 1 ? np : np / 8 + 1024);
   int nclusters = (int)n;
 
-  if (version == 1) {
+  if (version == 2) {
+    for (int64_t t = 0; t < np; ++t) {
+      pset_insert(&seen, pi[t], pj[t]);
+      item_t it = {as_value(score[t], score_f32), pi[t] < pj[t] ? pi[t] : pj[t], pi[t] < pj[t] ? pj[t] : pi[t]};
+      heap_push(&dyn, it);
+    }
+    int64_t left = n;
+    while (dyn.n > 0 && left != 0) {
+      item_t top = heap_pop(&dyn);
+      int64_t i = top.i, j = top.j;
+      if (i >= n || j >= n) continue;
+      if (cl[i] == i && cl[j] == j) {
+        if (!valid[i] || !valid[j]) continue;
+        left--;
+        if (sz[i] < sz[j]) { cl[i] = j; sz[j] += sz[i]; if (sz[j] >= cs) { valid[j] = 0; left--; } }
+        else { cl[j] = i; sz[i] += sz[j]; if (sz[i] >= cs) { valid[i] = 0; left--; } }
+      } else {
+        int64_t ri = uf_find(cl, i), rj = uf_find(cl, j);
+        if (!valid[ri] || !valid[rj]) continue;
+        if (ri != rj) {
+          int64_t a = ri < rj ? ri : rj, b = ri < rj ? rj : ri;
+          if (pset_insert(&seen, a, b)) {
+            item_t it = {as_value(orc_jaccard(rowptr, col, a, b, 0), score_f32), a, b};
+            heap_push(&dyn, it);
+          }
+        }
+      }
+    }
+  } else if (version == 1) {
     for (int64_t t = 0; t < np; ++t) {
       int64_t ii = pi[t], jj = pj[t];
       if (ii > jj) { int64_t s = ii; ii = jj; jj = s; }
@@ -1080,4 +1111,117 @@ void orc_features(int64_t n_rows, int64_t n_cols, const int64_t *rowptr, const i
   out[60] = (double)sv / (double)n_cols;
   free(rowlen); free(collen); free(offcnt); free(tiles); free(rbc); free(cbc); free(urb); free(ucb);
   free(rflag); free(cflag); free(dom); free(rmax); free(rmin); free(cmax); free(cmin);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Row-wise SpGEMM and the A * A^T top-k Jaccard front end                                */
+/* ------------------------------------------------------------------------------------ */
+
+/* The row-wise LeSpGEMM (src/spgemm.cpp:684-705) as its SPA kernel computes it
+ * (src/spgemm.cpp:175-236, spgemm_spa_numeric): per row, every product a_ik * b_kj added into
+ * C(i, j) in the order of A's row entries, then B's; C's columns are the symbolic union, ascending,
+ * zero sums kept.  *c_col / *c_val malloc'ed; c_rowptr has n_rows + 1 entries.
+ * Pinned: scipy's product (structure exact, values 1e-12) and the reference's own output for
+ * kernel flags 1, 2 and 3 (tests/golden/rowwise_*). */
+int64_t orc_spgemm_rows(int64_t n_rows, const int64_t *arp, const int64_t *acol, const double *aval, int64_t b_cols,
+                        const int64_t *brp, const int64_t *bcol, const double *bval, int64_t *c_rowptr,
+                        int64_t **c_col, double **c_val) {
+  int64_t *mark = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(b_cols ? b_cols : 1));
+  double *acc = (double *)xmalloc(sizeof(double) * (size_t)(b_cols ? b_cols : 1));
+  for (int64_t j = 0; j < b_cols; ++j) mark[j] = -1;
+  c_rowptr[0] = 0;
+  for (int64_t i = 0; i < n_rows; ++i) {           /* symbolic: distinct columns per row */
+    int64_t nz = 0;
+    for (int64_t e = arp[i]; e < arp[i + 1]; ++e)
+      for (int64_t f = brp[acol[e]]; f < brp[acol[e] + 1]; ++f)
+        if (mark[bcol[f]] != i) { mark[bcol[f]] = i; ++nz; }
+    c_rowptr[i + 1] = c_rowptr[i] + nz;
+  }
+  const int64_t nnz = c_rowptr[n_rows];
+  *c_col = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(nnz ? nnz : 1));
+  *c_val = (double *)xmalloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
+  for (int64_t j = 0; j < b_cols; ++j) mark[j] = -1;
+  for (int64_t i = 0; i < n_rows; ++i) {
+    int64_t *cc = *c_col + c_rowptr[i];
+    int64_t nz = 0;
+    for (int64_t e = arp[i]; e < arp[i + 1]; ++e)
+      for (int64_t f = brp[acol[e]]; f < brp[acol[e] + 1]; ++f) {
+        const int64_t j = bcol[f];
+        if (mark[j] != i) { mark[j] = i; acc[j] = 0.0; cc[nz++] = j; }
+        acc[j] += aval[e] * bval[f];
+      }
+    qsort(cc, (size_t)nz, sizeof(int64_t), cmp_i64);
+    for (int64_t t = 0; t < nz; ++t) (*c_val)[c_rowptr[i] + t] = acc[cc[t]];
+  }
+  free(mark);
+  free(acc);
+  return nnz;
+}
+
+typedef struct { double s; int64_t c; } cand_t;
+static int cmp_cand(const void *x, const void *y) {   /* score descending, column ascending */
+  const cand_t *a = (const cand_t *)x, *b = (const cand_t *)y;
+  if (a->s != b->s) return a->s > b->s ? -1 : 1;
+  return (a->c > b->c) - (a->c < b->c);
+}
+
+/* HashSpGEMMTopK (src/spgemm.cpp:707-842; selection src/spgemm_hash.cpp:99-108, 169-230): for
+ * row i the rows j != i sharing a column, intersection |A_i & A_j| counted with multiplicity
+ * (the product of the 0/1 patterns), union |A_i| + |A_j| - inter (stored entries), score
+ * 1 * inter / union in the value type (float with score_f32), the top_k by score descending then
+ * j ascending.  *c_col / *c_val malloc'ed; c_rowptr has n_rows + 1 entries.
+ * Pinned: the reference's own HashSpGEMMTopK output (tests/golden/topk_*) and a brute force over
+ * Python sets on small matrices. */
+int64_t orc_topk_jaccard(int64_t n_rows, int64_t n_cols, const int64_t *rowptr, const int64_t *col, int64_t top_k,
+                         int score_f32, int64_t *c_rowptr, int64_t **c_col, double **c_val) {
+  const int64_t nnz = rowptr[n_rows];
+  int64_t *tp = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(n_cols + 1));
+  int64_t *tr = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(nnz ? nnz : 1));
+  for (int64_t c = 0; c <= n_cols; ++c) tp[c] = 0;
+  for (int64_t e = 0; e < nnz; ++e) tp[col[e] + 1]++;
+  for (int64_t c = 0; c < n_cols; ++c) tp[c + 1] += tp[c];
+  int64_t *cur = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(n_cols ? n_cols : 1));
+  for (int64_t c = 0; c < n_cols; ++c) cur[c] = tp[c];
+  for (int64_t i = 0; i < n_rows; ++i)
+    for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) tr[cur[col[e]]++] = i;
+  double *cnt = (double *)xmalloc(sizeof(double) * (size_t)(n_rows ? n_rows : 1));
+  int64_t *seen = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(n_rows ? n_rows : 1));
+  int64_t *list = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(n_rows ? n_rows : 1));
+  cand_t *cand = (cand_t *)xmalloc(sizeof(cand_t) * (size_t)(n_rows ? n_rows : 1));
+  for (int64_t j = 0; j < n_rows; ++j) seen[j] = -1;
+  int64_t cap = n_rows * (top_k < n_rows ? top_k : n_rows) + 1;
+  *c_col = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)cap);
+  *c_val = (double *)xmalloc(sizeof(double) * (size_t)cap);
+  c_rowptr[0] = 0;
+  for (int64_t i = 0; i < n_rows; ++i) {
+    int64_t m = 0;
+    for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e)
+      for (int64_t f = tp[col[e]]; f < tp[col[e] + 1]; ++f) {
+        const int64_t j = tr[f];
+        if (seen[j] != i) { seen[j] = i; cnt[j] = 0.0; list[m++] = j; }
+        cnt[j] += 1.0;
+      }
+    int64_t q = 0;
+    const int64_t li = rowptr[i + 1] - rowptr[i];
+    for (int64_t t = 0; t < m; ++t) {
+      const int64_t j = list[t];
+      if (j == i) continue;
+      const int64_t u = li + (rowptr[j + 1] - rowptr[j]) - (int64_t)cnt[j];
+      double sc;
+      if (score_f32) sc = u == 0 ? 0.0 : (double)((float)1.0f * (float)cnt[j] / (float)u);
+      else sc = u == 0 ? 0.0 : 1.0 * cnt[j] / (double)u;
+      cand[q].s = sc;
+      cand[q].c = j;
+      ++q;
+    }
+    qsort(cand, (size_t)q, sizeof(cand_t), cmp_cand);
+    const int64_t keep = q < top_k ? q : top_k;
+    for (int64_t t = 0; t < keep; ++t) {
+      (*c_col)[c_rowptr[i] + t] = cand[t].c;
+      (*c_val)[c_rowptr[i] + t] = cand[t].s;
+    }
+    c_rowptr[i + 1] = c_rowptr[i] + keep;
+  }
+  free(tp); free(tr); free(cur); free(cnt); free(seen); free(list); free(cand);
+  return c_rowptr[n_rows];
 }

@@ -61,6 +61,13 @@ int64_t orc_hc(int version, int64_t n_rows, const int64_t *rowptr, const int64_t
                int score_f32, int64_t cluster_size, int64_t *perm, int64_t **offsets);
 double orc_jaccard(const int64_t *rowptr, const int64_t *col, int64_t a, int64_t b, int sorted_unique);
 
+/* ---- row-wise SpGEMM and the A * A^T top-k Jaccard front end ---- */
+int64_t orc_spgemm_rows(int64_t n_rows, const int64_t *arp, const int64_t *acol, const double *aval, int64_t b_cols,
+                        const int64_t *brp, const int64_t *bcol, const double *bval, int64_t *c_rowptr,
+                        int64_t **c_col, double **c_val);
+int64_t orc_topk_jaccard(int64_t n_rows, int64_t n_cols, const int64_t *rowptr, const int64_t *col, int64_t top_k,
+                         int score_f32, int64_t *c_rowptr, int64_t **c_col, double **c_val);
+
 /* ---- cluster format + cluster-wise SpGEMM ---- */
 int64_t orc_cluster_build(int64_t n_rows, const int64_t *rowptr, const int64_t *col, const double *val,
                           const int64_t *perm, int64_t nclusters, const int64_t *offsets,

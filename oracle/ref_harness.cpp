@@ -26,6 +26,15 @@
 //   bench_spmm_batch <list> <threads> <prec>
 //                         the same over a batch: <list> holds lines "<A.csr> <k>"; every
 //                         matrix is timed on one k-column product after a warm-up
+//   rowwise <A.csr> <B.csr> <outdir> <kernel 1|2|3> <prec>
+//                         the row-wise LeSpGEMM<sortOutput = true> (src/spgemm.cpp:684-705):
+//                         C's rowptr, col and values
+//   topk_hc <A.csr> <outdir> <top_k> <cluster_size> <prec>
+//                         HashSpGEMMTopK (C's rowptr, col, values), then the flow of
+//                         test/test_spgemm_hc.cpp on those pairs: C written as "%20.16g" text
+//                         and read back into map<pair, V> (test/test_spgemm_hc.cpp:37-85),
+//                         hierarchical_clustering_v0 and reordered_dict_to_permutation_and_offsets
+//                         (perm, offsets and the map's root keys)
 #include "SpOps.h"
 #include "spgemm.h"
 #include "spgemm_MinHashLSH.h"
@@ -39,6 +48,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <sstream>
 #include <string>
 #include <vector>
 #include <omp.h>
@@ -531,6 +542,63 @@ std::vector<double> read_f64(const char *path, size_t n) {
   return v;
 }
 
+template <typename V>
+void dump_csr(const std::string &out, const char *tag, const CSR_Matrix<int64_t, V> &C) {
+  std::vector<double> v(C.values, C.values + C.num_nnzs);
+  dump(out, (std::string(tag) + "_rowptr").c_str(), C.row_offset, (size_t)C.num_rows + 1);
+  dump(out, (std::string(tag) + "_col").c_str(), C.col_index, (size_t)C.num_nnzs);
+  dump(out, (std::string(tag) + "_val").c_str(), v.data(), v.size());
+}
+
+template <typename V>
+int run_rowwise(const HostCsr &ha, const HostCsr &hb, const std::string &out, int kernel) {
+  CSR_Matrix<int64_t, V> A = to_ref_csr<int64_t, V>(ha), B = to_ref_csr<int64_t, V>(hb), C;
+  LeSpGEMM<true, int64_t, V>(A, B, C, kernel);
+  dump_csr(out, "C", C);
+  delete_host_matrix(A);
+  delete_host_matrix(B);
+  delete_host_matrix(C);
+  return 0;
+}
+
+template <typename V>
+int run_topk_hc(const HostCsr &ha, const std::string &out, int64_t top_k, int64_t cluster_size) {
+  CSR_Matrix<int64_t, V> A = to_ref_csr<int64_t, V>(ha), C;
+  HashSpGEMMTopK<int64_t, V>(A, C, top_k);
+  dump_csr(out, "topk", C);
+  // the candidate file round trip of test/generate_candidate_pairs.cpp + test/test_spgemm_hc.cpp
+  std::map<std::pair<int64_t, int64_t>, V> close_pairs;
+  for (int64_t i = 0; i < C.num_rows; ++i)
+    for (int64_t e = C.row_offset[i]; e < C.row_offset[i + 1]; ++e) {
+      char buf[64];
+      std::snprintf(buf, sizeof buf, "%20.16g", (double)C.values[e]);
+      std::istringstream iss(buf);
+      V sim;
+      iss >> sim;
+      close_pairs[std::make_pair(i, (int64_t)C.col_index[e])] = sim;
+    }
+  std::vector<double> file_scores;
+  std::vector<int64_t> keys;
+  for (const auto &kv : close_pairs) {
+    keys.push_back(kv.first.first);
+    keys.push_back(kv.first.second);
+    file_scores.push_back((double)kv.second);
+  }
+  dump(out, "pair_keys", keys.data(), keys.size());
+  dump(out, "pair_scores", file_scores.data(), file_scores.size());
+  std::map<int64_t, std::vector<int64_t>> dict =
+      hierarchical_clustering_v0<int64_t, V>(A.row_offset, A.col_index, A.num_rows, close_pairs, cluster_size);
+  std::vector<int64_t> perm, offsets, roots;
+  reordered_dict_to_permutation_and_offsets<int64_t>(dict, A.num_rows, perm, offsets);
+  for (const auto &kv : dict) roots.push_back(kv.first);
+  dump(out, "perm", perm.data(), perm.size());
+  dump(out, "offsets", offsets.data(), offsets.size());
+  dump(out, "roots", roots.data(), roots.size());
+  delete_host_matrix(A);
+  delete_host_matrix(C);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -586,6 +654,17 @@ int main(int argc, char **argv) {
   if (mode == "bench_spmm_batch" && argc >= 5) {
     Le_set_thread_num(std::atoi(argv[3]));
     return std::atoi(argv[4]) == 32 ? bench_spmm_batch<float>(argv[2]) : bench_spmm_batch<double>(argv[2]);
+  }
+  if (mode == "rowwise" && argc >= 7) {
+    HostCsr A = read_csr_file(argv[2]);
+    HostCsr B = read_csr_file(argv[3]);
+    return std::atoi(argv[6]) == 32 ? run_rowwise<float>(A, B, argv[4], std::atoi(argv[5]))
+                                    : run_rowwise<double>(A, B, argv[4], std::atoi(argv[5]));
+  }
+  if (mode == "topk_hc" && argc >= 7) {
+    HostCsr A = read_csr_file(argv[2]);
+    return std::atoi(argv[6]) == 32 ? run_topk_hc<float>(A, argv[3], std::atoll(argv[4]), std::atoll(argv[5]))
+                                    : run_topk_hc<double>(A, argv[3], std::atoll(argv[4]), std::atoll(argv[5]));
   }
   if (mode == "features" && argc >= 3) {
     MTX<int, double> m(0);

@@ -3,6 +3,8 @@
 // Bit-exact restatement of the reference's merge process:
 //   hierarchical_clustering_v0_fast  include/spgemm_utility.h:743-915  (version 0)
 //   hierarchical_clustering_v1       include/spgemm_utility.h:609-733  (version 1)
+//   hierarchical_clustering_v0       include/spgemm_utility.h:517-606  (the map-based original;
+//                                    spops_hierarchical_clustering_v0, with the cluster roots)
 // It is inherently sequential (a priority-ordered union-find with on-the-fly exact Jaccard
 // for discovered root pairs) and stays on the host; the GPU pipeline around it produces
 // the pairs and consumes perm/offsets.
@@ -349,10 +351,80 @@ int64_t hc_v1(int64_t n, const Csr64 &A, int64_t np, const int64_t *pi, const in
   return emit(cl, n, perm, offsets);
 }
 
+// include/spgemm_utility.h:517-606.  The pairs arrive in the order of the caller's map
+// (keys as the map holds them, i > j allowed); each enters the queue as (score, (min, max)),
+// and a discovered root pair is scored only when its (min, max) key is not a key of the map
+// yet.  A pair whose rows are both roots merges them without the root search (a pair (i, i)
+// of a root therefore doubles its size, as in the reference); the loop runs while the queue
+// is not empty and the cluster count is not zero.
+template <typename V>
+int64_t hc_v0_map(int64_t n, const Csr64 &A, int64_t np, const int64_t *pi, const int64_t *pj, const V *score, int64_t cs,
+                  int64_t *perm, int64_t *offsets, int64_t *roots) {
+  std::vector<int64_t> cl(n), sz(n, 1);
+  std::vector<char> valid(n, 1);
+  for (int64_t i = 0; i < n; ++i) cl[i] = i;
+  std::priority_queue<Item<V>, std::vector<Item<V>>, ItemLess<V>> sims;
+  PairSet keys((size_t)np + 1024);
+  for (int64_t t = 0; t < np; ++t) {
+    keys.insert(pi[t], pj[t]);
+    sims.push(Item<V>{score[t], std::min(pi[t], pj[t]), std::max(pi[t], pj[t])});
+  }
+  int64_t nclusters = n;
+  while (!sims.empty() && nclusters != 0) {
+    const Item<V> top = sims.top();
+    sims.pop();
+    int64_t i = top.i, j = top.j;
+    if (i >= n || j >= n) continue;
+    if (cl[i] == i && cl[j] == j) {
+      if (!valid[i] || !valid[j]) continue;
+      nclusters--;
+      if (sz[i] < sz[j]) {
+        cl[i] = j; sz[j] += sz[i];
+        if (sz[j] >= cs) { valid[j] = 0; nclusters--; }
+      } else {
+        cl[j] = i; sz[i] += sz[j];
+        if (sz[i] >= cs) { valid[i] = 0; nclusters--; }
+      }
+    } else {
+      i = find_root(cl, i);
+      j = find_root(cl, j);
+      if (!valid[i] || !valid[j]) continue;
+      if (i != j) {
+        const int64_t a = std::min(i, j), b = std::max(i, j);
+        if (keys.insert(a, b)) sims.push(Item<V>{(V)jaccard(A, a, b, false), a, b});
+      }
+    }
+  }
+  const int64_t nc = emit(cl, n, perm, offsets);
+  for (int64_t c = 0; c < nc; ++c) roots[c] = find_root(cl, perm[offsets[c]]);   // the key of the reference's map
+  return nc;
+}
+
 }  // namespace
 }  // namespace spops
 
 using namespace spops;
+
+extern "C" int spops_hierarchical_clustering_v0(int64_t n, const int64_t *rowptr, const int64_t *col, int64_t np,
+                                                const int64_t *pi, const int64_t *pj, const void *score,
+                                                int score_type, int64_t cs, int64_t *perm, int64_t *offsets,
+                                                int64_t *roots, int64_t *nclusters) {
+  return guarded([&]() {
+    SPOPS_REQUIRE(n >= 0 && np >= 0, SPOPS_ERROR_INVALID_VALUE, "negative size");
+    SPOPS_REQUIRE(n < ((int64_t)1 << 31), SPOPS_ERROR_LIMIT, "n_rows must be < 2^31");
+    SPOPS_REQUIRE(rowptr && perm && offsets && roots && nclusters && (np == 0 || (pi && pj && score)),
+                  SPOPS_ERROR_INVALID_VALUE, "null pointer");
+    SPOPS_REQUIRE(score_type == SPOPS_VALUE_F64 || score_type == SPOPS_VALUE_F32, SPOPS_ERROR_INVALID_VALUE,
+                  "score_type must be F64 or F32");
+    for (int64_t t = 0; t < np; ++t)
+      SPOPS_REQUIRE(pi[t] >= 0 && pj[t] >= 0, SPOPS_ERROR_INVALID_VALUE, "negative row in pair %lld", (long long)t);
+    Csr64 A{rowptr, col};
+    *nclusters = score_type == SPOPS_VALUE_F64
+                     ? hc_v0_map<double>(n, A, np, pi, pj, static_cast<const double *>(score), cs, perm, offsets, roots)
+                     : hc_v0_map<float>(n, A, np, pi, pj, static_cast<const float *>(score), cs, perm, offsets, roots);
+    return SPOPS_SUCCESS;
+  });
+}
 
 extern "C" int spops_hierarchical_clustering(int version, int64_t n, const int64_t *rowptr, const int64_t *col,
                                              int64_t np, const int64_t *pi, const int64_t *pj, const void *score,

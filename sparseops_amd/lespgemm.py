@@ -10,6 +10,10 @@
     for c0, c1 in plan.panel_ranges(budget_bytes):
         P = plan.numeric(c0, c1, P)                                  # clusters [c0, c1) of C_cl
 
+    C       = spgemm_rows(dA, dB)                                    # row-wise LeSpGEMM (kernel flags 1-3)
+    Cand    = topk_jaccard(dA, top_k)                                # HashSpGEMMTopK candidate pairs
+    perm, offsets = hierarchical_clustering_v0(A, Cand.pairs, ...)   # map-based v0 over those pairs
+
 Every step runs in libsparseops_hip.so; this module only moves arrays.
 """
 from __future__ import annotations
@@ -81,6 +85,26 @@ def hierarchical_clustering(A: Csr, pi, pj, score, cluster_size: int = 8, versio
            pi.ctypes.data, pj.ctypes.data, score.ctypes.data, st, cluster_size, perm.ctypes.data,
            offsets.ctypes.data, ctypes.byref(nc))
     return perm, offsets[: nc.value + 1].copy()
+
+
+def hierarchical_clustering_v0(A: Csr, pi, pj, score, cluster_size: int = 8):
+    """The map-based hierarchical_clustering_v0 (include/spgemm_utility.h:517-606) over pairs in
+    the order of the caller's map (keys as it holds them): (perm, offsets, roots), clusters
+    ordered by their root row."""
+    rp = np.ascontiguousarray(A.rowptr, dtype=np.int64)
+    ci = np.ascontiguousarray(A.col, dtype=np.int64)
+    pi = np.ascontiguousarray(pi, dtype=np.int64)
+    pj = np.ascontiguousarray(pj, dtype=np.int64)
+    score = np.ascontiguousarray(score)
+    st = L.VALUE_F32 if score.dtype == np.float32 else L.VALUE_F64
+    perm = np.zeros(A.n_rows, np.int64)
+    offsets = np.zeros(A.n_rows + 1, np.int64)
+    roots = np.zeros(max(A.n_rows, 1), np.int64)
+    nc = c_i64()
+    L.call("spops_hierarchical_clustering_v0", A.n_rows, rp.ctypes.data, ci.ctypes.data, len(pi), pi.ctypes.data,
+           pj.ctypes.data, score.ctypes.data, st, cluster_size, perm.ctypes.data, offsets.ctypes.data,
+           roots.ctypes.data, ctypes.byref(nc))
+    return perm, offsets[: nc.value + 1].copy(), roots[: nc.value].copy()
 
 
 class ClusterMatrix:
@@ -330,6 +354,55 @@ def sample_ranges(cf, share: float, starts=(0.0, 0.5)):
         c1 = int(np.searchsorted(cum, base + share * tot)) + 1
         out.append((c0, min(c1, len(cum))))
     return out
+
+
+def _take_csr(n_rows, n_cols, nnz, drp, dcol, dval, vt) -> Csr:
+    """Host copy of three device arrays the library allocated, which are then freed."""
+    try:
+        rp = np.zeros(n_rows + 1, np.int64)
+        col = np.zeros(nnz, np.int64)
+        val = np.zeros(nnz, vt)
+        L.call("spops_memcpy_dtoh", rp.ctypes.data, drp, rp.nbytes, None)
+        if nnz:
+            L.call("spops_memcpy_dtoh", col.ctypes.data, dcol, col.nbytes, None)
+            L.call("spops_memcpy_dtoh", val.ctypes.data, dval, val.nbytes, None)
+    finally:
+        for p in (drp, dcol, dval):
+            L.lib.spops_free(p)
+    return Csr(n_rows, n_cols, rp, col, val)
+
+
+def spgemm_rows(dA: DeviceCsr, dB: DeviceCsr, stats=None) -> Csr:
+    """Row-wise C = A * B (the reference's LeSpGEMM, src/spgemm.cpp:684-705): columns ascending
+    in every row, zero sums kept.  A and B must share the value type."""
+    if dA.n_cols != dB.n_rows:
+        raise ValueError("Error: A.cols != B.rows")
+    if dA.value_type != dB.value_type:
+        raise ValueError("A and B must have the same value type")
+    nnz = c_i64()
+    drp, dcol, dval = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    st = (c_i64 * 16)()
+    L.call("spops_spgemm_csr", dA.n_rows, dA.n_cols, dA.nnz, dA.rowptr.data_ptr(), dA.col.data_ptr(),
+           dA.val.data_ptr(), dB.n_cols, dB.nnz, dB.rowptr.data_ptr(), dB.col.data_ptr(), dB.val.data_ptr(),
+           dA.value_type, ctypes.byref(nnz), ctypes.byref(drp), ctypes.byref(dcol), ctypes.byref(dval), st,
+           L.current_stream())
+    if stats is not None:
+        stats.update(_stats(st))
+    vt = np.float64 if dA.value_type == L.VALUE_F64 else np.float32
+    return _take_csr(dA.n_rows, dB.n_cols, nnz.value, drp, dcol, dval, vt)
+
+
+def topk_jaccard(dA: DeviceCsr, top_k: int = 7, precision: int = 64, budget_bytes: int = 0) -> Csr:
+    """HashSpGEMMTopK (src/spgemm.cpp:707-842): row i keeps the top_k rows j != i sharing a
+    column with it, by Jaccard score descending then j ascending; values are the scores
+    (float64, or float32 with precision 32)."""
+    nnz = c_i64()
+    drp, dcol, dval = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    vt = L.VALUE_F64 if precision == 64 else L.VALUE_F32
+    L.call("spops_spgemm_topk_jaccard", dA.n_rows, dA.n_cols, dA.nnz, dA.rowptr.data_ptr(), dA.col.data_ptr(), vt,
+           top_k, budget_bytes, ctypes.byref(nnz), ctypes.byref(drp), ctypes.byref(dcol), ctypes.byref(dval),
+           L.current_stream())
+    return _take_csr(dA.n_rows, dA.n_rows, nnz.value, drp, dcol, dval, np.float64 if precision == 64 else np.float32)
 
 
 def spgemm_flop(dA: DeviceCsr, dB: DeviceCsr) -> int:

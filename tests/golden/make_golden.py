@@ -219,16 +219,68 @@ def panels_case(name, scale, share=0.02):
     print("wrote", name, "ranges", picks, "nnzc", out["nnzc"])
 
 
+def rowwise_case(name, A, B, prec=64, full=False):
+    """The row-wise LeSpGEMM (kernel flags 1, 2, 3, sortOutput = true) on A * B.  A must not have
+    more columns than B: the reference sizes its hash tables from the products of A's columns
+    below B.num_cols only (src/spgemm_bin.cpp:100-104), and a wider A overflows a table, whose
+    probe loop then never ends."""
+    out = {"A_digest": digest(A.rowptr, A.col, A.val), "B_digest": digest(B.rowptr, B.col, B.val),
+           "precision": np.int64(prec)}
+    with tempfile.TemporaryDirectory() as d:
+        write_csr_bin(os.path.join(d, "A.csr"), A)
+        write_csr_bin(os.path.join(d, "B.csr"), B)
+        for kern in (1, 2, 3):
+            run(["rowwise", os.path.join(d, "A.csr"), os.path.join(d, "B.csr"), d, str(kern), str(prec)], d)
+            rp, col, val = rd(d, "C_rowptr", np.int64), rd(d, "C_col", np.int64), rd(d, "C_val", np.float64)
+            out[f"k{kern}_rowptr"] = rp
+            out[f"k{kern}_col_digest"] = digest(col)
+            rows = np.repeat(np.arange(A.n_rows), np.diff(rp))
+            out[f"k{kern}_rowsum"] = np.bincount(rows, weights=val, minlength=A.n_rows)
+            if full:
+                out[f"k{kern}_col"] = col
+                out[f"k{kern}_val"] = val
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
+
+
+def topk_case(name, A, top_k=7, cs=8, prec=64):
+    """HashSpGEMMTopK and the test_spgemm_hc flow on its pairs (text round trip,
+    hierarchical_clustering_v0, reordered_dict_to_permutation_and_offsets).  A needs at least as
+    many rows as columns (the hash-table sizing of rowwise_case, with B = A^T)."""
+    out = {"A_digest": digest(A.rowptr, A.col, A.val), "top_k": np.int64(top_k), "cluster_size": np.int64(cs),
+           "precision": np.int64(prec)}
+    with tempfile.TemporaryDirectory() as d:
+        write_csr_bin(os.path.join(d, "A.csr"), A)
+        run(["topk_hc", os.path.join(d, "A.csr"), d, str(top_k), str(cs), str(prec)], d)
+        out["topk_rowptr"] = rd(d, "topk_rowptr", np.int64)
+        out["topk_col"] = rd(d, "topk_col", np.int64)
+        out["topk_val"] = rd(d, "topk_val", np.float64)
+        out["pair_keys"] = rd(d, "pair_keys", np.int64).reshape(-1, 2)
+        out["pair_scores"] = rd(d, "pair_scores", np.float64)
+        out["perm"] = rd(d, "perm", np.int64)
+        out["offsets"] = rd(d, "offsets", np.int64)
+        out["roots"] = rd(d, "roots", np.int64)
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: run make -C oracle ref (needs the reference sources)")
     only = set(sys.argv[1:])
-    if only:   # regenerate the named groups only: spmv, lsh, features, classify, fingerprints, panels
+    if only:   # regenerate the named groups only: classify, fingerprints, panels, rowwise, topk
         groups = {"classify": lambda: (classify_case("classify_rmat10", rmat(10, 16, seed=10), [-1.0, 2e-4, 5e-5, 0.0]),
                                        classify_case("classify_rmat12", rmat(12, 16, seed=12), [-1.0, 2e-3, 1e-3, 2e-4])),
                   "fingerprints": lambda: (fingerprint_case("lsh_rmat14_fp", rmat(14, 16, seed=14)),
                                            fingerprint_case("lsh_rmat16_fp", rmat(16, 16, seed=16), block=16)),
-                  "panels": lambda: panels_case("lsh_rmat20_panels", 20)}
+                  "panels": lambda: panels_case("lsh_rmat20_panels", 20),
+                  "rowwise": lambda: (rowwise_case("rowwise_rmat10", rmat(10, 16, seed=10), rmat(10, 16, seed=10)),
+                                      rowwise_case("rowwise_rect_f64", random_rect(300, 200, 12, seed=5),
+                                                   random_rect(200, 500, 9, seed=6), full=True),
+                                      rowwise_case("rowwise_rect_f32", random_rect(300, 200, 12, seed=5),
+                                                   random_rect(200, 500, 9, seed=6), prec=32, full=True)),
+                  "topk": lambda: (topk_case("topk_rmat10", rmat(10, 16, seed=10)),
+                                   topk_case("topk_rmat10_f32", rmat(10, 16, seed=10), prec=32),
+                                   topk_case("topk_rmat12_k16", rmat(12, 16, seed=12), top_k=16, cs=4),
+                                   topk_case("topk_rect900x400", random_rect(900, 400, 10, seed=7), top_k=5))}
         for g in sorted(only):
             groups[g]()
         return

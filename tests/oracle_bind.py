@@ -24,7 +24,8 @@ def _load():
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "oracle"])
     lib = ctypes.CDLL(ORACLE_SO)
     for name in ["orc_sell_layout", "orc_ell_width", "orc_bsr_count", "orc_lsh_pairs", "orc_hc",
-                 "orc_cluster_build", "orc_spgemm_flop", "orc_cluster_classify", "orc_cluster_to_csr"]:
+                 "orc_cluster_build", "orc_spgemm_flop", "orc_cluster_classify", "orc_cluster_to_csr",
+                 "orc_spgemm_rows", "orc_topk_jaccard"]:
         getattr(lib, name).restype = i64
     lib.orc_band_hash.restype = ctypes.c_uint64
     lib.orc_jaccard.restype = c_dbl
@@ -166,6 +167,27 @@ def hc(A, pi, pj, score, cluster_size=8, version=0, score_f32=False):
     nc = lib.orc_hc(c_int(version), i64(A.n_rows), P(_i64(A.rowptr)), P(_i64(A.col)), i64(len(pi)), P(_i64(pi)),
                     P(_i64(pj)), P(_f64(score)), c_int(int(score_f32)), i64(cluster_size), P(perm), ctypes.byref(off))
     return perm, _take(off, nc + 1, ctypes.c_int64, np.int64)
+
+
+def spgemm_rows(A, B):
+    """Row-wise C = A * B as the reference's SPA kernel sums it (Csr with int64 indices)."""
+    from sparseops_amd.gen import Csr
+    rp = np.zeros(A.n_rows + 1, np.int64)
+    cc, cv = ctypes.c_void_p(), ctypes.c_void_p()
+    nnz = lib.orc_spgemm_rows(i64(A.n_rows), P(_i64(A.rowptr)), P(_i64(A.col)), P(_f64(A.val)), i64(B.n_cols),
+                              P(_i64(B.rowptr)), P(_i64(B.col)), P(_f64(B.val)), P(rp), ctypes.byref(cc),
+                              ctypes.byref(cv))
+    return Csr(A.n_rows, B.n_cols, rp, _take(cc, nnz, ctypes.c_int64, np.int64), _take(cv, nnz, ctypes.c_double, np.float64))
+
+
+def topk_jaccard(A, top_k, score_f32=False):
+    """HashSpGEMMTopK's candidate matrix (Csr, entries in score order)."""
+    from sparseops_amd.gen import Csr
+    rp = np.zeros(A.n_rows + 1, np.int64)
+    cc, cv = ctypes.c_void_p(), ctypes.c_void_p()
+    nnz = lib.orc_topk_jaccard(i64(A.n_rows), i64(A.n_cols), P(_i64(A.rowptr)), P(_i64(A.col)), i64(top_k),
+                               c_int(int(score_f32)), P(rp), ctypes.byref(cc), ctypes.byref(cv))
+    return Csr(A.n_rows, A.n_rows, rp, _take(cc, nnz, ctypes.c_int64, np.int64), _take(cv, nnz, ctypes.c_double, np.float64))
 
 
 def jaccard(A, a, b, sorted_unique=True):
