@@ -29,6 +29,9 @@
 //   rowwise <A.csr> <B.csr> <outdir> <kernel 1|2|3> <prec>
 //                         the row-wise LeSpGEMM<sortOutput = true> (src/spgemm.cpp:684-705):
 //                         C's rowptr, col and values
+//   read_mtx <A.mtx> <outdir> <compact 0|1>
+//                         the reference's read_csr_matrix<int64_t, double> (utils/sparse_io.cpp:
+//                         26-214): rowptr, col and values, dimensions in "dims"
 //   topk_hc <A.csr> <outdir> <top_k> <cluster_size> <prec>
 //                         HashSpGEMMTopK (C's rowptr, col, values), then the flow of
 //                         test/test_spgemm_hc.cpp on those pairs: C written as "%20.16g" text
@@ -654,6 +657,13 @@ int main(int argc, char **argv) {
   if (mode == "bench_spmm_batch" && argc >= 5) {
     Le_set_thread_num(std::atoi(argv[3]));
     return std::atoi(argv[4]) == 32 ? bench_spmm_batch<float>(argv[2]) : bench_spmm_batch<double>(argv[2]);
+  }
+  if (mode == "read_mtx" && argc >= 5) {
+    CSR_Matrix<int64_t, double> A = read_csr_matrix<int64_t, double>(argv[2], std::atoi(argv[4]) != 0);
+    const int64_t dims[3] = {(int64_t)A.num_rows, (int64_t)A.num_cols, (int64_t)A.num_nnzs};
+    dump(argv[3], "dims", dims, 3);
+    if (A.num_rows > 0) dump_csr(argv[3], "A", A);
+    return 0;
   }
   if (mode == "rowwise" && argc >= 7) {
     HostCsr A = read_csr_file(argv[2]);

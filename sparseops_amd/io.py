@@ -93,17 +93,25 @@ def read_mtx(path, index_dtype=np.int64, compact: bool = False) -> Csr:
 
 
 def sum_duplicates(A: Csr) -> Csr:
-    """Sort columns within rows and add duplicate entries (compact=true reading)."""
+    """compact=true reading (include/sparse_operation.h:21-73 through coo_to_csr): inside each
+    row the entries of one column are added in file order, exact-zero sums are dropped, and the
+    columns come out in reverse order of their first appearance (the order the reference's
+    linked list of seen columns yields)."""
+    nnz = A.nnz
     rows = np.repeat(np.arange(A.n_rows, dtype=np.int64), A.row_lengths())
-    key = rows * A.n_cols + A.col.astype(np.int64)
-    uk, inv = np.unique(key, return_inverse=True)
+    key = rows * max(A.n_cols, 1) + A.col.astype(np.int64)
+    uk, first, inv = np.unique(key, return_index=True, return_inverse=True)
     vals = np.zeros(uk.size)
-    np.add.at(vals, inv, A.val)
-    r, c = uk // A.n_cols, uk % A.n_cols
-    counts = np.bincount(r, minlength=A.n_rows)
+    np.add.at(vals, inv, A.val)                     # entries of a group added in entry order
+    r = uk // max(A.n_cols, 1)
+    keep = vals != 0.0
+    r, c, v, first = r[keep], (uk % max(A.n_cols, 1))[keep], vals[keep], first[keep]
+    order = np.lexsort((-first, r))                 # by row, then latest first appearance first
+    r, c, v = r[order], c[order], v[order]
+    counts = np.bincount(r, minlength=A.n_rows) if nnz else np.zeros(A.n_rows, np.int64)
     rowptr = np.zeros(A.n_rows + 1, dtype=np.int64)
     np.cumsum(counts, out=rowptr[1:])
-    return Csr(A.n_rows, A.n_cols, rowptr.astype(A.rowptr.dtype), c.astype(A.col.dtype), vals)
+    return Csr(A.n_rows, A.n_cols, rowptr.astype(A.rowptr.dtype), c.astype(A.col.dtype), v)
 
 
 def write_clustering(out_dir: str, name: str, perm, offsets) -> tuple[str, str]:

@@ -262,11 +262,30 @@ def topk_case(name, A, top_k=7, cs=8, prec=64):
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
 
 
+def reader_case(name="reader_mtx"):
+    """The reference's read_csr_matrix on the Matrix Market files of tests/golden/mtx/ (file
+    order kept, symmetric mirroring, pattern -> 1.0, duplicates kept or summed by compact)."""
+    src = os.path.join(OUT, "mtx")
+    out = {}
+    for fn in sorted(os.listdir(src)):
+        base = fn[:-4]
+        for compact in (0, 1):
+            with tempfile.TemporaryDirectory() as d:
+                run(["read_mtx", os.path.join(src, fn), d, str(compact)], d)
+                dims = rd(d, "dims", np.int64)
+                out[f"{base}_c{compact}_dims"] = dims
+                if dims[0] > 0:
+                    out[f"{base}_c{compact}_rowptr"] = rd(d, "A_rowptr", np.int64)
+                    out[f"{base}_c{compact}_col"] = rd(d, "A_col", np.int64)
+                    out[f"{base}_c{compact}_val"] = rd(d, "A_val", np.float64)
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: run make -C oracle ref (needs the reference sources)")
     only = set(sys.argv[1:])
-    if only:   # regenerate the named groups only: classify, fingerprints, panels, rowwise, topk
+    if only:   # regenerate the named groups only: classify, fingerprints, panels, reader, rowwise, topk
         groups = {"classify": lambda: (classify_case("classify_rmat10", rmat(10, 16, seed=10), [-1.0, 2e-4, 5e-5, 0.0]),
                                        classify_case("classify_rmat12", rmat(12, 16, seed=12), [-1.0, 2e-3, 1e-3, 2e-4])),
                   "fingerprints": lambda: (fingerprint_case("lsh_rmat14_fp", rmat(14, 16, seed=14)),
@@ -277,6 +296,7 @@ def main():
                                                    random_rect(200, 500, 9, seed=6), full=True),
                                       rowwise_case("rowwise_rect_f32", random_rect(300, 200, 12, seed=5),
                                                    random_rect(200, 500, 9, seed=6), prec=32, full=True)),
+                  "reader": reader_case,
                   "topk": lambda: (topk_case("topk_rmat10", rmat(10, 16, seed=10)),
                                    topk_case("topk_rmat10_f32", rmat(10, 16, seed=10), prec=32),
                                    topk_case("topk_rmat12_k16", rmat(12, 16, seed=12), top_k=16, cs=4),
