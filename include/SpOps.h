@@ -178,5 +178,6 @@ void SpOps_bsr(const V alpha, const BSR_Matrix<I, V> &b, const V *x, const V bet
 
 #include "spgemm.h"
 #include "spmv_testroutine.h"
+#include "sparse_features.h"
 
 #endif

@@ -29,6 +29,9 @@
 //   rowwise <A.csr> <B.csr> <outdir> <kernel 1|2|3> <prec>
 //                         the row-wise LeSpGEMM<sortOutput = true> (src/spgemm.cpp:684-705):
 //                         C's rowptr, col and values
+//   features_full <A.mtx> <matID>
+//                         MtxLoad, CalculateFeatures, CalculateTilesExtraFeatures, FeaturesPrint,
+//                         ExtraFeaturesPrint: the whole printout of test/Get_features.cpp
 //   read_mtx <A.mtx> <outdir> <compact 0|1>
 //                         the reference's read_csr_matrix<int64_t, double> (utils/sparse_io.cpp:
 //                         26-214): rowptr, col and values, dimensions in "dims"
@@ -675,6 +678,15 @@ int main(int argc, char **argv) {
     HostCsr A = read_csr_file(argv[2]);
     return std::atoi(argv[6]) == 32 ? run_topk_hc<float>(A, argv[3], std::atoll(argv[4]), std::atoll(argv[5]))
                                     : run_topk_hc<double>(A, argv[3], std::atoll(argv[4]), std::atoll(argv[5]));
+  }
+  if (mode == "features_full" && argc >= 4) {
+    MTX<int, double> m(std::atoi(argv[3]));
+    if (!m.MtxLoad(argv[2])) return 2;
+    m.CalculateFeatures();
+    m.CalculateTilesExtraFeatures(argv[2]);
+    m.FeaturesPrint();
+    m.ExtraFeaturesPrint();
+    return 0;
   }
   if (mode == "features" && argc >= 3) {
     MTX<int, double> m(0);

@@ -281,11 +281,25 @@ def reader_case(name="reader_mtx"):
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **out)
 
 
+def features_text_case(name, A=None, mtx=None):
+    """The reference's whole feature printout (test/Get_features.cpp: MtxLoad, CalculateFeatures,
+    CalculateTilesExtraFeatures, FeaturesPrint, ExtraFeaturesPrint) as text, for a generated
+    matrix or a file of tests/golden/mtx/."""
+    with tempfile.TemporaryDirectory() as d:
+        path = mtx if mtx else os.path.join(d, name + ".mtx")
+        if A is not None:
+            write_mtx(path, A)
+        text = subprocess.check_output([HARNESS, "features_full", path, "3"], text=True)
+    keep = [ln for ln in text.splitlines() if not ln.startswith("- ")]      # drop the file-path lines
+    with open(os.path.join(OUT, "features_text", name + ".txt"), "w") as f:
+        f.write("\n".join(keep) + "\n")
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: run make -C oracle ref (needs the reference sources)")
     only = set(sys.argv[1:])
-    if only:   # regenerate the named groups only: classify, fingerprints, panels, reader, rowwise, topk
+    if only:   # regenerate the named groups only: classify, featurestext, fingerprints, panels, reader, rowwise, topk
         groups = {"classify": lambda: (classify_case("classify_rmat10", rmat(10, 16, seed=10), [-1.0, 2e-4, 5e-5, 0.0]),
                                        classify_case("classify_rmat12", rmat(12, 16, seed=12), [-1.0, 2e-3, 1e-3, 2e-4])),
                   "fingerprints": lambda: (fingerprint_case("lsh_rmat14_fp", rmat(14, 16, seed=14)),
@@ -297,6 +311,14 @@ def main():
                                       rowwise_case("rowwise_rect_f32", random_rect(300, 200, 12, seed=5),
                                                    random_rect(200, 500, 9, seed=6), prec=32, full=True)),
                   "reader": reader_case,
+                  "featurestext": lambda: (
+                      os.makedirs(os.path.join(OUT, "features_text"), exist_ok=True),
+                      features_text_case("ft_rmat10", rmat(10, 16, seed=10)),
+                      features_text_case("ft_pl3000", powerlaw(3000, lmax=300, seed=2)),
+                      features_text_case("ft_rect700x1900", random_rect(700, 1900, 20, seed=1)),
+                      features_text_case("ft_lap40", laplacian_2d(40)),
+                      [features_text_case("ft_" + f[:-4], mtx=os.path.join(OUT, "mtx", f))
+                       for f in sorted(os.listdir(os.path.join(OUT, "mtx")))]),
                   "topk": lambda: (topk_case("topk_rmat10", rmat(10, 16, seed=10)),
                                    topk_case("topk_rmat10_f32", rmat(10, 16, seed=10), prec=32),
                                    topk_case("topk_rmat12_k16", rmat(12, 16, seed=12), top_k=16, cs=4),

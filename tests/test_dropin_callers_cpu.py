@@ -1,7 +1,8 @@
 """The reference's own command-line callers compile and link, unchanged, against the drop-in
 include/ and libsparseops_hip.so: benchmark_spmv_{csr,ell,sell_c_sigma,sell_c_R,bsr}.cpp,
 test_spgemm_hc_lsh.cpp, test_spgemm_lsh.cpp, record_permutation_lsh.cpp, test_bsr_format.cpp,
-generate_candidate_pairs.cpp and test_spgemm_hc.cpp from the reference's test/ directory.  Each source is compiled where it
+generate_candidate_pairs.cpp, test_spgemm_hc.cpp and Get_features.cpp from the reference's test/
+directory.  Each source is compiled where it
 lies, reached through a symlink in a scratch directory whose include/ links to this
 repository's headers, so its "../include/X.h" lines resolve to the drop-in (the recipe of
 oracle/Makefile's `callers` target).  Runs only where the reference sources are present."""
@@ -17,7 +18,7 @@ REF_TEST = "/root/reference/test"
 LIB = os.path.join(ROOT, "sparseops_amd", "libsparseops_hip.so")
 CALLERS = ["benchmark_spmv_csr", "benchmark_spmv_ell", "benchmark_spmv_sell_c_sigma", "benchmark_spmv_sell_c_R",
            "benchmark_spmv_bsr", "test_spgemm_hc_lsh", "test_spgemm_lsh", "record_permutation_lsh",
-           "test_bsr_format", "generate_candidate_pairs", "test_spgemm_hc"]
+           "test_bsr_format", "generate_candidate_pairs", "test_spgemm_hc", "Get_features"]
 
 pytestmark = pytest.mark.skipif(not os.path.isdir(REF_TEST), reason="the reference's callers are not present here")
 
@@ -60,7 +61,7 @@ def test_dropin_has_every_umbrella_header():
     want = ["SpOps.h", "general_config.h", "plat_config.h", "memopt.h", "mmio.h", "thread.h", "timer.h",
             "cmdline.h", "sparse_io.h", "sparse_format.h", "sparse_operation.h", "sparse_partition.h",
             "sparse_conversion.h", "spmv_benchmark.h", "spmv_testroutine.h", "spgemm.h", "spgemm_utility.h",
-            "spgemm_MinHashLSH.h", "spgemm_cluster.h"]
+            "spgemm_MinHashLSH.h", "spgemm_cluster.h", "sparse_features.h"]
     missing = [h for h in want if not os.path.exists(os.path.join(ROOT, "include", h))]
     assert not missing, missing
     assert shutil.which("g++")
