@@ -362,6 +362,35 @@ def lespgemm_panels_measure(scale, budget_gb, cpu_threads, cpu, cpu_share=0.02):
                   "nnzc": int(nnzc_t), "panels": len(ranges_t), "nnz_C": int(nnz_ct),
                   "roofline": spgemm_roofline(A.nnz, At.nnz, A.n_rows, At.n_rows, nnz_ct, A.n_rows, walls[-1])}
     del dAt
+    # the approximate GPU clustering behind hc_v = 3 (no reference counterpart): MinHash + LSH +
+    # rounds of mutual best pairs on the device, then the same product on its clusters (plan +
+    # panels, the second of two passes)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    perm_g, off_g, rounds_g, npairs_g = LG.cluster_rows_gpu(dA, 64, 16, 7, True, 8)
+    pre_g_ms = (time.perf_counter() - t) * 1e3
+    Acl_g = LG.ClusterMatrix.build(dA, perm_g, off_g)
+    Cg, walls_g = None, []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        pg = LG.SpgemmPlan(Acl_g, dA)
+        for c0, c1 in pg.panel_ranges(budget):
+            Cg = pg.numeric(c0, c1, Cg)
+        torch.cuda.synchronize()
+        walls_g.append((time.perf_counter() - t) * 1e3)
+        nnzc_g = pg.nnzc
+        pg.close()
+    Cg.close()
+    out["gpu_clustering"] = {
+        "workload": "hc_v = 3: constrained LSH pairs clustered on the GPU by rounds of mutual best pairs "
+                    "(cluster_size 8), then C = A*A on those clusters",
+        "genpairs_and_clustering_ms": round(pre_g_ms, 1), "rounds": int(rounds_g), "pairs": int(npairs_g),
+        "exact_genpairs_plus_hc_ms": round(genpairs_ms + hc_ms, 1),
+        "clusters": len(off_g) - 1, "clusters_exact_hc": len(offsets) - 1,
+        "nnzc_A": int(Acl_g.nnzc), "nnzc_A_exact_hc": int(Acl.nnzc), "nnzc_C": int(nnzc_g),
+        "gpu_ms": round(walls_g[-1], 2), "gpu_gflops": round(flop / walls_g[-1] / 1e6, 2)}
+    Acl_g.close()
     Acl.close()
     torch.cuda.synchronize()
     return out

@@ -338,6 +338,22 @@ int spops_hierarchical_clustering_v0(int64_t n_rows, const int64_t *rowptr, cons
                                      int64_t cluster_size, int64_t *perm, int64_t *offsets, int64_t *roots,
                                      int64_t *nclusters);
 
+/* Approximate row clustering on the GPU, an alternative to the sequential hierarchical
+ * clustering (include/spgemm_utility.h:743-915) behind a flag: no reference counterpart.  The
+ * candidate pairs (device arrays, e.g. from spops_pairs_device; scores >= 0 of score_type F64 or
+ * F32) are the edges of a graph over the rows; clusters merge in rounds in which every cluster
+ * picks its best edge to another cluster whose size keeps the pair within cluster_size (highest
+ * score, ties by the smaller splitmix64 hash of the (min id, max id) key) and clusters that picked
+ * each other merge.
+ * Rounds stop when none merges or after max_rounds (*rounds = rounds that merged).  perm[n_rows]
+ * and offsets[n_rows + 1] (host) as spops_hierarchical_clustering: clusters ordered by their
+ * smallest row, rows ascending inside; every cluster has at most cluster_size rows (the
+ * reference's can exceed it: it stops growing a cluster once it reaches the size).  Deterministic.
+ * Synchronous on `stream`. */
+int spops_cluster_matching(int64_t n_rows, int64_t npairs, const int64_t *d_pi, const int64_t *d_pj,
+                           const void *d_score, int score_type, int64_t cluster_size, int64_t max_rounds,
+                           int64_t *perm, int64_t *offsets, int64_t *nclusters, int64_t *rounds, void *stream);
+
 /* ---- LeSpGEMM: cluster format and cluster-wise SpGEMM (device) ----------------------------
  * A spops_cluster is a device CSR_VlengthCluster (include/spgemm_cluster.h:108-143):
  * cluster c = rows offsets[c]..offsets[c+1] of the permuted matrix, its distinct columns

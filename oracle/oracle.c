@@ -1225,3 +1225,87 @@ int64_t orc_topk_jaccard(int64_t n_rows, int64_t n_cols, const int64_t *rowptr, 
   free(tp); free(tr); free(cur); free(cnt); free(seen); free(list); free(cand);
   return c_rowptr[n_rows];
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* Approximate clustering by rounds of mutual best edges (no reference counterpart)      */
+/* ------------------------------------------------------------------------------------ */
+
+/* A bijective 64-bit mix of an edge's (min id, max id) key (the splitmix64 finalizer): the
+ * tie-break among equal scores, a pseudo-random strict order that keeps chains of equal-score
+ * pairs (the windowed LSH buckets of identical rows) from merging one pair per round. */
+static uint64_t edge_rank_hash(int64_t lo, int64_t hi) {
+  uint64_t x = (uint64_t)lo << 32 | (uint64_t)hi;
+  x ^= x >> 30; x *= 0xbf58476d1ce4e5b9ULL;
+  x ^= x >> 27; x *= 0x94d049bb133111ebULL;
+  x ^= x >> 31;
+  return x;
+}
+
+/* The GPU clustering of spops_cluster_matching, round by round: every cluster picks its best
+ * valid edge (the two clusters of a pair fit cluster_size together; highest score, ties by the
+ * smaller edge_rank_hash of its (min id, max id) key), clusters that picked each other merge into the smaller id, rows
+ * take their cluster's new id; until no merge or max_rounds.  Clusters ordered by id (their
+ * smallest row), rows ascending.  Returns the number of clusters; *offsets malloc'ed; *rounds =
+ * rounds that merged.  Pinned: a pure-Python restatement on small graphs and the invariants
+ * (permutation, sizes, every cluster connected by candidate pairs) in tests/test_cluster_matching_cpu.py. */
+int64_t orc_cluster_matching(int64_t n, int64_t np, const int64_t *pi, const int64_t *pj, const double *score,
+                             int64_t cs, int64_t max_rounds, int64_t *perm, int64_t **offsets, int64_t *rounds) {
+  int64_t *label = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(n ? n : 1));
+  int64_t *size = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(n ? n : 1));
+  int64_t *parent = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(n ? n : 1));
+  double *bs = (double *)xmalloc(sizeof(double) * (size_t)(n ? n : 1));
+  int64_t *be_lo = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(n ? n : 1));
+  int64_t *be_hi = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(n ? n : 1));
+  char *has = (char *)xmalloc((size_t)(n ? n : 1));
+  uint64_t *bh = (uint64_t *)xmalloc(sizeof(uint64_t) * (size_t)(n ? n : 1));
+  for (int64_t i = 0; i < n; ++i) { label[i] = i; size[i] = 1; }
+  *rounds = 0;
+  for (int64_t r = 0; r < max_rounds; ++r) {
+    for (int64_t c = 0; c < n; ++c) { has[c] = 0; parent[c] = c; }
+    for (int64_t t = 0; t < np; ++t) {
+      if (pi[t] < 0 || pi[t] >= n || pj[t] < 0 || pj[t] >= n) continue;
+      const int64_t a = label[pi[t]], b = label[pj[t]];
+      if (a == b || size[a] + size[b] > cs) continue;
+      const int64_t lo = a < b ? a : b, hi = a < b ? b : a;
+      const uint64_t h = edge_rank_hash(lo, hi);
+      const int64_t ends[2] = {a, b};
+      for (int e = 0; e < 2; ++e) {
+        const int64_t c = ends[e];
+        if (!has[c] || score[t] > bs[c] || (score[t] == bs[c] && h < bh[c])) {
+          has[c] = 1; bs[c] = score[t]; bh[c] = h; be_lo[c] = lo; be_hi[c] = hi;
+        }
+      }
+    }
+    int64_t merged = 0;
+    for (int64_t c = 0; c < n; ++c) {
+      if (!has[c] || c != be_hi[c]) continue;
+      const int64_t lo = be_lo[c];
+      if (!has[lo] || be_lo[lo] != lo || be_hi[lo] != c) continue;
+      parent[c] = lo;
+      ++merged;
+    }
+    for (int64_t c = 0; c < n; ++c)
+      if (parent[c] != c) size[parent[c]] += size[c];
+    for (int64_t i = 0; i < n; ++i) label[i] = parent[label[i]];
+    if (merged == 0) break;
+    ++*rounds;
+  }
+  /* clusters by id, rows ascending: a counting pass over the ids */
+  int64_t *count = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(n + 1));
+  for (int64_t i = 0; i <= n; ++i) count[i] = 0;
+  for (int64_t i = 0; i < n; ++i) count[label[i] + 1]++;
+  int64_t nc = 0;
+  for (int64_t i = 0; i < n; ++i) nc += count[i + 1] > 0;
+  *offsets = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(nc + 1));
+  int64_t *start = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(n ? n : 1));
+  int64_t pos = 0, k = 0;
+  for (int64_t id = 0; id < n; ++id) {
+    start[id] = pos;
+    if (count[id + 1] > 0) { (*offsets)[k++] = pos; pos += count[id + 1]; }
+  }
+  (*offsets)[nc] = n;
+  for (int64_t i = 0; i < n; ++i) perm[start[label[i]]++] = i;
+  free(label); free(size); free(parent); free(bs); free(be_lo); free(be_hi); free(has); free(count); free(start);
+  free(bh);
+  return nc;
+}

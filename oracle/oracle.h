@@ -65,6 +65,8 @@ double orc_jaccard(const int64_t *rowptr, const int64_t *col, int64_t a, int64_t
 int64_t orc_spgemm_rows(int64_t n_rows, const int64_t *arp, const int64_t *acol, const double *aval, int64_t b_cols,
                         const int64_t *brp, const int64_t *bcol, const double *bval, int64_t *c_rowptr,
                         int64_t **c_col, double **c_val);
+int64_t orc_cluster_matching(int64_t n, int64_t np, const int64_t *pi, const int64_t *pj, const double *score,
+                             int64_t cs, int64_t max_rounds, int64_t *perm, int64_t **offsets, int64_t *rounds);
 int64_t orc_topk_jaccard(int64_t n_rows, int64_t n_cols, const int64_t *rowptr, const int64_t *col, int64_t top_k,
                          int score_f32, int64_t *c_rowptr, int64_t **c_col, double **c_val);
 

@@ -101,6 +101,8 @@ _SIGNATURES = [
     ("spops_spgemm_flop", c_int, [c_i64, c_vp, c_i64, c_vp, c_vp, c_vp]),
     ("spops_hierarchical_clustering_v0", c_int, [c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_int, c_i64, c_vp, c_vp,
                                                  c_vp, c_vp]),
+    ("spops_cluster_matching", c_int, [c_i64, c_i64, c_vp, c_vp, c_vp, c_int, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp,
+                                       c_vp]),
     ("spops_spgemm_csr", c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_int,
                                  c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("spops_spgemm_topk_jaccard", c_int, [c_i64, c_i64, c_i64, c_vp, c_vp, c_int, c_i64, c_i64, c_vp, c_vp, c_vp,

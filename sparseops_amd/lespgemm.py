@@ -87,6 +87,49 @@ def hierarchical_clustering(A: Csr, pi, pj, score, cluster_size: int = 8, versio
     return perm, offsets[: nc.value + 1].copy()
 
 
+def cluster_matching(n_rows: int, pi: torch.Tensor, pj: torch.Tensor, score: torch.Tensor, cluster_size: int = 8,
+                     max_rounds: int = 256):
+    """Approximate clustering on the GPU (spops_cluster_matching) of device pair arrays (int64 i, j;
+    float64 or float32 scores >= 0): rounds of mutual best edges between clusters that fit
+    cluster_size.  Returns (perm, offsets, rounds)."""
+    for t in (pi, pj, score):
+        if not (t.is_cuda and t.is_contiguous()):
+            raise ValueError("pairs must be contiguous device tensors")
+    if pi.dtype != torch.int64 or pj.dtype != torch.int64 or score.dtype not in (torch.float64, torch.float32):
+        raise ValueError("pairs: int64 i, j and float64 / float32 scores")
+    st = L.VALUE_F64 if score.dtype == torch.float64 else L.VALUE_F32
+    perm = np.zeros(max(n_rows, 1), np.int64)
+    offsets = np.zeros(n_rows + 1, np.int64)
+    nc, rounds = c_i64(), c_i64()
+    L.call("spops_cluster_matching", n_rows, pi.numel(), pi.data_ptr(), pj.data_ptr(), score.data_ptr(), st,
+           cluster_size, max_rounds, perm.ctypes.data, offsets.ctypes.data, ctypes.byref(nc), ctypes.byref(rounds),
+           L.current_stream())
+    return perm[:n_rows].copy(), offsets[: nc.value + 1].copy(), rounds.value
+
+
+def cluster_rows_gpu(dA: DeviceCsr, k: int = 64, bands: int = 16, seed: int = 7, constrain: bool = True,
+                     cluster_size: int = 8, max_rounds: int = 256, score_f32: bool = False):
+    """MinHash + LSH pairs and the approximate GPU clustering, the pairs staying on the device:
+    (perm, offsets, rounds, npairs)."""
+    h = ctypes.c_void_p()
+    st = L.VALUE_F32 if score_f32 else L.VALUE_F64
+    L.call("spops_lsh_candidate_pairs", dA.rowptr.data_ptr(), dA.col.data_ptr(), L.INDEX_I64, dA.n_rows, k, bands,
+           ctypes.c_uint64(seed), int(constrain), st, ctypes.byref(h), L.current_stream())
+    try:
+        n = c_i64()
+        L.call("spops_pairs_count", h, ctypes.byref(n))
+        di, dj, ds = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        L.call("spops_pairs_device", h, ctypes.byref(di), ctypes.byref(dj), ctypes.byref(ds))
+        perm = np.zeros(max(dA.n_rows, 1), np.int64)
+        offsets = np.zeros(dA.n_rows + 1, np.int64)
+        nc, rounds = c_i64(), c_i64()
+        L.call("spops_cluster_matching", dA.n_rows, n.value, di, dj, ds, st, cluster_size, max_rounds,
+               perm.ctypes.data, offsets.ctypes.data, ctypes.byref(nc), ctypes.byref(rounds), L.current_stream())
+    finally:
+        L.lib.spops_pairs_destroy(h)
+    return perm[: dA.n_rows].copy(), offsets[: nc.value + 1].copy(), rounds.value, n.value
+
+
 def hierarchical_clustering_v0(A: Csr, pi, pj, score, cluster_size: int = 8):
     """The map-based hierarchical_clustering_v0 (include/spgemm_utility.h:517-606) over pairs in
     the order of the caller's map (keys as it holds them): (perm, offsets, roots), clusters
@@ -416,7 +459,8 @@ def lespgemm(A: Csr, B: Csr, k: int = 64, bands: int = 16, seed: int = 7, cluste
              device="cuda:0", precision: int = 64, iterations: int = 0):
     """Full test_spgemm_hc_lsh pipeline; returns host arrays and phase timings (ms).
 
-    hc_v: 0 = constrained LSH + v0_fast, 1 = full LSH + v1, 2 = full LSH + v0_fast.
+    hc_v: 0 = constrained LSH + v0_fast, 1 = full LSH + v1, 2 = full LSH + v0_fast,
+          3 = constrained LSH + the approximate GPU clustering (cluster_rows_gpu; no reference counterpart).
     iterations > 0 additionally times that many LeSpGEMM runs (symbolic + malloc + numeric).
     """
     if A.n_cols != B.n_rows:
@@ -429,11 +473,18 @@ def lespgemm(A: Csr, B: Csr, k: int = 64, bands: int = 16, seed: int = 7, cluste
     t = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    pi, pj, ps = lsh_candidate_pairs(dA, k, bands, seed, constrain=(hc_v == 0), score_f32=(precision == 32))
-    t["genpairs_ms"] = (time.perf_counter() - t0) * 1e3
-    t0 = time.perf_counter()
-    perm, offsets = hierarchical_clustering(A, pi, pj, ps, cluster_size, version=1 if hc_v == 1 else 0)
-    t["hc_ms"] = (time.perf_counter() - t0) * 1e3
+    if hc_v == 3:
+        pi = pj = ps = None
+        perm, offsets, rounds, _ = cluster_rows_gpu(dA, k, bands, seed, True, cluster_size,
+                                                    score_f32=(precision == 32))
+        t["genpairs_hc_ms"] = (time.perf_counter() - t0) * 1e3
+        t["hc_rounds"] = rounds
+    else:
+        pi, pj, ps = lsh_candidate_pairs(dA, k, bands, seed, constrain=(hc_v == 0), score_f32=(precision == 32))
+        t["genpairs_ms"] = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        perm, offsets = hierarchical_clustering(A, pi, pj, ps, cluster_size, version=1 if hc_v == 1 else 0)
+        t["hc_ms"] = (time.perf_counter() - t0) * 1e3
     t0 = time.perf_counter()
     Acl = ClusterMatrix.build(dA, perm, offsets)
     torch.cuda.synchronize()

@@ -25,7 +25,7 @@ def _load():
     lib = ctypes.CDLL(ORACLE_SO)
     for name in ["orc_sell_layout", "orc_ell_width", "orc_bsr_count", "orc_lsh_pairs", "orc_hc",
                  "orc_cluster_build", "orc_spgemm_flop", "orc_cluster_classify", "orc_cluster_to_csr",
-                 "orc_spgemm_rows", "orc_topk_jaccard"]:
+                 "orc_spgemm_rows", "orc_topk_jaccard", "orc_cluster_matching"]:
         getattr(lib, name).restype = i64
     lib.orc_band_hash.restype = ctypes.c_uint64
     lib.orc_jaccard.restype = c_dbl
@@ -188,6 +188,16 @@ def topk_jaccard(A, top_k, score_f32=False):
     nnz = lib.orc_topk_jaccard(i64(A.n_rows), i64(A.n_cols), P(_i64(A.rowptr)), P(_i64(A.col)), i64(top_k),
                                c_int(int(score_f32)), P(rp), ctypes.byref(cc), ctypes.byref(cv))
     return Csr(A.n_rows, A.n_rows, rp, _take(cc, nnz, ctypes.c_int64, np.int64), _take(cv, nnz, ctypes.c_double, np.float64))
+
+
+def cluster_matching(n_rows, pi, pj, score, cluster_size=8, max_rounds=256):
+    """(perm, offsets, rounds) of the mutual-best-edge clustering (float32 scores passed widened)."""
+    perm = np.zeros(max(n_rows, 1), np.int64)
+    off = ctypes.c_void_p()
+    rounds = i64()
+    nc = lib.orc_cluster_matching(i64(n_rows), i64(len(pi)), P(_i64(pi)), P(_i64(pj)), P(_f64(score)),
+                                  i64(cluster_size), i64(max_rounds), P(perm), ctypes.byref(off), ctypes.byref(rounds))
+    return perm[:n_rows], _take(off, nc + 1, ctypes.c_int64, np.int64), rounds.value
 
 
 def jaccard(A, a, b, sorted_unique=True):
