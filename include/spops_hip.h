@@ -206,6 +206,17 @@ int spops_spmm_typed(spops_matrix A, int64_t k, double alpha, const void *d_X, i
  * order them. */
 int spops_spmm_typed_noalloc(spops_matrix A, int64_t k, double alpha, const void *d_X, int x_type, double beta,
                              void *d_Y, int y_type, void *stream);
+/* No counterpart in the reference (for differentiable layers whose values change every
+ * step): spops_spmm_typed with the matrix values read from d_values (nnz elements of A's
+ * stored value type) instead of the stored ones, which are neither read nor changed.  As
+ * with spops_csr_set_values, d_values of a handle made by spops_csr_transpose are in the
+ * ORIGINAL matrix's entry order (the kernels read them through the kept permutation; such
+ * handles of >= 2^31 entries return SPOPS_ERROR_LIMIT).  Saves the copy of
+ * spops_csr_set_values before each product.  On a transposed handle the random value reads
+ * land inside the gather-bound product: at R-MAT scale 20 (31 M entries, k = 64) that measured
+ * slower than spops_csr_set_values followed by spops_spmm_typed. */
+int spops_spmm_values(spops_matrix A, const void *d_values, int64_t k, double alpha, const void *d_X, int x_type,
+                      double beta, void *d_Y, int y_type, void *stream);
 /* out[e] = sum_c G[row_e, c] * X[col_e, c] for every entry e of A (A's values unused);
  * G row-major (n_rows x k), X row-major (n_cols x k), out[nnz] */
 int spops_sddmm(spops_matrix A, int64_t k, const void *d_G, int g_type, const void *d_X, int x_type, void *d_out,
@@ -252,6 +263,11 @@ int spops_matrix_info(spops_matrix A, int64_t info[16]);
  *   entry counts; the H partial sums of a row are added in a fixed order.
  *   SPOPS_TUNE_SWEEP_MAX_ROWS: rows of the largest row block of the swept layout, 1..20480
  *   (default 20480: 160 KiB of FP64 sums in LDS).  Both apply to matrices created afterwards.
+ *   SPOPS_TUNE_SWEEP_SPLIT: how the swept layout forms its row blocks: 0 (default) consecutive
+ *   rows of equal entry counts as above; T > 0 two classes, the rows of at most T entries and
+ *   the longer ones, each class cut into blocks of the row limit in row order, and the blocks
+ *   of long rows (many entries per 128-byte line of x) shared by as many workgroups as their
+ *   entries ask for.  Applies to matrices created afterwards.
  *   SPOPS_TUNE_BSR_MFMA: which BSR matrices run on the matrix cores (v_mfma_f64_16x16x4f64,
  *   FP64 products and sums; dense blocks of at least 8 x 8 with bC % 4 == 0, bR <= 64,
  *   bR * bC <= 4096): 0 none, 1 (default) those whose blocks exceed the VALU kernel's 256
@@ -279,13 +295,16 @@ int spops_matrix_info(spops_matrix A, int64_t info[16]);
 #define SPOPS_TUNE_TASK_PRODUCTS 6
 #define SPOPS_TUNE_WIDE_CURSORS 7
 #define SPOPS_TUNE_HEAVY_PRODUCTS 8
+#define SPOPS_TUNE_SWEEP_SPLIT 9
 int spops_set_tuning(int key, int64_t value, int64_t *old);
 
 /* Device copies of the layout arrays a format conversion produced, for parity checks:
  * which=0: reorder (int32, n_rows; SELL only); which=1: chunk_len in the reference's
  * sense (int32, nchunks; SELL only, rounded to the reference alignment 256/8/sizeof(V));
  * which=2: the swept layout's parameters (int64[4]: row bits of a block-local row index,
- * workgroups per row block, row blocks, entry groups; zeros for the column-sliced layout).
+ * workgroups per row block (the largest count), row blocks, entry groups; zeros for the
+ * column-sliced layout); which=3: the swept layout's row classes (int64[4]: split length T,
+ * 0 for one class; workgroups; partial-plane elements; tasks of the plane-adding pass).
  * `host_out` must hold the element count given by spops_matrix_info. */
 int spops_matrix_layout(spops_matrix A, int which, void *host_out);
 

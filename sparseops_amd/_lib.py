@@ -19,7 +19,7 @@ INDEX_I32, INDEX_I64 = 0, 1
 VALUE_F64, VALUE_F32, VALUE_BF16 = 0, 1, 2
 FORMAT_CSR, FORMAT_SELL, FORMAT_ELL, FORMAT_BSR = 0, 1, 2, 3
 (TUNE_SLICE_BYTES, TUNE_DENSE_PRODUCTS, TUNE_SPMV_LAYOUT, TUNE_SWEEP_WORKGROUPS, TUNE_SWEEP_MAX_ROWS,
- TUNE_BSR_MFMA, TUNE_TASK_PRODUCTS, TUNE_WIDE_CURSORS, TUNE_HEAVY_PRODUCTS) = 0, 1, 2, 3, 4, 5, 6, 7, 8
+ TUNE_BSR_MFMA, TUNE_TASK_PRODUCTS, TUNE_WIDE_CURSORS, TUNE_HEAVY_PRODUCTS, TUNE_SWEEP_SPLIT) = range(10)
 
 c_i64 = ctypes.c_int64
 c_int = ctypes.c_int
@@ -58,6 +58,7 @@ _SIGNATURES = [
     ("spops_spmv", c_int, [c_vp, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
     ("spops_spmm", c_int, [c_vp, c_i64, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
     ("spops_spmm_typed", c_int, [c_vp, c_i64, c_dbl, c_vp, c_int, c_dbl, c_vp, c_int, c_vp]),
+    ("spops_spmm_values", c_int, [c_vp, c_vp, c_i64, c_dbl, c_vp, c_int, c_dbl, c_vp, c_int, c_vp]),
     ("spops_spmm_typed_noalloc", c_int, [c_vp, c_i64, c_dbl, c_vp, c_int, c_dbl, c_vp, c_int, c_vp]),
     ("spops_sddmm", c_int, [c_vp, c_i64, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_vp]),
     ("spops_csr_transpose", c_int, [c_vp, c_vp]),

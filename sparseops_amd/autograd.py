@@ -2,7 +2,7 @@
 
 The pattern of A is fixed and lives on the GPU; its values are a tensor that may require
 gradients.  Forward and backward call the C ABI of libsparseops_hip.so:
-  Y     = A X                      spops_spmm_typed on A
+  Y     = A X                      spops_spmm_values on A (vals read in place)
   dX    = A^T G                    spops_spmm_typed on a cached transposed handle whose values
                                    follow vals through the entry permutation
   dvals = SDDMM(G, X) on A's pattern   (dvals[e] = <G[row_e, :], X[col_e, :]>, spops_sddmm)
@@ -27,6 +27,7 @@ class SparsePattern:
         self.A = S.csr(A, value_type, spmv_layout=1)
         self.AT = self.A.transpose()
         self.n_rows, self.n_cols, self.nnz = A.n_rows, A.n_cols, A.nnz
+        self.value_dtype = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[value_type]
 
     def close(self):
         self.A.close()
@@ -37,9 +38,13 @@ class _SpMM(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vals: torch.Tensor, X: torch.Tensor, pat: SparsePattern, out_dtype):
         X = X.contiguous()
-        pat.A.set_values(vals.detach())
         Y = torch.empty(pat.n_rows, X.shape[1], dtype=out_dtype or X.dtype, device=X.device)
-        pat.A.spmm(X, Y)
+        v = vals.detach()
+        if v.dtype == pat.value_dtype and v.is_contiguous():
+            pat.A.spmm_values(v, X, Y)          # the values read in place, no copy
+        else:
+            pat.A.set_values(v)
+            pat.A.spmm(X, Y)
         ctx.save_for_backward(vals, X)
         ctx.pat = pat
         return Y
@@ -51,6 +56,8 @@ class _SpMM(torch.autograd.Function):
         G = G.contiguous()
         dvals = dX = None
         if ctx.needs_input_grad[1]:
+            # one permuting gather of the values, then the product on contiguous values (reading
+            # them through the permutation inside the gather-bound product measured slower)
             pat.AT.set_values(vals.detach())
             dX = torch.empty_like(X)
             pat.AT.spmm(G, dX)

@@ -15,8 +15,11 @@
 //   k_spmm_seg     : one workgroup per segment of kSpmmSegment entries of a long row, its
 //                    partial row of sums to a scratch buffer, then
 //   k_spmm_seg_sum : one workgroup per long row adds its segments in order (deterministic).
-//   k_sddmm        : entry-parallel, a group of L lanes per entry with 16-byte loads of the
-//                    G and X rows, cross-lane sum.
+//   k_sddmm_rows   : a group of L lanes per chunk of at most 64 entries of one row: the row
+//                    of G loaded once, then one 16-byte load of X's row per entry and lane,
+//                    cross-lane sum (k a multiple of the vector, at most 64 vectors);
+//   k_sddmm        : otherwise entry-parallel, a group of L lanes per entry with loads of
+//                    both the G and the X row, cross-lane sum.
 #include <algorithm>
 #include <memory>
 #include <mutex>
@@ -75,6 +78,7 @@ template <int C, int VEC> struct Vec {
 template <int VC, int XC, int YC, typename RP, int VEC>
 __global__ __launch_bounds__(kBlock) void k_spmm_rows(const RP *__restrict__ rowptr, const int *__restrict__ col,
                                                       const typename Ty<VC>::T *__restrict__ val,
+                                                      const int *__restrict__ vperm,
                                                       const int *__restrict__ order, int64_t n_short, int k, int L,
                                                       const typename Ty<XC>::T *__restrict__ X,
                                                       typename Ty<YC>::T *__restrict__ Y, double alpha, double beta) {
@@ -101,7 +105,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rows(const RP *__restrict__ row
       for (int u = 0; u < U; ++u) {
         const bool ok = p + u < p1;
         cc[u] = ok ? col[p + u] : 0;
-        vv[u] = ok ? (Acc)Ty<VC>::ld(val[p + u]) : Acc(0);
+        vv[u] = ok ? (Acc)Ty<VC>::ld(val[vperm ? (int64_t)vperm[p + u] : p + u]) : Acc(0);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -137,6 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rows(const RP *__restrict__ row
 template <int VC, int XC, typename RP, int VEC>
 __global__ __launch_bounds__(kBlock) void k_spmm_seg(const RP *__restrict__ rowptr, const int *__restrict__ col,
                                                      const typename Ty<VC>::T *__restrict__ val,
+                                                     const int *__restrict__ vperm,
                                                      const int *__restrict__ long_rows, const int2 *__restrict__ segs,
                                                      int k, int L, const typename Ty<XC>::T *__restrict__ X,
                                                      AccT<VC, XC> *__restrict__ part) {
@@ -163,8 +168,9 @@ __global__ __launch_bounds__(kBlock) void k_spmm_seg(const RP *__restrict__ rowp
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const bool ok = p + u * stride < q1;
-        cc[u] = ok ? col[p + u * stride] : 0;
-        vv[u] = ok ? (Acc)Ty<VC>::ld(val[p + u * stride]) : Acc(0);
+        const int64_t e = p + u * stride;
+        cc[u] = ok ? col[e] : 0;
+        vv[u] = ok ? (Acc)Ty<VC>::ld(val[vperm ? (int64_t)vperm[e] : e]) : Acc(0);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -273,6 +279,64 @@ __global__ __launch_bounds__(kBlock) void k_sddmm(const int *__restrict__ erow, 
   }
 }
 
+// The same by rows: a group of L lanes per task, a chunk of at most kSddmmChunk entries of one
+// row (tasks {row, first entry in the row}, the rows of a window longest first).  The group
+// loads its row of G once (lane q: columns q VEC .. q VEC + VEC - 1, all of k when
+// k <= L VEC), then per entry only the row of X; U entries in flight, each summed across the
+// group, and lane q writes the entries u with u % L == q.
+template <int GC, int XC, int OC, typename RP, int VEC>
+__global__ __launch_bounds__(kBlock) void k_sddmm_rows(const RP *__restrict__ rowptr, const int *__restrict__ col,
+                                                       const int2 *__restrict__ tasks, int64_t ntasks, int k, int L,
+                                                       const typename Ty<GC>::T *__restrict__ G,
+                                                       const typename Ty<XC>::T *__restrict__ X,
+                                                       typename Ty<OC>::T *__restrict__ out) {
+  using Acc = AccT<GC, XC>;
+  using GV = typename Vec<GC, VEC>::V;
+  using XV = typename Vec<XC, VEC>::V;
+  constexpr int U = 8;
+  const int lane = lane_id(), R = kWave / L, g = lane / L, q = lane % L;
+  const int64_t slot = ((int64_t)blockIdx.x * (kBlock / kWave) + wave_id()) * R + g;
+  if (slot >= ntasks) return;                       // lane groups past the last task
+  const int2 t = tasks[slot];
+  const int64_t row = t.x;
+  const int64_t p0 = (int64_t)rowptr[row] + t.y;
+  const int64_t p1 = std::min<int64_t>((int64_t)rowptr[row + 1], p0 + kSddmmChunk);
+  const int c = q * VEC;
+  const bool cv = c < k;
+  Acc gr[VEC];
+  {
+    GV t{};
+    if (cv) t = *reinterpret_cast<const GV *>(G + row * k + c);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) gr[e] = (Acc)Ty<GC>::ld(t.e[e]);
+  }
+  for (int64_t p = p0; p < p1; p += U) {
+    int cc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) cc[u] = p + u < p1 ? col[p + u] : 0;
+    XV xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      XV t{};
+      if (cv && p + u < p1) t = *reinterpret_cast<const XV *>(X + (int64_t)cc[u] * k + c);
+      xv[u] = t;
+    }
+    Acc part[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      part[u] = 0;
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) part[u] += gr[e] * (Acc)Ty<XC>::ld(xv[u].e[e]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      for (int m = 1; m < L; m <<= 1) part[u] += __shfl_xor(part[u], m, kWave);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (u % L == q && p + u < p1) out[p + u] = Ty<OC>::st((double)part[u]);
+  }
+}
+
 // erow[e] = row of entry e; one wave per row
 template <typename RP>
 __global__ __launch_bounds__(kBlock) void k_entry_rows(const RP *__restrict__ rowptr, int64_t n_rows,
@@ -327,18 +391,20 @@ inline int lanes_for(int k, int vec) {
 
 template <int VC, int XC, int YC, typename RP, int VEC>
 void spmm_launch_vec(const spops_matrix_s &A, int k, double alpha, const void *X, double beta, void *Y,
-                     hipStream_t s, void *ext_part) {
+                     hipStream_t s, void *ext_part, const void *ext_val) {
   using Acc = AccT<VC, XC>;
   const auto &C = A.csr;
   const RP *rp = C.rowptr.as<RP>();
-  const auto *v = C.val.as<typename Ty<VC>::T>();
+  const auto *v = ext_val ? static_cast<const typename Ty<VC>::T *>(ext_val) : C.val.as<typename Ty<VC>::T>();
+  // caller's values of a transposed handle are in the source matrix's entry order
+  const int *vp = ext_val && C.tperm.p ? C.tperm.as<int>() : nullptr;
   const auto *x = static_cast<const typename Ty<XC>::T *>(X);
   auto *y = static_cast<typename Ty<YC>::T *>(Y);
   const int L = lanes_for(k, VEC);
   if (C.nshort > 0) {
     const int64_t rows_per_block = (int64_t)(kBlock / kWave) * (kWave / L);
     hipLaunchKernelGGL((k_spmm_rows<VC, XC, YC, RP, VEC>), dim3(nblk(C.nshort, (int)rows_per_block)), dim3(kBlock), 0,
-                       s, rp, C.col.as<int>(), v, C.sorder.as<int>(), (int64_t)C.nshort, k, L, x, y, alpha, beta);
+                       s, rp, C.col.as<int>(), v, vp, C.sorder.as<int>(), (int64_t)C.nshort, k, L, x, y, alpha, beta);
     SPOPS_HIP_CHECK(hipGetLastError());
   }
   if (C.nslong > 0) {
@@ -351,7 +417,7 @@ void spmm_launch_vec(const spops_matrix_s &A, int k, double alpha, const void *X
       part = C.spmm_part.as<Acc>();
     }
     hipLaunchKernelGGL((k_spmm_seg<VC, XC, RP, VEC>), dim3((unsigned)C.nlseg), dim3(kBlock), 0, s, rp, C.col.as<int>(),
-                       v, C.slongs.as<int>(), C.lsegs.as<int2>(), k, L, x, part);
+                       v, vp, C.slongs.as<int>(), C.lsegs.as<int2>(), k, L, x, part);
     SPOPS_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL((k_spmm_seg_sum<Acc, YC>), dim3((unsigned)C.nslong), dim3(kBlock), 0, s, C.slongs.as<int>(),
                        C.lsegoff.as<int>(), part, k, y, alpha, beta);
@@ -363,41 +429,41 @@ void spmm_launch_vec(const spops_matrix_s &A, int k, double alpha, const void *X
 // and Y are aligned for it; otherwise one element per lane.
 template <int VC, int XC, int YC, typename RP>
 void spmm_launch(const spops_matrix_s &A, int k, double alpha, const void *X, double beta, void *Y, hipStream_t s,
-                 void *ext) {
+                 void *ext, const void *ev) {
   constexpr int V = 16 / (int)sizeof(typename Ty<XC>::T);
   if (k % V == 0 && aligned_to(X, 16) && aligned_to(Y, V * sizeof(typename Ty<YC>::T)))
-    spmm_launch_vec<VC, XC, YC, RP, V>(A, k, alpha, X, beta, Y, s, ext);
+    spmm_launch_vec<VC, XC, YC, RP, V>(A, k, alpha, X, beta, Y, s, ext, ev);
   else
-    spmm_launch_vec<VC, XC, YC, RP, 1>(A, k, alpha, X, beta, Y, s, ext);
+    spmm_launch_vec<VC, XC, YC, RP, 1>(A, k, alpha, X, beta, Y, s, ext, ev);
 }
 
 template <int VC, int XC, typename RP>
 void spmm_y(const spops_matrix_s &A, int k, double alpha, const void *X, double beta, void *Y, int yt, hipStream_t s,
-            void *ext) {
+            void *ext, const void *ev) {
   switch (yt) {
-    case SPOPS_VALUE_F64: spmm_launch<VC, XC, SPOPS_VALUE_F64, RP>(A, k, alpha, X, beta, Y, s, ext); break;
-    case SPOPS_VALUE_F32: spmm_launch<VC, XC, SPOPS_VALUE_F32, RP>(A, k, alpha, X, beta, Y, s, ext); break;
-    default: spmm_launch<VC, XC, SPOPS_VALUE_BF16, RP>(A, k, alpha, X, beta, Y, s, ext); break;
+    case SPOPS_VALUE_F64: spmm_launch<VC, XC, SPOPS_VALUE_F64, RP>(A, k, alpha, X, beta, Y, s, ext, ev); break;
+    case SPOPS_VALUE_F32: spmm_launch<VC, XC, SPOPS_VALUE_F32, RP>(A, k, alpha, X, beta, Y, s, ext, ev); break;
+    default: spmm_launch<VC, XC, SPOPS_VALUE_BF16, RP>(A, k, alpha, X, beta, Y, s, ext, ev); break;
   }
 }
 
 template <int VC, typename RP>
 void spmm_x(const spops_matrix_s &A, int k, double alpha, const void *X, int xt, double beta, void *Y, int yt,
-            hipStream_t s, void *ext) {
+            hipStream_t s, void *ext, const void *ev) {
   switch (xt) {
-    case SPOPS_VALUE_F64: spmm_y<VC, SPOPS_VALUE_F64, RP>(A, k, alpha, X, beta, Y, yt, s, ext); break;
-    case SPOPS_VALUE_F32: spmm_y<VC, SPOPS_VALUE_F32, RP>(A, k, alpha, X, beta, Y, yt, s, ext); break;
-    default: spmm_y<VC, SPOPS_VALUE_BF16, RP>(A, k, alpha, X, beta, Y, yt, s, ext); break;
+    case SPOPS_VALUE_F64: spmm_y<VC, SPOPS_VALUE_F64, RP>(A, k, alpha, X, beta, Y, yt, s, ext, ev); break;
+    case SPOPS_VALUE_F32: spmm_y<VC, SPOPS_VALUE_F32, RP>(A, k, alpha, X, beta, Y, yt, s, ext, ev); break;
+    default: spmm_y<VC, SPOPS_VALUE_BF16, RP>(A, k, alpha, X, beta, Y, yt, s, ext, ev); break;
   }
 }
 
 template <typename RP>
 void spmm_v(const spops_matrix_s &A, int k, double alpha, const void *X, int xt, double beta, void *Y, int yt,
-            hipStream_t s, void *ext) {
+            hipStream_t s, void *ext, const void *ev) {
   switch (A.value_type) {
-    case SPOPS_VALUE_F64: spmm_x<SPOPS_VALUE_F64, RP>(A, k, alpha, X, xt, beta, Y, yt, s, ext); break;
-    case SPOPS_VALUE_F32: spmm_x<SPOPS_VALUE_F32, RP>(A, k, alpha, X, xt, beta, Y, yt, s, ext); break;
-    default: spmm_x<SPOPS_VALUE_BF16, RP>(A, k, alpha, X, xt, beta, Y, yt, s, ext); break;
+    case SPOPS_VALUE_F64: spmm_x<SPOPS_VALUE_F64, RP>(A, k, alpha, X, xt, beta, Y, yt, s, ext, ev); break;
+    case SPOPS_VALUE_F32: spmm_x<SPOPS_VALUE_F32, RP>(A, k, alpha, X, xt, beta, Y, yt, s, ext, ev); break;
+    default: spmm_x<SPOPS_VALUE_BF16, RP>(A, k, alpha, X, xt, beta, Y, yt, s, ext, ev); break;
   }
 }
 
@@ -423,14 +489,33 @@ void sddmm_launch_vec(const spops_matrix_s &A, int k, const void *G, const void 
   SPOPS_HIP_CHECK(hipGetLastError());
 }
 
+template <int GC, int XC, int OC, typename RP, int VEC>
+void sddmm_rows_launch(const spops_matrix_s &A, int k, const void *G, const void *X, void *out, hipStream_t s) {
+  const auto &C = A.csr;
+  const int L = lanes_for(k, VEC);
+  const int64_t per_block = (int64_t)(kBlock / kWave) * (kWave / L);
+  hipLaunchKernelGGL((k_sddmm_rows<GC, XC, OC, RP, VEC>), dim3(nblk(C.nsdtasks, (int)per_block)), dim3(kBlock), 0, s,
+                     C.rowptr.as<RP>(), C.col.as<int>(), C.sdtasks.as<int2>(), C.nsdtasks, k, L,
+                     static_cast<const typename Ty<GC>::T *>(G), static_cast<const typename Ty<XC>::T *>(X),
+                     static_cast<typename Ty<OC>::T *>(out));
+  SPOPS_HIP_CHECK(hipGetLastError());
+}
+
 template <int GC, int XC, int OC>
 void sddmm_launch(const spops_matrix_s &A, int k, const void *G, const void *X, void *out, hipStream_t s) {
   if (A.nnz == 0) return;
+  // by rows when one lane group covers all k columns with 16-byte accesses of X (each row of
+  // G loaded once, V elements per lane); else entry-parallel
+  constexpr int V = 16 / (int)sizeof(typename Ty<XC>::T);
+  if (k % V == 0 && k <= kWave * V && aligned_to(X, 16) && aligned_to(G, V * sizeof(typename Ty<GC>::T))) {
+    if (A.csr.rowptr64) sddmm_rows_launch<GC, XC, OC, long, V>(A, k, G, X, out, s);
+    else sddmm_rows_launch<GC, XC, OC, int, V>(A, k, G, X, out, s);
+    return;
+  }
   if (A.csr.erow.p == nullptr) {
     if (A.csr.rowptr64) build_entry_rows<long>(A, s);
     else build_entry_rows<int>(A, s);
   }
-  constexpr int V = 16 / (int)sizeof(typename Ty<XC>::T);
   if (k % V == 0 && aligned_to(X, 16) && aligned_to(G, V * sizeof(typename Ty<GC>::T)))
     sddmm_launch_vec<GC, XC, OC, V>(A, k, G, X, out, s);
   else
@@ -490,7 +575,7 @@ void convert_values(const void *src, int st, const void *perm, bool perm64, int6
 }  // namespace
 
 void launch_spmm(const spops_matrix_s &A, int64_t k, double alpha, const void *X, int xt, double beta, void *Y, int yt,
-                 hipStream_t s, void *ext_part) {
+                 hipStream_t s, void *ext_part, const void *ext_val) {
   require_csr(&A);
   require_type(xt, "X");
   require_type(yt, "Y");
@@ -498,8 +583,8 @@ void launch_spmm(const spops_matrix_s &A, int64_t k, double alpha, const void *X
   SPOPS_REQUIRE((uint64_t)A.n_cols * (uint64_t)k < ((uint64_t)1 << 40) &&
                     (uint64_t)A.n_rows * (uint64_t)k < ((uint64_t)1 << 40),
                 SPOPS_ERROR_LIMIT, "n * k too large");
-  if (A.csr.rowptr64) spmm_v<long>(A, (int)k, alpha, X, xt, beta, Y, yt, s, ext_part);
-  else spmm_v<int>(A, (int)k, alpha, X, xt, beta, Y, yt, s, ext_part);
+  if (A.csr.rowptr64) spmm_v<long>(A, (int)k, alpha, X, xt, beta, Y, yt, s, ext_part, ext_val);
+  else spmm_v<int>(A, (int)k, alpha, X, xt, beta, Y, yt, s, ext_part, ext_val);
 }
 
 }  // namespace spops
@@ -541,7 +626,7 @@ extern "C" int spops_spmm(spops_matrix A, int64_t k, double alpha, const void *d
     SPOPS_REQUIRE((d_X != nullptr || A->n_cols == 0) && (d_Y != nullptr || A->n_rows == 0), SPOPS_ERROR_INVALID_VALUE,
                   "null X or Y");
     const int t = A->value_type == SPOPS_VALUE_F64 ? SPOPS_VALUE_F64 : SPOPS_VALUE_F32;
-    launch_spmm(*A, k, alpha, d_X, t, beta, d_Y, t, static_cast<hipStream_t>(stream), nullptr);
+    launch_spmm(*A, k, alpha, d_X, t, beta, d_Y, t, static_cast<hipStream_t>(stream), nullptr, nullptr);
     return SPOPS_SUCCESS;
   });
 }
@@ -552,7 +637,22 @@ extern "C" int spops_spmm_typed(spops_matrix A, int64_t k, double alpha, const v
     require_csr(A);
     SPOPS_REQUIRE((d_X != nullptr || A->n_cols == 0) && (d_Y != nullptr || A->n_rows == 0), SPOPS_ERROR_INVALID_VALUE,
                   "null X or Y");
-    launch_spmm(*A, k, alpha, d_X, x_type, beta, d_Y, y_type, static_cast<hipStream_t>(stream), nullptr);
+    launch_spmm(*A, k, alpha, d_X, x_type, beta, d_Y, y_type, static_cast<hipStream_t>(stream), nullptr, nullptr);
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_spmm_values(spops_matrix A, const void *d_values, int64_t k, double alpha, const void *d_X,
+                                 int x_type, double beta, void *d_Y, int y_type, void *stream) {
+  return guarded([&]() {
+    require_csr(A);
+    SPOPS_REQUIRE(d_values != nullptr || A->nnz == 0, SPOPS_ERROR_INVALID_VALUE, "null values");
+    SPOPS_REQUIRE(!A->csr.tperm64, SPOPS_ERROR_LIMIT,
+                  "spops_spmm_values: a transposed handle of >= 2^31 entries (use spops_csr_set_values)");
+    SPOPS_REQUIRE((d_X != nullptr || A->n_cols == 0) && (d_Y != nullptr || A->n_rows == 0), SPOPS_ERROR_INVALID_VALUE,
+                  "null X or Y");
+    launch_spmm(*A, k, alpha, d_X, x_type, beta, d_Y, y_type, static_cast<hipStream_t>(stream), nullptr,
+                A->nnz > 0 ? d_values : nullptr);
     return SPOPS_SUCCESS;
   });
 }
@@ -573,7 +673,7 @@ extern "C" int spops_spmm_typed_noalloc(spops_matrix A, int64_t k, double alpha,
                   need, A->csr.nlseg, (long long)k, kSpmmScratchBytes);
     void *part = A->csr.nlseg > 0 ? spmm_scratch_address() : nullptr;
     launch_spmm(*A, k, alpha, d_X, x_type, beta, d_Y, y_type, static_cast<hipStream_t>(stream),
-                part);
+                part, nullptr);
     return SPOPS_SUCCESS;
   });
 }

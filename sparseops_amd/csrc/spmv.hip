@@ -417,11 +417,14 @@ __global__ __launch_bounds__(kBlock) void k_slice_reduce(const double *__restric
 struct SweepArgs {
   const uint32_t *word;
   const void *val;
-  const int *gdesc, *wgrp, *rstart;
-  void *part;                 // H > 1: y type [H][n_rows], summed by k_sweep_reduce
+  const int *gdesc, *wgrp;
+  const int2 *wblk;           // per workgroup: {row block, share}
+  const int4 *binfo;          // per row block: {first position, rows, shares, first plane element}
+  const int *rowidx;          // position -> matrix row (nullptr: the identity)
+  void *part;                 // blocks of > 1 share: y-type planes, plane h at first + h * stride
   const void *x;
   void *y;
-  int n_rows, n_cols, rb, H;
+  int n_rows, n_cols, rb, pstride;   // pstride 0: a block's planes are `rows` apart
   double alpha, beta;
 };
 
@@ -437,8 +440,9 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep_spmv(const SweepArgs a) {
   __shared__ double acc[kSweepMaxRows];
   const int rb = a.rb;
   const uint32_t rmask = (1u << rb) - 1u;
-  const int blk = (int)blockIdx.x / a.H, h = (int)blockIdx.x % a.H;
-  const int r0 = a.rstart[blk], rows = a.rstart[blk + 1] - r0;
+  const int2 wb = a.wblk[blockIdx.x];
+  const int4 bi = a.binfo[wb.x];
+  const int r0 = bi.x, rows = bi.y, h = wb.y;
   for (int i = (int)threadIdx.x; i < rows; i += kSweepBlock) acc[i] = 0.0;
   __syncthreads();
   const int g0 = a.wgrp[blockIdx.x], g1 = a.wgrp[blockIdx.x + 1];
@@ -496,15 +500,16 @@ __global__ __launch_bounds__(kSweepBlock) void k_sweep_spmv(const SweepArgs a) {
     }
   }
   __syncthreads();
-  if (a.H == 1) {
+  if (bi.z == 1) {
     X *y = static_cast<X *>(a.y);
     for (int i = (int)threadIdx.x; i < rows; i += kSweepBlock) {
+      const int r = a.rowidx ? a.rowidx[r0 + i] : r0 + i;
       double o = a.alpha * acc[i];
-      if (a.beta != 0.0) o += a.beta * (double)y[r0 + i];
-      y[r0 + i] = (X)o;
+      if (a.beta != 0.0) o += a.beta * (double)y[r];
+      y[r] = (X)o;
     }
   } else {
-    X *mine = static_cast<X *>(a.part) + (long)h * a.n_rows + r0;
+    X *mine = static_cast<X *>(a.part) + bi.w + (long)h * (a.pstride ? a.pstride : rows);
     for (int i = (int)threadIdx.x; i < rows; i += kSweepBlock) mine[i] = (X)acc[i];
   }
 }
@@ -553,6 +558,26 @@ __global__ __launch_bounds__(kBlock) void k_sweep_reduce4(const f32x4 *__restric
   o.z = (float)o2;
   o.w = (float)o3;
   y[r] = o;
+}
+
+// The same for a layout whose blocks have their own share counts and planes: one thread per
+// row of a block of more than one share (task: {block, first local row, rows}); the row's
+// planes are `rows` apart, added in order h = 0..shares-1 in double.
+template <typename X>
+__global__ __launch_bounds__(kBlock) void k_sweep_gather(const int4 *__restrict__ tasks, const int4 *__restrict__ binfo,
+                                                        const X *__restrict__ part, const int *__restrict__ rowidx,
+                                                        X *__restrict__ y, double alpha, double beta) {
+  const int4 t = tasks[blockIdx.x];
+  if ((int)threadIdx.x >= t.z) return;
+  const int4 bi = binfo[t.x];
+  const int local = t.y + (int)threadIdx.x;
+  const X *p = part + bi.w + local;
+  double sum = 0.0;
+  for (int h = 0; h < bi.z; ++h) sum += (double)__builtin_nontemporal_load(p + (long)h * bi.y);
+  const int r = rowidx ? rowidx[bi.x + local] : bi.x + local;
+  double o = alpha * sum;
+  if (beta != 0.0) o += beta * (double)y[r];
+  y[r] = (X)o;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -772,21 +797,29 @@ void build_csr_plan(const std::vector<int64_t> &rp, int64_t n_rows, std::vector<
 }
 
 void upload_csr_plan(CsrDev &P, const std::vector<int64_t> &rp, int64_t n_rows) {
-  std::vector<int2> tasks, segs;
-  std::vector<int> longs, order, slongs, segoff{0};
+  std::vector<int2> tasks, segs, sdt;
+  std::vector<int> longs, order, slongs, segoff{0}, win;
   build_csr_plan(rp, n_rows, tasks, longs);
   for (int64_t w0 = 0; w0 < n_rows; w0 += kSpmmWindow) {
     const size_t at = order.size();
+    win.clear();
     for (int64_t r = w0; r < std::min(n_rows, w0 + kSpmmWindow); ++r) {
       const int64_t len = rp[r + 1] - rp[r];
+      if (len > 0) win.push_back((int)r);
       if (len <= kSpmmLongRow) { order.push_back((int)r); continue; }
       slongs.push_back((int)r);
       for (int64_t off = 0; off < len; off += kSpmmSegment) segs.push_back(make_int2((int)slongs.size() - 1, (int)off));
       segoff.push_back((int)segs.size());
     }
-    std::stable_sort(order.begin() + at, order.end(),
-                     [&](int a, int b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
+    auto longer = [&](int a, int b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; };
+    std::stable_sort(order.begin() + at, order.end(), longer);
+    // SDDMM: every row with entries in chunks of at most kSddmmChunk, longest rows first
+    std::stable_sort(win.begin(), win.end(), longer);
+    for (int r : win)
+      for (int64_t off = 0; off < rp[r + 1] - rp[r]; off += kSddmmChunk) sdt.push_back(make_int2(r, (int)off));
   }
+  P.sdtasks.upload(sdt);
+  P.nsdtasks = (int64_t)sdt.size();
   P.tasks.upload(tasks);
   P.longs.upload(longs);
   P.sorder.upload(order);
@@ -815,6 +848,7 @@ int64_t g_dense_products = (int64_t)1 << 22;
 int64_t g_sweep_mode = 0;
 int64_t g_sweep_workgroups = kSweepTargetWorkgroups;
 int64_t g_sweep_max_rows = kSweepMaxRows;
+int64_t g_sweep_split = 0;
 int64_t g_bsr_mfma = 1;   // 0 never, 1 blocks past the VALU kernel's limit, 2 every dense block >= 8 x 8
 
 bool bsr_use_mfma(int64_t bR, int64_t bC) {
@@ -845,21 +879,21 @@ static void radix_sort_by(std::vector<int32_t> &idx, const std::vector<uint32_t>
   }
 }
 
-// The swept layout (see SweepDev).  pos[r] is the SELL position of original row r, len[r] its length.
-// With slot_src, the source entry of every slot (-1 for padding) is kept on the device, so
-// that new values of the source arrays can be copied into the layout (CSR handles).
-static void build_sweep_layout(spops_matrix_s &M, const RowSource &src, const std::vector<int64_t> &pos,
-                               const std::vector<int64_t> &len, int vt, bool slot_src = false) {
-  auto &W = M.sweep;
-  const int64_t n = M.n_rows, m = M.n_cols;
-  // row blocks: H workgroups per block and about g_sweep_workgroups / H blocks of equal entry
-  // counts, each at most g_sweep_max_rows rows
-  const int64_t rmax = g_sweep_max_rows;
+// The blocks of a swept layout: positions (row order of the blocks), block starts, shares.
+struct SweepPlan {
+  std::vector<int32_t> order;      // position -> row; empty: the identity
+  std::vector<int32_t> rstart{0};  // per block + 1: first position
+  std::vector<int> shares;         // per block
+  int64_t split = 0;               // 0: one class
+};
+
+// One class: about btarget blocks of consecutive rows with equal entry counts (each at most
+// rmax rows), H shares each.
+static SweepPlan sweep_plan_one_class(const std::vector<int64_t> &cum, int64_t n, int64_t rmax, int64_t wgs) {
+  SweepPlan P;
   const int64_t bmin = std::max<int64_t>(1, (n + rmax - 1) / rmax);
-  const int H = (int)std::min<int64_t>(kSweepMaxSplit, std::max<int64_t>(1, g_sweep_workgroups / bmin));
-  const int64_t btarget = std::max<int64_t>(bmin, g_sweep_workgroups / H);
-  std::vector<int64_t> cum(n + 1, 0);
-  for (int64_t r = 0; r < n; ++r) cum[r + 1] = cum[r] + len[r];
+  const int H = (int)std::min<int64_t>(kSweepMaxSplit, std::max<int64_t>(1, wgs / bmin));
+  const int64_t btarget = std::max<int64_t>(bmin, wgs / H);
   // greedy cut: a block takes rows while it has at most `cap` entries and rmax rows (and at
   // least one row); the smallest cap that needs at most btarget blocks, by bisection (the
   // greedy count falls as cap grows, and cap = nnz needs bmin <= btarget blocks)
@@ -880,9 +914,73 @@ static void build_sweep_layout(spops_matrix_s &M, const RowSource &src, const st
     if (cut(mid, nullptr) <= btarget) hi = mid;
     else lo = mid + 1;
   }
-  std::vector<int32_t> rstart{0};
-  cut(lo, &rstart);
-  if (n == 0) rstart.push_back(0);
+  cut(lo, &P.rstart);
+  if (n == 0) P.rstart.push_back(0);
+  P.shares.assign(P.rstart.size() - 1, H);
+  return P;
+}
+
+// Two classes: rows of at most T entries, then the longer rows (each run in row order), cut
+// into blocks of about rmax rows; shares handed out one at a time to the block with the
+// largest load (entries per share + rows) until `wgs` workgroups are used.
+static SweepPlan sweep_plan_two_class(const std::vector<int64_t> &len, int64_t n, int64_t rmax, int64_t wgs,
+                                      int64_t T) {
+  SweepPlan P;
+  P.split = T;
+  P.order.reserve((size_t)n);
+  for (int pass = 0; pass < 2; ++pass)
+    for (int64_t r = 0; r < n; ++r)
+      if ((len[r] <= T) == (pass == 0)) P.order.push_back((int32_t)r);
+  std::vector<int64_t> E;
+  int64_t p0 = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    int64_t nc = 0;
+    for (int64_t r = 0; r < n; ++r) nc += (len[r] <= T) == (pass == 0);
+    const int64_t nb = (nc + rmax - 1) / rmax;
+    for (int64_t k = 0; k < nb; ++k) {
+      const int64_t a = p0 + nc * k / nb, b = p0 + nc * (k + 1) / nb;
+      int64_t e = 0;
+      for (int64_t q = a; q < b; ++q) e += len[P.order[(size_t)q]];
+      P.rstart.push_back((int32_t)b);
+      E.push_back(e);
+    }
+    p0 += nc;
+  }
+  if (n == 0) { P.rstart.push_back(0); E.push_back(0); }
+  const size_t nb = E.size();
+  P.shares.assign(nb, 1);
+  auto load = [&](size_t b) {
+    return (double)E[b] / P.shares[b] + (double)(P.rstart[b + 1] - P.rstart[b]);
+  };
+  std::vector<std::pair<double, size_t>> heap;
+  for (size_t b = 0; b < nb; ++b) heap.push_back({load(b), b});
+  std::make_heap(heap.begin(), heap.end());
+  for (int64_t used = (int64_t)nb; used < wgs && !heap.empty(); ++used) {
+    std::pop_heap(heap.begin(), heap.end());
+    const size_t b = heap.back().second;
+    heap.pop_back();
+    if (P.shares[b] >= kSweepMaxShares || P.shares[b] * (int64_t)kSweepGroup >= E[b]) { --used; continue; }
+    ++P.shares[b];
+    heap.push_back({load(b), b});
+    std::push_heap(heap.begin(), heap.end());
+  }
+  return P;
+}
+
+// The swept layout (see SweepDev).  pos[r] is the SELL position of original row r, len[r] its length.
+// With slot_src, the source entry of every slot (-1 for padding) is kept on the device, so
+// that new values of the source arrays can be copied into the layout (CSR handles).
+static void build_sweep_layout(spops_matrix_s &M, const RowSource &src, const std::vector<int64_t> &pos,
+                               const std::vector<int64_t> &len, int vt, bool slot_src = false) {
+  auto &W = M.sweep;
+  const int64_t n = M.n_rows, m = M.n_cols;
+  const int64_t rmax = g_sweep_max_rows;
+  std::vector<int64_t> cum(n + 1, 0);
+  for (int64_t r = 0; r < n; ++r) cum[r + 1] = cum[r] + len[r];
+  const SweepPlan plan = g_sweep_split > 0 ? sweep_plan_two_class(len, n, rmax, g_sweep_workgroups, g_sweep_split)
+                                            : sweep_plan_one_class(cum, n, rmax, g_sweep_workgroups);
+  const std::vector<int32_t> &rstart = plan.rstart;
+  const std::vector<int32_t> &order = plan.order;
   const int64_t n_rb = (int64_t)rstart.size() - 1;
   int64_t maxrows = 1;
   for (int64_t b = 0; b < n_rb; ++b) maxrows = std::max<int64_t>(maxrows, rstart[b + 1] - rstart[b]);
@@ -903,15 +1001,17 @@ static void build_sweep_layout(spops_matrix_s &M, const RowSource &src, const st
   std::vector<int32_t> bsrc, ssrc;
   for (int64_t b = 0; b < n_rb; ++b) {
     bcol.clear(); bword.clear(); bval.clear(); bsrc.clear();
-    for (int64_t r = rstart[b]; r < rstart[b + 1]; ++r)
+    for (int64_t q = rstart[b]; q < rstart[b + 1]; ++q) {
+      const int64_t r = order.empty() ? q : order[(size_t)q];
       for (int64_t k = 0; k < len[r]; ++k) {
         int64_t c; double v;
         src.entry(pos[r], k, c, v);
         bcol.push_back((uint32_t)c);
-        bword.push_back((uint32_t)(((uint64_t)c & (((uint64_t)1 << cbits) - 1)) << rb) | (uint32_t)(r - rstart[b]));
+        bword.push_back((uint32_t)(((uint64_t)c & (((uint64_t)1 << cbits) - 1)) << rb) | (uint32_t)(q - rstart[b]));
         bval.push_back(v);
         if (slot_src) bsrc.push_back((int32_t)src.entry_index(pos[r], k));
       }
+    }
     idx.resize(bcol.size());
     std::iota(idx.begin(), idx.end(), 0);
     radix_sort_by(idx, bcol, colbits);                             // by column; rows ascend among equals
@@ -942,11 +1042,36 @@ static void build_sweep_layout(spops_matrix_s &M, const RowSource &src, const st
   const int64_t ngroups = (int64_t)gdesc.size();
   SPOPS_REQUIRE(ngroups * kSweepGroup < (int64_t)INT32_MAX, SPOPS_ERROR_LIMIT,
                 "the swept layout holds >= 2^31 entry slots");
-  const int Hu = ngroups == 0 ? 1 : H;
-  std::vector<int32_t> wgrp((size_t)(n_rb * Hu + 1));
-  for (int64_t b = 0; b < n_rb; ++b)
-    for (int h = 0; h < Hu; ++h) wgrp[(size_t)(b * Hu + h)] = (int32_t)(bg[b] + (bg[b + 1] - bg[b]) * h / Hu);
-  wgrp[(size_t)(n_rb * Hu)] = (int32_t)ngroups;
+  // workgroups (block-major, shares consecutive) and the planes of the blocks of several shares
+  const bool one_class = order.empty();
+  std::vector<int32_t> wgrp;
+  std::vector<int2> wblk;
+  std::vector<int4> binfo((size_t)n_rb), rtasks;
+  int64_t pel = 0;
+  int maxH = 1;
+  for (int64_t b = 0; b < n_rb; ++b) {
+    const int Hb = ngroups == 0 ? 1 : plan.shares[(size_t)b];
+    const int32_t rows = rstart[b + 1] - rstart[b];
+    maxH = std::max(maxH, Hb);
+    for (int h = 0; h < Hb; ++h) {
+      wgrp.push_back((int32_t)(bg[b] + (bg[b + 1] - bg[b]) * h / Hb));
+      wblk.push_back(make_int2((int)b, h));
+    }
+    int64_t first = 0;
+    if (Hb > 1) {
+      if (one_class) {
+        first = rstart[b];                                         // planes [H][n_rows]
+      } else {
+        first = pel;                                               // planes [shares][rows] per block
+        pel += (int64_t)Hb * rows;
+        for (int32_t l0 = 0; l0 < rows; l0 += kBlock) rtasks.push_back(make_int4((int)b, l0, std::min(kBlock, rows - l0), 0));
+      }
+    }
+    binfo[(size_t)b] = make_int4(rstart[b], rows, Hb, (int)first);
+  }
+  if (one_class && maxH > 1) pel = (int64_t)maxH * n;
+  SPOPS_REQUIRE(pel < (int64_t)INT32_MAX, SPOPS_ERROR_LIMIT, "the swept layout's partial planes hold >= 2^31 elements");
+  wgrp.push_back((int32_t)ngroups);
   std::vector<uint8_t> val;
   val.reserve(dval.size() * value_bytes(vt));
   for (double v : dval) pack_values(val, vt, v);
@@ -955,15 +1080,24 @@ static void build_sweep_layout(spops_matrix_s &M, const RowSource &src, const st
   if (slot_src) W.src.upload(ssrc);
   W.gdesc.upload(gdesc);
   W.wgrp.upload(wgrp);
-  W.rstart.upload(rstart);
-  if (Hu > 1) W.part.alloc((size_t)Hu * (size_t)n * xy_bytes(vt));
+  W.wblk.upload(wblk);
+  W.binfo.upload(binfo);
+  if (!one_class) {
+    W.rowidx.upload(order);
+    W.rtasks.upload(rtasks);
+  }
+  if (pel > 0) W.part.alloc((size_t)pel * xy_bytes(vt));
   W.rb = rb;
-  W.H = Hu;
+  W.H = maxH;
   W.n_rb = (int)n_rb;
   W.ngroups = (int)ngroups;
+  W.nwg = (int)wblk.size();
+  W.nrtasks = (int)rtasks.size();
+  W.split = plan.split;
+  W.plane_elems = pel;
   M.slots = ngroups * kSweepGroup;
-  M.dev_bytes = W.word.bytes + W.val.bytes + W.src.bytes + W.gdesc.bytes + W.wgrp.bytes + W.rstart.bytes +
-                W.part.bytes;
+  M.dev_bytes = W.word.bytes + W.val.bytes + W.src.bytes + W.gdesc.bytes + W.wgrp.bytes + W.wblk.bytes +
+                W.binfo.bytes + W.rowidx.bytes + W.rtasks.bytes + W.part.bytes;
 }
 
 // Column slices of the sliced layout: P from the x footprint of one XCD's rows (128-byte lines
@@ -1246,19 +1380,25 @@ static void launch_spmv_vt(const spops_matrix_s &M, double alpha, const void *x,
     wa.val = W.val.p;
     wa.gdesc = W.gdesc.as<int>();
     wa.wgrp = W.wgrp.as<int>();
-    wa.rstart = W.rstart.as<int>();
+    wa.wblk = W.wblk.as<int2>();
+    wa.binfo = W.binfo.as<int4>();
+    wa.rowidx = W.split > 0 ? W.rowidx.as<int>() : nullptr;
     wa.part = W.part.p;
     wa.x = x;
     wa.y = y;
     wa.n_rows = (int)M.n_rows;
     wa.n_cols = (int)M.n_cols;
     wa.rb = W.rb;
-    wa.H = W.H;
+    wa.pstride = W.split > 0 ? 0 : (int)M.n_rows;
     wa.alpha = alpha;
     wa.beta = beta;
-    const dim3 grid((unsigned)(W.n_rb * W.H));
-    hipLaunchKernelGGL((k_sweep_spmv<VT>), grid, dim3(kSweepBlock), 0, s, wa);
-    if (W.H > 1) {
+    hipLaunchKernelGGL((k_sweep_spmv<VT>), dim3((unsigned)W.nwg), dim3(kSweepBlock), 0, s, wa);
+    if (W.split > 0 && W.nrtasks > 0) {
+      SPOPS_HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL((k_sweep_gather<typename T::X>), dim3((unsigned)W.nrtasks), dim3(kBlock), 0, s,
+                         W.rtasks.as<int4>(), W.binfo.as<int4>(), W.part.as<typename T::X>(), W.rowidx.as<int>(), yy,
+                         alpha, beta);
+    } else if (W.split == 0 && W.H > 1) {
       SPOPS_HIP_CHECK(hipGetLastError());
       if (std::is_same<typename T::X, float>::value && M.n_rows % 4 == 0 && ((uintptr_t)W.part.p % 16) == 0 &&
           ((uintptr_t)y % 16) == 0) {
@@ -1704,6 +1844,10 @@ extern "C" int spops_set_tuning(int key, int64_t value, int64_t *old) {
                     "swept row block size must be in [1, %d]", kSweepMaxRows);
       if (old) *old = g_sweep_max_rows;
       g_sweep_max_rows = value;
+    } else if (key == SPOPS_TUNE_SWEEP_SPLIT) {
+      SPOPS_REQUIRE(value >= 0, SPOPS_ERROR_INVALID_VALUE, "swept split length must be >= 0");
+      if (old) *old = g_sweep_split;
+      g_sweep_split = value;
     } else if (key == SPOPS_TUNE_BSR_MFMA) {
       SPOPS_REQUIRE(value >= 0 && value <= 2, SPOPS_ERROR_INVALID_VALUE, "BSR MFMA mode must be 0, 1 or 2");
       if (old) *old = g_bsr_mfma;
@@ -1735,12 +1879,15 @@ extern "C" int spops_matrix_layout(spops_matrix A, int which, void *host_out) {
   return guarded([&]() {
     SPOPS_REQUIRE(A != nullptr && host_out != nullptr, SPOPS_ERROR_INVALID_VALUE, "null argument");
     SPOPS_REQUIRE(A->format == SPOPS_FORMAT_SELL ||
-                      (which == 2 && (A->format == SPOPS_FORMAT_ELL || A->format == SPOPS_FORMAT_CSR)),
+                      (which >= 2 && (A->format == SPOPS_FORMAT_ELL || A->format == SPOPS_FORMAT_CSR)),
                   SPOPS_ERROR_NOT_SUPPORTED, "layout arrays exist for SELL only");
     if (which == 0) std::memcpy(host_out, A->sell.ref_reorder.data(), A->sell.ref_reorder.size() * 4);
     else if (which == 1) std::memcpy(host_out, A->sell.ref_chunk_len.data(), A->sell.ref_chunk_len.size() * 4);
     else if (which == 2) {
       const int64_t d[4] = {A->sweep.rb, A->sweep.H, A->sweep.n_rb, A->sweep.ngroups};
+      std::memcpy(host_out, d, sizeof(d));
+    } else if (which == 3) {
+      const int64_t d[4] = {A->sweep.split, A->sweep.nwg, A->sweep.plane_elems, A->sweep.nrtasks};
       std::memcpy(host_out, d, sizeof(d));
     }
     else throw_error(SPOPS_ERROR_INVALID_VALUE, "unknown layout array %d", which);

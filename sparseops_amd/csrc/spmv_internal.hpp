@@ -12,6 +12,7 @@ constexpr int64_t kCsrLongRow = 2048;  // longer rows: one workgroup each
 constexpr int64_t kSpmmLongRow = 256;  // SpMM: longer rows are summed in segments
 constexpr int64_t kSpmmSegment = 512;  // SpMM: entries per segment of a long row
 constexpr int64_t kSpmmWindow = 4096;  // SpMM: short rows are ordered by length inside windows of this many rows
+constexpr int64_t kSddmmChunk = 64;    // SDDMM by rows: entries of a row per lane-group task
 constexpr int kBsrMaxBlock = 256;      // bR * bC limit of the wave-per-block-row VALU kernel
 // MFMA BSR kernel: blocks with bR >= 8, bC >= 8, bC % 4 == 0, at most these sizes
 constexpr int kBsrMfmaMinDim = 8, kBsrMfmaMaxRows = 64, kBsrMfmaMaxBlock = 4096;
@@ -33,6 +34,8 @@ struct CsrDev {
   DevBuf lsegoff;               // SpMM: first segment of each long row (nslong + 1 ints)
   mutable DevBuf spmm_part;     // SpMM: partial sums of the segments (grown on demand)
   mutable DevBuf erow;          // SDDMM: row of every entry (built on first use)
+  DevBuf sdtasks;               // SDDMM by rows: int2 {row, first entry in the row}, chunks of kSddmmChunk
+  int64_t nsdtasks = 0;
   bool rowptr64 = false, tperm64 = false;
   int ntasks = 0, nlong = 0, nshort = 0, nslong = 0, nlseg = 0;
 };
@@ -65,36 +68,53 @@ struct SliceDev {
 };
 
 // Swept layout (SELL-C-sigma, SELL-C-R and ELL of matrices whose rows gather from all over x):
-// rows are cut into blocks of consecutive rows with about equal entry counts, at most
-// kSweepMaxRows rows (whose FP64 sums fill the 160 KiB of LDS of a CU); a workgroup adds the
-// products of its block into LDS, and the block's entries are stored sorted by column, so the
-// 64 lanes of a gather instruction read 64 consecutive entries of one sweep over x and fall on
-// few cache lines.  An entry is one uint32 word, (column - cb * 2^(32 - rb)) << rb | row in
-// the block (rb: bits of the largest block), plus its value; cb, the column block, comes from
-// the entry's group.  Entries are packed in groups of kSweepGroup (one column block per
-// group, the last group of a column block padded), lane-interleaved so that each 128-bit load
-// of a wave reads one contiguous block and gather u of lane l takes the group's sorted entry
-// 64 u + l.  H workgroups share a row block (contiguous group ranges, H from the CU count);
-// with H > 1 each writes a partial plane (in the y type) and a second pass adds the H planes
-// in order h = 0..H-1 and writes y.  (An in-kernel "last workgroup adds" handoff needs
-// agent-scope release/acquire, which on this chip writes back and invalidates the XCD's L2
-// and with it the cached x: measured slower.)
+// rows are cut into blocks of at most kSweepMaxRows rows (whose FP64 sums fill the 160 KiB of
+// LDS of a CU); a workgroup adds the products of its block into LDS, and the block's entries
+// are stored sorted by column, so the 64 lanes of a gather instruction read 64 consecutive
+// entries of one sweep over x and fall on few cache lines.  An entry is one uint32 word,
+// (column - cb * 2^(32 - rb)) << rb | row in the block (rb: bits of the largest block), plus
+// its value; cb, the column block, comes from the entry's group.  Entries are packed in
+// groups of kSweepGroup (one column block per group, the last group of a column block
+// padded), lane-interleaved so that each 128-bit load of a wave reads one contiguous block
+// and gather u of lane l takes the group's sorted entry 64 u + l.  A block's groups are cut
+// into shares (contiguous group ranges), one workgroup each; a block of one share writes y,
+// a block of more shares writes one partial plane per share (in the y type) and a second
+// pass adds them in share order.  Two ways to form the blocks:
+//  * one class: consecutive rows with about equal entry counts, H shares each (H from the CU
+//    count), planes [H][n_rows];
+//  * two classes (split length T > 0): the rows of at most T entries and the longer rows form
+//    separate runs (each in row order), cut into blocks of about kSweepMaxRows rows.  A block
+//    of long rows holds many entries per 128-byte line of x, so its sweep fetches each line
+//    of x for many entries; such blocks get as many shares as their entry counts ask for
+//    (planes per block, [shares][rows]), and a row map sends each position to its row.
+// The second form cuts the gathered 64-byte pieces of x by ~40 % on a power-law matrix
+// (n = 2^21, mean 15.5) but measured slower there (108 against 86 us): the shares' entry counts
+// are less even than the one-class cut's, and blocks of one share sweep all of x, which one
+// XCD's L2 does not hold.  So one class is the default.  (An in-kernel "last workgroup adds"
+// handoff needs agent-scope release/acquire, which on this chip writes back and invalidates
+// the XCD's L2 and with it the cached x: measured slower.)
 constexpr int kSweepMaxRows = 20480;          // 20480 x 8 bytes = 160 KiB of LDS
 constexpr int64_t kSweepRowsTypical = 16384;  // rows of a block for the layout choice
 constexpr int kSweepBlock = 1024;             // threads per workgroup (one workgroup per CU)
 constexpr int kSweepLaneEntries = 8;
 constexpr int kSweepGroup = kWave * kSweepLaneEntries;
 constexpr int kSweepTargetWorkgroups = 256;   // one per CU (default of g_sweep_workgroups)
-constexpr int kSweepMaxSplit = 16;            // H limit
+constexpr int kSweepMaxSplit = 16;            // shares of a one-class block
+constexpr int kSweepMaxShares = 64;           // shares of a two-class block
 
 struct SweepDev {
   DevBuf word, val;           // per group: kSweepGroup words / values, lane-interleaved
   DevBuf src;                 // int32 per slot (CSR handles): its CSR entry, -1 for padding
   DevBuf gdesc;               // int per group: column block | valid entries << 16
   DevBuf wgrp;                // int per workgroup + 1: first group
-  DevBuf rstart;              // int per row block + 1: first row
-  DevBuf part;                // y type [H][n_rows] (H > 1)
-  int rb = 0, H = 0, n_rb = 0, ngroups = 0;
+  DevBuf wblk;                // int2 per workgroup: {row block, share}
+  DevBuf binfo;               // int4 per row block: {first position, rows, shares, first plane element}
+  DevBuf rowidx;              // int per position: its row (two classes only)
+  DevBuf rtasks;              // int4 per task of the plane-adding pass (two classes only)
+  DevBuf part;                // partial planes (y type)
+  int rb = 0, H = 0, n_rb = 0, ngroups = 0, nwg = 0, nrtasks = 0;
+  int64_t split = 0;          // T of the two-class form, 0 for one class
+  int64_t plane_elems = 0;    // elements of all partial planes
 };
 
 struct BsrDev {
@@ -123,6 +143,8 @@ extern int64_t g_sweep_mode;
 // swept layout: workgroups aimed for per launch (H = this / fewest row blocks, 1..kSweepMaxSplit)
 extern int64_t g_sweep_workgroups;
 extern int64_t g_sweep_max_rows;     // swept layout: rows of the largest row block
+// swept layout: 0 one class, T > 0 two classes split at row length T
+extern int64_t g_sweep_split;
 // BSR matrices created from now on with dense blocks of at least 8 x 8 (bC % 4 == 0) on the
 // MFMA kernel: 0 never, 1 (default) when the VALU kernel cannot hold the block (more than
 // kBsrMaxBlock entries), 2 always

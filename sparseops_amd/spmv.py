@@ -102,6 +102,22 @@ class SpMatrix:
                Y.data_ptr(), TORCH_TYPES[Y.dtype], s)
         return Y
 
+    def spmm_values(self, vals: torch.Tensor, X: torch.Tensor, Y: torch.Tensor, alpha: float = 1.0,
+                    beta: float = 0.0, stream=None):
+        """spmm with the matrix values read from `vals` (nnz elements of the stored value type)
+        instead of the stored ones, which stay unchanged; for a transpose() handle, vals are in
+        the original matrix's entry order (as for set_values)."""
+        k = _rhs_check(X, self.n_cols, "X")
+        if Y.dim() != 2 or Y.shape[0] != self.n_rows or Y.shape[1] != k or not Y.is_contiguous() or not Y.is_cuda:
+            raise ValueError("Y must be a contiguous device tensor of shape (n_rows, k)")
+        if (vals.numel() != self.nnz or not vals.is_contiguous() or not vals.is_cuda
+                or TORCH_TYPES.get(vals.dtype) != self.info[4]):
+            raise ValueError("vals must be a contiguous device tensor of nnz elements of the stored value type")
+        s = L.current_stream() if stream is None else stream
+        L.call("spops_spmm_values", self._h, vals.data_ptr(), k, float(alpha), X.data_ptr(), TORCH_TYPES[X.dtype],
+               float(beta), Y.data_ptr(), TORCH_TYPES[Y.dtype], s)
+        return Y
+
     def sddmm(self, G: torch.Tensor, X: torch.Tensor, out: torch.Tensor, stream=None):
         """out[e] = <G[row_e, :], X[col_e, :]> over A's pattern (the gradient of the values)."""
         k = _rhs_check(X, self.n_cols, "X")
@@ -143,6 +159,16 @@ class SpMatrix:
         out = np.zeros(4, dtype=np.int64)
         L.call("spops_matrix_layout", self._h, 2, out.ctypes.data)
         return tuple(int(v) for v in out) if out[1] > 0 else None
+
+    @property
+    def sweep_classes(self):
+        """Row classes of the swept layout (split length T, 0 for one class; workgroups;
+        partial-plane elements; tasks of the plane-adding pass), or None without one."""
+        if self.sweep is None:
+            return None
+        out = np.zeros(4, dtype=np.int64)
+        L.call("spops_matrix_layout", self._h, 3, out.ctypes.data)
+        return tuple(int(v) for v in out)
 
     def layout(self, which: int) -> np.ndarray:
         n = self.n_rows if which == 0 else self.info[14]
@@ -269,6 +295,12 @@ def bsr_mfma(mode: str) -> tuning:
 def sweep_workgroups(n: int) -> tuning:
     """Workgroups a swept-layout SpMV aims for (default 256); row blocks are shared by up to 16."""
     return tuning(L.TUNE_SWEEP_WORKGROUPS, n)
+
+
+def sweep_split(T: int) -> tuning:
+    """Row blocks of the swept layout: 0 (default) consecutive rows of equal entry counts,
+    T > 0 the rows of at most T entries and the longer rows in separate blocks."""
+    return tuning(L.TUNE_SWEEP_SPLIT, T)
 
 
 def SpOps(alpha: float, A: SpMatrix, x: torch.Tensor, beta: float, y: torch.Tensor):

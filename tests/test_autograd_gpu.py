@@ -81,13 +81,18 @@ def test_spmm_mixed_types(mats, xt, yt, dev):
     _close(Yt.double().cpu().numpy(), want, scale, tol)
 
 
-@pytest.mark.parametrize("k", [1, 16, 64, 70])
-@pytest.mark.parametrize("gt", [torch.float64, torch.float32, torch.bfloat16])
-def test_sddmm(mats, k, gt, dev):
-    A = mats["rmat"]
+@pytest.mark.parametrize("k", [1, 16, 64, 70, 512, 520])
+@pytest.mark.parametrize("gt,xt", [(torch.float64, torch.float64), (torch.float32, torch.float32),
+                                   (torch.bfloat16, torch.bfloat16), (torch.float32, torch.bfloat16)])
+@pytest.mark.parametrize("name", ["rmat", "edge"])
+def test_sddmm(mats, name, k, gt, xt, dev):
+    """By rows (each G row loaded once; the 512-entry segments of rows longer than 256 entries)
+    when k is a multiple of X's 16-byte vector and at most 64 vectors; entry-parallel otherwise
+    (k = 1, and k = 70 / 512 / 520 where the vector does not divide k or 64 vectors fall short)."""
+    A = mats[name]
     rng = np.random.default_rng(k + 1)
     G = torch.tensor(rng.standard_normal((A.n_rows, k)), device=dev).to(gt)
-    X = torch.tensor(rng.standard_normal((A.n_cols, k)), device=dev).to(gt)
+    X = torch.tensor(rng.standard_normal((A.n_cols, k)), device=dev).to(xt)
     M = S.csr(A, "f64")
     out = torch.zeros(A.nnz, dtype=torch.float64 if gt == torch.float64 else torch.float32, device=dev)
     M.sddmm(G, X, out)
@@ -95,6 +100,51 @@ def test_sddmm(mats, k, gt, dev):
     want = orc.sddmm(A, Gn, Xn)
     scale = orc.sddmm(A, np.abs(Gn), np.abs(Xn))
     _close(out.double().cpu().numpy(), want, scale, 1e-13 if gt == torch.float64 else 2e-6)
+
+
+@pytest.mark.parametrize("vt,xt", [("f64", torch.float64), ("f32", torch.float32), ("f32", torch.bfloat16),
+                                   ("bf16", torch.bfloat16)])
+@pytest.mark.parametrize("name", ["edge", "rmat"])
+def test_spmm_values_in_place(mats, name, vt, xt, dev):
+    """spops_spmm_values reads the values from the caller's array: the same product as
+    set_values + spmm on those values, bit for bit, and the stored values stay as they were."""
+    A = mats[name]
+    k = 64
+    rng = np.random.default_rng(7)
+    tdt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[vt]
+    vals = torch.tensor(rng.standard_normal(A.nnz), device=dev).to(tdt)
+    X = torch.tensor(rng.standard_normal((A.n_cols, k)), device=dev).to(xt)
+    M = S.csr(A, vt)
+    yt = torch.float64 if vt == "f64" else torch.float32
+    Y0 = torch.tensor(rng.standard_normal((A.n_rows, k)), device=dev).to(yt)
+    Y1 = Y0.clone()
+    M.spmm_values(vals, X, Y1, 0.5, 0.25)
+    Ys = Y0.clone()
+    M.spmm(X, Ys, 0.5, 0.25)                      # the stored values: A.val
+    R = S.csr(A, vt)
+    R.set_values(vals)
+    Y2 = Y0.clone()
+    R.spmm(X, Y2, 0.5, 0.25)
+    torch.cuda.synchronize()
+    assert torch.equal(Y1, Y2)
+    Ystored = Y0.clone()
+    S.csr(A, vt).spmm(X, Ystored, 0.5, 0.25)
+    assert torch.equal(Ys, Ystored)
+    # a transposed handle reads the values (in A's entry order) through its permutation
+    AT, AT2 = M.transpose(), S.csr(A, vt).transpose()
+    AT2.set_values(vals)
+    Gm = torch.tensor(rng.standard_normal((A.n_rows, k)), device=dev).to(xt)
+    D0 = torch.tensor(rng.standard_normal((A.n_cols, k)), device=dev).to(yt)
+    D1, D2 = D0.clone(), D0.clone()
+    AT.spmm_values(vals, Gm, D1, 1.5, -0.5)
+    AT2.spmm(Gm, D2, 1.5, -0.5)
+    torch.cuda.synchronize()
+    assert torch.equal(D1, D2)
+    with pytest.raises(ValueError):
+        M.spmm_values(vals[:-1], X, Y1)
+    other = torch.float32 if vt != "f32" else torch.float64
+    with pytest.raises(ValueError):
+        M.spmm_values(vals.to(other), X, Y1)
 
 
 @pytest.mark.parametrize("name", ["edge", "rmat"])

@@ -412,6 +412,30 @@ def test_swept(mats, name, vt, fmt, wgs, maxrows, dev):
         np.testing.assert_array_equal(M.layout(1), cl)
 
 
+@pytest.mark.parametrize("T,wgs,maxrows", [(4, 256, 20480), (12, 64, 1000), (64, 7, 777), (100000, 256, 3000)])
+@pytest.mark.parametrize("vt", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("name", ["pl", "edge", "wide"])
+@pytest.mark.parametrize("fmt", ["sell", "ell", "csr"])
+def test_swept_two_class(mats, name, vt, fmt, T, wgs, maxrows, dev):
+    """The two-class swept layout: rows of at most T entries and the longer rows in separate
+    row blocks (row map from positions to rows), blocks of one share writing y and blocks of
+    several shares writing planes that the gathering pass adds; T above every row length puts
+    all rows in one class of row-limit blocks.  alpha/beta, beta == 0 (y not read), repeats."""
+    A = _sweep_matrix(mats, name)
+    with S.spmv_layout("swept"), S.sweep_workgroups(wgs), S.sweep_max_rows(maxrows), S.sweep_split(T):
+        M = (S.csr(A, vt) if fmt == "csr" else
+             S.csr_to_sell_c_sigma(A, 256, 32, vt) if fmt == "sell" else S.csr_to_ell(A, vt))
+    split, nwg, planes, tasks = M.sweep_classes
+    rbits, H, n_rb, groups = M.sweep
+    assert split == T and nwg >= n_rb and n_rb >= -(-A.n_rows // maxrows) and groups >= -(-A.nnz // 512)
+    assert nwg <= max(wgs, n_rb) and (H > 1) == (planes > 0) and (tasks > 0) == (planes > 0)
+    x = uniform_x(A.n_cols, 8)
+    for _ in range(2):
+        _check(A, M, x, _y0(A.n_rows), vt, dev=dev)
+    y = _run(M, x, np.full(A.n_rows, np.nan), dev, 1.0, 0.0)
+    assert np.all(np.isfinite(y))
+
+
 @pytest.mark.parametrize("n", [20001, 20002, 20003, 20004])
 @pytest.mark.parametrize("vt", ["f32", "bf16", "f64"])
 def test_swept_reduce_row_counts(n, vt, dev):
