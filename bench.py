@@ -540,7 +540,7 @@ def batch_measure(count, steps, warmup, cpu, cpu_threads):
     import numpy as np
     import torch
     from sparseops_amd import _lib as L
-    from sparseops_amd.batch import BatchSpMM
+    from sparseops_amd.batch import FORMATS as FORMAT_NAMES, BatchSpMM
     from sparseops_amd.gen import Csr, band_batch
     t = time.time()
     mats, ks = band_batch(count, seed=0)
@@ -551,24 +551,46 @@ def batch_measure(count, steps, warmup, cpu, cpu_threads):
     X, Y = B.new_x(), B.new_y()
     X.copy_(torch.randn(X.numel(), generator=torch.Generator().manual_seed(0)).to(X.device))
     s = torch.cuda.Stream()
-    torch.cuda.synchronize()
-    B.capture(X, Y, s.cuda_stream)
-    ev = [ctypes.c_void_p() for _ in range(2)]
-    for e in ev:
-        L.call("spops_event_create", ctypes.byref(e))
-    for _ in range(warmup):
-        B.replay(s.cuda_stream)
-    s.synchronize()
-    L.call("spops_event_record", ev[0], s.cuda_stream)
-    for _ in range(steps):
-        B.replay(s.cuda_stream)
-    L.call("spops_event_record", ev[1], s.cuda_stream)
-    s.synchronize()
-    ms = ctypes.c_float()
-    L.call("spops_event_elapsed_ms", ctypes.byref(ms), ev[0], ev[1])
-    for e in ev:
-        L.call("spops_event_destroy", e)
-    step_ms = ms.value / steps
+
+    def replay_ms(Bx, Yx):
+        """Mean ms of one graph replay of Bx's launch (HIP events on the replay stream)."""
+        torch.cuda.synchronize()
+        Bx.capture(X, Yx, s.cuda_stream)
+        ev = [ctypes.c_void_p() for _ in range(2)]
+        for e in ev:
+            L.call("spops_event_create", ctypes.byref(e))
+        for _ in range(warmup):
+            Bx.replay(s.cuda_stream)
+        s.synchronize()
+        L.call("spops_event_record", ev[0], s.cuda_stream)
+        for _ in range(steps):
+            Bx.replay(s.cuda_stream)
+        L.call("spops_event_record", ev[1], s.cuda_stream)
+        s.synchronize()
+        ms = ctypes.c_float()
+        L.call("spops_event_elapsed_ms", ctypes.byref(ms), ev[0], ev[1])
+        for e in ev:
+            L.call("spops_event_destroy", e)
+        return ms.value / steps
+
+    step_ms = replay_ms(B, Y)
+    # the autotuner's choice per class, and the same batch on the untuned default rule
+    classes = {}
+    for i in range(count):
+        c = classes.setdefault(str(int(B.cls[i])), {"matrices": 0, "plans": {}})
+        c["matrices"] += 1
+        key = f"{FORMAT_NAMES.get(int(B.fmt[i]), B.fmt[i])}/lanes={int(B.lanes[i])}/sigma={int(B.sigma[i])}"
+        c["plans"][key] = c["plans"].get(key, 0) + 1
+    t = time.time()
+    B0 = BatchSpMM(mats, ks, tune=False)
+    Y0 = B0.new_y()
+    untuned_ms = replay_ms(B0, Y0)
+    untuned = {"ms_per_step": round(untuned_ms, 4), "gbs": round(B0.algo_bytes / untuned_ms / 1e6, 1),
+               "create_s": round(time.time() - t, 1), "tuned_speedup": round(untuned_ms / step_ms, 3)}
+    fmts0 = B0.formats()
+    untuned["formats"] = {f: fmts0.count(f) for f in set(fmts0)}
+    B0.close()
+    del Y0
     fmts = B.formats()
     out = {"workload": f"{count} band matrices (n log-uniform 1k-64k, half-width 1-64, 0-8 off-band entries per row), "
                        f"k U{{1..32}}, FP32 values / X / accumulation, one persistent launch replayed from a HIP graph",
@@ -577,7 +599,8 @@ def batch_measure(count, steps, warmup, cpu, cpu_threads):
            "gbs": round(B.algo_bytes / step_ms / 1e6, 1),
            "pct_hbm_peak": round(100 * B.algo_bytes / step_ms / 1e6 / HBM_PEAK_GBS, 2),
            "gflops": round(B.flop / step_ms / 1e6, 1), "tasks": B.ntasks,
-           "formats": {f: fmts.count(f) for f in set(fmts)}, "generate_s": round(gen_s, 1),
+           "formats": {f: fmts.count(f) for f in set(fmts)}, "autotuner_classes": classes,
+           "untuned_default_rule": untuned, "generate_s": round(gen_s, 1),
            "create_and_tune_s": round(create_s, 1), "kernel": "k_batch_spmm"}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as orc

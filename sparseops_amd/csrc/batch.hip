@@ -141,6 +141,124 @@ __device__ __forceinline__ void chunk_rows(const BatchMat &m, int64_t cidx, int 
   }
 }
 
+// A wave's chunks ch0, ch0 + chstep, ... (< ch1) of one matrix whose row lanes cover all ld
+// columns in one block (L VEC NV >= ld): the same products as chunk_rows, with the entry
+// pieces software-pipelined -- the loads of the next piece (of this chunk or the next one)
+// are issued before the current piece is read from LDS, so a piece's memory latency overlaps
+// the previous piece's work instead of adding to it.  Registers hold one piece in flight.
+template <int VEC, int NV>
+__device__ __forceinline__ void chunk_run(const BatchMat &m, int ch0, int chstep, int ch1, const int2 *__restrict__ ent,
+                                          const int64_t *__restrict__ cbase, const int *__restrict__ cwidth,
+                                          const int *__restrict__ crow, const float *__restrict__ Xm,
+                                          float *__restrict__ Ym, float alpha, float beta, int lane, int2 *stage) {
+  using V = typename FV<VEC>::V;
+  constexpr int U = VEC * NV >= 8 ? 4 : 8;            // entries per step (register budget)
+  constexpr int NB = kPiece / 2 / kWave;              // 16-byte loads per lane for a full piece
+  const int L = m.lanes, R = kWave / L, rl = lane / L, q = lane % L;
+  const int ld = m.ld, c0 = q * VEC * NV;
+  const int pmax = kPiece / R;
+  const rsrc_t rx = make_rsrc(Xm, (uint64_t)m.n_cols * ld * 4);
+  bu4 *st4 = reinterpret_cast<bu4 *>(stage);
+  if (ch0 >= ch1) return;
+  // piece geometry of a chunk: width (entry columns), columns per piece, the entry resource
+  int width = 0, pj = 0;
+  rsrc_t re;
+  auto geometry = [&](int ch, int &wd, int &pjr, rsrc_t &res) {
+    const int64_t cidx = m.cb + ch;
+    wd = __builtin_amdgcn_readfirstlane(cwidth[cidx]);
+    const int npc = (wd + pmax - 1) / max(pmax, 1);
+    pjr = npc > 1 ? (wd + npc - 1) / npc : pmax;
+    res = make_rsrc(ent + cbase[cidx], (uint64_t)wd * R * 8);
+  };
+  auto issue = [&](const rsrc_t &res, int j0, int pjr, bu4 (&t)[NB]) {
+    const uint32_t boff = (uint32_t)(j0 * R) * 8;       // 16-byte aligned: R is even
+    const int nld = (pjr * R + 2 * kWave - 1) / (2 * kWave);
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+      if (i < nld) t[i] = bload<bu4>(res, boff + (uint32_t)(i * kWave + lane) * 16);
+  };
+  int ch = ch0, j0 = 0;
+  geometry(ch, width, pj, re);
+  bu4 t[NB];
+  issue(re, 0, pj, t);
+  float acc[NV][VEC];
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) acc[v][e] = 0.f;
+  while (true) {
+    {
+      const int nld = (pj * R + 2 * kWave - 1) / (2 * kWave);
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        if (i < nld) st4[i * kWave + lane] = t[i];
+    }
+    const int jn = min(pj, width - j0);
+    // the next piece: later in this chunk, else the first of the wave's next chunk
+    int nch = ch, nj0 = j0 + pj, nwidth = width, npj = pj;
+    rsrc_t nre = re;
+    const bool last_of_chunk = nj0 >= width;
+    if (last_of_chunk) {
+      nch = ch + chstep;
+      nj0 = 0;
+    }
+    const bool more = nch < ch1;
+    if (more) {
+      if (last_of_chunk) geometry(nch, nwidth, npj, nre);
+      issue(nre, nj0, npj, t);
+    }
+    for (int j = 0; j < jn; j += U) {
+      int2 en[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) en[u] = j + u < jn ? stage[(j + u) * R + rl] : make_int2(-1, 0);
+      V xv[U][NV];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) xv[u][v] = bload<V>(rx, (uint32_t)((en[u].x * ld + c0 + v * VEC) * 4));
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float a = __int_as_float(en[u].y);
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) acc[v][e] = __builtin_fmaf(a, xv[u][v].e[e], acc[v][e]);
+      }
+    }
+    if (last_of_chunk) {
+      const int slot = ch * R + rl;
+      const int row = slot < m.n_rows ? crow[m.rp + slot] : -1;
+      if (row >= 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          if (c0 + v * VEC >= ld) continue;
+          V *dst = reinterpret_cast<V *>(Ym + (int64_t)row * ld + c0 + v * VEC);
+          V o;
+          if (beta != 0.f) {
+            const V old = *dst;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) o.e[e] = alpha * acc[v][e] + beta * old.e[e];
+          } else {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) o.e[e] = alpha * acc[v][e];
+          }
+          *dst = o;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) acc[v][e] = 0.f;
+    }
+    if (!more) break;
+    ch = nch;
+    j0 = nj0;
+    width = nwidth;
+    pj = npj;
+    re = nre;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_batch_spmm(const BatchMat *__restrict__ mats, const int4 *__restrict__ tasks,
                                                        int ntasks, const int *__restrict__ rowptr,
                                                        const int *__restrict__ col, const float *__restrict__ val,
@@ -197,6 +315,16 @@ __global__ __launch_bounds__(kBlock) void k_batch_spmm(const BatchMat *__restric
       }
     } else {
       const int sel = m.vec * 4 + m.nv;               // wave-uniform
+      if (m.lanes * m.vec * m.nv >= m.ld) {
+        const int ch0 = start + w, ch1 = start + count, cs = kBlock / kWave;
+        switch (sel) {
+          case 1 * 4 + 1: chunk_run<1, 1>(m, ch0, cs, ch1, ent, cbase, cwidth, crow, Xm, Ym, alpha, beta, lane, stage[w]); break;
+          case 2 * 4 + 1: chunk_run<2, 1>(m, ch0, cs, ch1, ent, cbase, cwidth, crow, Xm, Ym, alpha, beta, lane, stage[w]); break;
+          case 4 * 4 + 2: chunk_run<4, 2>(m, ch0, cs, ch1, ent, cbase, cwidth, crow, Xm, Ym, alpha, beta, lane, stage[w]); break;
+          default: chunk_run<4, 1>(m, ch0, cs, ch1, ent, cbase, cwidth, crow, Xm, Ym, alpha, beta, lane, stage[w]); break;
+        }
+        continue;
+      }
       for (int ch = start + w; ch < start + count; ch += kBlock / kWave) {
         const int64_t cidx = m.cb + ch;
         switch (sel) {
@@ -372,10 +500,12 @@ void build_batch(const std::vector<HostMat> &ms, const std::vector<MatPlan> &pla
           }
         }
       }
+      // a task: at least one chunk per wave of the workgroup (the waves split a task's chunks),
+      // more while they fit kBatchTaskEntries
       int64_t c = 0;
       while (c < nch) {
         int64_t c1 = c, acc = 0;
-        while (c1 < nch && (acc == 0 || acc + ccost[c1] <= kBatchTaskEntries)) acc += ccost[c1++];
+        while (c1 < nch && (c1 - c < kBlock / kWave || acc + ccost[c1] <= kBatchTaskEntries)) acc += ccost[c1++];
         mq[i].push_back(make_int4((int)i, (int)c, (int)(c1 - c), 1));
         c = c1;
       }
