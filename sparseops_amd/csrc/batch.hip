@@ -141,81 +141,71 @@ __device__ __forceinline__ void chunk_rows(const BatchMat &m, int64_t cidx, int 
   }
 }
 
-// A wave's chunks ch0, ch0 + chstep, ... (< ch1) of one matrix whose row lanes cover all ld
-// columns in one block (L VEC NV >= ld): the same products as chunk_rows, with the entry
-// pieces software-pipelined -- the loads of the next piece (of this chunk or the next one)
-// are issued before the current piece is read from LDS, so a piece's memory latency overlaps
-// the previous piece's work instead of adding to it.  Registers hold one piece in flight.
+// X rows staged in LDS per round of a workgroup (kBlock / kWave consecutive chunks, one per
+// wave): the rows that the round's near-diagonal entries touch, when they fit.
+constexpr int kWinFloats = 6144;   // 24 KiB
+
+// One chunk of R = 64 / L rows whose row lanes cover all ld columns (L VEC NV >= ld), with X
+// rows [wlo, wlo + wn) in the workgroup's LDS window `xw` (row stride ld): an entry whose
+// column falls in the window reads X there, the others (the far entries, kept at the end of
+// each row, and padding) gather from memory in steps that hold any.  Otherwise as chunk_rows.
 template <int VEC, int NV>
-__device__ __forceinline__ void chunk_run(const BatchMat &m, int ch0, int chstep, int ch1, const int2 *__restrict__ ent,
-                                          const int64_t *__restrict__ cbase, const int *__restrict__ cwidth,
-                                          const int *__restrict__ crow, const float *__restrict__ Xm,
-                                          float *__restrict__ Ym, float alpha, float beta, int lane, int2 *stage) {
+__device__ __forceinline__ void chunk_rows_win(const BatchMat &m, int64_t cidx, int chunk, const int2 *__restrict__ ent,
+                                               const int64_t *__restrict__ cbase, const int *__restrict__ cwidth,
+                                               const int *__restrict__ crow, const float *__restrict__ Xm,
+                                               float *__restrict__ Ym, float alpha, float beta, int lane, int2 *stage,
+                                               const float *xw, int wlo, int wn) {
   using V = typename FV<VEC>::V;
   constexpr int U = VEC * NV >= 8 ? 4 : 8;            // entries per step (register budget)
-  constexpr int NB = kPiece / 2 / kWave;              // 16-byte loads per lane for a full piece
   const int L = m.lanes, R = kWave / L, rl = lane / L, q = lane % L;
+  const int width = __builtin_amdgcn_readfirstlane(cwidth[cidx]);
+  const int slot = chunk * R + rl;
+  const int row = slot < m.n_rows ? crow[m.rp + slot] : -1;
   const int ld = m.ld, c0 = q * VEC * NV;
-  const int pmax = kPiece / R;
+  const int pmax = kPiece / R, npc = (width + pmax - 1) / max(pmax, 1);
+  const int pj = npc > 1 ? (width + npc - 1) / npc : pmax;
+  const rsrc_t re = make_rsrc(ent + cbase[cidx], (uint64_t)width * R * 8);
   const rsrc_t rx = make_rsrc(Xm, (uint64_t)m.n_cols * ld * 4);
   bu4 *st4 = reinterpret_cast<bu4 *>(stage);
-  if (ch0 >= ch1) return;
-  // piece geometry of a chunk: width (entry columns), columns per piece, the entry resource
-  int width = 0, pj = 0;
-  rsrc_t re;
-  auto geometry = [&](int ch, int &wd, int &pjr, rsrc_t &res) {
-    const int64_t cidx = m.cb + ch;
-    wd = __builtin_amdgcn_readfirstlane(cwidth[cidx]);
-    const int npc = (wd + pmax - 1) / max(pmax, 1);
-    pjr = npc > 1 ? (wd + npc - 1) / npc : pmax;
-    res = make_rsrc(ent + cbase[cidx], (uint64_t)wd * R * 8);
-  };
-  auto issue = [&](const rsrc_t &res, int j0, int pjr, bu4 (&t)[NB]) {
-    const uint32_t boff = (uint32_t)(j0 * R) * 8;       // 16-byte aligned: R is even
-    const int nld = (pjr * R + 2 * kWave - 1) / (2 * kWave);
-#pragma unroll
-    for (int i = 0; i < NB; ++i)
-      if (i < nld) t[i] = bload<bu4>(res, boff + (uint32_t)(i * kWave + lane) * 16);
-  };
-  int ch = ch0, j0 = 0;
-  geometry(ch, width, pj, re);
-  bu4 t[NB];
-  issue(re, 0, pj, t);
   float acc[NV][VEC];
 #pragma unroll
   for (int v = 0; v < NV; ++v)
 #pragma unroll
     for (int e = 0; e < VEC; ++e) acc[v][e] = 0.f;
-  while (true) {
-    {
-      const int nld = (pj * R + 2 * kWave - 1) / (2 * kWave);
+  for (int j0 = 0; j0 < width; j0 += pj) {
+    const uint32_t boff = (uint32_t)(j0 * R) * 8;             // 16-byte aligned: R is even
+    const int nld = (pj * R + 2 * kWave - 1) / (2 * kWave);    // 16-byte loads per lane (wave-uniform)
+    bu4 t[kPiece / 2 / kWave];
 #pragma unroll
-      for (int i = 0; i < NB; ++i)
-        if (i < nld) st4[i * kWave + lane] = t[i];
-    }
+    for (int i = 0; i < kPiece / 2 / kWave; ++i)
+      if (i < nld) t[i] = bload<bu4>(re, boff + (uint32_t)(i * kWave + lane) * 16);
+#pragma unroll
+    for (int i = 0; i < kPiece / 2 / kWave; ++i)
+      if (i < nld) st4[i * kWave + lane] = t[i];
     const int jn = min(pj, width - j0);
-    // the next piece: later in this chunk, else the first of the wave's next chunk
-    int nch = ch, nj0 = j0 + pj, nwidth = width, npj = pj;
-    rsrc_t nre = re;
-    const bool last_of_chunk = nj0 >= width;
-    if (last_of_chunk) {
-      nch = ch + chstep;
-      nj0 = 0;
-    }
-    const bool more = nch < ch1;
-    if (more) {
-      if (last_of_chunk) geometry(nch, nwidth, npj, nre);
-      issue(nre, nj0, npj, t);
-    }
     for (int j = 0; j < jn; j += U) {
       int2 en[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) en[u] = j + u < jn ? stage[(j + u) * R + rl] : make_int2(-1, 0);
       V xv[U][NV];
+      uint32_t d[U];
+      bool far = false;
 #pragma unroll
-      for (int u = 0; u < U; ++u)
+      for (int u = 0; u < U; ++u) {
+        d[u] = (uint32_t)(en[u].x - wlo);
+        const bool in = d[u] < (uint32_t)wn;
+        far |= !in;
+        const uint32_t li = in ? d[u] : 0u;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) xv[u][v] = bload<V>(rx, (uint32_t)((en[u].x * ld + c0 + v * VEC) * 4));
+        for (int v = 0; v < NV; ++v) xv[u][v] = *reinterpret_cast<const V *>(xw + li * ld + c0 + v * VEC);
+      }
+      if (__any(far)) {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (d[u] >= (uint32_t)wn)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) xv[u][v] = bload<V>(rx, (uint32_t)((en[u].x * ld + c0 + v * VEC) * 4));
+      }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const float a = __int_as_float(en[u].y);
@@ -225,37 +215,23 @@ __device__ __forceinline__ void chunk_run(const BatchMat &m, int ch0, int chstep
           for (int e = 0; e < VEC; ++e) acc[v][e] = __builtin_fmaf(a, xv[u][v].e[e], acc[v][e]);
       }
     }
-    if (last_of_chunk) {
-      const int slot = ch * R + rl;
-      const int row = slot < m.n_rows ? crow[m.rp + slot] : -1;
-      if (row >= 0) {
+  }
+  if (row >= 0) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          if (c0 + v * VEC >= ld) continue;
-          V *dst = reinterpret_cast<V *>(Ym + (int64_t)row * ld + c0 + v * VEC);
-          V o;
-          if (beta != 0.f) {
-            const V old = *dst;
+    for (int v = 0; v < NV; ++v) {
+      if (c0 + v * VEC >= ld) continue;
+      V *dst = reinterpret_cast<V *>(Ym + (int64_t)row * ld + c0 + v * VEC);
+      V o;
+      if (beta != 0.f) {
+        const V old = *dst;
 #pragma unroll
-            for (int e = 0; e < VEC; ++e) o.e[e] = alpha * acc[v][e] + beta * old.e[e];
-          } else {
+        for (int e = 0; e < VEC; ++e) o.e[e] = alpha * acc[v][e] + beta * old.e[e];
+      } else {
 #pragma unroll
-            for (int e = 0; e < VEC; ++e) o.e[e] = alpha * acc[v][e];
-          }
-          *dst = o;
-        }
+        for (int e = 0; e < VEC; ++e) o.e[e] = alpha * acc[v][e];
       }
-#pragma unroll
-      for (int v = 0; v < NV; ++v)
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) acc[v][e] = 0.f;
+      *dst = o;
     }
-    if (!more) break;
-    ch = nch;
-    j0 = nj0;
-    width = nwidth;
-    pj = npj;
-    re = nre;
   }
 }
 
@@ -264,9 +240,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_spmm(const BatchMat *__restric
                                                        const int *__restrict__ col, const float *__restrict__ val,
                                                        const int2 *__restrict__ ent, const int64_t *__restrict__ cbase,
                                                        const int *__restrict__ cwidth, const int *__restrict__ crow,
+                                                       const int2 *__restrict__ cwin,
                                                        const float *__restrict__ X, float *__restrict__ Y, float alpha,
                                                        float beta) {
   __shared__ int2 stage[kBlock / kWave][kPiece];
+  __shared__ bu4 xwin4[kWinFloats / 4];
   const int lane = lane_id(), w = wave_id();
   for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
     const int4 tk = tasks[t];
@@ -316,12 +294,41 @@ __global__ __launch_bounds__(kBlock) void k_batch_spmm(const BatchMat *__restric
     } else {
       const int sel = m.vec * 4 + m.nv;               // wave-uniform
       if (m.lanes * m.vec * m.nv >= m.ld) {
-        const int ch0 = start + w, ch1 = start + count, cs = kBlock / kWave;
-        switch (sel) {
-          case 1 * 4 + 1: chunk_run<1, 1>(m, ch0, cs, ch1, ent, cbase, cwidth, crow, Xm, Ym, alpha, beta, lane, stage[w]); break;
-          case 2 * 4 + 1: chunk_run<2, 1>(m, ch0, cs, ch1, ent, cbase, cwidth, crow, Xm, Ym, alpha, beta, lane, stage[w]); break;
-          case 4 * 4 + 2: chunk_run<4, 2>(m, ch0, cs, ch1, ent, cbase, cwidth, crow, Xm, Ym, alpha, beta, lane, stage[w]); break;
-          default: chunk_run<4, 1>(m, ch0, cs, ch1, ent, cbase, cwidth, crow, Xm, Ym, alpha, beta, lane, stage[w]); break;
+        // rounds of one chunk per wave, the X rows of the round's near entries in LDS
+        const int end = start + count;
+        const float *xw = reinterpret_cast<const float *>(xwin4);
+        for (int base = start; base < end; base += kBlock / kWave) {
+          int lo = INT32_MAX, hi = -1;
+          for (int qq = 0; qq < kBlock / kWave && base + qq < end; ++qq) {
+            const int2 b = cwin[m.cb + base + qq];
+            if (b.x >= 0 && b.y >= b.x) {
+              lo = min(lo, b.x);
+              hi = max(hi, b.y);
+            }
+          }
+          lo &= ~3;                                   // 16-byte aligned start (4 rows)
+          int wn = hi >= lo ? hi - lo + 1 : 0;
+          if ((int64_t)wn * m.ld > kWinFloats) wn = 0;
+          wn = __builtin_amdgcn_readfirstlane(wn);
+          lo = __builtin_amdgcn_readfirstlane(lo);
+          __syncthreads();                            // the previous round's window reads are done
+          if (wn > 0) {
+            const rsrc_t rxw = make_rsrc(Xm, (uint64_t)m.n_cols * m.ld * 4);
+            const int n4 = (wn * m.ld + 3) / 4;
+            for (int i = (int)threadIdx.x; i < n4; i += kBlock)
+              xwin4[i] = bload<bu4>(rxw, (uint32_t)lo * (uint32_t)m.ld * 4u + (uint32_t)i * 16u);
+          }
+          __syncthreads();
+          const int ch = base + w;
+          if (ch < end) {
+            const int64_t cidx = m.cb + ch;
+            switch (sel) {
+              case 1 * 4 + 1: chunk_rows_win<1, 1>(m, cidx, ch, ent, cbase, cwidth, crow, Xm, Ym, alpha, beta, lane, stage[w], xw, lo, wn); break;
+              case 2 * 4 + 1: chunk_rows_win<2, 1>(m, cidx, ch, ent, cbase, cwidth, crow, Xm, Ym, alpha, beta, lane, stage[w], xw, lo, wn); break;
+              case 4 * 4 + 2: chunk_rows_win<4, 2>(m, cidx, ch, ent, cbase, cwidth, crow, Xm, Ym, alpha, beta, lane, stage[w], xw, lo, wn); break;
+              default: chunk_rows_win<4, 1>(m, cidx, ch, ent, cbase, cwidth, crow, Xm, Ym, alpha, beta, lane, stage[w], xw, lo, wn); break;
+            }
+          }
         }
         continue;
       }
@@ -354,7 +361,7 @@ struct MatPlan {
 };
 
 struct BatchDev {
-  DevBuf mats, tasks, rowptr, col, val, ent, cbase, cwidth, crow;
+  DevBuf mats, tasks, rowptr, col, val, ent, cbase, cwidth, crow, cwin;
   int ntasks = 0, grid = 0;
   int64_t x_len = 0, y_len = 0, flop = 0, algo_bytes = 0;
   std::vector<BatchMat> hmats;
@@ -399,6 +406,7 @@ void build_batch(const std::vector<HostMat> &ms, const std::vector<MatPlan> &pla
   const size_t nm = ms.size();
   std::vector<BatchMat> hm(nm);
   std::vector<int32_t> rowptr, col, cwidth, crow;
+  std::vector<int2> cwin;                       // per chunk: column range of its near entries
   std::vector<float> val;
   std::vector<int2> ent;
   std::vector<int64_t> cbase;
@@ -484,6 +492,16 @@ void build_batch(const std::vector<HostMat> &ms, const std::vector<MatPlan> &pla
           wdt = std::max(wdt, M.rp[r + 1] - M.rp[r]);
         }
         SPOPS_REQUIRE(wdt * R * 8 < (int64_t)kMaxBufBytes, SPOPS_ERROR_LIMIT, "matrix %zu: chunk of 2 GiB", i);
+        int64_t wlo = INT32_MAX, whi = -1;
+        for (int64_t l = 0; l < R && c * R + l < M.n_rows; ++l) {
+          const int64_t r = order[c * R + l];
+          for (int64_t j = M.rp[r]; j < M.rp[r + 1]; ++j)
+            if (dist[j] <= near) {
+              wlo = std::min<int64_t>(wlo, M.ci[j]);
+              whi = std::max<int64_t>(whi, M.ci[j]);
+            }
+        }
+        cwin.push_back(whi >= 0 ? make_int2((int)wlo, (int)whi) : make_int2(-1, -1));
         cbase.push_back((int64_t)ent.size());
         cwidth.push_back((int32_t)wdt);
         ccost[c] = wdt * R + R;
@@ -540,6 +558,7 @@ void build_batch(const std::vector<HostMat> &ms, const std::vector<MatPlan> &pla
   B.cbase.upload(cbase);
   B.cwidth.upload(cwidth);
   B.crow.upload(crow);
+  B.cwin.upload(cwin);
   B.ntasks = (int)ht.size();
   // persistent grid: every workgroup resident at once (the task striding is static), a
   // multiple of 8 so that task t runs on XCD t % 8
@@ -560,7 +579,8 @@ void launch_batch(const BatchDev &B, const float *X, float *Y, float alpha, floa
   if (B.ntasks == 0) return;
   hipLaunchKernelGGL(k_batch_spmm, dim3((unsigned)B.grid), dim3(kBlock), 0, s, B.mats.as<BatchMat>(),
                      B.tasks.as<int4>(), B.ntasks, B.rowptr.as<int>(), B.col.as<int>(), B.val.as<float>(),
-                     B.ent.as<int2>(), B.cbase.as<int64_t>(), B.cwidth.as<int>(), B.crow.as<int>(), X, Y, alpha, beta);
+                     B.ent.as<int2>(), B.cbase.as<int64_t>(), B.cwidth.as<int>(), B.crow.as<int>(), B.cwin.as<int2>(),
+                     X, Y, alpha, beta);
   SPOPS_HIP_CHECK(hipGetLastError());
 }
 
