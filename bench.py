@@ -210,6 +210,92 @@ def lespgemm_measure(scale, iterations, cpu_threads, cpu):
     return out
 
 
+def rowwise_topk_measure(scale, iterations, cpu_threads, cpu):
+    """The row-wise LeSpGEMM C = A*A (the reference's LeSpGEMM<sortOutput = true>, kernels 1-2;
+    spops_spgemm_csr on the device: A's duplicates merged, one row per cluster, the cluster
+    pipeline, CSR out) and the A*A^T top-k Jaccard candidate front end (HashSpGEMMTopK, top 7;
+    spops_spgemm_topk_jaccard) on R-MAT(scale, edge factor 16), FP64, device arrays in and out,
+    timed with HIP events around each call (output allocation included), beside the reference's
+    OpenMP kernels on this host's cores."""
+    import ctypes
+    import torch
+    from sparseops_amd import _lib as L
+    from sparseops_amd import lespgemm as LG
+    from sparseops_amd.gen import rmat
+    from sparseops_amd.io import write_csr_bin
+    A = rmat(scale, 16, seed=0)
+    dev = torch.device("cuda:0")
+    dA = LG.DeviceCsr(A, dev)
+    flop = LG.spgemm_flop(dA, dA)
+    stream = L.current_stream()
+    ev = [ctypes.c_void_p() for _ in range(2)]
+    for e in ev:
+        L.call("spops_event_create", ctypes.byref(e))
+
+    def timed(call):
+        """Mean ms of `iterations` calls after one warm-up; each returns device outputs, freed after."""
+        tot, nnz = 0.0, 0
+        for it in range(iterations + 1):
+            outs = [ctypes.c_void_p() for _ in range(3)]
+            n = ctypes.c_int64()
+            torch.cuda.synchronize()
+            L.call("spops_event_record", ev[0], stream)
+            call(n, outs)
+            L.call("spops_event_record", ev[1], stream)
+            torch.cuda.synchronize()
+            ms = ctypes.c_float()
+            L.call("spops_event_elapsed_ms", ctypes.byref(ms), ev[0], ev[1])
+            if it > 0:
+                tot += ms.value
+            nnz = n.value
+            for o in outs:
+                L.lib.spops_free(o)
+        return tot / iterations, nnz
+
+    st = (ctypes.c_int64 * 16)()
+    rw_ms, rw_nnz = timed(lambda n, o: L.call(
+        "spops_spgemm_csr", dA.n_rows, dA.n_cols, dA.nnz, dA.rowptr.data_ptr(), dA.col.data_ptr(), dA.val.data_ptr(),
+        dA.n_cols, dA.nnz, dA.rowptr.data_ptr(), dA.col.data_ptr(), dA.val.data_ptr(), L.VALUE_F64,
+        ctypes.byref(n), ctypes.byref(o[0]), ctypes.byref(o[1]), ctypes.byref(o[2]), st, stream))
+    tk_ms, tk_nnz = timed(lambda n, o: L.call(
+        "spops_spgemm_topk_jaccard", dA.n_rows, dA.n_cols, dA.nnz, dA.rowptr.data_ptr(), dA.col.data_ptr(),
+        L.VALUE_F64, 7, 0, ctypes.byref(n), ctypes.byref(o[0]), ctypes.byref(o[1]), ctypes.byref(o[2]), stream))
+    for e in ev:
+        L.call("spops_event_destroy", e)
+    out = {"workload": f"R-MAT scale {scale} (n={A.n_rows}, nnz={A.nnz}), FP64: row-wise C = A*A and the A*A^T "
+                       f"top-7 Jaccard candidates",
+           "flop": int(flop), "rowwise_gpu_ms": round(rw_ms, 3), "rowwise_gpu_gflops": round(flop / rw_ms / 1e6, 2),
+           "rowwise_nnz_C": int(rw_nnz), "topk_gpu_ms": round(tk_ms, 3), "topk_nnz": int(tk_nnz),
+           "iterations": iterations}
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if cpu and os.path.exists(exe):
+        with tempfile.TemporaryDirectory(dir="/tmp") as d:
+            path = os.path.join(d, "A.csr")
+            write_csr_bin(path, A)
+            env = cpu_env(cpu_threads)
+            for name, args in (("rowwise_kernel1", ["bench_rowwise", path, path, str(cpu_threads), "1", "1"]),
+                               ("rowwise_kernel2", ["bench_rowwise", path, path, str(cpu_threads), "1", "2"]),
+                               ("topk", ["bench_topk", path, str(cpu_threads), "1", "7"])):
+                p = subprocess.run([exe] + args, capture_output=True, text=True, env=env, timeout=900)
+                if p.returncode != 0:
+                    log(f"cpu {name} failed: {p.stderr[-300:]}")
+                    continue
+                c = json.loads(p.stdout.strip().splitlines()[-1])
+                ms = c["seconds"] * 1e3
+                o = {**HOST, "ms": round(ms, 2), "threads": c["threads"], "kind": "reference", "nnz_C": c["nnz_C"]}
+                if name.startswith("rowwise"):
+                    o["gflops"] = round(flop / ms / 1e6, 3)
+                    o["gpu_over_cpu"] = round(ms / rw_ms, 1)
+                    o["nnz_C_equal"] = c["nnz_C"] == rw_nnz
+                else:
+                    o["gpu_over_cpu"] = round(ms / tk_ms, 1)
+                    o["nnz_equal"] = c["nnz_C"] == tk_nnz
+                out["cpu_" + name] = o
+    del dA
+    torch.cuda.synchronize()
+    return out
+
+
 def lespgemm_panels_measure(scale, budget_gb, cpu_threads, cpu, cpu_share=0.02):
     """LeSpGEMM C = A*A on R-MAT(scale, edge factor 16) (k=64, bands=16, seed=7,
     cluster_size=8, hc_v=0, kernel 3) when C is too large for one piece: the plan (B
@@ -731,6 +817,8 @@ def main():
                     help="R-MAT scale of the panelled LeSpGEMM line (0: skip)")
     ap.add_argument("--lespgemm-budget-gb", type=float, default=32.0, help="output bytes per LeSpGEMM panel")
     ap.add_argument("--lespgemm-iters", type=int, default=5)
+    ap.add_argument("--rowwise-scale", type=int, default=16,
+                    help="R-MAT scale of the row-wise LeSpGEMM / A*A^T top-k line (0: skip)")
     ap.add_argument("--autograd-scale", type=int, default=20, help="R-MAT scale of the autograd SpMM line (0: skip)")
     ap.add_argument("--batch-count", type=int, default=512, help="matrices of the batched SpMM line (0: skip)")
     ap.add_argument("--bsr-n", type=int, default=1 << 18, help="rows of the dense-block BSR line (0: skip)")
@@ -825,6 +913,12 @@ def main():
         lg = lespgemm_measure(args.lespgemm_scale, args.lespgemm_iters, args.cpu_threads, not args.no_cpu)
         log(f"lespgemm done in {time.time() - t:.1f}s: {lg}")
 
+    rt = None
+    if args.rowwise_scale > 0:
+        t = time.time()
+        rt = rowwise_topk_measure(args.rowwise_scale, 3, args.cpu_threads, not args.no_cpu)
+        log(f"rowwise / topk done in {time.time() - t:.1f}s: {rt}")
+
     lp = None
     if args.lespgemm_panels_scale > 0:
         t = time.time()
@@ -899,6 +993,7 @@ def main():
                                                 if "csrrows_f64" in res and "sell_f32" in res else None),
         "lespgemm": lg,
         "lespgemm_panels": lp,
+        "rowwise_topk": rt,
         "autograd_spmm": ag,
         "batch_spmm": bt,
         "bsr_dense_blocks": bs,

@@ -35,6 +35,11 @@
 //   read_mtx <A.mtx> <outdir> <compact 0|1>
 //                         the reference's read_csr_matrix<int64_t, double> (utils/sparse_io.cpp:
 //                         26-214): rowptr, col and values, dimensions in "dims"
+//   bench_rowwise <A.csr> <B.csr> <threads> <iterations> <kernel 1|2|3>
+//                         the row-wise LeSpGEMM<sortOutput = true> in FP64: one warm-up, then the mean
+//                         wall time of the iterations and C's nonzero count
+//   bench_topk <A.csr> <threads> <iterations> <top_k>
+//                         HashSpGEMMTopK in FP64 (A A^T top-k Jaccard candidates): the same timing
 //   topk_hc <A.csr> <outdir> <top_k> <cluster_size> <prec>
 //                         HashSpGEMMTopK (C's rowptr, col, values), then the flow of
 //                         test/test_spgemm_hc.cpp on those pairs: C written as "%20.16g" text
@@ -567,6 +572,45 @@ int run_rowwise(const HostCsr &ha, const HostCsr &hb, const std::string &out, in
   return 0;
 }
 
+int bench_rowwise(const HostCsr &ha, const HostCsr &hb, int iterations, int kernel) {
+  CSR_Matrix<int64_t, double> A = to_ref_csr<int64_t, double>(ha), B = to_ref_csr<int64_t, double>(hb);
+  int64_t nnz = 0;
+  double total = 0.0;
+  for (int it = 0; it <= iterations; ++it) {
+    CSR_Matrix<int64_t, double> C;
+    const double t0 = omp_get_wtime();
+    LeSpGEMM<true, int64_t, double>(A, B, C, kernel);
+    const double t = omp_get_wtime() - t0;
+    if (it > 0) total += t;
+    nnz = C.num_nnzs;
+    delete_host_matrix(C);
+  }
+  std::printf("{\"threads\": %d, \"iterations\": %d, \"kernel\": %d, \"seconds\": %.9e, \"nnz_C\": %lld}\n",
+              Le_get_thread_num(), iterations, kernel, total / std::max(iterations, 1), (long long)nnz);
+  delete_host_matrix(A);
+  delete_host_matrix(B);
+  return 0;
+}
+
+int bench_topk(const HostCsr &ha, int iterations, int64_t top_k) {
+  CSR_Matrix<int64_t, double> A = to_ref_csr<int64_t, double>(ha);
+  int64_t nnz = 0;
+  double total = 0.0;
+  for (int it = 0; it <= iterations; ++it) {
+    CSR_Matrix<int64_t, double> C;
+    const double t0 = omp_get_wtime();
+    HashSpGEMMTopK<int64_t, double>(A, C, top_k);
+    const double t = omp_get_wtime() - t0;
+    if (it > 0) total += t;
+    nnz = C.num_nnzs;
+    delete_host_matrix(C);
+  }
+  std::printf("{\"threads\": %d, \"iterations\": %d, \"top_k\": %lld, \"seconds\": %.9e, \"nnz_C\": %lld}\n",
+              Le_get_thread_num(), iterations, (long long)top_k, total / std::max(iterations, 1), (long long)nnz);
+  delete_host_matrix(A);
+  return 0;
+}
+
 template <typename V>
 int run_topk_hc(const HostCsr &ha, const std::string &out, int64_t top_k, int64_t cluster_size) {
   CSR_Matrix<int64_t, V> A = to_ref_csr<int64_t, V>(ha), C;
@@ -673,6 +717,17 @@ int main(int argc, char **argv) {
     HostCsr B = read_csr_file(argv[3]);
     return std::atoi(argv[6]) == 32 ? run_rowwise<float>(A, B, argv[4], std::atoi(argv[5]))
                                     : run_rowwise<double>(A, B, argv[4], std::atoi(argv[5]));
+  }
+  if (mode == "bench_rowwise" && argc >= 7) {
+    HostCsr A = read_csr_file(argv[2]);
+    HostCsr B = read_csr_file(argv[3]);
+    Le_set_thread_num(std::atoi(argv[4]));
+    return bench_rowwise(A, B, std::atoi(argv[5]), std::atoi(argv[6]));
+  }
+  if (mode == "bench_topk" && argc >= 6) {
+    HostCsr A = read_csr_file(argv[2]);
+    Le_set_thread_num(std::atoi(argv[3]));
+    return bench_topk(A, std::atoi(argv[4]), std::atoll(argv[5]));
   }
   if (mode == "topk_hc" && argc >= 7) {
     HostCsr A = read_csr_file(argv[2]);
