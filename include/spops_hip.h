@@ -221,6 +221,19 @@ int spops_spmm_values(spops_matrix A, const void *d_values, int64_t k, double al
  * G row-major (n_rows x k), X row-major (n_cols x k), out[nnz] */
 int spops_sddmm(spops_matrix A, int64_t k, const void *d_G, int g_type, const void *d_X, int x_type, void *d_out,
                 int out_type, void *stream);
+/* The backward of Y = A X in one pass over AT = A^T (a handle made by spops_csr_transpose, its
+ * values set): dX = AT G (n_cols(A) x k, in X's type; alpha 1, beta 0) and dvals[e] =
+ * sum_c G[row_e, c] * X[col_e, c] for every entry e of A, in A's entry order and AT's value
+ * type.  Each row of G gathered for dX also serves the dot products, and X's rows are read once
+ * per row of AT; the dot products are written in AT's entry order into a buffer the handle
+ * keeps (nnz values, allocated on the first call with the inverse permutation) and gathered into
+ * A's order, so calls on one AT handle must be stream-ordered.  No reference counterpart (the
+ * differentiable form of SpOps_csr).  Returns
+ * SPOPS_ERROR_NOT_SUPPORTED, having done nothing, unless k is a multiple of the elements of G in
+ * 16 bytes and at most 64 times that, G is 16-byte aligned and X, dX are aligned to that many
+ * elements of their type; spops_spmm_typed on AT plus spops_sddmm on A compute the same. */
+int spops_spmm_sddmm_transposed(spops_matrix AT, int64_t k, const void *d_G, int g_type, const void *d_X, int x_type,
+                                void *d_dX, void *d_dvals, void *stream);
 /* AT = A^T as a new CSR handle; AT's entry e is A's entry perm[e] (kept inside AT), rows of
  * AT hold A's entries in A's row order */
 int spops_csr_transpose(spops_matrix A, spops_matrix *AT);

@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsparseops_hip.so")
 
 SUCCESS = 0
+NOT_SUPPORTED = 4
 INDEX_I32, INDEX_I64 = 0, 1
 VALUE_F64, VALUE_F32, VALUE_BF16 = 0, 1, 2
 FORMAT_CSR, FORMAT_SELL, FORMAT_ELL, FORMAT_BSR = 0, 1, 2, 3
@@ -61,6 +62,7 @@ _SIGNATURES = [
     ("spops_spmm_values", c_int, [c_vp, c_vp, c_i64, c_dbl, c_vp, c_int, c_dbl, c_vp, c_int, c_vp]),
     ("spops_spmm_typed_noalloc", c_int, [c_vp, c_i64, c_dbl, c_vp, c_int, c_dbl, c_vp, c_int, c_vp]),
     ("spops_sddmm", c_int, [c_vp, c_i64, c_vp, c_int, c_vp, c_int, c_vp, c_int, c_vp]),
+    ("spops_spmm_sddmm_transposed", c_int, [c_vp, c_i64, c_vp, c_int, c_vp, c_int, c_vp, c_vp, c_vp]),
     ("spops_csr_transpose", c_int, [c_vp, c_vp]),
     ("spops_csr_set_values", c_int, [c_vp, c_vp, c_int, c_vp]),
     ("spops_matrix_features", c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_i64, c_i64, c_vp, c_vp]),

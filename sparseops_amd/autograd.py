@@ -6,6 +6,8 @@ gradients.  Forward and backward call the C ABI of libsparseops_hip.so:
   dX    = A^T G                    spops_spmm_typed on a cached transposed handle whose values
                                    follow vals through the entry permutation
   dvals = SDDMM(G, X) on A's pattern   (dvals[e] = <G[row_e, :], X[col_e, :]>, spops_sddmm)
+When both gradients are needed they come from one pass over the transposed handle
+(spops_spmm_sddmm_transposed), where k and the alignment allow it.
 Accumulation is FP64 when the values or X are FP64, otherwise FP32 (BF16 X / G are widened).
 The transpose semantics are the reference's csr_transpose (include/sparse_conversion.h:2067-2108)
 without its duplicate merging: the pattern has no duplicates.
@@ -55,6 +57,16 @@ class _SpMM(torch.autograd.Function):
         pat = ctx.pat
         G = G.contiguous()
         dvals = dX = None
+        if ctx.needs_input_grad[0] and ctx.needs_input_grad[1] and vals.dtype == pat.value_dtype:
+            # both gradients in one pass over A^T: each gathered row of G serves dX and dvals
+            pat.AT.set_values(vals.detach())
+            dX = torch.empty_like(X)
+            dvals = torch.empty(pat.nnz, dtype=vals.dtype, device=vals.device)
+            if pat.AT.spmm_sddmm_transposed(G, X, dX, dvals):
+                return dvals, dX, None, None
+            pat.AT.spmm(G, dX)
+            pat.A.sddmm(G, X, dvals)
+            return dvals, dX, None, None
         if ctx.needs_input_grad[1]:
             # one permuting gather of the values, then the product on contiguous values (reading
             # them through the permutation inside the gather-bound product measured slower)

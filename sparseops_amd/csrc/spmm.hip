@@ -20,6 +20,10 @@
 //                    cross-lane sum (k a multiple of the vector, at most 64 vectors);
 //   k_sddmm        : otherwise entry-parallel, a group of L lanes per entry with loads of
 //                    both the G and the X row, cross-lane sum.
+//   k_spmm_sddmm_rows / k_spmm_sddmm_seg : the differentiable backward in one pass over a
+//                    transposed handle: k_spmm_rows / k_spmm_seg for dX = A^T G plus each entry's
+//                    dot product with the row of X (loaded once per row of A^T), written in
+//                    A^T's order and gathered into A's by the inverse permutation.
 #include <algorithm>
 #include <memory>
 #include <mutex>
@@ -220,6 +224,208 @@ __global__ __launch_bounds__(kBlock) void k_spmm_seg_sum(const int *__restrict__
     if (beta != 0.0) o += beta * Ty<YC>::ld(Y[row * k + c]);
     Y[row * k + c] = Ty<YC>::st(o);
   }
+}
+
+// Sums U per-lane partials (one per entry of a step) over the L lanes of each lane group (L a
+// power of two, groups aligned to L).  Each step pairs lanes q and q ^ m and halves the entries
+// a lane holds: the lane keeps the lower or upper half (by bit m of q) and adds the partner's
+// values of that half, so the U sums take U - 1 + log2(L / U) shuffles (instead of U log2 L).
+// Afterwards lane q holds the complete sums of entries base + i in part[i], i < U / min(L, U);
+// the return value is base.  Lanes with q >= U hold copies.
+template <int U, typename A>
+__device__ __forceinline__ int group_sum_entries(A (&part)[U], int q, int L) {
+  int base = 0;
+#pragma unroll
+  for (int s = U / 2, m = 1; s >= 1; s >>= 1, m <<= 1) {
+    if (m < L) {
+      const bool hi = (q & m) != 0;
+#pragma unroll
+      for (int i = 0; i < s; ++i) {
+        const A send = hi ? part[i] : part[i + s];
+        const A keep = hi ? part[i + s] : part[i];
+        part[i] = keep + __shfl_xor(send, m, kWave);
+      }
+      if (hi) base += s;
+    }
+  }
+  for (int m = U; m < L; m <<= 1) part[0] += __shfl_xor(part[0], m, kWave);
+  return base;
+}
+
+// out[p + base + i] = part[i] for the entries a lane holds after group_sum_entries (p + u < p1)
+template <int U, int OC, typename A>
+__device__ __forceinline__ void group_write_entries(const A (&part)[U], int base, int q, int L, int64_t p, int64_t p1,
+                                                    int64_t stride, typename Ty<OC>::T *__restrict__ out) {
+  if (q >= U) return;
+  const int held = L >= U ? 1 : U / L;
+#pragma unroll
+  for (int i = 0; i < U; ++i) {
+    const int64_t e = p + (int64_t)(base + i) * stride;
+    if (i < held && e < p1) out[e] = Ty<OC>::st((double)part[i]);
+  }
+}
+
+// The backward of the differentiable SpMM in one pass over the transposed handle AT (its rows
+// are the source matrix's columns): dX = AT G as k_spmm_rows computes it (alpha 1, beta 0), and
+// for each entry e of AT, dT[e] = <G[col_e, :], X[row, :]>, the SDDMM of the source matrix's
+// entry tperm[e], in AT's entry order (a gather through the inverse permutation puts it in the
+// source's order).  Each gathered row of G serves both products, and the row of X is loaded
+// once per AT row, so the SDDMM's gathers of X rows disappear.  One lane group covers all k
+// columns (L VEC >= k); the U dot products of a step are summed by group_sum_entries.
+template <int VC, int GC, int XC, typename RP, int VEC, int U>
+__global__ __launch_bounds__(kBlock) void k_spmm_sddmm_rows(const RP *__restrict__ rowptr, const int *__restrict__ col,
+                                                            const typename Ty<VC>::T *__restrict__ val,
+                                                            const int *__restrict__ order, int64_t n_short, int k, int L,
+                                                            const typename Ty<GC>::T *__restrict__ G,
+                                                            const typename Ty<XC>::T *__restrict__ X,
+                                                            typename Ty<XC>::T *__restrict__ dX,
+                                                            typename Ty<VC>::T *__restrict__ dT) {
+  using Acc = AccT<VC, GC>;
+  using Ad = AccT<GC, XC>;
+  using GV = typename Vec<GC, VEC>::V;
+  using XV = typename Vec<XC, VEC>::V;
+  const int lane = lane_id(), R = kWave / L, g = lane / L, q = lane % L;
+  const int64_t slot = ((int64_t)blockIdx.x * (kBlock / kWave) + wave_id()) * R + g;
+  if (slot >= n_short) return;                      // lane groups past the last row
+  const int64_t row = order[slot];
+  const int64_t p0 = (int64_t)rowptr[row], p1 = (int64_t)rowptr[row + 1];
+  const int c = q * VEC;
+  const bool cv = c < k;
+  Ad xr[VEC];
+  {
+    XV t{};
+    if (cv) t = *reinterpret_cast<const XV *>(X + row * k + c);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) xr[e] = (Ad)Ty<XC>::ld(t.e[e]);
+  }
+  Acc acc[VEC];
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) acc[e] = 0;
+  for (int64_t p = p0; p < p1; p += U) {
+    int cc[U];
+    Acc vv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool ok = p + u < p1;
+      cc[u] = ok ? col[p + u] : 0;
+      vv[u] = ok ? (Acc)Ty<VC>::ld(val[p + u]) : Acc(0);
+    }
+    GV gv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      GV t{};
+      if (cv && p + u < p1) t = *reinterpret_cast<const GV *>(G + (int64_t)cc[u] * k + c);
+      gv[u] = t;
+    }
+    Ad part[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      part[u] = 0;
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) {
+        acc[e] = vv[u] * (Acc)Ty<GC>::ld(gv[u].e[e]) + acc[e];
+        part[u] += (Ad)Ty<GC>::ld(gv[u].e[e]) * xr[e];
+      }
+    }
+    const int base = group_sum_entries<U>(part, q, L);
+    group_write_entries<U, VC>(part, base, q, L, p, p1, 1, dT);
+  }
+  if (cv) {
+    XV o;
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) o.e[e] = Ty<XC>::st((double)acc[e]);
+    *reinterpret_cast<XV *>(dX + row * k + c) = o;
+  }
+}
+
+// The same for the segments of AT's long rows (k_spmm_seg's partition of the entries): the
+// segment's partial row of dX to part[], and each entry's dot product with the row of X
+// (loaded once per segment) summed over its slot's L lanes and written to dT[e].
+template <int VC, int GC, int XC, typename RP, int VEC>
+__global__ __launch_bounds__(kBlock) void k_spmm_sddmm_seg(const RP *__restrict__ rowptr, const int *__restrict__ col,
+                                                           const typename Ty<VC>::T *__restrict__ val,
+                                                           const int *__restrict__ long_rows,
+                                                           const int2 *__restrict__ segs, int k, int L,
+                                                           const typename Ty<GC>::T *__restrict__ G,
+                                                           const typename Ty<XC>::T *__restrict__ X,
+                                                           AccT<VC, GC> *__restrict__ part,
+                                                           typename Ty<VC>::T *__restrict__ dT) {
+  using Acc = AccT<VC, GC>;
+  using Ad = AccT<GC, XC>;
+  using GV = typename Vec<GC, VEC>::V;
+  using XV = typename Vec<XC, VEC>::V;
+  constexpr int W = kBlock / kWave, U = 4;
+  __shared__ Acc red[W][kWave * VEC];
+  const int2 sg = segs[blockIdx.x];
+  const int64_t row = long_rows[sg.x];
+  const int64_t q0 = (int64_t)rowptr[row] + sg.y;
+  const int64_t q1 = std::min<int64_t>((int64_t)rowptr[row + 1], q0 + kSpmmSegment);
+  const int lane = lane_id(), w = wave_id(), E = kWave / L, es = lane / L, q = lane % L;
+  const int64_t stride = (int64_t)W * E;
+  const int c = q * VEC;
+  const bool cv = c < k;
+  Ad xr[VEC];
+  {
+    XV t{};
+    if (cv) t = *reinterpret_cast<const XV *>(X + row * k + c);
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) xr[e] = (Ad)Ty<XC>::ld(t.e[e]);
+  }
+  Acc acc[VEC];
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) acc[e] = 0;
+  for (int64_t p = q0 + (int64_t)w * E + es; p < q1; p += stride * U) {
+    int cc[U];
+    Acc vv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t e = p + u * stride;
+      const bool ok = e < q1;
+      cc[u] = ok ? col[e] : 0;
+      vv[u] = ok ? (Acc)Ty<VC>::ld(val[e]) : Acc(0);
+    }
+    GV gv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      GV t{};
+      if (cv && p + u * stride < q1) t = *reinterpret_cast<const GV *>(G + (int64_t)cc[u] * k + c);
+      gv[u] = t;
+    }
+    Ad pd[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      pd[u] = 0;
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) {
+        acc[e] = vv[u] * (Acc)Ty<GC>::ld(gv[u].e[e]) + acc[e];
+        pd[u] += (Ad)Ty<GC>::ld(gv[u].e[e]) * xr[e];
+      }
+    }
+    const int base = group_sum_entries<U>(pd, q, L);
+    group_write_entries<U, VC>(pd, base, q, L, p, q1, stride, dT);
+  }
+  for (int m = L; m < kWave; m <<= 1)
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) acc[e] += __shfl_xor(acc[e], m, kWave);
+  if (es == 0)
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) red[w][q * VEC + e] = acc[e];
+  __syncthreads();
+  if (w == 0 && es == 0 && cv) {
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+      Acc t = red[0][q * VEC + e];
+#pragma unroll
+      for (int v = 1; v < W; ++v) t += red[v][q * VEC + e];
+      part[(int64_t)blockIdx.x * k + c + e] = t;
+    }
+  }
+}
+
+// inv[perm[e]] = e (the inverse of a transposed handle's entry permutation)
+__global__ __launch_bounds__(kBlock) void k_invert_perm(const int *__restrict__ perm, int64_t n, int *__restrict__ inv) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) inv[perm[i]] = (int)i;
 }
 
 // out[e] = sum_c G[row_e, c] * X[col_e, c], entry-parallel: a group of L lanes per entry
@@ -563,6 +769,76 @@ void convert_d(const void *src, const void *perm, bool perm64, int64_t n, void *
   }
 }
 
+// The fused backward on a transposed handle (k_spmm_sddmm_rows / _seg, then k_spmm_seg_sum for
+// the long rows' dX), the dot products in AT's order into the handle's buffer, then gathered
+// into the source matrix's order through the inverse permutation (built on first use); the
+// caller has checked that one lane group covers k with 16-byte G loads.
+template <int VC, int GC, int XC>
+void spmm_sddmm_launch(const spops_matrix_s &AT, int k, const void *G, const void *X, void *dX, void *dvals,
+                       hipStream_t s) {
+  constexpr int V = 16 / (int)sizeof(typename Ty<GC>::T);
+  using Acc = AccT<VC, GC>;
+  using VT = typename Ty<VC>::T;
+  const auto &C = AT.csr;
+  const int L = lanes_for(k, V);
+  const int *rp = C.rowptr.as<int>();
+  const auto *v = C.val.as<VT>();
+  const auto *g = static_cast<const typename Ty<GC>::T *>(G);
+  const auto *x = static_cast<const typename Ty<XC>::T *>(X);
+  auto *dx = static_cast<typename Ty<XC>::T *>(dX);
+  if (AT.nnz > 0) {
+    if (C.tinv.p == nullptr) {
+      C.tinv.alloc((size_t)AT.nnz * 4);
+      hipLaunchKernelGGL(k_invert_perm, dim3(nblk(AT.nnz, kBlock)), dim3(kBlock), 0, s, C.tperm.as<int>(), AT.nnz,
+                         C.tinv.as<int>());
+      SPOPS_HIP_CHECK(hipGetLastError());
+    }
+    if (C.tdv.bytes < (size_t)AT.nnz * sizeof(VT)) C.tdv.alloc((size_t)AT.nnz * sizeof(VT));
+  }
+  VT *dT = C.tdv.as<VT>();
+  if (C.nshort > 0) {
+    const int64_t rows_per_block = (int64_t)(kBlock / kWave) * (kWave / L);
+    // four entries per step: 59 VGPRs, eight waves per SIMD (eight entries: 99 VGPRs, four waves,
+    // 610 against 536 us at R-MAT 20)
+    hipLaunchKernelGGL((k_spmm_sddmm_rows<VC, GC, XC, int, V, 4>), dim3(nblk(C.nshort, (int)rows_per_block)),
+                       dim3(kBlock), 0, s, rp, C.col.as<int>(), v, C.sorder.as<int>(), (int64_t)C.nshort, k, L, g, x, dx,
+                       dT);
+    SPOPS_HIP_CHECK(hipGetLastError());
+  }
+  if (C.nslong > 0) {
+    const size_t need = (size_t)C.nlseg * (size_t)k * sizeof(Acc);
+    if (C.spmm_part.bytes < need) C.spmm_part.alloc(need);
+    Acc *part = C.spmm_part.as<Acc>();
+    hipLaunchKernelGGL((k_spmm_sddmm_seg<VC, GC, XC, int, V>), dim3((unsigned)C.nlseg), dim3(kBlock), 0, s, rp,
+                       C.col.as<int>(), v, C.slongs.as<int>(), C.lsegs.as<int2>(), k, L, g, x, part, dT);
+    SPOPS_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL((k_spmm_seg_sum<Acc, XC>), dim3((unsigned)C.nslong), dim3(kBlock), 0, s, C.slongs.as<int>(),
+                       C.lsegoff.as<int>(), part, k, dx, 1.0, 0.0);
+    SPOPS_HIP_CHECK(hipGetLastError());
+  }
+  if (AT.nnz > 0) convert_launch<VC, VC>(dT, C.tinv.p, false, AT.nnz, dvals, s);
+}
+
+template <int VC, int GC>
+void spmm_sddmm_x(const spops_matrix_s &AT, int k, const void *G, const void *X, int xt, void *dX, void *dvals,
+                  hipStream_t s) {
+  switch (xt) {
+    case SPOPS_VALUE_F64: spmm_sddmm_launch<VC, GC, SPOPS_VALUE_F64>(AT, k, G, X, dX, dvals, s); break;
+    case SPOPS_VALUE_F32: spmm_sddmm_launch<VC, GC, SPOPS_VALUE_F32>(AT, k, G, X, dX, dvals, s); break;
+    default: spmm_sddmm_launch<VC, GC, SPOPS_VALUE_BF16>(AT, k, G, X, dX, dvals, s); break;
+  }
+}
+
+template <int VC>
+void spmm_sddmm_g(const spops_matrix_s &AT, int k, const void *G, int gt, const void *X, int xt, void *dX,
+                  void *dvals, hipStream_t s) {
+  switch (gt) {
+    case SPOPS_VALUE_F64: spmm_sddmm_x<VC, SPOPS_VALUE_F64>(AT, k, G, X, xt, dX, dvals, s); break;
+    case SPOPS_VALUE_F32: spmm_sddmm_x<VC, SPOPS_VALUE_F32>(AT, k, G, X, xt, dX, dvals, s); break;
+    default: spmm_sddmm_x<VC, SPOPS_VALUE_BF16>(AT, k, G, X, xt, dX, dvals, s); break;
+  }
+}
+
 void convert_values(const void *src, int st, const void *perm, bool perm64, int64_t n, void *dst, int dt,
                     hipStream_t s) {
   switch (st) {
@@ -693,6 +969,40 @@ extern "C" int spops_sddmm(spops_matrix A, int64_t k, const void *d_G, int g_typ
       case SPOPS_VALUE_F64: sddmm_x<SPOPS_VALUE_F64>(*A, (int)k, d_G, d_X, x_type, d_out, out_type, s); break;
       case SPOPS_VALUE_F32: sddmm_x<SPOPS_VALUE_F32>(*A, (int)k, d_G, d_X, x_type, d_out, out_type, s); break;
       default: sddmm_x<SPOPS_VALUE_BF16>(*A, (int)k, d_G, d_X, x_type, d_out, out_type, s); break;
+    }
+    return SPOPS_SUCCESS;
+  });
+}
+
+extern "C" int spops_spmm_sddmm_transposed(spops_matrix AT, int64_t k, const void *d_G, int g_type, const void *d_X,
+                                          int x_type, void *d_dX, void *d_dvals, void *stream) {
+  return guarded([&]() {
+    require_csr(AT);
+    require_type(g_type, "G");
+    require_type(x_type, "X");
+    SPOPS_REQUIRE(AT->csr.tperm.p != nullptr || AT->nnz == 0, SPOPS_ERROR_INVALID_VALUE,
+                  "spops_spmm_sddmm_transposed: AT must come from spops_csr_transpose");
+    SPOPS_REQUIRE(!AT->csr.tperm64, SPOPS_ERROR_NOT_SUPPORTED,
+                  "spops_spmm_sddmm_transposed: a transposed handle of >= 2^31 entries");
+    SPOPS_REQUIRE(k > 0 && k < (1 << 20), SPOPS_ERROR_INVALID_VALUE, "k must be in [1, 2^20)");
+    SPOPS_REQUIRE((d_G != nullptr || AT->n_cols == 0) && (d_X != nullptr && d_dX != nullptr || AT->n_rows == 0) &&
+                      (d_dvals != nullptr || AT->nnz == 0),
+                  SPOPS_ERROR_INVALID_VALUE, "null G, X, dX or dvals");
+    // one lane group over all k columns with 16-byte loads of G and aligned rows of X and dX
+    const size_t gs = value_bytes(g_type), xs = value_bytes(x_type);
+    const int64_t V = 16 / (int64_t)gs;
+    const size_t xa = (size_t)V * xs;
+    SPOPS_REQUIRE(k % V == 0 && k <= kWave * V && ((uintptr_t)d_G % 16) == 0 && ((uintptr_t)d_X % xa) == 0 &&
+                      ((uintptr_t)d_dX % xa) == 0,
+                  SPOPS_ERROR_NOT_SUPPORTED,
+                  "spops_spmm_sddmm_transposed: k = %lld needs to be a multiple of %lld and at most %lld, G 16-byte "
+                  "aligned and X, dX %zu-byte aligned (use spops_spmm_typed and spops_sddmm)",
+                  (long long)k, (long long)V, (long long)(kWave * V), xa);
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    switch (AT->value_type) {
+      case SPOPS_VALUE_F64: spmm_sddmm_g<SPOPS_VALUE_F64>(*AT, (int)k, d_G, g_type, d_X, x_type, d_dX, d_dvals, s); break;
+      case SPOPS_VALUE_F32: spmm_sddmm_g<SPOPS_VALUE_F32>(*AT, (int)k, d_G, g_type, d_X, x_type, d_dX, d_dvals, s); break;
+      default: spmm_sddmm_g<SPOPS_VALUE_BF16>(*AT, (int)k, d_G, g_type, d_X, x_type, d_dX, d_dvals, s); break;
     }
     return SPOPS_SUCCESS;
   });

@@ -34,6 +34,8 @@ struct CsrDev {
   DevBuf lsegoff;               // SpMM: first segment of each long row (nslong + 1 ints)
   mutable DevBuf spmm_part;     // SpMM: partial sums of the segments (grown on demand)
   mutable DevBuf erow;          // SDDMM: row of every entry (built on first use)
+  mutable DevBuf tinv;          // transposed view: the inverse of tperm (built on first use)
+  mutable DevBuf tdv;           // transposed view: the fused backward's dot products in its own entry order
   DevBuf sdtasks;               // SDDMM by rows: int2 {row, first entry in the row}, chunks of kSddmmChunk
   int64_t nsdtasks = 0;
   bool rowptr64 = false, tperm64 = false;

@@ -130,6 +130,30 @@ class SpMatrix:
                out.data_ptr(), TORCH_TYPES[out.dtype], s)
         return out
 
+    def spmm_sddmm_transposed(self, G: torch.Tensor, X: torch.Tensor, dX: torch.Tensor, dvals: torch.Tensor,
+                              stream=None) -> bool:
+        """On a transpose() handle AT of A: dX = AT G and dvals[e] = <G[row_e, :], X[col_e, :]> over
+        A's entries (A's entry order, this handle's value type) in one pass.  Returns False, having
+        done nothing, where the fused kernels do not apply (k or alignment); then spmm on this
+        handle and sddmm on A compute the same."""
+        k = _rhs_check(G, self.n_cols, "G")
+        if X.dim() != 2 or X.shape != (self.n_rows, k) or not X.is_contiguous() or not X.is_cuda:
+            raise ValueError("X must be a contiguous device tensor of shape (n_rows, k)")
+        if dX.shape != X.shape or dX.dtype != X.dtype or not dX.is_contiguous() or not dX.is_cuda:
+            raise ValueError("dX must be a contiguous device tensor shaped and typed like X")
+        if (dvals.numel() != self.nnz or not dvals.is_contiguous() or not dvals.is_cuda
+                or TORCH_TYPES.get(dvals.dtype) != self.info[4]):
+            raise ValueError("dvals must be a contiguous device tensor of nnz elements of the stored value type")
+        if X.dtype not in TORCH_TYPES:
+            raise TypeError("X must be float64, float32 or bfloat16")
+        s = L.current_stream() if stream is None else stream
+        st = L.lib.spops_spmm_sddmm_transposed(self._h, k, G.data_ptr(), TORCH_TYPES[G.dtype], X.data_ptr(),
+                                               TORCH_TYPES[X.dtype], dX.data_ptr(), dvals.data_ptr(), s)
+        if st == L.NOT_SUPPORTED:
+            return False
+        L.check(st, "spops_spmm_sddmm_transposed")
+        return True
+
     def transpose(self) -> "SpMatrix":
         """A^T as a CSR handle whose values follow this matrix's (see set_values)."""
         h = ctypes.c_void_p()

@@ -269,3 +269,93 @@ def test_spmm_sddmm_unaligned_views(mats, offset, dev):
     _close(Y.double().cpu().numpy(), orc.spmm(Ar, X64), orc.spmm(absA, np.abs(X64)), 2e-6)
     _close(out.double().cpu().numpy(), orc.sddmm(A, G64, X64), orc.sddmm(A, np.abs(G64), np.abs(X64)), 2e-6)
     assert torch.all(yb[:offset] == 0) and torch.all(yb[offset + A.n_rows * k:] == 0)
+
+
+def _hub_columns():
+    """The transpose of _edge(): columns of 2600 and 4100 entries, so A^T has rows past the
+    256-entry limit (the segment kernels), besides empty and short ones."""
+    A = _edge(5)
+    trp, tc, tv, _ = orc.transpose(A)
+    return Csr(A.n_cols, A.n_rows, trp.astype(np.int32), tc.astype(np.int32), tv)
+
+
+@pytest.mark.parametrize("k", [8, 64, "max"])
+@pytest.mark.parametrize("vt,gt,xt", [("f64", torch.float64, torch.float64), ("f32", torch.float32, torch.float32),
+                                      ("f32", torch.float32, torch.bfloat16), ("f32", torch.bfloat16, torch.bfloat16),
+                                      ("bf16", torch.float32, torch.bfloat16)])
+@pytest.mark.parametrize("name", ["hub", "rmat"])
+def test_spmm_sddmm_transposed(mats, name, vt, gt, xt, k, dev):
+    """One pass over A^T for both gradients: dX equals the separate product on A^T bit for
+    bit (the same sums in the same order), dvals (in A's entry order) equals the oracle's
+    SDDMM and the separate SDDMM on A within the accumulation bound."""
+    A = _hub_columns() if name == "hub" else mats[name]
+    V = 16 // torch.tensor([], dtype=gt).element_size()
+    k = 64 * V if k == "max" else k
+    rng = np.random.default_rng(k)
+    tdt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[vt]
+    vals = torch.tensor(rng.standard_normal(A.nnz), device=dev).to(tdt)
+    G = torch.tensor(rng.standard_normal((A.n_rows, k)), device=dev).to(gt)
+    X = torch.tensor(rng.standard_normal((A.n_cols, k)), device=dev).to(xt)
+    M = S.csr(A, vt, spmv_layout=1)
+    M.set_values(vals)
+    AT = M.transpose()
+    AT.set_values(vals)
+    dX = torch.full_like(X, float("nan"))
+    dv = torch.full((A.nnz,), float("nan"), dtype=tdt, device=dev)
+    assert AT.spmm_sddmm_transposed(G, X, dX, dv)
+    dX2 = torch.full_like(X, float("nan"))
+    AT.spmm(G, dX2)
+    dv2 = torch.zeros(A.nnz, dtype=tdt, device=dev)
+    M.sddmm(G, X, dv2)
+    torch.cuda.synchronize()
+    assert torch.equal(dX, dX2)
+    Gn, Xn = G.double().cpu().numpy(), X.double().cpu().numpy()
+    tol = 1e-13 if gt == torch.float64 else 2e-6
+    if vt == "bf16":
+        tol += 2 ** -8                                  # the BF16 output rounds each value once
+    scale = orc.sddmm(A, np.abs(Gn), np.abs(Xn))
+    _close(dv.double().cpu().numpy(), orc.sddmm(A, Gn, Xn), scale, tol)
+    _close(dv.double().cpu().numpy(), dv2.double().cpu().numpy(), scale, 2 * tol)
+
+
+@pytest.mark.parametrize("k", ["odd", "wide"])
+def test_spmm_sddmm_transposed_declines(mats, k, dev):
+    """k not a multiple of G's 16-byte vector, or wider than 64 vectors: the call returns False
+    and writes nothing; a handle that is not a transpose is refused."""
+    A = mats["rmat"]
+    k = 6 if k == "odd" else 260
+    rng = np.random.default_rng(3)
+    M = S.csr(A, "f32", spmv_layout=1)
+    AT = M.transpose()
+    G = torch.tensor(rng.standard_normal((A.n_rows, k)), device=dev).float()
+    X = torch.tensor(rng.standard_normal((A.n_cols, k)), device=dev).float()
+    dX = torch.full_like(X, 7.0)
+    dv = torch.full((A.nnz,), 7.0, device=dev)
+    assert not AT.spmm_sddmm_transposed(G, X, dX, dv)
+    torch.cuda.synchronize()
+    assert torch.all(dX == 7.0) and torch.all(dv == 7.0)
+    with pytest.raises(Exception):
+        M.spmm_sddmm_transposed(X, G, dX, dv)        # A is square: the shapes pass, the handle does not
+
+
+def test_layer_backward_fused_matches_separate(dev):
+    """The layer's backward (one fused pass) gives the gradients of the separate calls."""
+    A = rmat_symnorm(11, 16, seed=3)
+    pat = AG.SparsePattern(A, "f32")
+    rng = np.random.default_rng(4)
+    vals = torch.tensor(A.val, device=dev, requires_grad=True)
+    X = torch.tensor(rng.standard_normal((A.n_cols, 64)), device=dev).to(torch.bfloat16).requires_grad_()
+    G = torch.tensor(rng.standard_normal((A.n_rows, 64)), device=dev).float()
+    Y = AG.spmm(pat, vals, X, out_dtype=torch.float32)
+    Y.backward(G)
+    AT = pat.AT
+    AT.set_values(vals.detach())
+    dX = torch.empty_like(X)
+    AT.spmm(G, dX)
+    dv = torch.empty(A.nnz, device=dev)
+    pat.A.sddmm(G, X.detach(), dv)
+    torch.cuda.synchronize()
+    assert torch.equal(X.grad, dX)
+    Gn, Xn = G.double().cpu().numpy(), X.detach().double().cpu().numpy()
+    A64 = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, A.val.astype(np.float64))
+    _close(vals.grad.double().cpu().numpy(), dv.double().cpu().numpy(), orc.sddmm(A64, np.abs(Gn), np.abs(Xn)), 4e-6)
