@@ -226,9 +226,11 @@ int spops_sddmm(spops_matrix A, int64_t k, const void *d_G, int g_type, const vo
  * sum_c G[row_e, c] * X[col_e, c] for every entry e of A, in A's entry order and AT's value
  * type.  Each row of G gathered for dX also serves the dot products, and X's rows are read once
  * per row of AT; the dot products are written in AT's entry order into a buffer the handle
- * keeps (nnz values, allocated on the first call with the inverse permutation) and gathered into
- * A's order, so calls on one AT handle must be stream-ordered.  No reference counterpart (the
- * differentiable form of SpOps_csr).  Returns
+ * keeps and moved into A's order by a two-pass permutation (coalesced passes through LDS instead
+ * of a random gather).  The first call builds both permutations of the handle on the host (about
+ * 16 bytes per entry of device memory, plus two nnz-value buffers); afterwards
+ * spops_csr_set_values on the handle uses the forward one too.  Calls on one AT handle must be
+ * stream-ordered.  No reference counterpart (the differentiable form of SpOps_csr).  Returns
  * SPOPS_ERROR_NOT_SUPPORTED, having done nothing, unless k is a multiple of the elements of G in
  * 16 bytes and at most 64 times that, G is 16-byte aligned and X, dX are aligned to that many
  * elements of their type; spops_spmm_typed on AT plus spops_sddmm on A compute the same. */

@@ -422,10 +422,85 @@ __global__ __launch_bounds__(kBlock) void k_spmm_sddmm_seg(const RP *__restrict_
   }
 }
 
-// inv[perm[e]] = e (the inverse of a transposed handle's entry permutation)
-__global__ __launch_bounds__(kBlock) void k_invert_perm(const int *__restrict__ perm, int64_t n, int *__restrict__ inv) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) inv[perm[i]] = (int)i;
+// The two passes of a PermPlan (bit copies of T-sized elements, tiles of TS elements: TS * sizeof(T)
+// bytes of LDS).  A thread handles elements t + i kBlock of the tile (every load instruction
+// coalesced), sixteen at a time: all sixteen loads are issued before their LDS stores, so a
+// workgroup keeps 16 K loads in flight instead of one per thread (at R-MAT 20 pass 1 took 91 us
+// with batches of eight against 74 us, pass 2 303 us with one load per thread in each step's
+// chain against 116 us).  Pass 1: workgroup t takes
+// sources t TS.., orders them in LDS by destination tile and writes the ordered tile back to
+// the same range of the staging array (coalesced).
+constexpr int kPermBatch = 16;
+
+template <typename T, int TS>
+__global__ __launch_bounds__(kBlock) void k_perm_tiles(const T *__restrict__ x, const uint16_t *__restrict__ loc1,
+                                                       int64_t n, T *__restrict__ stg) {
+  constexpr int B = kPermBatch;
+  static_assert(TS % (kBlock * B) == 0, "whole batches per tile");
+  __shared__ T lds[TS];
+  const int64_t j0 = (int64_t)blockIdx.x * TS;
+  const int cnt = (int)std::min<int64_t>(TS, n - j0);
+  // 32-bit offsets into buffer resources over the tile (offsets past it load 0, not stored)
+  const rsrc_t rx = make_rsrc(x + j0, (uint64_t)cnt * sizeof(T));
+  const rsrc_t rl = make_rsrc(loc1 + j0, (uint64_t)cnt * 2);
+  const int t = (int)threadIdx.x;
+  for (int q0 = 0; q0 < TS; q0 += kBlock * B) {
+    T v[B];
+    uint16_t l[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      const uint32_t q = (uint32_t)(q0 + t + i * kBlock);
+      v[i] = bload<T>(rx, q * (uint32_t)sizeof(T));
+      l[i] = bload<uint16_t>(rl, q * 2u);
+    }
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      if (q0 + t + i * kBlock < cnt) lds[l[i]] = v[i];
+  }
+  __syncthreads();
+  T *ss = stg + (int64_t)blockIdx.x * (TS + kPermPad);
+  for (int q = t; q < cnt; q += kBlock) ss[q] = lds[q];
+}
+
+// Pass 2: workgroup b takes destination tile b: its elements, which lie in every staged source
+// tile as one run (src2: their staging addresses, run after run), to their places in LDS
+// (loc2), then the tile out (coalesced).
+template <typename T, int TS>
+__global__ __launch_bounds__(kBlock) void k_perm_buckets(const T *__restrict__ stg, int64_t n_stage,
+                                                         const int *__restrict__ src2,
+                                                         const uint16_t *__restrict__ loc2, int64_t n,
+                                                         T *__restrict__ y) {
+  // per batch: the addresses, then the data loads, then the LDS stores (32-bit offsets into
+  // buffer resources: the staging array stays below 2 GiB; offsets past the tile load 0 and
+  // are not stored)
+  constexpr int B = kPermBatch;
+  static_assert(TS % (kBlock * B) == 0, "whole batches per tile");
+  __shared__ T lds[TS];
+  const int64_t k0 = (int64_t)blockIdx.x * TS;
+  const int cnt = (int)std::min<int64_t>(TS, n - k0);
+  const rsrc_t ra = make_rsrc(src2 + k0, (uint64_t)cnt * 4);
+  const rsrc_t rl = make_rsrc(loc2 + k0, (uint64_t)cnt * 2);
+  const rsrc_t rs = make_rsrc(stg, (uint64_t)n_stage * sizeof(T));
+  const int t = (int)threadIdx.x;
+  for (int q0 = 0; q0 < TS; q0 += kBlock * B) {
+    uint32_t a[B];
+    uint16_t l[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+      const uint32_t q = (uint32_t)(q0 + t + i * kBlock);
+      a[i] = bload<uint32_t>(ra, q * 4u);
+      l[i] = bload<uint16_t>(rl, q * 2u);
+    }
+    T v[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i) v[i] = bload<T>(rs, a[i] * (uint32_t)sizeof(T));
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      if (q0 + t + i * kBlock < cnt) lds[l[i]] = v[i];
+  }
+  __syncthreads();
+  T *ys = y + k0;
+  for (int q = t; q < cnt; q += kBlock) ys[q] = lds[q];
 }
 
 // out[e] = sum_c G[row_e, c] * X[col_e, c], entry-parallel: a group of L lanes per entry
@@ -769,6 +844,86 @@ void convert_d(const void *src, const void *perm, bool perm64, int64_t n, void *
   }
 }
 
+// The plan of y[i] = x[g[i]] (g a permutation of 0..n-1; see PermPlan).  Source j goes to
+// destination d = inv[j], tile d / S.  Pass 1 orders each source tile by destination tile
+// (loc1, stable); jof[a] is the source at staging address a.  Pass 2's list for destination tile
+// b walks the staging addresses in order and keeps those whose source goes to b (src2), with
+// their places in the tile (loc2).
+void build_perm_plan(const std::vector<int32_t> &g, int64_t S, PermPlan &P) {
+  const int64_t n = (int64_t)g.size();
+  const int64_t nt = (n + S - 1) / S;
+  std::vector<int32_t> inv((size_t)n), jof((size_t)n), src2((size_t)n);
+  std::vector<uint16_t> loc1((size_t)n), loc2((size_t)n);
+  for (int64_t i = 0; i < n; ++i) inv[(size_t)g[(size_t)i]] = (int32_t)i;
+  std::vector<int64_t> cnt((size_t)nt + 1);
+  for (int64_t j0 = 0; j0 < n; j0 += S) {
+    const int64_t j1 = std::min(n, j0 + S);
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int64_t j = j0; j < j1; ++j) ++cnt[(size_t)(inv[(size_t)j] / S) + 1];
+    for (int64_t b = 0; b < nt; ++b) cnt[(size_t)b + 1] += cnt[(size_t)b];
+    for (int64_t j = j0; j < j1; ++j) {
+      const int64_t r = cnt[(size_t)(inv[(size_t)j] / S)]++;   // place in the tile ordered by destination tile
+      loc1[(size_t)j] = (uint16_t)r;
+      jof[(size_t)(j0 + r)] = (int32_t)j;
+    }
+  }
+  std::vector<int64_t> cur((size_t)nt);
+  for (int64_t b = 0; b < nt; ++b) cur[(size_t)b] = b * S;
+  for (int64_t a = 0; a < n; ++a) {
+    const int64_t d = inv[(size_t)jof[(size_t)a]];
+    const int64_t r = cur[(size_t)(d / S)]++;
+    src2[(size_t)r] = (int32_t)((a / S) * (S + kPermPad) + a % S);
+    loc2[(size_t)r] = (uint16_t)(d % S);
+  }
+  P.loc1.upload(loc1);
+  P.src2.upload(src2);
+  P.loc2.upload(loc2);
+  P.n = n;
+  P.tile = S;
+}
+
+// tile of the two-pass permutation: 128 KiB of LDS (16384 elements of 8 bytes, 32768 of 4 or 2)
+inline int64_t perm_tile(size_t esize) { return esize == 8 ? kPermTile : 2 * kPermTile; }
+
+// Both permutations of a transposed handle (its values from the source order, and its entry
+// order back to the source's), built from tperm on first use.
+void ensure_perm_plans(const spops_matrix_s &AT) {
+  const auto &C = AT.csr;
+  if (C.pin.n == AT.nnz && C.pout.n == AT.nnz) return;
+  const int64_t S = perm_tile(value_bytes(AT.value_type));
+  std::vector<int32_t> tp((size_t)AT.nnz), ti((size_t)AT.nnz);
+  if (AT.nnz) SPOPS_HIP_CHECK(hipMemcpy(tp.data(), C.tperm.p, (size_t)AT.nnz * 4, hipMemcpyDeviceToHost));
+  for (int64_t e = 0; e < AT.nnz; ++e) ti[(size_t)tp[(size_t)e]] = (int32_t)e;
+  build_perm_plan(tp, S, C.pin);
+  build_perm_plan(ti, S, C.pout);
+}
+
+void perm_apply(const PermPlan &P, const void *x, void *y, size_t esize, DevBuf &stage, hipStream_t s) {
+  const int64_t n = P.n;
+  if (n <= 0) return;
+  SPOPS_REQUIRE(P.tile == perm_tile(esize), SPOPS_ERROR_INVALID_VALUE, "permutation plan built for another element size");
+  const unsigned tiles = (unsigned)((n + P.tile - 1) / P.tile);
+  const size_t need = (size_t)tiles * (size_t)(P.tile + kPermPad) * esize;
+  SPOPS_REQUIRE(need < kMaxBufBytes, SPOPS_ERROR_LIMIT, "two-pass permutation: staging of >= 2 GiB");
+  if (stage.bytes < need) stage.alloc(need);
+  const auto *l1 = P.loc1.as<uint16_t>();
+  const auto *l2 = P.loc2.as<uint16_t>();
+  const int *s2 = P.src2.as<int>();
+  auto run = [&](auto tag, auto ts) {
+    using T = decltype(tag);
+    constexpr int TS = decltype(ts)::value;
+    hipLaunchKernelGGL((k_perm_tiles<T, TS>), dim3(tiles), dim3(kBlock), 0, s, static_cast<const T *>(x), l1, n,
+                       stage.as<T>());
+    SPOPS_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL((k_perm_buckets<T, TS>), dim3(tiles), dim3(kBlock), 0, s, stage.as<T>(),
+                       (int64_t)(need / esize), s2, l2, n, static_cast<T *>(y));
+    SPOPS_HIP_CHECK(hipGetLastError());
+  };
+  if (esize == 8) run(uint64_t{}, std::integral_constant<int, kPermTile>{});
+  else if (esize == 4) run(uint32_t{}, std::integral_constant<int, 2 * kPermTile>{});
+  else run(uint16_t{}, std::integral_constant<int, 2 * kPermTile>{});
+}
+
 // The fused backward on a transposed handle (k_spmm_sddmm_rows / _seg, then k_spmm_seg_sum for
 // the long rows' dX), the dot products in AT's order into the handle's buffer, then gathered
 // into the source matrix's order through the inverse permutation (built on first use); the
@@ -787,12 +942,7 @@ void spmm_sddmm_launch(const spops_matrix_s &AT, int k, const void *G, const voi
   const auto *x = static_cast<const typename Ty<XC>::T *>(X);
   auto *dx = static_cast<typename Ty<XC>::T *>(dX);
   if (AT.nnz > 0) {
-    if (C.tinv.p == nullptr) {
-      C.tinv.alloc((size_t)AT.nnz * 4);
-      hipLaunchKernelGGL(k_invert_perm, dim3(nblk(AT.nnz, kBlock)), dim3(kBlock), 0, s, C.tperm.as<int>(), AT.nnz,
-                         C.tinv.as<int>());
-      SPOPS_HIP_CHECK(hipGetLastError());
-    }
+    ensure_perm_plans(AT);
     if (C.tdv.bytes < (size_t)AT.nnz * sizeof(VT)) C.tdv.alloc((size_t)AT.nnz * sizeof(VT));
   }
   VT *dT = C.tdv.as<VT>();
@@ -816,7 +966,7 @@ void spmm_sddmm_launch(const spops_matrix_s &AT, int k, const void *G, const voi
                        C.lsegoff.as<int>(), part, k, dx, 1.0, 0.0);
     SPOPS_HIP_CHECK(hipGetLastError());
   }
-  if (AT.nnz > 0) convert_launch<VC, VC>(dT, C.tinv.p, false, AT.nnz, dvals, s);
+  if (AT.nnz > 0) perm_apply(C.pout, dT, dvals, sizeof(VT), C.pstage, s);
 }
 
 template <int VC, int GC>
@@ -1037,8 +1187,12 @@ extern "C" int spops_csr_set_values(spops_matrix A, const void *d_values, int va
     SPOPS_REQUIRE(A->nnz == 0 || d_values != nullptr, SPOPS_ERROR_INVALID_VALUE, "null values");
     const bool perm = A->csr.tperm.p != nullptr;
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    convert_values(d_values, values_type, perm ? A->csr.tperm.p : nullptr, A->csr.tperm64, A->nnz, A->csr.val.p,
-                   A->value_type, s);
+    if (perm && values_type == A->value_type && A->csr.pin.n == A->nnz)
+      // a transposed handle that ran the fused backward keeps the two-pass form of its permutation
+      perm_apply(A->csr.pin, d_values, A->csr.val.p, value_bytes(A->value_type), A->csr.pstage, s);
+    else
+      convert_values(d_values, values_type, perm ? A->csr.tperm.p : nullptr, A->csr.tperm64, A->nnz, A->csr.val.p,
+                     A->value_type, s);
     // the swept SpMV layout's copy follows on the same stream, so any later SpMV ordered after
     // this call (or captured after it) reads the new values; handles made for SpMM / SDDMM only
     // (spops_csr_create_layout with spmv_layout 1) have no such copy and pay nothing here

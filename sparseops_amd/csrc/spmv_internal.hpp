@@ -25,6 +25,22 @@ constexpr int kSliceTaskRows = 256;       // rows of a multi-row wave task (row 
 // entries per lane of a multi-row task: 16 (8 for FP64 values, which need twice the registers)
 constexpr int slice_lane_entries(int vt) { return vt == SPOPS_VALUE_F64 ? 8 : 16; }
 
+// A fixed permutation y[i] = x[g[i]] of n elements applied as two passes instead of a random
+// gather (one cache line per element): pass 1 takes tiles of consecutive sources, orders each in
+// LDS by destination tile (loc1: the element's place there) and writes it back to its range of
+// a staging array (coalesced); in the staging array each destination tile's elements then form
+// one run per source tile.  Pass 2 takes one destination tile per workgroup, reads its runs
+// (src2: the staging addresses, run after run), places the elements in LDS at their
+// destinations (loc2) and writes the tile out (coalesced).  Built on the host once per
+// permutation, for one element size (tiles of 128 KiB).
+constexpr int kPermTile = 16384;
+constexpr int kPermPad = 96;     // staging elements between source tiles (breaks the power-of-two stride)
+struct PermPlan {
+  DevBuf loc1, src2, loc2;    // uint16 [n], int32 [n], uint16 [n]
+  int64_t n = -1;             // -1: not built
+  int64_t tile = 0;           // elements per tile (kPermTile for 8-byte elements, twice that otherwise)
+};
+
 struct CsrDev {
   DevBuf rowptr, col, val, tasks, longs;
   DevBuf tperm;                 // transposed view: entry e holds the source matrix's entry tperm[e]
@@ -34,8 +50,9 @@ struct CsrDev {
   DevBuf lsegoff;               // SpMM: first segment of each long row (nslong + 1 ints)
   mutable DevBuf spmm_part;     // SpMM: partial sums of the segments (grown on demand)
   mutable DevBuf erow;          // SDDMM: row of every entry (built on first use)
-  mutable DevBuf tinv;          // transposed view: the inverse of tperm (built on first use)
   mutable DevBuf tdv;           // transposed view: the fused backward's dot products in its own entry order
+  mutable DevBuf pstage;        // transposed view: staging of the two-pass permutations
+  mutable PermPlan pin, pout;   // transposed view: source order -> its order (values), and back (built on first use)
   DevBuf sdtasks;               // SDDMM by rows: int2 {row, first entry in the row}, chunks of kSddmmChunk
   int64_t nsdtasks = 0;
   bool rowptr64 = false, tperm64 = false;

@@ -359,3 +359,40 @@ def test_layer_backward_fused_matches_separate(dev):
     Gn, Xn = G.double().cpu().numpy(), X.detach().double().cpu().numpy()
     A64 = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, A.val.astype(np.float64))
     _close(vals.grad.double().cpu().numpy(), dv.double().cpu().numpy(), orc.sddmm(A64, np.abs(Gn), np.abs(Xn)), 4e-6)
+
+
+@pytest.mark.parametrize("vt", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("name", ["hub", "rmat"])
+def test_transposed_values_after_fused_backward(mats, name, vt, dev):
+    """After a fused backward the transposed handle keeps its permutation in two-pass form:
+    later set_values calls use it and must give the values the plain gather gives (bit for
+    bit, seen through the product), and later fused calls stay correct."""
+    A = _hub_columns() if name == "hub" else mats[name]
+    k = 16
+    rng = np.random.default_rng(11)
+    tdt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[vt]
+    gt = torch.float64 if vt == "f64" else torch.float32
+    M = S.csr(A, vt, spmv_layout=1)
+    AT = M.transpose()
+    G = torch.tensor(rng.standard_normal((A.n_rows, k)), device=dev).to(gt)
+    X = torch.tensor(rng.standard_normal((A.n_cols, k)), device=dev).to(gt)
+    dX = torch.empty_like(X)
+    dv = torch.empty(A.nnz, dtype=tdt, device=dev)
+    assert AT.spmm_sddmm_transposed(G, X, dX, dv)     # builds the two-pass plans
+    for step in range(3):
+        vals = torch.tensor(rng.standard_normal(A.nnz), device=dev).to(tdt)
+        AT.set_values(vals)                            # the plan's path
+        fresh = S.csr(A, vt, spmv_layout=1).transpose()
+        fresh.set_values(vals)                         # the gather's path
+        Y1, Y2 = torch.empty_like(X), torch.empty_like(X)
+        AT.spmm(G, Y1)
+        fresh.spmm(G, Y2)
+        dv.fill_(float("nan"))
+        assert AT.spmm_sddmm_transposed(G, X, dX, dv)
+        torch.cuda.synchronize()
+        assert torch.equal(Y1, Y2)
+        assert torch.equal(dX, Y2)
+        Gn, Xn = G.double().cpu().numpy(), X.double().cpu().numpy()
+        tol = (1e-13 if vt == "f64" else 2e-6) + (2 ** -8 if vt == "bf16" else 0)
+        _close(dv.double().cpu().numpy(), orc.sddmm(A, Gn, Xn), orc.sddmm(A, np.abs(Gn), np.abs(Xn)), tol)
+        fresh.close()
