@@ -10,7 +10,7 @@
 //   k_spmm_rows    : one row per group of L lanes (64 / L rows per wave), each lane covering
 //                    VEC consecutive columns with one 16-byte load per entry when k and the
 //                    alignment of X and Y allow it (else one column per lane), eight row
-//                    entries in flight; rows ordered by length inside windows of
+//                    entries in flight (four with 2-byte X); rows ordered by length inside windows of
 //                    kSpmmWindow rows.  Rows longer than kSpmmLongRow entries are left to
 //   k_spmm_seg     : one workgroup per segment of kSpmmSegment entries of a long row, its
 //                    partial row of sums to a scratch buffer, then
@@ -79,7 +79,7 @@ template <int C, int VEC> struct Vec {
 // so the rows of a wave have about the same length).  A row is served by a group of L lanes
 // (64 / L rows per wave); lane q of the group covers columns c .. c + VEC - 1 with
 // c = cb + q VEC, loading them as one vector per entry.  U entries of the row are in flight.
-template <int VC, int XC, int YC, typename RP, int VEC>
+template <int VC, int XC, int YC, typename RP, int VEC, int U>
 __global__ __launch_bounds__(kBlock) void k_spmm_rows(const RP *__restrict__ rowptr, const int *__restrict__ col,
                                                       const typename Ty<VC>::T *__restrict__ val,
                                                       const int *__restrict__ vperm,
@@ -89,7 +89,6 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rows(const RP *__restrict__ row
   using Acc = AccT<VC, XC>;
   using XV = typename Vec<XC, VEC>::V;
   using YV = typename Vec<YC, VEC>::V;
-  constexpr int U = 8;
   const int lane = lane_id(), R = kWave / L, g = lane / L, q = lane % L;
   const int64_t slot = ((int64_t)blockIdx.x * (kBlock / kWave) + wave_id()) * R + g;
   if (slot >= n_short) return;                      // lane groups past the last row
@@ -142,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rows(const RP *__restrict__ row
 // wave's 64 / L entry slots and the 4 waves each take every (4 * 64 / L)-th entry; the slots
 // are summed with cross-lane adds, the waves through LDS, and the segment's row of partial
 // sums is written to part[segment * k ...].
-template <int VC, int XC, typename RP, int VEC>
+template <int VC, int XC, typename RP, int VEC, int U>
 __global__ __launch_bounds__(kBlock) void k_spmm_seg(const RP *__restrict__ rowptr, const int *__restrict__ col,
                                                      const typename Ty<VC>::T *__restrict__ val,
                                                      const int *__restrict__ vperm,
@@ -151,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_seg(const RP *__restrict__ rowp
                                                      AccT<VC, XC> *__restrict__ part) {
   using Acc = AccT<VC, XC>;
   using XV = typename Vec<XC, VEC>::V;
-  constexpr int W = kBlock / kWave, U = 4;
+  constexpr int W = kBlock / kWave;
   __shared__ Acc red[W][kWave * VEC];
   const int2 sg = segs[blockIdx.x];
   const int64_t row = long_rows[sg.x];
@@ -682,10 +681,15 @@ void spmm_launch_vec(const spops_matrix_s &A, int k, double alpha, const void *X
   const auto *x = static_cast<const typename Ty<XC>::T *>(X);
   auto *y = static_cast<typename Ty<YC>::T *>(Y);
   const int L = lanes_for(k, VEC);
+  // entries in flight per row: with vectors of eight 2-byte elements (eight FP32 sums per lane)
+  // eight entries take 100 VGPRs (4 waves per SIMD) and four 64 (8 waves): R-MAT 20, k = 64,
+  // BF16 X 0.81 -> 0.67 ms; FP32 / FP64 X keep eight (4 there were 1-5 % slower)
+  constexpr int kRowSteps = VEC >= 8 ? 4 : 8, kSegSteps = VEC >= 8 ? 2 : 4;
   if (C.nshort > 0) {
     const int64_t rows_per_block = (int64_t)(kBlock / kWave) * (kWave / L);
-    hipLaunchKernelGGL((k_spmm_rows<VC, XC, YC, RP, VEC>), dim3(nblk(C.nshort, (int)rows_per_block)), dim3(kBlock), 0,
-                       s, rp, C.col.as<int>(), v, vp, C.sorder.as<int>(), (int64_t)C.nshort, k, L, x, y, alpha, beta);
+    hipLaunchKernelGGL((k_spmm_rows<VC, XC, YC, RP, VEC, kRowSteps>), dim3(nblk(C.nshort, (int)rows_per_block)),
+                       dim3(kBlock), 0, s, rp, C.col.as<int>(), v, vp, C.sorder.as<int>(), (int64_t)C.nshort, k, L, x, y,
+                       alpha, beta);
     SPOPS_HIP_CHECK(hipGetLastError());
   }
   if (C.nslong > 0) {
@@ -697,8 +701,8 @@ void spmm_launch_vec(const spops_matrix_s &A, int k, double alpha, const void *X
       if (C.spmm_part.bytes < need) C.spmm_part.alloc(need);
       part = C.spmm_part.as<Acc>();
     }
-    hipLaunchKernelGGL((k_spmm_seg<VC, XC, RP, VEC>), dim3((unsigned)C.nlseg), dim3(kBlock), 0, s, rp, C.col.as<int>(),
-                       v, vp, C.slongs.as<int>(), C.lsegs.as<int2>(), k, L, x, part);
+    hipLaunchKernelGGL((k_spmm_seg<VC, XC, RP, VEC, kSegSteps>), dim3((unsigned)C.nlseg), dim3(kBlock), 0, s, rp,
+                       C.col.as<int>(), v, vp, C.slongs.as<int>(), C.lsegs.as<int2>(), k, L, x, part);
     SPOPS_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL((k_spmm_seg_sum<Acc, YC>), dim3((unsigned)C.nslong), dim3(kBlock), 0, s, C.slongs.as<int>(),
                        C.lsegoff.as<int>(), part, k, y, alpha, beta);
