@@ -233,7 +233,8 @@ int spops_sddmm(spops_matrix A, int64_t k, const void *d_G, int g_type, const vo
  * stream-ordered.  No reference counterpart (the differentiable form of SpOps_csr).  Returns
  * SPOPS_ERROR_NOT_SUPPORTED, having done nothing, unless k is a multiple of the elements of G in
  * 16 bytes and at most 64 times that, G is 16-byte aligned and X, dX are aligned to that many
- * elements of their type; spops_spmm_typed on AT plus spops_sddmm on A compute the same. */
+ * elements of their type, and the handle's values take less than 2 GiB (the permutation's
+ * staging); spops_spmm_typed on AT plus spops_sddmm on A compute the same. */
 int spops_spmm_sddmm_transposed(spops_matrix AT, int64_t k, const void *d_G, int g_type, const void *d_X, int x_type,
                                 void *d_dX, void *d_dvals, void *stream);
 /* AT = A^T as a new CSR handle; AT's entry e is A's entry perm[e] (kept inside AT), rows of

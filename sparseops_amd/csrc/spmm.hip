@@ -1152,6 +1152,13 @@ extern "C" int spops_spmm_sddmm_transposed(spops_matrix AT, int64_t k, const voi
                   "spops_spmm_sddmm_transposed: k = %lld needs to be a multiple of %lld and at most %lld, G 16-byte "
                   "aligned and X, dX %zu-byte aligned (use spops_spmm_typed and spops_sddmm)",
                   (long long)k, (long long)V, (long long)(kWave * V), xa);
+    // the two-pass permutation's staging array is addressed with 32-bit offsets
+    const int64_t S = perm_tile(value_bytes(AT->value_type));
+    const int64_t tiles = (AT->nnz + S - 1) / S;
+    SPOPS_REQUIRE((uint64_t)tiles * (uint64_t)(S + kPermPad) * value_bytes(AT->value_type) < kMaxBufBytes,
+                  SPOPS_ERROR_NOT_SUPPORTED,
+                  "spops_spmm_sddmm_transposed: %lld entries need a staging array of >= 2 GiB (use spops_spmm_typed and "
+                  "spops_sddmm)", (long long)AT->nnz);
     const hipStream_t s = static_cast<hipStream_t>(stream);
     switch (AT->value_type) {
       case SPOPS_VALUE_F64: spmm_sddmm_g<SPOPS_VALUE_F64>(*AT, (int)k, d_G, g_type, d_X, x_type, d_dX, d_dvals, s); break;
