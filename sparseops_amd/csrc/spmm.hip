@@ -562,9 +562,9 @@ __global__ __launch_bounds__(kBlock) void k_sddmm(const int *__restrict__ erow, 
 // The same by rows: a group of L lanes per task, a chunk of at most kSddmmChunk entries of one
 // row (tasks {row, first entry in the row}, the rows of a window longest first).  The group
 // loads its row of G once (lane q: columns q VEC .. q VEC + VEC - 1, all of k when
-// k <= L VEC), then per entry only the row of X; U entries in flight, each summed across the
-// group, and lane q writes the entries u with u % L == q.
-template <int GC, int XC, int OC, typename RP, int VEC>
+// k <= L VEC), then per entry only the row of X; U entries in flight, summed across the group
+// by group_sum_entries (each lane q < U then writes the entries it holds).
+template <int GC, int XC, int OC, typename RP, int VEC, int U>
 __global__ __launch_bounds__(kBlock) void k_sddmm_rows(const RP *__restrict__ rowptr, const int *__restrict__ col,
                                                        const int2 *__restrict__ tasks, int64_t ntasks, int k, int L,
                                                        const typename Ty<GC>::T *__restrict__ G,
@@ -573,7 +573,6 @@ __global__ __launch_bounds__(kBlock) void k_sddmm_rows(const RP *__restrict__ ro
   using Acc = AccT<GC, XC>;
   using GV = typename Vec<GC, VEC>::V;
   using XV = typename Vec<XC, VEC>::V;
-  constexpr int U = 8;
   const int lane = lane_id(), R = kWave / L, g = lane / L, q = lane % L;
   const int64_t slot = ((int64_t)blockIdx.x * (kBlock / kWave) + wave_id()) * R + g;
   if (slot >= ntasks) return;                       // lane groups past the last task
@@ -608,12 +607,8 @@ __global__ __launch_bounds__(kBlock) void k_sddmm_rows(const RP *__restrict__ ro
 #pragma unroll
       for (int e = 0; e < VEC; ++e) part[u] += gr[e] * (Acc)Ty<XC>::ld(xv[u].e[e]);
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      for (int m = 1; m < L; m <<= 1) part[u] += __shfl_xor(part[u], m, kWave);
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-      if (u % L == q && p + u < p1) out[p + u] = Ty<OC>::st((double)part[u]);
+    const int base = group_sum_entries<U>(part, q, L);
+    group_write_entries<U, OC>(part, base, q, L, p, p1, 1, out);
   }
 }
 
@@ -779,7 +774,9 @@ void sddmm_rows_launch(const spops_matrix_s &A, int k, const void *G, const void
   const auto &C = A.csr;
   const int L = lanes_for(k, VEC);
   const int64_t per_block = (int64_t)(kBlock / kWave) * (kWave / L);
-  hipLaunchKernelGGL((k_sddmm_rows<GC, XC, OC, RP, VEC>), dim3(nblk(C.nsdtasks, (int)per_block)), dim3(kBlock), 0, s,
+  // entries per step: four with vectors of eight 2-byte elements (eight waves per SIMD), else eight
+  constexpr int U = VEC >= 8 ? 4 : 8;
+  hipLaunchKernelGGL((k_sddmm_rows<GC, XC, OC, RP, VEC, U>), dim3(nblk(C.nsdtasks, (int)per_block)), dim3(kBlock), 0, s,
                      C.rowptr.as<RP>(), C.col.as<int>(), C.sdtasks.as<int2>(), C.nsdtasks, k, L,
                      static_cast<const typename Ty<GC>::T *>(G), static_cast<const typename Ty<XC>::T *>(X),
                      static_cast<typename Ty<OC>::T *>(out));
