@@ -198,7 +198,7 @@ int spops_spmm_typed(spops_matrix A, int64_t k, double alpha, const void *d_X, i
 /* No counterpart in the reference.  For callers that cannot allocate device memory (code
  * under HIP graph capture, serving loops, harnesses that forbid hipMalloc): spops_spmm_typed
  * with the same arguments and results, computed by the same kernels, whose only scratch (the
- * partial sums of rows longer than 256 entries, one row of k sums per 512-entry segment) is
+ * partial sums of rows longer than 256 entries, one row of k sums per 1024-entry segment) is
  * a static device array of 64 MiB instead of a per-handle buffer grown on demand.  It makes
  * no allocation.  A request needing more (segments x k x 8 bytes for FP64 values or X, x 4
  * otherwise) returns SPOPS_ERROR_LIMIT.  Every call on one GPU shares that GPU's array:

@@ -10,7 +10,9 @@ namespace spops {
 constexpr int kCsrStreamNnz = 512;     // entries staged per wave stream block
 constexpr int64_t kCsrLongRow = 2048;  // longer rows: one workgroup each
 constexpr int64_t kSpmmLongRow = 256;  // SpMM: longer rows are summed in segments
-constexpr int64_t kSpmmSegment = 512;  // SpMM: entries per segment of a long row
+// SpMM: entries per segment of a long row (R-MAT 20, k = 64: forward 0.72 -> 0.68 ms against 512;
+// 2048 no better)
+constexpr int64_t kSpmmSegment = 1024;
 constexpr int64_t kSpmmWindow = 4096;  // SpMM: short rows are ordered by length inside windows of this many rows
 constexpr int64_t kSddmmChunk = 64;    // SDDMM by rows: entries of a row per lane-group task
 constexpr int kBsrMaxBlock = 256;      // bR * bC limit of the wave-per-block-row VALU kernel

@@ -86,7 +86,7 @@ def test_spmm_mixed_types(mats, xt, yt, dev):
                                    (torch.bfloat16, torch.bfloat16), (torch.float32, torch.bfloat16)])
 @pytest.mark.parametrize("name", ["rmat", "edge"])
 def test_sddmm(mats, name, k, gt, xt, dev):
-    """By rows (each G row loaded once; the 512-entry segments of rows longer than 256 entries)
+    """By rows (each G row loaded once per chunk of at most 64 entries of a row)
     when k is a multiple of X's 16-byte vector and at most 64 vectors; entry-parallel otherwise
     (k = 1, and k = 70 / 512 / 520 where the vector does not divide k or 64 vectors fall short)."""
     A = mats[name]

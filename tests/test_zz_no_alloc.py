@@ -56,13 +56,14 @@ def test_noalloc_matches_spmm_typed_twice(vt, xt, yt, dev):
 
 
 def test_noalloc_rejects_beyond_its_scratch(dev):
-    """One row of 2^20 entries is 2048 segments: k = 8193 FP64 sums need 134 MB > 64 MiB."""
+    """One row of 2^20 entries is 1024 segments: k = 16384 FP64 sums need 134 MB > 64 MiB (the call is
+    refused before it touches X or Y)."""
     n = 1 << 20
     A = Csr(1, n, np.array([0, n], np.int64), np.arange(n, dtype=np.int64), np.ones(n))
     M = S.csr(A, "f64", spmv_layout=1)
     X = torch.zeros(16, dtype=torch.float64, device=dev)
     Y = torch.zeros(16, dtype=torch.float64, device=dev)
-    st = L.lib.spops_spmm_typed_noalloc(M._h, 8193, ctypes.c_double(1.0), ctypes.c_void_p(X.data_ptr()), L.VALUE_F64,
+    st = L.lib.spops_spmm_typed_noalloc(M._h, 16384, ctypes.c_double(1.0), ctypes.c_void_p(X.data_ptr()), L.VALUE_F64,
                                         ctypes.c_double(0.0), ctypes.c_void_p(Y.data_ptr()), L.VALUE_F64,
                                         ctypes.c_void_p(L.current_stream()))
     assert st == 5 and b"static scratch" in L.lib.spops_last_error()
