@@ -228,7 +228,8 @@ int spops_sddmm(spops_matrix A, int64_t k, const void *d_G, int g_type, const vo
  * per row of AT; the dot products are written in AT's entry order into a buffer the handle
  * keeps and moved into A's order by a two-pass permutation (coalesced passes through LDS instead
  * of a random gather).  The first call builds both permutations of the handle on the host (about
- * 16 bytes per entry of device memory, plus two nnz-value buffers); afterwards
+ * 16 bytes per entry of device memory, plus two nnz-value buffers; it allocates and copies
+ * synchronously, so it cannot be captured into a graph, later calls can); afterwards
  * spops_csr_set_values on the handle uses the forward one too.  Calls on one AT handle must be
  * stream-ordered.  No reference counterpart (the differentiable form of SpOps_csr).  Returns
  * SPOPS_ERROR_NOT_SUPPORTED, having done nothing, unless k is a multiple of the elements of G in
