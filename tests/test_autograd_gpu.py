@@ -396,3 +396,40 @@ def test_transposed_values_after_fused_backward(mats, name, vt, dev):
         tol = (1e-13 if vt == "f64" else 2e-6) + (2 ** -8 if vt == "bf16" else 0)
         _close(dv.double().cpu().numpy(), orc.sddmm(A, Gn, Xn), orc.sddmm(A, np.abs(Gn), np.abs(Xn)), tol)
         fresh.close()
+
+
+def test_fused_backward_graph_replays(mats, dev):
+    """After its first call (which builds the handle's permutations), the fused backward can be
+    captured: three replays, with eager work and synchronisations between them, each equal to
+    the eager call bit for bit."""
+    A = _hub_columns()
+    k = 32
+    rng = np.random.default_rng(21)
+    M = S.csr(A, "f32", spmv_layout=1)
+    AT = M.transpose()
+    vals = torch.tensor(rng.standard_normal(A.nnz), device=dev).float()
+    AT.set_values(vals)
+    G = torch.tensor(rng.standard_normal((A.n_rows, k)), device=dev).float()
+    X = torch.tensor(rng.standard_normal((A.n_cols, k)), device=dev).to(torch.bfloat16)
+    dX0, dv0 = torch.empty_like(X), torch.empty(A.nnz, device=dev)
+    assert AT.spmm_sddmm_transposed(G, X, dX0, dv0)     # eager: builds the plans
+    torch.cuda.synchronize()
+    dX, dv = torch.empty_like(X), torch.empty(A.nnz, device=dev)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            AT.set_values(vals, stream=s.cuda_stream)
+            AT.spmm_sddmm_transposed(G, X, dX, dv, stream=s.cuda_stream)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        dX.fill_(float("nan"))
+        dv.fill_(float("nan"))
+        junk = torch.randn(1000, device=dev) * 2.0          # eager work between the replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(dX, dX0), rep
+        assert torch.equal(dv, dv0), rep
+        del junk
