@@ -484,7 +484,8 @@ def lespgemm_panels_measure(scale, budget_gb, cpu_threads, cpu, cpu_share=0.02):
 
 def autograd_measure(scale, k, steps, cpu, cpu_threads):
     """The differentiable SpMM layer Y = A(vals) X: R-MAT(scale, edge factor 32) with symmetric
-    normalised FP32 values, X and G of width k in BF16, FP32 accumulation.  Forward and backward
+    normalised FP32 values, X of width k in BF16, FP32 accumulation, Y rounded once to BF16 (so
+    its gradient G is BF16, as the workload states).  Forward and backward
     (dX = A^T G and dvals = SDDMM(G, X)) timed with HIP events; the CPU leg is the oracle's
     single-thread CSR SpMM on the first 1/64 of the rows, scaled to the whole matrix."""
     import ctypes
@@ -500,7 +501,7 @@ def autograd_measure(scale, k, steps, cpu, cpu_threads):
     vals = torch.tensor(A.val, device=dev, requires_grad=True)
     X = torch.tensor(rng.standard_normal((A.n_cols, k), dtype=np.float32), device=dev).to(torch.bfloat16)
     X.requires_grad_()
-    G = torch.tensor(rng.standard_normal((A.n_rows, k), dtype=np.float32), device=dev).to(torch.bfloat16).float()
+    G = torch.tensor(rng.standard_normal((A.n_rows, k), dtype=np.float32), device=dev).to(torch.bfloat16)
     stream = torch.cuda.current_stream().cuda_stream
     ev = [ctypes.c_void_p() for _ in range(3)]
     for e in ev:
@@ -512,7 +513,7 @@ def autograd_measure(scale, k, steps, cpu, cpu_threads):
         X.grad = None
         torch.cuda.synchronize()
         L.call("spops_event_record", ev[0], stream)
-        Y = AG.spmm(pat, vals, X, out_dtype=torch.float32)
+        Y = AG.spmm(pat, vals, X, out_dtype=torch.bfloat16)
         L.call("spops_event_record", ev[1], stream)
         Y.backward(G)
         L.call("spops_event_record", ev[2], stream)
@@ -526,10 +527,12 @@ def autograd_measure(scale, k, steps, cpu, cpu_threads):
         L.call("spops_event_destroy", e)
     fw, bw = fw / steps, bw / steps
     n, nnz = A.n_rows, A.nnz
-    b_fwd = nnz * 8 + (n + 1) * 4 + n * k * 2 + n * k * 4
-    b_bwd = (nnz * 8 + (n + 1) * 4 + n * k * 4 + n * k * 2) + (nnz * 4 + (n + 1) * 4 + n * k * 4 + n * k * 2 + nnz * 4)
+    # forward: entries (column + FP32 value), row pointers, X read, Y written; backward: dX = A^T G
+    # (entries, row pointers, G read, dX written) and dvals (columns, row pointers, G, X, dvals)
+    b_fwd = nnz * 8 + (n + 1) * 4 + n * k * 2 + n * k * 2
+    b_bwd = (nnz * 8 + (n + 1) * 4 + n * k * 2 + n * k * 2) + (nnz * 4 + (n + 1) * 4 + n * k * 2 + n * k * 2 + nnz * 4)
     out = {"workload": f"Y = A X, R-MAT scale {scale} edge factor 32 (n={n}, nnz={nnz}), symmetric-normalised FP32 "
-                       f"values, X/G BF16 width {k}, FP32 accumulation; backward dX = A^T G, dvals = SDDMM(G, X)",
+                       f"values, X/Y/G BF16 width {k}, FP32 accumulation; backward dX = A^T G, dvals = SDDMM(G, X)",
            "forward_ms": round(fw, 3), "backward_ms": round(bw, 3),
            "forward_gbs": round(b_fwd / fw / 1e6, 1), "backward_gbs": round(b_bwd / bw / 1e6, 1),
            "forward_gflops": round(2 * nnz * k / fw / 1e6, 1),

@@ -213,6 +213,39 @@ def test_backward_matches_cpu_reference_bf16(dev):
     _close(vals.grad.double().cpu().numpy(), orc.sddmm(A64, Gn, Xn), orc.sddmm(A64, np.abs(Gn), np.abs(Xn)), 2e-6)
 
 
+def test_backward_bf16_output_matches_cpu_reference(dev):
+    """The layer with BF16 output: Y rounded once to BF16 from FP32 sums, so its gradient G
+    arrives in BF16 and the fused backward gathers BF16 rows of G.  Y, dX = A^T G and
+    dvals = SDDMM(G, X) against the CPU oracle on the same rounded inputs."""
+    A = rmat_symnorm(12, 16, seed=5)
+    pat = AG.SparsePattern(A, "f32")
+    rng = np.random.default_rng(6)
+    vals = torch.tensor(A.val, device=dev, requires_grad=True)
+    X = torch.tensor(rng.standard_normal((A.n_cols, 64)), device=dev).to(torch.bfloat16).requires_grad_()
+    Y = AG.spmm(pat, vals, X, out_dtype=torch.bfloat16)
+    assert Y.dtype == torch.bfloat16
+    G = torch.tensor(rng.standard_normal((A.n_rows, 64)), device=dev).to(torch.bfloat16)
+    Y.backward(G)
+    Xn, Gn = X.detach().double().cpu().numpy(), G.double().cpu().numpy()
+    A64 = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, A.val.astype(np.float64))
+    absA = Csr(A.n_rows, A.n_cols, A.rowptr, A.col, np.abs(A64.val))
+    _close(Y.detach().double().cpu().numpy(), orc.spmm(A64, Xn), orc.spmm(absA, np.abs(Xn)), 2 ** -8 + 2e-6)
+    trp, tc, tv, tp = orc.transpose(A64)
+    want = orc.spmm(Csr(A.n_cols, A.n_rows, trp, tc, tv), Gn)
+    scale = orc.spmm(Csr(A.n_cols, A.n_rows, trp, tc, np.abs(tv)), np.abs(Gn))
+    assert X.grad.dtype == torch.bfloat16 and vals.grad.dtype == torch.float32
+    _close(X.grad.double().cpu().numpy(), want, scale, 2 ** -8 + 2e-6)
+    _close(vals.grad.double().cpu().numpy(), orc.sddmm(A64, Gn, Xn), orc.sddmm(A64, np.abs(Gn), np.abs(Xn)), 2e-6)
+    # the same G as FP32 values takes the FP32-gather form of the fused pass: equal to within
+    # FP32 summation order
+    vals2 = torch.tensor(A.val, device=dev, requires_grad=True)
+    X2 = X.detach().clone().requires_grad_()
+    AG.spmm(pat, vals2, X2, out_dtype=torch.float32).backward(G.float())
+    _close(vals2.grad.double().cpu().numpy(), vals.grad.double().cpu().numpy(),
+           orc.sddmm(A64, np.abs(Gn), np.abs(Xn)), 2e-6)
+    _close(X2.grad.double().cpu().numpy(), X.grad.double().cpu().numpy(), scale, 2 ** -7)
+
+
 @pytest.mark.parametrize("k", [1, 5, 8, 33])
 def test_spmm_writes_nothing_past_y(k, dev):
     """Y sits inside a larger buffer whose tail must stay untouched (rows past n_rows in the
