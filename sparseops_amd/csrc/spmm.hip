@@ -339,8 +339,9 @@ __global__ __launch_bounds__(kBlock) void k_spmm_sddmm_rows(const RP *__restrict
 
 // The same for the segments of AT's long rows (k_spmm_seg's partition of the entries): the
 // segment's partial row of dX to part[], and each entry's dot product with the row of X
-// (loaded once per segment) summed over its slot's L lanes and written to dT[e].
-template <int VC, int GC, int XC, typename RP, int VEC>
+// (loaded once per segment) summed over its slot's L lanes and written to dT[e], U entries per
+// slot and step.
+template <int VC, int GC, int XC, typename RP, int VEC, int U>
 __global__ __launch_bounds__(kBlock) void k_spmm_sddmm_seg(const RP *__restrict__ rowptr, const int *__restrict__ col,
                                                            const typename Ty<VC>::T *__restrict__ val,
                                                            const int *__restrict__ long_rows,
@@ -353,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_sddmm_seg(const RP *__restrict_
   using Ad = AccT<GC, XC>;
   using GV = typename Vec<GC, VEC>::V;
   using XV = typename Vec<XC, VEC>::V;
-  constexpr int W = kBlock / kWave, U = 4;
+  constexpr int W = kBlock / kWave;
   __shared__ Acc red[W][kWave * VEC];
   const int2 sg = segs[blockIdx.x];
   const int64_t row = long_rows[sg.x];
@@ -949,8 +950,9 @@ void spmm_sddmm_launch(const spops_matrix_s &AT, int k, const void *G, const voi
   VT *dT = C.tdv.as<VT>();
   if (C.nshort > 0) {
     const int64_t rows_per_block = (int64_t)(kBlock / kWave) * (kWave / L);
-    // four entries per step: 59 VGPRs, eight waves per SIMD (eight entries: 99 VGPRs, four waves,
-    // 610 against 536 us at R-MAT 20)
+    // four entries per step (FP32 G: 59 VGPRs, eight waves per SIMD; eight entries: 99 VGPRs, four
+    // waves, 610 against 536 us at R-MAT 20; BF16 G: 83 VGPRs, five waves, 460 us, and two entries
+    // with eight waves 487 us)
     hipLaunchKernelGGL((k_spmm_sddmm_rows<VC, GC, XC, int, V, 4>), dim3(nblk(C.nshort, (int)rows_per_block)),
                        dim3(kBlock), 0, s, rp, C.col.as<int>(), v, C.sorder.as<int>(), (int64_t)C.nshort, k, L, g, x, dx,
                        dT);
@@ -960,7 +962,9 @@ void spmm_sddmm_launch(const spops_matrix_s &AT, int k, const void *G, const voi
     const size_t need = (size_t)C.nlseg * (size_t)k * sizeof(Acc);
     if (C.spmm_part.bytes < need) C.spmm_part.alloc(need);
     Acc *part = C.spmm_part.as<Acc>();
-    hipLaunchKernelGGL((k_spmm_sddmm_seg<VC, GC, XC, int, V>), dim3((unsigned)C.nlseg), dim3(kBlock), 0, s, rp,
+    // two entries per step with BF16 G (56 VGPRs, eight waves per SIMD; four: 77 VGPRs, six waves,
+    // 418 against 371 us at R-MAT 20, k = 64), four with FP32 / FP64 G
+    hipLaunchKernelGGL((k_spmm_sddmm_seg<VC, GC, XC, int, V, (V >= 8 ? 2 : 4)>), dim3((unsigned)C.nlseg), dim3(kBlock), 0, s, rp,
                        C.col.as<int>(), v, C.slongs.as<int>(), C.lsegs.as<int2>(), k, L, g, x, part, dT);
     SPOPS_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL((k_spmm_seg_sum<Acc, XC>), dim3((unsigned)C.nslong), dim3(kBlock), 0, s, C.slongs.as<int>(),
