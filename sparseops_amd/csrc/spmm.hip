@@ -264,6 +264,25 @@ __device__ __forceinline__ void group_write_entries(const A (&part)[U], int base
   }
 }
 
+// <a, b> over VEC BF16 elements with v_dot2c_f32_bf16 (two products and an FP32 addend per
+// instruction, each product exact in FP32), in two chains; the operands stay packed, so no
+// conversion of either row to FP32 is needed for the dot product.
+typedef __bf16 __attribute__((ext_vector_type(2))) bf16x2_t;
+__device__ __forceinline__ bf16x2_t bf16_pair(uint16_t lo, uint16_t hi) {
+  return __builtin_bit_cast(bf16x2_t, (uint32_t)lo | ((uint32_t)hi << 16));
+}
+template <int VEC>
+__device__ __forceinline__ float dot_bf16(const uint16_t (&a)[VEC], const uint16_t (&b)[VEC]) {
+  static_assert(VEC % 4 == 0, "pairs of pairs");
+  float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+  for (int i = 0; i < VEC; i += 4) {
+    s0 = __builtin_amdgcn_fdot2_f32_bf16(bf16_pair(a[i], a[i + 1]), bf16_pair(b[i], b[i + 1]), s0, false);
+    s1 = __builtin_amdgcn_fdot2_f32_bf16(bf16_pair(a[i + 2], a[i + 3]), bf16_pair(b[i + 2], b[i + 3]), s1, false);
+  }
+  return s0 + s1;
+}
+
 // The backward of the differentiable SpMM in one pass over the transposed handle AT (its rows
 // are the source matrix's columns): dX = AT G as k_spmm_rows computes it (alpha 1, beta 0), and
 // for each entry e of AT, dT[e] = <G[col_e, :], X[row, :]>, the SDDMM of the source matrix's
@@ -290,13 +309,13 @@ __global__ __launch_bounds__(kBlock) void k_spmm_sddmm_rows(const RP *__restrict
   const int64_t p0 = (int64_t)rowptr[row], p1 = (int64_t)rowptr[row + 1];
   const int c = q * VEC;
   const bool cv = c < k;
+  // BF16 G and X: dot products on the packed pairs (v_dot2c_f32_bf16)
+  constexpr bool kDot2 = GC == SPOPS_VALUE_BF16 && XC == SPOPS_VALUE_BF16 && VEC % 4 == 0;
   Ad xr[VEC];
-  {
-    XV t{};
-    if (cv) t = *reinterpret_cast<const XV *>(X + row * k + c);
+  XV xraw{};
+  if (cv) xraw = *reinterpret_cast<const XV *>(X + row * k + c);
 #pragma unroll
-    for (int e = 0; e < VEC; ++e) xr[e] = (Ad)Ty<XC>::ld(t.e[e]);
-  }
+  for (int e = 0; e < VEC; ++e) xr[e] = (Ad)Ty<XC>::ld(xraw.e[e]);
   Acc acc[VEC];
 #pragma unroll
   for (int e = 0; e < VEC; ++e) acc[e] = 0;
@@ -319,11 +338,17 @@ __global__ __launch_bounds__(kBlock) void k_spmm_sddmm_rows(const RP *__restrict
     Ad part[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      part[u] = 0;
+      if constexpr (kDot2) {
+        part[u] = dot_bf16<VEC>(gv[u].e, xraw.e);
 #pragma unroll
-      for (int e = 0; e < VEC; ++e) {
-        acc[e] = vv[u] * (Acc)Ty<GC>::ld(gv[u].e[e]) + acc[e];
-        part[u] += (Ad)Ty<GC>::ld(gv[u].e[e]) * xr[e];
+        for (int e = 0; e < VEC; ++e) acc[e] = vv[u] * (Acc)Ty<GC>::ld(gv[u].e[e]) + acc[e];
+      } else {
+        part[u] = 0;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+          acc[e] = vv[u] * (Acc)Ty<GC>::ld(gv[u].e[e]) + acc[e];
+          part[u] += (Ad)Ty<GC>::ld(gv[u].e[e]) * xr[e];
+        }
       }
     }
     const int base = group_sum_entries<U>(part, q, L);
@@ -364,13 +389,13 @@ __global__ __launch_bounds__(kBlock) void k_spmm_sddmm_seg(const RP *__restrict_
   const int64_t stride = (int64_t)W * E;
   const int c = q * VEC;
   const bool cv = c < k;
+  // BF16 G and X: dot products on the packed pairs (v_dot2c_f32_bf16)
+  constexpr bool kDot2 = GC == SPOPS_VALUE_BF16 && XC == SPOPS_VALUE_BF16 && VEC % 4 == 0;
   Ad xr[VEC];
-  {
-    XV t{};
-    if (cv) t = *reinterpret_cast<const XV *>(X + row * k + c);
+  XV xraw{};
+  if (cv) xraw = *reinterpret_cast<const XV *>(X + row * k + c);
 #pragma unroll
-    for (int e = 0; e < VEC; ++e) xr[e] = (Ad)Ty<XC>::ld(t.e[e]);
-  }
+  for (int e = 0; e < VEC; ++e) xr[e] = (Ad)Ty<XC>::ld(xraw.e[e]);
   Acc acc[VEC];
 #pragma unroll
   for (int e = 0; e < VEC; ++e) acc[e] = 0;
@@ -394,11 +419,17 @@ __global__ __launch_bounds__(kBlock) void k_spmm_sddmm_seg(const RP *__restrict_
     Ad pd[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      pd[u] = 0;
+      if constexpr (kDot2) {
+        pd[u] = dot_bf16<VEC>(gv[u].e, xraw.e);
 #pragma unroll
-      for (int e = 0; e < VEC; ++e) {
-        acc[e] = vv[u] * (Acc)Ty<GC>::ld(gv[u].e[e]) + acc[e];
-        pd[u] += (Ad)Ty<GC>::ld(gv[u].e[e]) * xr[e];
+        for (int e = 0; e < VEC; ++e) acc[e] = vv[u] * (Acc)Ty<GC>::ld(gv[u].e[e]) + acc[e];
+      } else {
+        pd[u] = 0;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) {
+          acc[e] = vv[u] * (Acc)Ty<GC>::ld(gv[u].e[e]) + acc[e];
+          pd[u] += (Ad)Ty<GC>::ld(gv[u].e[e]) * xr[e];
+        }
       }
     }
     const int base = group_sum_entries<U>(pd, q, L);
@@ -583,13 +614,13 @@ __global__ __launch_bounds__(kBlock) void k_sddmm_rows(const RP *__restrict__ ro
   const int64_t p1 = std::min<int64_t>((int64_t)rowptr[row + 1], p0 + kSddmmChunk);
   const int c = q * VEC;
   const bool cv = c < k;
+  // BF16 G and X: dot products on the packed pairs (v_dot2c_f32_bf16)
+  constexpr bool kDot2 = GC == SPOPS_VALUE_BF16 && XC == SPOPS_VALUE_BF16 && VEC % 4 == 0;
   Acc gr[VEC];
-  {
-    GV t{};
-    if (cv) t = *reinterpret_cast<const GV *>(G + row * k + c);
+  GV graw{};
+  if (cv) graw = *reinterpret_cast<const GV *>(G + row * k + c);
 #pragma unroll
-    for (int e = 0; e < VEC; ++e) gr[e] = (Acc)Ty<GC>::ld(t.e[e]);
-  }
+  for (int e = 0; e < VEC; ++e) gr[e] = (Acc)Ty<GC>::ld(graw.e[e]);
   for (int64_t p = p0; p < p1; p += U) {
     int cc[U];
 #pragma unroll
@@ -604,9 +635,13 @@ __global__ __launch_bounds__(kBlock) void k_sddmm_rows(const RP *__restrict__ ro
     Acc part[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      part[u] = 0;
+      if constexpr (kDot2) {
+        part[u] = dot_bf16<VEC>(graw.e, xv[u].e);
+      } else {
+        part[u] = 0;
 #pragma unroll
-      for (int e = 0; e < VEC; ++e) part[u] += gr[e] * (Acc)Ty<XC>::ld(xv[u].e[e]);
+        for (int e = 0; e < VEC; ++e) part[u] += gr[e] * (Acc)Ty<XC>::ld(xv[u].e[e]);
+      }
     }
     const int base = group_sum_entries<U>(part, q, L);
     group_write_entries<U, OC>(part, base, q, L, p, p1, 1, out);
@@ -951,8 +986,8 @@ void spmm_sddmm_launch(const spops_matrix_s &AT, int k, const void *G, const voi
   if (C.nshort > 0) {
     const int64_t rows_per_block = (int64_t)(kBlock / kWave) * (kWave / L);
     // four entries per step (FP32 G: 59 VGPRs, eight waves per SIMD; eight entries: 99 VGPRs, four
-    // waves, 610 against 536 us at R-MAT 20; BF16 G: 83 VGPRs, five waves, 460 us, and two entries
-    // with eight waves 487 us)
+    // waves, 610 against 536 us at R-MAT 20; BF16 G and X with packed dot products: 63 VGPRs, eight
+    // waves, 378 us, against 83 VGPRs, five waves, 461 us with the products in FP32)
     hipLaunchKernelGGL((k_spmm_sddmm_rows<VC, GC, XC, int, V, 4>), dim3(nblk(C.nshort, (int)rows_per_block)),
                        dim3(kBlock), 0, s, rp, C.col.as<int>(), v, C.sorder.as<int>(), (int64_t)C.nshort, k, L, g, x, dx,
                        dT);
@@ -962,9 +997,10 @@ void spmm_sddmm_launch(const spops_matrix_s &AT, int k, const void *G, const voi
     const size_t need = (size_t)C.nlseg * (size_t)k * sizeof(Acc);
     if (C.spmm_part.bytes < need) C.spmm_part.alloc(need);
     Acc *part = C.spmm_part.as<Acc>();
-    // two entries per step with BF16 G (56 VGPRs, eight waves per SIMD; four: 77 VGPRs, six waves,
-    // 418 against 371 us at R-MAT 20, k = 64), four with FP32 / FP64 G
-    hipLaunchKernelGGL((k_spmm_sddmm_seg<VC, GC, XC, int, V, (V >= 8 ? 2 : 4)>), dim3((unsigned)C.nlseg), dim3(kBlock), 0, s, rp,
+    // four entries per step, but two with BF16 G and FP32 / FP64 X (56 VGPRs, eight waves per SIMD;
+    // four: 77 VGPRs, six waves, 418 against 371 us at R-MAT 20, k = 64); BF16 G and X with packed
+    // dot products: 57 VGPRs at four, 301 us
+    hipLaunchKernelGGL((k_spmm_sddmm_seg<VC, GC, XC, int, V, (V >= 8 && XC != SPOPS_VALUE_BF16 ? 2 : 4)>), dim3((unsigned)C.nlseg), dim3(kBlock), 0, s, rp,
                        C.col.as<int>(), v, C.slongs.as<int>(), C.lsegs.as<int2>(), k, L, g, x, part, dT);
     SPOPS_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL((k_spmm_seg_sum<Acc, XC>), dim3((unsigned)C.nslong), dim3(kBlock), 0, s, C.slongs.as<int>(),
