@@ -454,20 +454,23 @@ __global__ __launch_bounds__(kBlock) void k_spmm_sddmm_seg(const RP *__restrict_
 }
 
 // The two passes of a PermPlan (bit copies of T-sized elements, tiles of TS elements: TS * sizeof(T)
-// bytes of LDS).  A thread handles elements t + i kBlock of the tile (every load instruction
-// coalesced), sixteen at a time: all sixteen loads are issued before their LDS stores, so a
-// workgroup keeps 16 K loads in flight instead of one per thread (at R-MAT 20 pass 1 took 91 us
+// bytes of LDS).  A thread handles elements t + i kPermThreads of the tile (every load
+// instruction coalesced), sixteen at a time: all sixteen loads are issued before their LDS
+// stores, so a thread keeps sixteen loads in flight instead of one (at R-MAT 20 pass 1 took 91 us
 // with batches of eight against 74 us, pass 2 303 us with one load per thread in each step's
-// chain against 116 us).  Pass 1: workgroup t takes
+// chain against 116 us).  The tile's LDS leaves one workgroup per CU, so it has kPermThreads =
+// 1024 threads (16 waves) to keep enough loads in flight: 256 threads took 74.8 / 117.0 us for
+// the two passes, 512 61.9 / 97.1, 1024 58.3 / 93.6.  Pass 1: workgroup t takes
 // sources t TS.., orders them in LDS by destination tile and writes the ordered tile back to
 // the same range of the staging array (coalesced).
 constexpr int kPermBatch = 16;
+constexpr int kPermThreads = 1024;
 
 template <typename T, int TS>
-__global__ __launch_bounds__(kBlock) void k_perm_tiles(const T *__restrict__ x, const uint16_t *__restrict__ loc1,
+__global__ __launch_bounds__(kPermThreads) void k_perm_tiles(const T *__restrict__ x, const uint16_t *__restrict__ loc1,
                                                        int64_t n, T *__restrict__ stg) {
   constexpr int B = kPermBatch;
-  static_assert(TS % (kBlock * B) == 0, "whole batches per tile");
+  static_assert(TS % (kPermThreads * B) == 0, "whole batches per tile");
   __shared__ T lds[TS];
   const int64_t j0 = (int64_t)blockIdx.x * TS;
   const int cnt = (int)std::min<int64_t>(TS, n - j0);
@@ -475,29 +478,29 @@ __global__ __launch_bounds__(kBlock) void k_perm_tiles(const T *__restrict__ x, 
   const rsrc_t rx = make_rsrc(x + j0, (uint64_t)cnt * sizeof(T));
   const rsrc_t rl = make_rsrc(loc1 + j0, (uint64_t)cnt * 2);
   const int t = (int)threadIdx.x;
-  for (int q0 = 0; q0 < TS; q0 += kBlock * B) {
+  for (int q0 = 0; q0 < TS; q0 += kPermThreads * B) {
     T v[B];
     uint16_t l[B];
 #pragma unroll
     for (int i = 0; i < B; ++i) {
-      const uint32_t q = (uint32_t)(q0 + t + i * kBlock);
+      const uint32_t q = (uint32_t)(q0 + t + i * kPermThreads);
       v[i] = bload<T>(rx, q * (uint32_t)sizeof(T));
       l[i] = bload<uint16_t>(rl, q * 2u);
     }
 #pragma unroll
     for (int i = 0; i < B; ++i)
-      if (q0 + t + i * kBlock < cnt) lds[l[i]] = v[i];
+      if (q0 + t + i * kPermThreads < cnt) lds[l[i]] = v[i];
   }
   __syncthreads();
   T *ss = stg + (int64_t)blockIdx.x * (TS + kPermPad);
-  for (int q = t; q < cnt; q += kBlock) ss[q] = lds[q];
+  for (int q = t; q < cnt; q += kPermThreads) ss[q] = lds[q];
 }
 
 // Pass 2: workgroup b takes destination tile b: its elements, which lie in every staged source
 // tile as one run (src2: their staging addresses, run after run), to their places in LDS
 // (loc2), then the tile out (coalesced).
 template <typename T, int TS>
-__global__ __launch_bounds__(kBlock) void k_perm_buckets(const T *__restrict__ stg, int64_t n_stage,
+__global__ __launch_bounds__(kPermThreads) void k_perm_buckets(const T *__restrict__ stg, int64_t n_stage,
                                                          const int *__restrict__ src2,
                                                          const uint16_t *__restrict__ loc2, int64_t n,
                                                          T *__restrict__ y) {
@@ -505,7 +508,7 @@ __global__ __launch_bounds__(kBlock) void k_perm_buckets(const T *__restrict__ s
   // buffer resources: the staging array stays below 2 GiB; offsets past the tile load 0 and
   // are not stored)
   constexpr int B = kPermBatch;
-  static_assert(TS % (kBlock * B) == 0, "whole batches per tile");
+  static_assert(TS % (kPermThreads * B) == 0, "whole batches per tile");
   __shared__ T lds[TS];
   const int64_t k0 = (int64_t)blockIdx.x * TS;
   const int cnt = (int)std::min<int64_t>(TS, n - k0);
@@ -513,12 +516,12 @@ __global__ __launch_bounds__(kBlock) void k_perm_buckets(const T *__restrict__ s
   const rsrc_t rl = make_rsrc(loc2 + k0, (uint64_t)cnt * 2);
   const rsrc_t rs = make_rsrc(stg, (uint64_t)n_stage * sizeof(T));
   const int t = (int)threadIdx.x;
-  for (int q0 = 0; q0 < TS; q0 += kBlock * B) {
+  for (int q0 = 0; q0 < TS; q0 += kPermThreads * B) {
     uint32_t a[B];
     uint16_t l[B];
 #pragma unroll
     for (int i = 0; i < B; ++i) {
-      const uint32_t q = (uint32_t)(q0 + t + i * kBlock);
+      const uint32_t q = (uint32_t)(q0 + t + i * kPermThreads);
       a[i] = bload<uint32_t>(ra, q * 4u);
       l[i] = bload<uint16_t>(rl, q * 2u);
     }
@@ -527,11 +530,11 @@ __global__ __launch_bounds__(kBlock) void k_perm_buckets(const T *__restrict__ s
     for (int i = 0; i < B; ++i) v[i] = bload<T>(rs, a[i] * (uint32_t)sizeof(T));
 #pragma unroll
     for (int i = 0; i < B; ++i)
-      if (q0 + t + i * kBlock < cnt) lds[l[i]] = v[i];
+      if (q0 + t + i * kPermThreads < cnt) lds[l[i]] = v[i];
   }
   __syncthreads();
   T *ys = y + k0;
-  for (int q = t; q < cnt; q += kBlock) ys[q] = lds[q];
+  for (int q = t; q < cnt; q += kPermThreads) ys[q] = lds[q];
 }
 
 // out[e] = sum_c G[row_e, c] * X[col_e, c], entry-parallel: a group of L lanes per entry
@@ -949,10 +952,10 @@ void perm_apply(const PermPlan &P, const void *x, void *y, size_t esize, DevBuf 
   auto run = [&](auto tag, auto ts) {
     using T = decltype(tag);
     constexpr int TS = decltype(ts)::value;
-    hipLaunchKernelGGL((k_perm_tiles<T, TS>), dim3(tiles), dim3(kBlock), 0, s, static_cast<const T *>(x), l1, n,
+    hipLaunchKernelGGL((k_perm_tiles<T, TS>), dim3(tiles), dim3(kPermThreads), 0, s, static_cast<const T *>(x), l1, n,
                        stage.as<T>());
     SPOPS_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL((k_perm_buckets<T, TS>), dim3(tiles), dim3(kBlock), 0, s, stage.as<T>(),
+    hipLaunchKernelGGL((k_perm_buckets<T, TS>), dim3(tiles), dim3(kPermThreads), 0, s, stage.as<T>(),
                        (int64_t)(need / esize), s2, l2, n, static_cast<T *>(y));
     SPOPS_HIP_CHECK(hipGetLastError());
   };
